@@ -1,0 +1,1522 @@
+// CPU restatement of the Multivariate Cauchy Estimator term-propagation step
+// (natsnyder1/CauchyFriendly, include/cauchy_estimator.hpp and the headers it
+// drives).  TEST INFRASTRUCTURE ONLY: this is the checker that the HIP product
+// path is compared against; the product never links or calls it.
+//
+// It is a from-scratch restatement, single threaded, that reproduces the
+// reference's observable semantics, including the parts that depend on its
+// pthread decomposition:
+//   * the term ORDER produced by the T-thread TP->MU->MUC split
+//     (cauchy_estimator.hpp:854-855, 898-937, 1604, 1685-1704),
+//   * the 8-chunk fast-term-reduction and its flag-array merge tree, used when a
+//     shape bucket holds more than 10000 terms (term_reduction.hpp:592-695),
+//   * the per-thread b_pert vectors of the cell enumerator (cauchy_estimator.hpp:150-161).
+// Arithmetic follows the reference's operation order; the library is built with
+// -ffp-contract=off and uses GCC's complex type so that complex division is the
+// same libgcc routine the reference calls.
+#include "mce_oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+typedef __complex__ double cx;
+
+// libm's complex modulus, the routine the reference's cabs() resolves to.
+extern "C" double cabs(__complex__ double);
+
+namespace {
+
+inline cx mk(double r, double i) {
+    cx v;
+    __real__ v = r;
+    __imag__ v = i;
+    return v;
+}
+inline double cre(cx v) { return __real__ v; }
+inline double cim(cx v) { return __imag__ v; }
+inline cx cconj(cx v) {
+    __imag__ v = -__imag__ v;
+    return v;
+}
+
+// include/cauchy_constants.hpp:25-116
+const double kCoalignEps = 1e-8;
+const double kMuEps = 1e-10;
+const double kApproxEps = 1e-15;
+const double kPluEps = 1e-15;
+const double kCondEps = 1e12;
+const double kReductionEps = 1e-8;
+const double kInvTwoPi = 1.0 / (2.0 * M_PI);
+const int kCpus = 8;
+const int kMinTermsTp = 200;
+const int kMinTermsFtr = 10000;
+const double kFzImagToReal = 1e-3;
+const double kMeanImagHard = 0.001;
+const double kMeanImagToReal = 1e-1;
+const double kCovImagHard = 2000;
+const double kCovImagToReal = 10;
+const double kCovEigTol = -1e-5;
+enum {
+    kErrCovAny = 0, kErrCovFinal = 1, kErrCovNotFinal = 2, kErrCovDne = 3,
+    kErrMeanAny = 4, kErrMeanFinal = 5, kErrMeanNotFinal = 6, kErrMeanDne = 7,
+    kErrFzUnstable = 8, kErrFzNeg = 9
+};
+
+int g_search_order[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+
+// ---------------------------------------------------------------- small numerics
+// Sequential dot product, cauchy_linalg.hpp:559-565.
+inline double dotp(const double* x, const double* y, int n) {
+    double z = 0.0;
+    for (int i = 0; i < n; i++) z += x[i] * y[i];
+    return z;
+}
+
+// Column-sum one-norm, cauchy_linalg.hpp:1301-1314.
+double one_norm(const double* A, int n) {
+    double best = -1;
+    for (int c = 0; c < n; c++) {
+        double s = 0;
+        for (int r = 0; r < n; r++) s += fabs(A[r * n + c]);
+        if (s > best) best = s;
+    }
+    return best;
+}
+
+// Partial-pivot LU in place, cauchy_linalg.hpp:1180-1232.  Returns false when singular.
+bool lu_factor(double* A, int* piv, int n, double tol) {
+    double row[32];
+    for (int j = 0; j < n; ++j) {
+        double pv = tol;
+        int pi = -1;
+        for (int i = j; i < n; ++i)
+            if (fabs(A[i * n + j]) > fabs(pv)) {
+                pv = A[i * n + j];
+                pi = i;
+            }
+        if (pi == -1) return false;
+        if (pi != j) {
+            memcpy(row, A + j * n, n * sizeof(double));
+            memcpy(A + j * n, A + pi * n, n * sizeof(double));
+            memcpy(A + pi * n, row, n * sizeof(double));
+        }
+        piv[j] = pi;
+        for (int k = j + 1; k < n; ++k) {
+            A[k * n + j] /= A[j * n + j];
+            double t = A[k * n + j];
+            for (int q = j + 1; q < n; q++) A[k * n + q] -= t * A[j * n + q];
+        }
+    }
+    return true;
+}
+
+// Unit-lower forward / upper back substitution on packed LU, cauchy_linalg.hpp:1234-1256.
+void lu_forward(const double* LU, double* b, int n) {
+    for (int i = 0; i < n; i++) {
+        double s = b[i];
+        for (int j = 0; j < i; j++) s -= LU[i * n + j] * b[j];
+        b[i] = s;
+    }
+}
+void lu_backward(const double* LU, double* b, int n) {
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = n - 1; j > i; j--) s -= LU[i * n + j] * b[j];
+        b[i] = s / LU[i * n + i];
+    }
+}
+
+// Inverse row permutation of LAPACK-style pivots, cauchy_linalg.hpp:1258-1277.
+void pivot_inverse(const int* piv, int* pt, int n) {
+    int preg[32];
+    for (int i = 0; i < n; i++) preg[i] = i;
+    for (int i = 0; i < n; i++) std::swap(preg[i], preg[piv[i]]);
+    for (int i = 0; i < n; i++) pt[preg[i]] = i;
+}
+
+// One-norm condition number through an explicit inverse, cauchy_linalg.hpp:1337-1392.
+// A is left holding its LU factors.
+double cond1(double* A, int* piv, int n) {
+    double na = one_norm(A, n);
+    if (!lu_factor(A, piv, n, kPluEps)) return DBL_MAX;
+    double W[32 * 32];
+    memset(W, 0, n * n * sizeof(double));
+    int pt[32];
+    pivot_inverse(piv, pt, n);
+    for (int i = 0; i < n; i++) {
+        W[i * n + pt[i]] = 1;
+        lu_forward(A, W + i * n, n);
+        lu_backward(A, W + i * n, n);
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = i; j < n; j++) std::swap(W[i * n + j], W[j * n + i]);
+    return na * one_norm(W, n);
+}
+
+// PLU solve, cauchy_linalg.hpp:1280-1299.
+void lu_solve(const double* LU, const int* piv, const double* b, double* x, int n) {
+    int pt[32];
+    pivot_inverse(piv, pt, n);
+    for (int i = 0; i < n; i++) x[pt[i]] = b[i];
+    lu_forward(LU, x, n);
+    lu_backward(LU, x, n);
+}
+
+// Symmetric eigenvalues (cyclic Jacobi).  The reference uses tred2/tqli
+// (eig_solve.hpp:404-428); eigenvalues only feed the sign test against
+// COV_EIGENVALUE_TOLERANCE, so any accurate solver gives the same flags.
+void sym_eigvals(const double* S, double* ev, int n) {
+    std::vector<double> a(S, S + n * n);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+        if (off < 1e-300) break;
+        for (int pp = 0; pp < n; pp++)
+            for (int qq = pp + 1; qq < n; qq++) {
+                double apq = a[pp * n + qq];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (a[qq * n + qq] - a[pp * n + pp]) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = a[k * n + pp], akq = a[k * n + qq];
+                    a[k * n + pp] = c * akp - s * akq;
+                    a[k * n + qq] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a[pp * n + k], aqk = a[qq * n + k];
+                    a[pp * n + k] = c * apk - s * aqk;
+                    a[qq * n + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) ev[i] = a[i * n + i];
+}
+
+// Binomial coefficient and cell counts, cell_enumeration.hpp:11-56.
+unsigned long long binom(int n, int k) {
+    unsigned long long r = 1;
+    if (k > n - k) k = n - k;
+    for (int i = 0; i < k; ++i) {
+        r *= (n - i);
+        r /= (i + 1);
+    }
+    return r;
+}
+int cells_central(int m, int d) {
+    if (m < d) return 1 << m;
+    unsigned long long s = 0;
+    for (int i = 0; i < d; i++) s += binom(m - 1, i);
+    return (int)(2 * s);
+}
+int cells_general(int m, int d) {
+    if (m < d) return 1 << m;
+    unsigned long long s = 0;
+    for (int i = 0; i < d + 1; i++) s += binom(m, i);
+    return (int)s;
+}
+
+// All k-subsets of [0, n) in lexicographic order (the order std::prev_permutation
+// walks the bitmask in, cell_enumeration.hpp:71-92).
+std::vector<int> lex_combos(int n, int k) {
+    std::vector<int> out;
+    if (k > n || k <= 0) return out;
+    std::vector<int> c(k);
+    for (int i = 0; i < k; i++) c[i] = i;
+    while (true) {
+        out.insert(out.end(), c.begin(), c.end());
+        int i = k - 1;
+        while (i >= 0 && c[i] == n - k + i) i--;
+        if (i < 0) break;
+        c[i]++;
+        for (int j = i + 1; j < k; j++) c[j] = c[j - 1] + 1;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- g-tables
+struct GTab {
+    std::vector<uint32_t> key;  // ascending
+    std::vector<cx> val;
+};
+typedef std::shared_ptr<GTab> GTabP;
+typedef std::shared_ptr<std::vector<int>> BTabP;
+
+// Binary search on an ascending key array, gtable.hpp:283-301.
+int bsearch_key(const uint32_t* keys, uint32_t target, int size) {
+    int lo = 0, hi = size - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) / 2;
+        uint32_t k = keys[mid];
+        if (k == target) return mid;
+        if (k > target) hi = mid - 1;
+        else lo = mid + 1;
+    }
+    return -1;
+}
+
+// Half-storage numerator lookup, eval_gs.hpp:94-153 (g_num_binsearch, HALF_STORAGE).
+cx lookup_num(int enc, int top_bit, int rev_mask, const GTab* gp, int size) {
+    if (enc & top_bit) {
+        int i = bsearch_key(gp->key.data(), (uint32_t)(rev_mask ^ enc), size);
+        return i < 0 ? mk(0, 0) : cconj(gp->val[i]);
+    }
+    int i = bsearch_key(gp->key.data(), (uint32_t)enc, size);
+    return i < 0 ? mk(0, 0) : gp->val[i];
+}
+
+// ---------------------------------------------------------------- terms
+struct Term {
+    int m = 0;
+    std::vector<double> A, p, q, b;
+    double c = 0, dv = 0;
+    int phc = 0, pbc = 0, z = 0;
+    int enc_lhp = 0;
+    unsigned hflag = 0;
+    bool is_new = false;
+    bool has_cmap = false;
+    std::vector<uint8_t> cmap;
+    std::vector<int8_t> csmap;
+    GTabP gp;
+    int cells_p = 0;
+    BTabP B;
+    int cells = 0;
+    GTabP g;
+};
+
+struct Est {
+    int d, cmcc, pncc, p, n_mu, shape_range, max_shape;
+    int master_step = 0, Nt = 1, errs = 0, win_num = 0;
+    bool skip_post_mu = false;
+    std::vector<std::vector<Term>> bucket;  // by shape m
+    std::vector<double> root, A0, p0, b0;
+    std::vector<std::vector<double>> bpert;  // [kCpus][max_shape]
+    std::vector<std::vector<int>> combos;    // per m: binom(m,d) x d
+    std::vector<int> cen, gen;
+    cx fz, last_fz;
+    std::vector<cx> mean, cov, last_mean, last_cov;
+    double gscale = 0;
+    int threads_used = 1;
+    long long counters[3] = {0, 0, 0};
+    std::vector<int> skip_counts_cache;
+};
+
+double rand_u() { return ((double)rand() + 1.0) / ((double)RAND_MAX + 1.0); }  // random_variables.hpp:22-26
+
+// L1 row normalisation, cauchy_term.hpp:458-472.
+void normalize_rows(Term& t, int d, bool set_q) {
+    if (set_q) t.q.assign(t.p.begin(), t.p.begin() + t.m);
+    for (int i = 0; i < t.m; i++) {
+        double n1 = 0;
+        for (int j = 0; j < d; j++) n1 += fabs(t.A[i * d + j]);
+        t.p[i] *= n1;
+        for (int j = 0; j < d; j++) t.A[i * d + j] /= n1;
+    }
+}
+
+// Coalignment gate: all coordinates equal (pos) or opposite (neg) within eps,
+// cauchy_term.hpp:494-505.
+inline void coalign_gates(const double* r, const double* c, int d, bool& pos, bool& neg) {
+    pos = true;
+    neg = true;
+    for (int l = 0; l < d; l++) {
+        if (pos) pos &= fabs(r[l] - c[l]) < kCoalignEps;
+        if (neg) neg &= fabs(r[l] + c[l]) < kCoalignEps;
+        if (!(pos || neg)) break;
+    }
+}
+
+// Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.
+int precoalign_gamma(const double* Gamma, const double* beta, int pncc, int d,
+                     std::vector<double>& G, std::vector<double>& bt) {
+    G.assign(pncc * d, 0);
+    bt.assign(pncc, 0);
+    for (int i = 0; i < pncc; i++) {
+        for (int j = 0; j < d; j++) G[i * d + j] = Gamma[j * pncc + i];
+        bt[i] = beta[i];
+    }
+    for (int i = 0; i < pncc; i++) {
+        double nf = 0;
+        for (int j = 0; j < d; j++) nf += fabs(G[i * d + j]);
+        for (int j = 0; j < d; j++) G[i * d + j] /= nf;
+        bt[i] *= nf;
+    }
+    std::vector<char> F(pncc, 1);
+    for (int i = 0; i < pncc - 1; i++) {
+        if (!F[i]) continue;
+        for (int j = i + 1; j < pncc; j++) {
+            if (!F[j]) continue;
+            bool pos, neg;
+            coalign_gates(&G[i * d], &G[j * d], d, pos, neg);
+            if (pos || neg) {
+                F[j] = 0;
+                bt[i] += bt[j];
+            }
+        }
+    }
+    int cnt = 0;
+    for (int i = 0; i < pncc; i++) cnt += F[i];
+    if (cnt != pncc) {
+        int u = 1;
+        for (int j = 1; j < pncc; j++)
+            if (F[j]) {
+                if (u < j) {
+                    memcpy(&G[u * d], &G[j * d], d * sizeof(double));
+                    bt[u] = bt[j];
+                }
+                u++;
+            }
+        return cnt;
+    }
+    return pncc;
+}
+
+// A <- A Phi^T, b <- Phi b (+ B u), cauchy_term.hpp:435-456.
+void time_prop(Term& t, int d, const double* Phi, const double* Bm, const double* u, int cmcc) {
+    std::vector<double> W(t.A.begin(), t.A.begin() + t.m * d);
+    for (int i = 0; i < t.m; i++)
+        for (int j = 0; j < d; j++) {
+            double s = 0.0;
+            for (int k = 0; k < d; k++) s += W[i * d + k] * Phi[k + j * d];
+            t.A[i * d + j] = s;
+        }
+    std::vector<double> w(t.b.begin(), t.b.begin() + d);
+    for (int i = 0; i < d; i++) {
+        double s = 0.0;
+        for (int j = 0; j < d; j++) s += Phi[i * d + j] * w[j];
+        t.b[i] = s;
+    }
+    if (cmcc > 0) {
+        for (int i = 0; i < d; i++) {
+            double s = 0.0;
+            for (int j = 0; j < cmcc; j++) s += Bm[i * cmcc + j] * u[j];
+            w[i] = s;
+        }
+        for (int i = 0; i < d; i++) t.b[i] += 1.0 * w[i];
+    }
+}
+
+// Merge Gamma rows into coaligned hyperplanes or append them, cauchy_term.hpp:475-531.
+int tp_coalign(Term& t, int d, const std::vector<double>& G, const std::vector<double>& bt, int ng) {
+    normalize_rows(t, d, false);
+    std::vector<char> F(ng, 1);
+    for (int j = 0; j < ng; j++)
+        for (int k = 0; k < t.m; k++) {
+            if (!F[j]) continue;
+            bool pos, neg;
+            coalign_gates(&G[j * d], &t.A[k * d], d, pos, neg);
+            if (pos || neg) {
+                F[j] = 0;
+                t.p[k] += bt[j];
+            }
+        }
+    for (int i = 0; i < ng; i++)
+        if (F[i]) {
+            memcpy(&t.A[t.m * d], &G[i * d], d * sizeof(double));
+            t.p[t.m++] = bt[i];
+        }
+    return t.m;
+}
+
+// Cells of the time-propagated arrangement by vertex enumeration,
+// cell_enumeration.hpp:676-855 (make_time_prop_btable, STABLE_SOLVE branch).
+void tp_btable(Est& E, Term& t, const std::vector<double>& bpert) {
+    const int m = t.m, d = E.d;
+    std::vector<int>& out = *t.B;
+    if (m < d) {
+        // trivial set; the reference leaves cells_gtable unchanged here
+        int n = std::max(1 << (m - 1), t.cells);
+        out.resize(n);
+        for (int i = 0; i < n; i++) out[i] = i;
+        E.counters[2]++;
+        return;
+    }
+    const int phc = t.phc;
+    const std::vector<int>& comb = E.combos[m];
+    const int ncomb = (int)(comb.size() / d);
+    const int cells_gen = E.gen[m];
+    const int top_phc = 1 << (phc - 1), rev_phc = (1 << phc) - 1;
+    const int top_m = 1 << (m - 1), rev_m = (1 << m) - 1;
+    std::vector<char> seen((size_t)1 << m, 0);
+    std::vector<int> found;
+    std::vector<char> inset((size_t)1 << m, 0);
+    double Ac[32 * 32], bc[32], v[32];
+    int piv[32];
+    for (int i = 0; i < ncomb; i++) {
+        const int* cb = &comb[i * d];
+        for (int j = 0; j < d; j++) {
+            memcpy(Ac + j * d, &t.A[cb[j] * d], d * sizeof(double));
+            bc[j] = bpert[cb[j]];
+        }
+        if (cond1(Ac, piv, d) > kCondEps) continue;
+        lu_solve(Ac, piv, bc, v, d);
+        int niv = 0;
+        for (int ac = 0, ci = 0; ac < m; ac++) {
+            if (ci < d && cb[ci] == ac) {
+                ci++;
+                continue;
+            }
+            if ((dotp(&t.A[ac * d], v, d) - bpert[ac]) < 0) niv |= (1 << ac);
+        }
+        for (int s = 0; s < (1 << d); s++) {
+            int sv = niv;
+            for (int k = 0; k < d; k++)
+                if ((s >> k) & 1) sv |= (1 << cb[k]);
+            if (seen[sv]) continue;
+            seen[sv] = 1;
+            int psv = sv & rev_phc;
+            if (psv & top_phc) psv ^= rev_phc;
+            if (bsearch_key(t.gp->key.data(), (uint32_t)psv, t.cells_p) >= 0) {
+                found.push_back(sv);
+                inset[sv] = 1;
+            }
+        }
+        if ((int)found.size() == cells_gen) break;
+    }
+    std::vector<char> used(found.size(), 0);
+    int cnt = 0;
+    out.clear();
+    for (size_t i = 0; i < found.size(); i++) {
+        int b = found[i];
+        int br = b ^ rev_m;
+        if (!inset[b]) continue;  // already consumed as an antipode
+        if (inset[br]) {
+            inset[b] = 0;
+            inset[br] = 0;
+            out.push_back((b & top_m) ? br : b);
+            cnt++;
+        }
+    }
+    t.cells = cnt;
+}
+
+// Measurement update: the parent becomes child m, the integrable hyperplanes
+// spawn children 0..m-1, cauchy_term.hpp:84-310.
+int msmt_update(Term& P, std::vector<Term>& kids, int d, int cap, double msmt, const double* H, double gamma,
+                bool first, bool last) {
+    const int m = P.m;
+    std::vector<double> mu(P.A.begin(), P.A.begin() + m * d);
+    mu.resize((m + 1) * d, 0.0);
+    std::vector<double> rho(P.p.begin(), P.p.begin() + m);
+    rho.push_back(gamma);
+    std::vector<int> sgn(m);
+    std::vector<char> integ(m + 1, 1);
+    for (int l = 0; l < m; l++) {
+        double* mul = &mu[l * d];
+        double h = dotp(H, mul, d);
+        double ah = fabs(h);
+        if (ah < kMuEps) {
+            rho[l] = P.p[l];
+            sgn[l] = 1;
+            integ[l] = 0;
+        } else {
+            double r = 1.0 / h;
+            for (int i = 0; i < d; i++) mul[i] *= r;
+            rho[l] = P.p[l] * ah;
+            sgn[l] = (h > 0) ? 1 : -1;
+            integ[l] = 1;
+        }
+    }
+    const double zeta = msmt - dotp(H, P.b.data(), d);
+    const size_t base = kids.size();
+    for (int t = 0; t < m; t++)
+        if (integ[t]) kids.emplace_back();
+    size_t ci = base;
+    std::vector<double> bpar(P.b.begin(), P.b.begin() + d);
+    for (int t = 0; t <= m; t++) {
+        if (!integ[t]) continue;
+        Term& C = (t == m) ? P : kids[ci++];
+        if (t != m) {
+            C.m = m;
+            C.A.assign((size_t)cap * d, 0.0);
+            C.p.assign(cap, 0.0);
+            C.b.assign(d, 0.0);
+        }
+        if (!first) {
+            C.gp = P.gp;
+            C.cells_p = P.cells_p;
+            C.phc = P.phc;
+            C.pbc = m;
+            C.z = t;
+        }
+        C.c = zeta;
+        C.dv = rho[t];
+        const double* mut = &mu[t * d];
+        for (int i = 0; i < d; i++) C.b[i] = bpar[i] + zeta * mut[i];
+        unsigned hf = 0;
+        int l = 0;
+        for (int ll = 0; ll < m + 1; ll++) {
+            if (ll == t) continue;
+            C.p[l] = rho[ll];
+            if (integ[ll]) {
+                for (int i = 0; i < d; i++) C.A[l * d + i] = mu[ll * d + i] - mut[i];
+            } else {
+                for (int i = 0; i < d; i++) C.A[l * d + i] = mu[ll * d + i];
+                hf |= (1u << l);
+            }
+            l++;
+        }
+        C.hflag = hf;
+    }
+    P.is_new = false;
+    const int nkids = (int)(kids.size() - base);
+    if (!first) {
+        int sgn_enc = 0;
+        for (int l = 0; l < m; l++)
+            if (sgn[l] == -1) sgn_enc |= (1 << l);
+        P.enc_lhp = sgn_enc;
+        if (P.phc < m) P.enc_lhp &= (1 << P.phc) - 1;
+        if (sgn_enc & (1 << (m - 1))) sgn_enc ^= (1 << m) - 1;
+        for (size_t i = base; i < kids.size(); i++) {
+            kids[i].enc_lhp = P.enc_lhp;
+            kids[i].is_new = true;
+            kids[i].B = P.B;
+            kids[i].cells = P.cells;
+        }
+        if (!last) {
+            std::vector<int>& Bv = *P.B;
+            for (int i = 0; i < P.cells; i++) Bv[i] ^= sgn_enc;
+        }
+    }
+    return nkids;
+}
+
+// g-value and y_ei of one term at the fixed root point, cauchy_term.hpp:312-401.
+cx eval_g_yei(const Est& E, const Term& t, cx* yei, bool first) {
+    const int m = t.m, d = E.d;
+    std::vector<double> s(m), ty(d, 0.0);
+    double ygi;
+    if (t.hflag) {
+        ygi = 0;
+        for (int l = 0; l < m; l++) {
+            s[l] = dotp(&t.A[l * d], E.root.data(), d) > 0 ? 1 : -1;
+            double sc = t.p[l] * s[l];
+            for (int i = 0; i < d; i++) ty[i] += sc * t.A[l * d + i];
+            if (!(t.hflag & (1u << l))) ygi += t.p[l] * s[l];
+        }
+    } else {
+        for (int l = 0; l < m; l++) {
+            s[l] = dotp(&t.A[l * d], E.root.data(), d) > 0 ? 1 : -1;
+            double sc = t.p[l] * s[l];
+            for (int i = 0; i < d; i++) ty[i] += sc * t.A[l * d + i];
+        }
+        ygi = dotp(t.p.data(), s.data(), m);
+    }
+    cx gp, gm;
+    if (first) {
+        gp = mk(1, 0);
+        gm = mk(1, 0);
+    } else {
+        const int top = 1 << (t.phc - 1), rev = (1 << t.phc) - 1;
+        int lp = 0, lm = 0, k = 0;
+        for (int l = 0; l < m; l++) {
+            if (k < t.phc) {
+                if (k == t.z) {
+                    lm |= (1 << k);
+                    k++;
+                    if (k == t.phc) continue;
+                }
+                if (s[l] < 0) {
+                    lp |= (1 << k);
+                    lm |= (1 << k);
+                }
+                k++;
+            }
+        }
+        gp = lookup_num(lp ^ t.enc_lhp, top, rev, t.gp.get(), t.cells_p);
+        gm = lookup_num(lm ^ t.enc_lhp, top, rev, t.gp.get(), t.cells_p);
+    }
+    cx g = gp / mk(ygi + t.dv, t.c) - gm / mk(ygi - t.dv, t.c);
+    g *= kInvTwoPi;
+    for (int j = 0; j < d; j++) yei[j] = mk(-ty[j], t.b[j]);
+    return g;
+}
+
+// fz / mean / covariance accumulation of one term, cauchy_estimator.hpp:307-338.
+void accumulate(const Est& E, const Term& t, bool first, cx& fz, cx* mean, cx* cov) {
+    const int d = E.d;
+    std::vector<cx> y(d);
+    cx g = eval_g_yei(E, t, y.data(), first);
+    fz += g;
+    for (int j = 0; j < d; j++) {
+        cx yj = y[j];
+        mean[j] += g * yj;
+        for (int k = 0; k < d; k++) cov[j * d + k] -= g * yj * y[k];
+    }
+}
+
+// Measurement-update coalignment, cauchy_term.hpp:533-745.  Returns the new shape.
+int mu_coalign(Term& t, int d) {
+    normalize_rows(t, d, true);
+    const int m = t.m;
+    std::vector<char> F(m, 1);
+    t.cmap.assign(m, 255);
+    t.csmap.assign(m, 1);
+    std::vector<char> hu(m, 0);
+    if (t.hflag)
+        for (int j = 0; j < m; j++) hu[j] = (t.hflag >> j) & 1;
+    int u = 0;
+    for (int j = 0; j < m - 1; j++) {
+        if (!F[j]) continue;
+        t.cmap[j] = (uint8_t)u;
+        for (int k = j + 1; k < m; k++) {
+            if (!F[k]) continue;
+            bool pos, neg;
+            coalign_gates(&t.A[j * d], &t.A[k * d], d, pos, neg);
+            if (pos) {
+                if (t.hflag) {
+                    bool hj = hu[j], hk = hu[k];
+                    if (!hj && !hk) t.q[j] += t.q[k];
+                    else if (hj && !hk) {
+                        t.q[j] = t.q[k];
+                        hu[j] = 0;
+                    } else if (!hj && hk) {
+                        hu[k] = 0;
+                    } else {
+                        hu[k] = 0;
+                        t.q[j] += t.q[k];
+                    }
+                } else {
+                    t.q[j] += t.q[k];
+                }
+                F[k] = 0;
+                t.p[j] += t.p[k];
+                t.cmap[k] = (uint8_t)u;
+                t.csmap[k] = 1;
+            }
+            if (neg) {
+                if (t.hflag) {
+                    bool hj = hu[j], hk = hu[k];
+                    if (!hj && !hk) t.q[j] -= t.q[k];
+                    else if (hj && !hk) {
+                        t.q[j] = -t.q[k];
+                        hu[j] = 0;
+                    } else if (!hj && hk) {
+                        hu[k] = 0;
+                    } else {
+                        fprintf(stderr, "mce_oracle: two H-orthogonal hyperplanes coalign negatively\n");
+                        abort();
+                    }
+                } else {
+                    t.q[j] -= t.q[k];
+                }
+                F[k] = 0;
+                t.p[j] += t.p[k];
+                t.cmap[k] = (uint8_t)u;
+                t.csmap[k] = -1;
+            }
+        }
+        u += 1;
+    }
+    if (F[m - 1]) t.cmap[m - 1] = (uint8_t)u;
+    int ns = 0;
+    for (int i = 0; i < m; i++) ns += F[i];
+    if (ns != m) {
+        int w = 1;
+        for (int j = 1; j < m; j++)
+            if (F[j]) {
+                if (w < j) {
+                    memcpy(&t.A[w * d], &t.A[j * d], d * sizeof(double));
+                    t.p[w] = t.p[j];
+                    t.q[w] = t.q[j];
+                }
+                w++;
+            }
+        if (t.hflag) {
+            unsigned nh = 0;
+            int c = 0;
+            for (int j = 0; j < m; j++)
+                if (F[j]) nh |= ((unsigned)hu[j] << c++);
+            t.hflag = nh;
+        }
+        t.m = ns;
+    }
+    return t.m;
+}
+
+// g-table of a first-step term, flattening.hpp:14-67.
+void make_gtable_first(Term& t, int d, double gscale) {
+    const int m = d;
+    const int n = t.cells;
+    t.g = std::make_shared<GTab>();
+    t.g->key.resize(n);
+    t.g->val.resize(n);
+    for (int j = 0; j < n; j++) {
+        double ygi = 0;
+        for (int k = 0; k < m; k++)
+            if (!(t.hflag & (1u << k))) {
+                double sb = ((j >> k) & 1) == 0 ? 1 : -1;
+                ygi += t.p[k] * sb;
+            }
+        cx v = 1.0 / mk(ygi + t.dv, t.c) - 1.0 / mk(ygi - t.dv, t.c);
+        v *= gscale;
+        t.g->key[j] = (uint32_t)j;
+        t.g->val[j] = v;
+    }
+}
+
+// g-table of a term after MU over its B-table, flattening.hpp:77-255.
+// Returns true when the term is negligible (term approximation).
+bool make_gtable(Term& t, double gscale) {
+    const int m = t.m, phc = t.phc;
+    const int top_phc = 1 << (phc - 1), rev_phc = (1 << phc) - 1;
+    double ps = 0;
+    for (int i = 0; i < m; i++) ps += t.p[i];
+    ps *= ps;
+    bool negl = true;
+    const int n = t.cells;
+    std::vector<std::pair<uint32_t, cx>> kv(n);
+    const std::vector<int>& Bv = *t.B;
+    std::vector<int> sb(m);
+    for (int j = 0; j < n; j++) {
+        const int be = Bv[j];
+        double ygi = 0;
+        for (int k = 0; k < m; k++) {
+            sb[k] = ((be >> k) & 1) ? -1 : 1;
+            if (!(t.hflag & (1u << k))) ygi += t.q[k] * sb[k];
+        }
+        int lp = 0, lm = 0;
+        if (!t.is_new) {
+            lp = be & rev_phc;
+            lm = lp;
+        } else {
+            int k = 0, l = 0;
+            while (k < phc) {
+                if (k == t.z) {
+                    lm |= (1 << k);
+                    k++;
+                    if (k == phc) break;
+                }
+                int bv = t.has_cmap ? sb[t.cmap[l]] * t.csmap[l] : sb[l];
+                if (bv < 0) {
+                    lp |= (1 << k);
+                    lm |= (1 << k);
+                }
+                k++;
+                l++;
+            }
+        }
+        cx gpv = lookup_num(lp ^ t.enc_lhp, top_phc, rev_phc, t.gp.get(), t.cells_p);
+        cx gmv = lookup_num(lm ^ t.enc_lhp, top_phc, rev_phc, t.gp.get(), t.cells_p);
+        cx v = gpv / mk(ygi + t.dv, t.c) - gmv / mk(ygi - t.dv, t.c);
+        v *= gscale;
+        kv[j] = std::make_pair((uint32_t)be, v);
+        if (negl && (ps * cabs(v)) > kApproxEps) negl = false;
+    }
+    std::sort(kv.begin(), kv.end(),
+              [](const std::pair<uint32_t, cx>& a, const std::pair<uint32_t, cx>& b) { return a.first < b.first; });
+    t.g = std::make_shared<GTab>();
+    t.g->key.resize(n);
+    t.g->val.resize(n);
+    for (int j = 0; j < n; j++) {
+        t.g->key[j] = kv[j].first;
+        t.g->val[j] = kv[j].second;
+    }
+    return negl;
+}
+
+// Orientation mask between two equal-up-to-sign arrangements: sign of the first
+// element that is non-negligible in both rows, cell_enumeration.hpp:586-599.
+int orientation_mask(const double* Ai, const double* Aj, int m, int d) {
+    int sig = 0;
+    for (int k = 0; k < m; k++) {
+        int l = 0;
+        const int kd = k * d;
+        while (kd + l < m * d && ((fabs(Ai[kd + l]) < kReductionEps) || (fabs(Aj[kd + l]) < kReductionEps))) l++;
+        if (kd + l >= m * d) continue;
+        if ((Ai[kd + l] * Aj[kd + l]) < 0) sig |= (1 << k);
+    }
+    return sig;
+}
+
+// Re-orient a B-table, cell_enumeration.hpp:584-625.  dst == nullptr updates src in place.
+void update_btable(const double* Ai, std::vector<int>& src, const double* Aj, std::vector<int>* dst, int ncells,
+                   int m, int d) {
+    int sig = orientation_mask(Ai, Aj, m, d);
+    if (sig & (1 << (m - 1))) sig ^= (1 << m) - 1;
+    if (dst == nullptr) {
+        if (sig)
+            for (int k = 0; k < ncells; k++) src[k] ^= sig;
+    } else {
+        if ((int)dst->size() < ncells) dst->resize(ncells);
+        for (int k = 0; k < ncells; k++) (*dst)[k] = src[k] ^ sig;
+    }
+}
+
+// Sum a member's g-table into the root's, flattening.hpp:259-315.
+void add_gtables(Term& ti, const Term& tj, int d) {
+    const int m = ti.m;
+    const int size = ti.cells;
+    const int top = 1 << (m - 1), rev = (1 << m) - 1;
+    const int sig = orientation_mask(ti.A.data(), tj.A.data(), m, d);
+    const std::vector<int>& Bi = *ti.B;
+    for (int k = 0; k < ti.cells; k++) {
+        int bi = Bi[k];
+        int bj = bi ^ sig;
+        bool cj = false;
+        if (bj & top) {
+            cj = true;
+            bj ^= rev;
+        }
+        int ii = bsearch_key(ti.g->key.data(), (uint32_t)bi, size);
+        int jj = bsearch_key(tj.g->key.data(), (uint32_t)bj, std::min(size, (int)tj.g->key.size()));
+        if (ii > -1 && jj > -1) ti.g->val[ii] += cj ? cconj(tj.g->val[jj]) : tj.g->val[jj];
+    }
+}
+
+// B-table of a new child from its parent's (post-MU) B-table, cell_enumeration.hpp:206-371.
+void child_btable(Term& t, int d, const std::vector<int>& Bmu, int cells_parent) {
+    const int m = t.m;
+    std::vector<int>& out = *t.B;
+    out.clear();
+    if (m <= d) {
+        int n = 1 << (m - 1);
+        for (int i = 0; i < n; i++) out.push_back(i);
+        t.cells = n;
+        return;
+    }
+    const int pbc = t.pbc, z = t.z;
+    const bool coal = m < pbc;
+    const int* members = Bmu.data();
+    // (value, index) sorted by value, then index: a lookup returns the LAST index
+    // holding a value, as the reference's overwrite-on-insert hash table does.
+    std::vector<std::pair<int, int>> byval(cells_parent);
+    for (int j = 0; j < cells_parent; j++) byval[j] = std::make_pair(members[j], j);
+    std::sort(byval.begin(), byval.end());
+    const int mask_z = 1 << z, top = 1 << (pbc - 1), rev = (1 << pbc) - 1, low = (1 << z) - 1;
+    std::vector<char> done(cells_parent, 0);
+    std::vector<int> uc;
+    for (int j = 0; j < cells_parent; j++) {
+        if (done[j]) continue;
+        int b = members[j];
+        int q = b ^ mask_z;
+        if (q & top) q ^= rev;
+        auto it = std::upper_bound(byval.begin(), byval.end(), std::make_pair(q, INT32_MAX));
+        int hit = -1;
+        if (it != byval.begin() && (it - 1)->first == q) hit = (it - 1)->second;
+        if (hit < 0) continue;
+        done[j] = 1;
+        done[hit] = 1;
+        int zb = (b & mask_z) >> z;
+        int c1 = ((b >> (z + 1)) << z) | (b & low) | (zb << (pbc - 1));
+        int c2 = c1 ^ top;
+        uc.push_back((c1 & top) ? (c1 ^ rev) : c1);
+        uc.push_back((c2 & top) ? (c2 ^ rev) : c2);
+    }
+    if (coal) {
+        std::vector<char> Fc(pbc, 1);
+        std::vector<int> bitpos;
+        for (int j = 0; j < pbc; j++) {
+            int ci = t.cmap[j];
+            if (Fc[ci]) {
+                Fc[ci] = 0;
+                bitpos.push_back(j);
+            }
+        }
+        std::vector<int> seen;
+        seen.reserve(uc.size());
+        for (size_t j = 0; j < uc.size(); j++) {
+            int b = uc[j], bc = 0;
+            for (size_t l = 0; l < bitpos.size(); l++)
+                if (b & (1 << bitpos[l])) bc |= (1 << l);
+            seen.push_back(bc);
+        }
+        // first occurrence of each projected sign vector, in generation order
+        std::vector<std::pair<int, int>> sv(seen.size());
+        for (size_t j = 0; j < seen.size(); j++) sv[j] = std::make_pair(seen[j], (int)j);
+        std::sort(sv.begin(), sv.end());
+        std::vector<char> keep(seen.size(), 0);
+        for (size_t j = 0; j < sv.size(); j++)
+            if (j == 0 || sv[j].first != sv[j - 1].first) keep[sv[j].second] = 1;
+        for (size_t j = 0; j < seen.size(); j++)
+            if (keep[j]) out.push_back(seen[j]);
+        t.cells = (int)out.size();
+    } else {
+        out = uc;
+        t.cells = (int)out.size();
+    }
+}
+
+// ---------------------------------------------------------------- fast term reduction
+// Greedy reduction over roots [start, end) into flag array F, term_reduction.hpp:159-276.
+// Candidate j of root i: j > i, F[j] == j, and on every axis (b_i - ep) < b_j <= (b_i + ep)
+// (the window ftr_binary_search selects, term_reduction.hpp:89-157), then the p check and
+// the element-wise |A_i[k] -/+ A_j[k]| < ep check over the first m flattened entries.
+void ftr_range(const std::vector<Term>& T, std::vector<int>& F, const std::vector<int>& order0,
+               const std::vector<double>& vals, int d, int m, int start, int end) {
+    const double ep = kReductionEps;
+    const int a0 = g_search_order[0];
+    for (int i = start; i < end; i++) {
+        if (F[i] != i) continue;
+        const Term& ti = T[i];
+        const double lo = ti.b[a0] - ep, hi = ti.b[a0] + ep;
+        int r0 = (int)(std::upper_bound(vals.begin(), vals.end(), lo) - vals.begin());
+        int r1 = (int)(std::upper_bound(vals.begin(), vals.end(), hi) - vals.begin());
+        // construct_candidate_list only scans a window holding at least two points
+        // (term_reduction.hpp:120: gti - lti > 2)
+        if (r1 - r0 < 2) continue;
+        for (int r = r0; r < r1; r++) {
+            int j = order0[r];
+            if (j <= i || F[j] != j) continue;
+            bool ok = true;
+            for (int a = 1; a < d && ok; a++) {
+                int ax = g_search_order[a];
+                double l2 = ti.b[ax] - ep, h2 = ti.b[ax] + ep;
+                double v = T[j].b[ax];
+                if (!(v > l2 && v <= h2)) ok = false;
+            }
+            if (!ok) continue;
+            for (int k = 0; k < m; k++)
+                if (fabs(T[j].p[k] - ti.p[k]) > ep) {
+                    ok = false;
+                    break;
+                }
+            if (!ok) continue;
+            for (int k = 0; k < m; k++) {
+                double x = ti.A[k], y = T[j].A[k];
+                bool pos = fabs(x - y) < ep, neg = fabs(x + y) < ep;
+                if (d == 0) pos = neg = true;
+                if (!(pos || neg)) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok) F[j] = i;
+        }
+    }
+}
+
+std::vector<int> fast_term_reduction(const std::vector<Term>& T, int d, int m) {
+    const int n = (int)T.size();
+    std::vector<int> order0(n);
+    for (int i = 0; i < n; i++) order0[i] = i;
+    const int a0 = g_search_order[0];
+    std::stable_sort(order0.begin(), order0.end(), [&](int x, int y) { return T[x].b[a0] < T[y].b[a0]; });
+    std::vector<double> vals(n);
+    for (int r = 0; r < n; r++) vals[r] = T[order0[r]].b[a0];
+    std::vector<int> F(n);
+    for (int i = 0; i < n; i++) F[i] = i;
+    if (n > kMinTermsFtr) {
+        // 8 chunks with private flag arrays, then the pairwise merge tree
+        // (term_reduction.hpp:592-695).
+        const int chunk = n / kCpus;
+        std::vector<std::vector<int>> Fs(kCpus, F);
+        std::vector<int> starts(kCpus);
+        for (int c = 0; c < kCpus; c++) {
+            starts[c] = c * chunk;
+            int end = (c == kCpus - 1) ? n : (c + 1) * chunk;
+            ftr_range(T, Fs[c], order0, vals, d, m, starts[c], end);
+        }
+        for (int lvl = 0; (1 << lvl) < kCpus; lvl++) {
+            int cmps = kCpus / (1 << (lvl + 1));
+            for (int k = 0; k < cmps; k++) {
+                int rt = k * (1 << (lvl + 1));
+                int cm = rt + (1 << lvl);
+                std::vector<int>& F1 = Fs[rt];
+                const std::vector<int>& F2 = Fs[cm];
+                for (int i = starts[cm]; i < n; i++) {
+                    if (F2[i] == -1 || F2[i] == i || F1[i] != i) continue;
+                    F1[i] = (F1[F2[i]] == F2[i]) ? F2[i] : F1[F2[i]];
+                }
+            }
+        }
+        return Fs[0];
+    }
+    ftr_range(T, F, order0, vals, d, m, 0, n);
+    return F;
+}
+
+// ---------------------------------------------------------------- step phases
+void finalize_and_check(Est& E) {
+    const int d = E.d;
+    E.gscale = kInvTwoPi / cre(E.fz);
+    cx ifz = mk(0, cre(E.fz));
+    for (int i = 0; i < d; i++) E.mean[i] /= ifz;
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) E.cov[i * d + j] = (E.cov[i * d + j] / E.fz) - E.mean[i] * E.mean[j];
+    // moments_numerical_check, cauchy_estimator.hpp:359-461
+    const bool first_of_step = (E.master_step % E.p) == 0;
+    const bool not_last = (E.master_step % E.p) != (E.p - 1);
+    const bool last = (E.master_step % E.p) == (E.p - 1);
+    if (first_of_step) {
+        E.errs |= (1 << kErrMeanDne);
+        E.errs |= (1 << kErrCovDne);
+        E.errs &= ~(1 << kErrMeanFinal);
+        E.errs &= ~(1 << kErrMeanNotFinal);
+        E.errs &= ~(1 << kErrCovNotFinal);
+        E.errs &= ~(1 << kErrCovFinal);
+    }
+    int ne = 0;
+    if (cre(E.fz) <= 0) ne |= (1 << kErrFzNeg);
+    if (fabs(cim(E.fz) / (1e-15 + cre(E.fz))) > kFzImagToReal) ne |= (1 << kErrFzUnstable);
+    bool mean_ok = true;
+    for (int i = 0; i < d; i++) {
+        double r = fabs(cre(E.mean[i])), im = fabs(cim(E.mean[i]));
+        double ratio = im / (1e-15 + r);
+        if ((ratio > kMeanImagToReal) || (im > kMeanImagHard)) {
+            ne |= (1 << kErrMeanAny);
+            if (not_last) ne |= (1 << kErrMeanNotFinal);
+            if (last) ne |= (1 << kErrMeanFinal);
+            mean_ok = false;
+        }
+    }
+    // covariance_checker, cauchy_util.hpp:1882-2004
+    int cf = 0;
+    {
+        std::vector<double> cr(d * d), ci(d * d), ev(d);
+        for (int i = 0; i < d * d; i++) {
+            cr[i] = cre(E.cov[i]);
+            ci[i] = cim(E.cov[i]);
+        }
+        sym_eigvals(cr.data(), ev.data(), d);
+        for (int i = 0; i < d; i++)
+            if (ev[i] < kCovEigTol) cf |= 1;
+        for (int i = 0; i < d - 1; i++) {
+            double sii = sqrt(cr[i * d + i]);
+            for (int j = i + 1; j < d; j++) {
+                double sjj = sqrt(cr[j * d + j]);
+                double corr = cr[i * d + j] / (sii * sjj);
+                if (fabs(corr) > 1) cf |= 2;
+            }
+        }
+        for (int i = 0; i < d; i++)
+            for (int j = i; j < d; j++) {
+                double ratio = fabs(ci[i * d + j]) / (fabs(cr[i * d + j]) + 1e-15);
+                if (ratio > kCovImagToReal) cf |= 4;
+            }
+        for (int i = 0; i < d * d; i++)
+            if (fabs(ci[i]) > kCovImagHard) cf |= 8;
+    }
+    bool cov_ok = true;
+    if (cf) {
+        ne |= (1 << kErrCovAny);
+        if (not_last) ne |= (1 << kErrCovNotFinal);
+        if (last) ne |= (1 << kErrCovFinal);
+        cov_ok = false;
+    }
+    E.errs |= ne;
+    if (mean_ok) E.errs &= ~(1 << kErrMeanDne);
+    if (cov_ok) E.errs &= ~(1 << kErrCovDne);
+    if (mean_ok) E.last_mean = E.mean;
+    else E.mean = E.last_mean;
+    if (cov_ok) E.last_cov = E.cov;
+    else E.cov = E.last_cov;
+    if (!mean_ok && !cov_ok) E.fz = E.last_fz;
+    else E.last_fz = E.fz;
+}
+
+Term first_term(const Est& E) {
+    Term t;
+    const int d = E.d;
+    t.m = d;
+    t.A.assign((size_t)E.max_shape * d, 0.0);
+    t.p.assign(E.max_shape, 0.0);
+    t.b.assign(d, 0.0);
+    memcpy(t.A.data(), E.A0.data(), d * d * sizeof(double));
+    memcpy(t.p.data(), E.p0.data(), d * sizeof(double));
+    memcpy(t.b.data(), E.b0.data(), d * sizeof(double));
+    return t;
+}
+
+void step_first(Est& E, double msmt, const double* H, double gamma) {
+    const int d = E.d;
+    std::vector<Term> kids;
+    Term& P = E.bucket[d][0];
+    msmt_update(P, kids, d, E.max_shape, msmt, H, gamma, true, false);
+    for (auto& k : kids) E.bucket[d].push_back(std::move(k));
+    E.Nt = (int)E.bucket[d].size();
+    // compute_moments(true), cauchy_estimator.hpp:524-602
+    E.fz = mk(0, 0);
+    std::fill(E.mean.begin(), E.mean.end(), mk(0, 0));
+    std::fill(E.cov.begin(), E.cov.end(), mk(0, 0));
+    for (auto& t : E.bucket[d]) accumulate(E, t, true, E.fz, E.mean.data(), E.cov.data());
+    E.gscale = kInvTwoPi / cre(E.fz);
+    cx ifz = mk(0, cre(E.fz));
+    for (int i = 0; i < d; i++) E.mean[i] /= ifz;
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) E.cov[i * d + j] = (E.cov[i * d + j] / E.fz) - E.mean[i] * E.mean[j];
+    for (auto& t : E.bucket[d]) {
+        t.cells = cells_central(d, d) / 2;
+        t.B = std::make_shared<std::vector<int>>(t.cells);
+        for (int j = 0; j < t.cells; j++) (*t.B)[j] = j;
+        make_gtable_first(t, d, E.gscale);
+        t.phc = t.m;
+        t.cells_p = t.cells;
+        t.gp = t.g;
+        t.g.reset();
+        t.is_new = false;
+    }
+}
+
+struct StepRecord {
+    cx fz_pre;
+    std::vector<cx> mean, cov;
+    int nt_mu;
+    std::vector<int> shape_mu;
+};
+
+void step_tp_to_muc(Est& E, double msmt, const double* Phi, const double* Gamma, const double* beta,
+                    const double* H, double gamma, const double* Bm, const double* u) {
+    const int d = E.d, sr = E.shape_range;
+    const bool with_tp = (E.master_step % E.p) == 0;
+    std::vector<double> G, bt;
+    int ng = 0;
+    if (with_tp) ng = precoalign_gamma(Gamma, beta, E.pncc, d, G, bt);
+    const bool threaded = E.Nt >= kMinTermsTp;
+    int T = 1;
+    if (threaded) {
+        int chunks = (E.Nt + kMinTermsTp - 1) / kMinTermsTp;
+        T = chunks > kCpus ? kCpus : chunks;
+    }
+    E.threads_used = threaded ? T : 1;
+    std::vector<std::vector<std::vector<Term>>> per_tid(T, std::vector<std::vector<Term>>(sr));
+    std::vector<cx> fz_c(T, mk(0, 0));
+    std::vector<std::vector<cx>> mean_c(T, std::vector<cx>(d, mk(0, 0))), cov_c(T, std::vector<cx>(d * d, mk(0, 0)));
+    std::vector<int> skip_counts(sr, 0);
+    for (int tid = 0; tid < T; tid++) {
+        std::vector<Term> kids_all;
+        std::vector<std::vector<Term>>& nb = per_tid[tid];
+        for (int m = 1; m < sr; m++) {
+            std::vector<Term>& bk = E.bucket[m];
+            for (int i = tid; i < (int)bk.size(); i += T) {
+                Term& P = bk[i];
+                if (with_tp) {
+                    time_prop(P, d, Phi, Bm, u, E.cmcc);
+                    tp_coalign(P, d, G, bt, ng);
+                    if (!E.skip_post_mu) {
+                        if (threaded && P.m <= d) {
+                            auto nbt = std::make_shared<std::vector<int>>(P.cells_p);
+                            for (int j = 0; j < P.cells_p; j++) (*nbt)[j] = j;
+                            P.B = nbt;
+                            E.counters[2]++;
+                        } else if (P.m == P.phc) {
+                            auto nbt = std::make_shared<std::vector<int>>(P.B->begin(), P.B->begin() + P.cells_p);
+                            P.B = nbt;
+                        } else {
+                            P.B = std::make_shared<std::vector<int>>();
+                            tp_btable(E, P, E.bpert[threaded ? tid : 0]);
+                        }
+                    }
+                }
+                const int m_pre = P.m;
+                std::vector<Term> kids;
+                int nk = msmt_update(P, kids, d, E.max_shape, msmt, H, gamma, false, E.skip_post_mu);
+                accumulate(E, P, false, fz_c[tid], mean_c[tid].data(), cov_c[tid].data());
+                for (auto& k : kids) accumulate(E, k, false, fz_c[tid], mean_c[tid].data(), cov_c[tid].data());
+                if (!E.skip_post_mu) {
+                    normalize_rows(P, d, true);
+                    for (auto& k : kids) {
+                        mu_coalign(k, d);
+                        k.has_cmap = k.m < m_pre;
+                    }
+                    for (auto& k : kids) kids_all.push_back(std::move(k));
+                } else {
+                    skip_counts[P.m] += nk + 1;
+                }
+            }
+        }
+        if (!E.skip_post_mu) {
+            // parents of this tid by their new shape, then its children (cauchy_estimator.hpp:1683-1704)
+            for (int m = 1; m < sr; m++) {
+                std::vector<Term>& bk = E.bucket[m];
+                for (int i = tid; i < (int)bk.size(); i += T) nb[bk[i].m].push_back(std::move(bk[i]));
+            }
+            for (auto& k : kids_all) nb[k.m].push_back(std::move(k));
+        }
+    }
+    E.fz = mk(0, 0);
+    std::fill(E.mean.begin(), E.mean.end(), mk(0, 0));
+    std::fill(E.cov.begin(), E.cov.end(), mk(0, 0));
+    for (int tid = 0; tid < T; tid++) {
+        E.fz += fz_c[tid];
+        for (int j = 0; j < d; j++) E.mean[j] += mean_c[tid][j];
+        for (int j = 0; j < d * d; j++) E.cov[j] += cov_c[tid][j];
+    }
+    finalize_and_check(E);
+    if (!E.skip_post_mu) {
+        std::vector<std::vector<Term>> nbk(sr);
+        E.Nt = 0;
+        for (int m = 0; m < sr; m++) {
+            for (int tid = 0; tid < T; tid++)
+                for (auto& t : per_tid[tid][m]) nbk[m].push_back(std::move(t));
+            E.Nt += (int)nbk[m].size();
+        }
+        E.bucket.swap(nbk);
+    } else {
+        E.Nt = 0;
+        for (int m = 0; m < sr; m++) E.Nt += skip_counts[m];
+        E.skip_counts_cache = skip_counts;
+    }
+}
+
+void ftr_and_gtables(Est& E) {
+    const int d = E.d, sr = E.shape_range;
+    int nt = 0;
+    for (int m = 0; m < sr; m++) {
+        std::vector<Term>& T = E.bucket[m];
+        const int N = (int)T.size();
+        if (N == 0) continue;
+        std::vector<int> F = fast_term_reduction(T, d, m);
+        // ForwardFlagArray, cauchy_util.hpp:349-408
+        std::vector<std::vector<int>> members(N);
+        for (int j = 0; j < N; j++)
+            if (F[j] != j) members[F[j]].push_back(j);
+        std::vector<Term> out;
+        // make_gtables, flattening.hpp:318-568 (output order = ascending root index
+        // also for the threaded variant, flattening.hpp:615-694)
+        for (int j = 0; j < N; j++) {
+            if (F[j] != j) continue;
+            int rt = j;
+            Term* cj = &T[j];
+            if (cj->is_new) {
+                BTabP parentB = cj->B;
+                int np = cj->cells;
+                cj->B = std::make_shared<std::vector<int>>();
+                child_btable(*cj, d, *parentB, np);
+            }
+            if (make_gtable(*cj, E.gscale)) rt = -1;
+            const std::vector<int>& mem = members[j];
+            size_t k = 0;
+            if (rt == -1) {
+                E.counters[0]++;
+                int ncg = cj->cells;
+                BTabP btg = cj->B;
+                const double* Alfr = cj->A.data();
+                while (k < mem.size()) {
+                    int cp = mem[k++];
+                    Term* ck = &T[cp];
+                    if (ck->is_new) {
+                        ck->B = btg;
+                        ck->cells = ncg;
+                        update_btable(Alfr, *ck->B, ck->A.data(), nullptr, ck->cells, m, d);
+                    } else {
+                        E.counters[1]++;
+                        btg = ck->B;
+                        if (ck->cells != ncg) {
+                            E.counters[2]++;
+                            ncg = ck->cells;
+                        }
+                    }
+                    if (!make_gtable(*ck, E.gscale)) {
+                        rt = cp;
+                        cj = ck;
+                        break;
+                    } else {
+                        Alfr = ck->A.data();
+                    }
+                }
+            }
+            while (k < mem.size()) {
+                int cp = mem[k++];
+                Term* ck = &T[cp];
+                if (ck->is_new) {
+                    ck->cells = cj->cells;
+                    auto nb = std::make_shared<std::vector<int>>();
+                    update_btable(cj->A.data(), *cj->B, ck->A.data(), nb.get(), ck->cells, m, d);
+                    ck->B = nb;
+                } else {
+                    E.counters[1]++;
+                    if (ck->cells != cj->cells) {
+                        E.counters[2]++;
+                        if (ck->cells > cj->cells) {
+                            ck->cells = cj->cells;
+                            update_btable(cj->A.data(), *cj->B, ck->A.data(), ck->B.get(), ck->cells, m, d);
+                        } else {
+                            ck->cells = cj->cells;
+                            auto nb = std::make_shared<std::vector<int>>();
+                            update_btable(cj->A.data(), *cj->B, ck->A.data(), nb.get(), ck->cells, m, d);
+                            ck->B = nb;
+                        }
+                    }
+                }
+                if (!make_gtable(*ck, E.gscale)) add_gtables(*cj, *ck, d);
+            }
+            if (rt != -1) {
+                // become_parent, cauchy_term.hpp:748-755
+                cj->phc = cj->m;
+                cj->cells_p = cj->cells;
+                cj->gp = cj->g;
+                cj->g.reset();
+                cj->is_new = false;
+                cj->has_cmap = false;
+                out.push_back(*cj);
+            }
+        }
+        nt += (int)out.size();
+        T.swap(out);
+    }
+    E.Nt = nt;
+    E.fz = mk(1, 0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C API
+extern "C" {
+
+void* oracle_create(int num_steps, int d, int cmcc, int pncc, int p, const double* A0, const double* p0,
+                    const double* b0, int use_seed, unsigned int seed) {
+    if (use_seed) srand(seed);
+    Est* E = new Est;
+    E->d = d;
+    E->cmcc = cmcc;
+    E->pncc = pncc;
+    E->p = p;
+    E->n_mu = p * num_steps;
+    int maxhp = d > 1 ? (num_steps - 1) * pncc + d : d + pncc;
+    E->shape_range = maxhp + 1;
+    E->max_shape = maxhp;
+    if (maxhp > 31) {
+        delete E;
+        return nullptr;
+    }
+    E->A0.assign(A0, A0 + d * d);
+    E->p0.assign(p0, p0 + d);
+    E->b0.assign(b0, b0 + d);
+    E->root.resize(d);
+    for (int i = 0; i < d; i++) E->root[i] = 1.0 + rand_u();
+    E->bpert.assign(kCpus, std::vector<double>(maxhp));
+    for (int h = 0; h < kCpus; h++)
+        for (int i = 0; i < maxhp; i++) E->bpert[h][i] = 2 * rand_u() - 1;
+    E->combos.resize(E->shape_range);
+    E->cen.resize(E->shape_range);
+    E->gen.resize(E->shape_range);
+    for (int m = 0; m < E->shape_range; m++) {
+        E->cen[m] = cells_central(m, d);
+        E->gen[m] = cells_general(m, d);
+        if (m > d) E->combos[m] = lex_combos(m, d);
+    }
+    E->bucket.assign(E->shape_range, std::vector<Term>());
+    E->bucket[d].push_back(first_term(*E));
+    E->mean.assign(d, mk(0, 0));
+    E->cov.assign(d * d, mk(0, 0));
+    E->last_mean = E->mean;
+    E->last_cov = E->cov;
+    E->fz = mk(0, 0);
+    E->last_fz = mk(0, 0);
+    return E;
+}
+
+void oracle_destroy(void* h) { delete (Est*)h; }
+int oracle_shape_range(void* h) { return ((Est*)h)->shape_range; }
+int oracle_master_step(void* h) { return ((Est*)h)->master_step; }
+int oracle_num_terms(void* h) { return ((Est*)h)->Nt; }
+
+void oracle_root_point(void* h, double* out) {
+    Est* E = (Est*)h;
+    memcpy(out, E->root.data(), E->d * sizeof(double));
+}
+
+void oracle_b_pert(void* h, int tid, double* out) {
+    Est* E = (Est*)h;
+    memcpy(out, E->bpert[tid].data(), E->max_shape * sizeof(double));
+}
+
+void oracle_set_random_draws(void* h, const double* root_point, const double* b_pert_8xmax) {
+    Est* E = (Est*)h;
+    memcpy(E->root.data(), root_point, E->d * sizeof(double));
+    for (int t = 0; t < kCpus; t++) memcpy(E->bpert[t].data(), b_pert_8xmax + t * E->max_shape, E->max_shape * sizeof(double));
+}
+
+static void put(double* o, cx v) {
+    o[0] = cre(v);
+    o[1] = cim(v);
+}
+
+int oracle_step(void* h, double msmt, const double* Phi, const double* Gamma, const double* beta, const double* H,
+                double gamma, const double* Bm, const double* u, double* out_mom, int* out_cnt) {
+    Est& E = *(Est*)h;
+    const int d = E.d, sr = E.shape_range;
+    if (E.errs & (1 << kErrFzNeg)) return -1;
+    if (E.master_step == E.n_mu) return -1;
+    E.skip_post_mu = (E.master_step == (E.n_mu - 1));
+    E.threads_used = 1;
+    std::vector<int> shape_mu(sr, 0);
+    if (E.master_step == 0) {
+        step_first(E, msmt, H, gamma);
+        put(out_mom + 2, E.fz);
+        for (int i = 0; i < d; i++) put(out_mom + 6 + 2 * i, E.mean[i]);
+        for (int i = 0; i < d * d; i++) put(out_mom + 6 + 2 * d + 2 * i, E.cov[i]);
+        out_cnt[1] = E.Nt;
+        shape_mu[d] = E.Nt;
+        E.fz = mk(1, 0);
+        E.last_mean = E.mean;
+        E.last_cov = E.cov;
+        E.last_fz = E.fz;
+    } else {
+        step_tp_to_muc(E, msmt, Phi, Gamma, beta, H, gamma, Bm, u);
+        put(out_mom + 2, E.fz);
+        for (int i = 0; i < d; i++) put(out_mom + 6 + 2 * i, E.mean[i]);
+        for (int i = 0; i < d * d; i++) put(out_mom + 6 + 2 * d + 2 * i, E.cov[i]);
+        out_cnt[1] = E.Nt;
+        if (E.skip_post_mu) {
+            shape_mu = E.skip_counts_cache;
+        } else {
+            for (int m = 0; m < sr; m++) shape_mu[m] = (int)E.bucket[m].size();
+            ftr_and_gtables(E);
+        }
+    }
+    E.master_step++;
+    put(out_mom + 0, E.fz);
+    put(out_mom + 4, E.last_fz);
+    out_cnt[0] = E.errs;
+    out_cnt[2] = E.Nt;
+    out_cnt[3] = E.threads_used;
+    for (int m = 0; m < sr; m++) {
+        out_cnt[4 + m] = shape_mu[m];
+        out_cnt[4 + sr + m] = E.skip_post_mu ? shape_mu[m] : (int)E.bucket[m].size();
+    }
+    return E.errs;
+}
+
+void oracle_public_moments(void* h, double* fz, double* xhat, double* Phat) {
+    Est& E = *(Est*)h;
+    fz[0] = cre(E.fz);
+    fz[1] = cim(E.fz);
+    for (int i = 0; i < E.d; i++) xhat[i] = cre(E.mean[i]);
+    for (int i = 0; i < E.d * E.d; i++) Phat[i] = cre(E.cov[i]);
+}
+
+int oracle_term(void* h, int shape, int idx, int* info, double* A, double* p, double* b, unsigned int* keys,
+                double* gvals) {
+    Est& E = *(Est*)h;
+    if (shape < 0 || shape >= E.shape_range) return -1;
+    if (idx < 0 || idx >= (int)E.bucket[shape].size()) return -1;
+    const Term& t = E.bucket[shape][idx];
+    const int d = E.d;
+    info[0] = t.m;
+    info[1] = t.cells_p;
+    info[2] = t.phc;
+    if (A) memcpy(A, t.A.data(), t.m * d * sizeof(double));
+    if (p) memcpy(p, t.p.data(), t.m * sizeof(double));
+    if (b) memcpy(b, t.b.data(), d * sizeof(double));
+    if (keys && gvals && t.gp) {
+        for (int i = 0; i < t.cells_p; i++) {
+            keys[i] = t.gp->key[i];
+            gvals[2 * i] = cre(t.gp->val[i]);
+            gvals[2 * i + 1] = cim(t.gp->val[i]);
+        }
+    }
+    return 0;
+}
+
+int oracle_set_tr_search_idxs_ordering(const int* ordering, int n) {
+    if (n > 12) return 1;
+    for (int i = 0; i < n; i++) g_search_order[i] = ordering[i];
+    return 0;
+}
+
+void oracle_path_counters(void* h, long long* out3) {
+    Est& E = *(Est*)h;
+    for (int i = 0; i < 3; i++) out3[i] = E.counters[i];
+}
+
+}  // extern "C"
