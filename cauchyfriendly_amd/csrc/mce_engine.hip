@@ -1,0 +1,1376 @@
+// Host side of the MI355X MCE engine: term pools in HBM, the per-update kernel
+// sequence, and the C ABI of include/cauchy_mi355x.h.
+//
+// One Engine is one CauchyEstimator (include/cauchy_estimator.hpp:47-1432).  Its
+// terms never leave the GPU; the host only sequences kernels, reads back a few
+// counters (slot totals, per-shape counts, reduction rounds) and finalises the
+// 1 + d + d^2 moment sums.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cauchy_mi355x.h"
+#include "mce_kernels.h"
+
+using namespace mce;
+
+namespace {
+
+std::string g_last_error;
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e_) + "' at " + \
+                                     #x);                                                           \
+    } while (0)
+
+int ceil_div(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+// Grow-only device buffer.
+template <class T>
+struct DBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        // kernels still in flight may read the old allocation
+        HIPCHK(hipDeviceSynchronize());
+        if (ptr) HIPCHK(hipFree(ptr));
+        size_t nn = std::max(n, cap + cap / 2);
+        nn = std::max<size_t>(nn, 64);
+        HIPCHK(hipMalloc(&ptr, nn * sizeof(T)));
+        cap = nn;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+struct PoolStore {
+    int MM = 0, d = 0;
+    int64_t cap = 0;
+    DBuf<double> A, p, q, b, c, dv;
+    DBuf<int> m, phc, pbc, z, lhp, gp_n, bt_n, bt_sgn, tid;
+    DBuf<uint32_t> hflag;
+    DBuf<uint8_t> flags, cmap, mode;
+    DBuf<int8_t> csmap;
+    DBuf<int64_t> gp_off, bt_off;
+    void reserve(int64_t n) {
+        if (n <= cap) return;
+        int64_t nn = std::max<int64_t>(n, cap + cap / 2);
+        A.reserve((size_t)nn * MM * d);
+        p.reserve((size_t)nn * MM);
+        q.reserve((size_t)nn * MM);
+        b.reserve((size_t)nn * d);
+        c.reserve(nn);
+        dv.reserve(nn);
+        m.reserve(nn);
+        phc.reserve(nn);
+        pbc.reserve(nn);
+        z.reserve(nn);
+        lhp.reserve(nn);
+        gp_n.reserve(nn);
+        bt_n.reserve(nn);
+        bt_sgn.reserve(nn);
+        tid.reserve(nn);
+        hflag.reserve(nn);
+        flags.reserve(nn);
+        cmap.reserve((size_t)nn * MM);
+        mode.reserve(nn);
+        csmap.reserve((size_t)nn * MM);
+        gp_off.reserve(nn);
+        bt_off.reserve(nn);
+        cap = nn;
+    }
+    TermPool view() const {
+        TermPool t;
+        t.cap = cap;
+        t.MM = MM;
+        t.d = d;
+        t.A = A.ptr;
+        t.p = p.ptr;
+        t.q = q.ptr;
+        t.b = b.ptr;
+        t.c = c.ptr;
+        t.dv = dv.ptr;
+        t.m = m.ptr;
+        t.phc = phc.ptr;
+        t.pbc = pbc.ptr;
+        t.z = z.ptr;
+        t.lhp = lhp.ptr;
+        t.hflag = hflag.ptr;
+        t.flags = flags.ptr;
+        t.cmap = cmap.ptr;
+        t.csmap = csmap.ptr;
+        t.gp_off = gp_off.ptr;
+        t.gp_n = gp_n.ptr;
+        t.bt_off = bt_off.ptr;
+        t.bt_n = bt_n.ptr;
+        t.bt_sgn = bt_sgn.ptr;
+        t.tid = tid.ptr;
+        t.mode = mode.ptr;
+        return t;
+    }
+};
+
+// ---------------------------------------------------------------- host numerics
+unsigned long long binom(int n, int k) {
+    unsigned long long r = 1;
+    if (k > n - k) k = n - k;
+    for (int i = 0; i < k; ++i) {
+        r *= (n - i);
+        r /= (i + 1);
+    }
+    return r;
+}
+int cells_central(int m, int d) {
+    if (m < d) return 1 << m;
+    unsigned long long s = 0;
+    for (int i = 0; i < d; i++) s += binom(m - 1, i);
+    return (int)(2 * s);
+}
+
+double rand_u() { return ((double)rand() + 1.0) / ((double)RAND_MAX + 1.0); }  // random_variables.hpp:22-26
+
+// Symmetric eigenvalues by cyclic Jacobi; they only feed the sign test of
+// covariance_checker (cauchy_util.hpp:1882-2004).
+void sym_eigvals(const double* S, double* ev, int n) {
+    std::vector<double> a(S, S + n * n);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+        if (off < 1e-300) break;
+        for (int pp = 0; pp < n; pp++)
+            for (int qq = pp + 1; qq < n; qq++) {
+                double apq = a[pp * n + qq];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (a[qq * n + qq] - a[pp * n + pp]) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = a[k * n + pp], akq = a[k * n + qq];
+                    a[k * n + pp] = c * akp - s * akq;
+                    a[k * n + qq] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a[pp * n + k], aqk = a[qq * n + k];
+                    a[pp * n + k] = c * apk - s * aqk;
+                    a[qq * n + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) ev[i] = a[i * n + i];
+}
+
+int g_search_order[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+
+// ---------------------------------------------------------------- profiling
+enum Phase { kPhTp, kPhEnum, kPhMu, kPhMom, kPhGather, kPhFtr, kPhGBtab, kPhGEval, kPhCompact, kNumPhases };
+
+struct Profiler {
+    bool on = false;
+    double ms[kNumPhases] = {0};
+    double total = 0;
+    long long geval_launches = 0;
+    double geval_bytes = 0;
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    hipEvent_t start_ev = nullptr;
+    void begin(hipStream_t s) {
+        if (!on) return;
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, s));
+        start_ev = e;
+        marks.clear();
+    }
+    void mark(int ph, hipStream_t s) {
+        if (!on) return;
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, s));
+        marks.push_back(std::make_pair(ph, e));
+    }
+    void end(hipStream_t s) {
+        if (!on || !start_ev) return;
+        HIPCHK(hipStreamSynchronize(s));
+        hipEvent_t prev = start_ev;
+        for (auto& pe : marks) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, prev, pe.second));
+            ms[pe.first] += t;
+            total += t;
+            if (prev != start_ev) (void)hipEventDestroy(prev);
+            prev = pe.second;
+        }
+        if (prev != start_ev) (void)hipEventDestroy(prev);
+        (void)hipEventDestroy(start_ev);
+        start_ev = nullptr;
+        marks.clear();
+    }
+};
+
+// ---------------------------------------------------------------- engine
+struct Engine {
+    int d, cmcc, pncc, p, n_mu, sr, MM;
+    int master_step = 0, Nt = 1, errs = 0, win_num = 0;
+    bool skip = false;
+    std::vector<double> A0, p0, b0, root, bpert;
+    cplx fz, last_fz;
+    std::vector<cplx> mean, cov, last_mean, last_cov;
+    double gscale = 0;
+    std::vector<int> in_count;  // per-shape term counts of the parent pool
+    // diagnostics of the last update
+    bool have_diag = false;
+    std::vector<double> diag_mom;
+    std::vector<int> diag_cnt;
+    int threads_used = 1;
+
+    hipStream_t st = nullptr;
+    PoolStore in, out;
+    DBuf<uint32_t> gk[2];
+    DBuf<cplx> gv[2];
+    int gcur = 0;
+    DBuf<uint32_t> barena, b2arena, tmpk;
+    DBuf<cplx> scratch, tmpv;
+    DBuf<StepConst> dconst;
+    DBuf<double> dbpert, gy, mom_part, mom_out;
+    DBuf<uint8_t> dcombos;
+    DBuf<int64_t> dcombo_off;
+    DBuf<int> dcombo_cnt, dcen_half, din_off;
+    DBuf<int> slots, slot_base, enum_list, counters, hist;
+    DBuf<int64_t> btcap64, bt_base;
+    DBuf<int> btcap;
+    DBuf<uint32_t> keys, skeys, iota, perm;
+    // reduction
+    DBuf<uint64_t> fkeys, fskeys;
+    DBuf<uint32_t> fidx, fsidx;
+    DBuf<double> fvals;
+    DBuf<int> ecnt, eoff, edges, indeg, inoff, cursor, Fc;
+    DBuf<uint8_t> st0, st1, isroot;
+    DBuf<uint32_t> memkey, memval, smkey, smval, roots, jiota;
+    DBuf<int> memcnt, memoff;
+    // groups (all buckets of an update)
+    DBuf<int> grp_root, grp_moff, grp_mcnt, grp_m, mem_slots, b2n, surv, gxor, sort_list, spos, sflag;
+    DBuf<int64_t> ub64, b2off, goff, b2n64;
+    DBuf<uint8_t> cub_tmp;
+    Profiler prof;
+
+    Engine(int num_steps, int d_, int cmcc_, int pncc_, int p_, const double* A0_, const double* p0_,
+           const double* b0_)
+        : d(d_), cmcc(cmcc_), pncc(pncc_), p(p_) {
+        if (d < 1 || d > kMaxD) throw std::runtime_error("state dimension outside 1..8 is not supported");
+        n_mu = p * num_steps;
+        const int maxhp = d > 1 ? (num_steps - 1) * pncc + d : d + pncc;
+        sr = maxhp + 1;
+        MM = maxhp;
+        if (maxhp > 31) throw std::runtime_error("largest hyperplane count exceeds 31");
+        if (maxhp > 20) throw std::runtime_error("largest hyperplane count above 20 is not supported by the LDS cell tables");
+        A0.assign(A0_, A0_ + d * d);
+        p0.assign(p0_, p0_ + d);
+        b0.assign(b0_, b0_ + d);
+        // construction-time draws in the reference's order (cauchy_estimator.hpp:127-161)
+        root.resize(d);
+        for (int i = 0; i < d; i++) root[i] = 1.0 + rand_u();
+        bpert.assign(8 * MM, 0.0);
+        for (int h = 0; h < 8; h++)
+            for (int i = 0; i < MM; i++) bpert[h * MM + i] = 2 * rand_u() - 1;
+        mean.assign(d, cmake(0, 0));
+        cov.assign(d * d, cmake(0, 0));
+        last_mean = mean;
+        last_cov = cov;
+        fz = cmake(0, 0);
+        last_fz = cmake(0, 0);
+        HIPCHK(hipStreamCreate(&st));
+        in.MM = out.MM = MM;
+        in.d = out.d = d;
+        // combinations in lexicographic order per shape (cell_enumeration.hpp:71-92)
+        std::vector<uint8_t> cmb;
+        std::vector<int64_t> coff(sr + 1, 0);
+        std::vector<int> ccnt(sr, 0), cen(sr + 1, 0);
+        for (int m = 0; m < sr; m++) {
+            coff[m] = (int64_t)cmb.size();
+            cen[m] = cells_central(m, d) / 2;
+            if (m <= d) continue;
+            std::vector<int> c(d);
+            for (int i = 0; i < d; i++) c[i] = i;
+            while (true) {
+                for (int i = 0; i < d; i++) cmb.push_back((uint8_t)c[i]);
+                ccnt[m]++;
+                int i = d - 1;
+                while (i >= 0 && c[i] == m - d + i) i--;
+                if (i < 0) break;
+                c[i]++;
+                for (int j = i + 1; j < d; j++) c[j] = c[j - 1] + 1;
+            }
+        }
+        coff[sr] = (int64_t)cmb.size();
+        dcombos.reserve(std::max<size_t>(cmb.size(), 1));
+        if (!cmb.empty()) HIPCHK(hipMemcpy(dcombos.ptr, cmb.data(), cmb.size(), hipMemcpyHostToDevice));
+        dcombo_off.reserve(sr + 1);
+        HIPCHK(hipMemcpy(dcombo_off.ptr, coff.data(), (sr + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        dcombo_cnt.reserve(sr);
+        HIPCHK(hipMemcpy(dcombo_cnt.ptr, ccnt.data(), sr * sizeof(int), hipMemcpyHostToDevice));
+        dcen_half.reserve(sr + 1);
+        HIPCHK(hipMemcpy(dcen_half.ptr, cen.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice));
+        dbpert.reserve(8 * MM);
+        upload_draws();
+        dconst.reserve(1);
+        counters.reserve(16);
+        hist.reserve(sr + 1);
+        din_off.reserve(sr + 2);
+        reset_terms();
+    }
+
+    ~Engine() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void upload_draws() {
+        HIPCHK(hipMemcpy(dbpert.ptr, bpert.data(), 8 * MM * sizeof(double), hipMemcpyHostToDevice));
+    }
+
+    // First term (setup_first_term, cauchy_term.hpp:772-785).
+    void reset_terms() {
+        in.reserve(1);
+        std::vector<double> A(MM * d, 0.0), pp(MM, 0.0);
+        memcpy(A.data(), A0.data(), d * d * sizeof(double));
+        memcpy(pp.data(), p0.data(), d * sizeof(double));
+        HIPCHK(hipMemcpy(in.A.ptr, A.data(), MM * d * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(in.p.ptr, pp.data(), MM * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(in.b.ptr, b0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+        int mm = d, zero = 0;
+        int64_t z64 = 0;
+        HIPCHK(hipMemcpy(in.m.ptr, &mm, sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(in.gp_n.ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(in.gp_off.ptr, &z64, sizeof(int64_t), hipMemcpyHostToDevice));
+        in_count.assign(sr, 0);
+        in_count[d] = 1;
+        Nt = 1;
+        master_step = 0;
+        errs = 0;
+    }
+
+    template <class F>
+    void cub(F f) {
+        size_t bytes = 0;
+        HIPCHK(f((void*)nullptr, bytes));
+        cub_tmp.reserve(bytes + 16);
+        HIPCHK(f((void*)cub_tmp.ptr, bytes));
+    }
+
+    template <class T>
+    T read1(const T* dptr) {
+        T v;
+        HIPCHK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return v;
+    }
+
+    // Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.
+    int precoalign(const double* Gamma, const double* beta, StepConst& sc) {
+        std::vector<double> G(pncc * d), bt(pncc);
+        for (int i = 0; i < pncc; i++) {
+            for (int j = 0; j < d; j++) G[i * d + j] = Gamma[j * pncc + i];
+            bt[i] = beta[i];
+        }
+        for (int i = 0; i < pncc; i++) {
+            double nf = 0;
+            for (int j = 0; j < d; j++) nf += fabs(G[i * d + j]);
+            for (int j = 0; j < d; j++) G[i * d + j] /= nf;
+            bt[i] *= nf;
+        }
+        std::vector<char> F(pncc, 1);
+        for (int i = 0; i < pncc - 1; i++) {
+            if (!F[i]) continue;
+            for (int j = i + 1; j < pncc; j++) {
+                if (!F[j]) continue;
+                bool pos = true, neg = true;
+                for (int l = 0; l < d; l++) {
+                    if (pos) pos &= fabs(G[i * d + l] - G[j * d + l]) < kCoalignEps;
+                    if (neg) neg &= fabs(G[i * d + l] + G[j * d + l]) < kCoalignEps;
+                    if (!(pos || neg)) break;
+                }
+                if (pos || neg) {
+                    F[j] = 0;
+                    bt[i] += bt[j];
+                }
+            }
+        }
+        int cnt = 0;
+        for (int i = 0; i < pncc; i++) cnt += F[i];
+        if (cnt != pncc) {
+            int u = 1;
+            for (int j = 1; j < pncc; j++)
+                if (F[j]) {
+                    if (u < j) {
+                        memcpy(&G[u * d], &G[j * d], d * sizeof(double));
+                        bt[u] = bt[j];
+                    }
+                    u++;
+                }
+        }
+        for (int i = 0; i < cnt; i++) {
+            for (int j = 0; j < d; j++) sc.G[i * d + j] = G[i * d + j];
+            sc.beta[i] = bt[i];
+        }
+        return cnt;
+    }
+
+    // finalize_cached_moments + moments_numerical_check, cauchy_estimator.hpp:340-461.
+    void finalize_and_check() {
+        gscale = kInvTwoPi / fz.re;
+        const cplx ifz = cmake(0, fz.re);
+        for (int i = 0; i < d; i++) mean[i] = cdiv(mean[i], ifz);
+        for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++) cov[i * d + j] = csub(cdiv(cov[i * d + j], fz), cmul(mean[i], mean[j]));
+        const bool first_of_step = (master_step % p) == 0;
+        const bool not_last = (master_step % p) != (p - 1);
+        const bool last = (master_step % p) == (p - 1);
+        if (first_of_step) {
+            errs |= (1 << 7);
+            errs |= (1 << 3);
+            errs &= ~(1 << 5);
+            errs &= ~(1 << 6);
+            errs &= ~(1 << 2);
+            errs &= ~(1 << 1);
+        }
+        int ne = 0;
+        if (fz.re <= 0) ne |= (1 << 9);
+        if (fabs(fz.im / (1e-15 + fz.re)) > 1e-3) ne |= (1 << 8);
+        bool mean_ok = true;
+        for (int i = 0; i < d; i++) {
+            double r = fabs(mean[i].re), im = fabs(mean[i].im);
+            double ratio = im / (1e-15 + r);
+            if ((ratio > 1e-1) || (im > 0.001)) {
+                ne |= (1 << 4);
+                if (not_last) ne |= (1 << 6);
+                if (last) ne |= (1 << 5);
+                mean_ok = false;
+            }
+        }
+        int cf = 0;
+        {
+            std::vector<double> cr(d * d), ci(d * d), ev(d);
+            for (int i = 0; i < d * d; i++) {
+                cr[i] = cov[i].re;
+                ci[i] = cov[i].im;
+            }
+            sym_eigvals(cr.data(), ev.data(), d);
+            for (int i = 0; i < d; i++)
+                if (ev[i] < -1e-5) cf |= 1;
+            for (int i = 0; i < d - 1; i++) {
+                double sii = sqrt(cr[i * d + i]);
+                for (int j = i + 1; j < d; j++) {
+                    double sjj = sqrt(cr[j * d + j]);
+                    double corr = cr[i * d + j] / (sii * sjj);
+                    if (fabs(corr) > 1) cf |= 2;
+                }
+            }
+            for (int i = 0; i < d; i++)
+                for (int j = i; j < d; j++)
+                    if (fabs(ci[i * d + j]) / (fabs(cr[i * d + j]) + 1e-15) > 10) cf |= 4;
+            for (int i = 0; i < d * d; i++)
+                if (fabs(ci[i]) > 2000) cf |= 8;
+        }
+        bool cov_ok = true;
+        if (cf) {
+            ne |= 1;
+            if (not_last) ne |= (1 << 2);
+            if (last) ne |= (1 << 1);
+            cov_ok = false;
+        }
+        errs |= ne;
+        if (mean_ok) errs &= ~(1 << 7);
+        if (cov_ok) errs &= ~(1 << 3);
+        if (mean_ok) last_mean = mean;
+        else mean = last_mean;
+        if (cov_ok) last_cov = cov;
+        else cov = last_cov;
+        if (!mean_ok && !cov_ok) fz = last_fz;
+        else last_fz = fz;
+    }
+
+    // ------------------------------------------------------------ dispatch helpers
+#define MCE_DISPATCH_D(D_, ...)                         \
+    switch (D_) {                                       \
+        case 1: { constexpr int DD = 1; __VA_ARGS__; } break; \
+        case 2: { constexpr int DD = 2; __VA_ARGS__; } break; \
+        case 3: { constexpr int DD = 3; __VA_ARGS__; } break; \
+        case 4: { constexpr int DD = 4; __VA_ARGS__; } break; \
+        case 5: { constexpr int DD = 5; __VA_ARGS__; } break; \
+        case 6: { constexpr int DD = 6; __VA_ARGS__; } break; \
+        case 7: { constexpr int DD = 7; __VA_ARGS__; } break; \
+        case 8: { constexpr int DD = 8; __VA_ARGS__; } break; \
+        default: throw std::runtime_error("unsupported state dimension"); \
+    }
+
+    void scan_i32_to_i32_incl(const int* in_, int* out_, int n) {
+        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::InclusiveSum(t, b, in_, out_, n, st); });
+    }
+    void scan_i64_excl(const int64_t* in_, int64_t* out_, int n) {
+        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
+    }
+    void scan_i32_excl(const int* in_, int* out_, int n) {
+        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
+    }
+
+    // ------------------------------------------------------------ one scalar measurement update
+    int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
+               double gamma, const double* Bm, const double* u) {
+        if (errs & (1 << 9)) {
+            fprintf(stderr, "[Window %d:] ERROR_FZ_NEGATIVE triggered. Cannot continue stepping until this estimator has been reset!\n", win_num);
+            return errs;
+        }
+        if (master_step == n_mu)
+            throw std::runtime_error("master_step == num_estimation_steps: the estimator must be reset before stepping again");
+        skip = (master_step == n_mu - 1);
+        const bool first = (master_step == 0);
+        const bool with_tp = !first && (master_step % p) == 0;
+        const bool threaded = !first && Nt >= 200;
+        int T = 1;
+        if (threaded) {
+            int chunks = (Nt + 199) / 200;
+            T = chunks > 8 ? 8 : chunks;
+        }
+        threads_used = threaded ? T : 1;
+        StepConst sc;
+        memset(&sc, 0, sizeof(sc));
+        for (int i = 0; i < d * d; i++) sc.Phi[i] = Phi ? Phi[i] : 0.0;
+        for (int i = 0; i < d; i++) sc.H[i] = Hrow[i];
+        sc.ng = with_tp ? precoalign(Gamma, beta, sc) : 0;
+        if (with_tp && cmcc > 0 && Bm && u)
+            for (int i = 0; i < d; i++) {
+                double s = 0.0;
+                for (int j = 0; j < cmcc; j++) s += Bm[i * cmcc + j] * u[j];
+                sc.Bu[i] = s;
+            }
+        for (int i = 0; i < d; i++) sc.root[i] = root[i];
+        sc.msmt = msmt;
+        sc.gamma = gamma;
+        sc.d = d;
+        sc.cmcc = (with_tp && Bm && u) ? cmcc : 0;
+        sc.with_tp = with_tp;
+        sc.skip = skip;
+        sc.first = first;
+        sc.T = T;
+        sc.threaded = threaded;
+        for (int i = 0; i < d; i++) sc.order[i] = g_search_order[i];
+        HIPCHK(hipMemcpyAsync(dconst.ptr, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        std::vector<int> in_off(sr + 1, 0);
+        for (int m = 0; m < sr; m++) in_off[m + 1] = in_off[m] + in_count[m];
+        HIPCHK(hipMemcpyAsync(din_off.ptr, in_off.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+        const int np = Nt;
+        prof.begin(st);
+
+        // ---- time propagation and B-tables of the parents
+        slots.reserve(np);
+        slot_base.reserve(np + 1);
+        btcap64.reserve(np);
+        bt_base.reserve(np);
+        btcap.reserve(np);
+        TermPool IN = in.view();
+        HIPCHK(hipMemsetAsync(counters.ptr, 0, 16 * sizeof(int), st));
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
+                                             din_off.ptr, dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(slot_base.ptr, 0, sizeof(int), st));
+        scan_i32_to_i32_incl(slots.ptr, slot_base.ptr + 1, np);
+        const int nslots = read1(slot_base.ptr + np);
+        prof.mark(kPhTp, st);
+        if (!skip && !first) {
+            // B arena offsets (64-bit)
+            widen(btcap.ptr, btcap64.ptr, np);
+            scan_i64_excl(btcap64.ptr, bt_base.ptr, np);
+            int64_t last_off = read1(bt_base.ptr + np - 1);
+            int last_cap = read1(btcap.ptr + np - 1);
+            barena.reserve((size_t)(last_off + last_cap) + 1);
+            enum_list.reserve(np);
+            HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
+            hipLaunchKernelGGL(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
+                               gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr);
+            HIPCHK(hipGetLastError());
+            const int nenum = read1(counters.ptr);
+            if (nenum > 0) {
+                const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
+                const size_t lds = (size_t)words * 4;
+                MCE_DISPATCH_D(d, {
+                    HIPCHK(hipFuncSetAttribute((const void*)k_btab_enum<DD>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    hipLaunchKernelGGL(k_btab_enum<DD>, dim3(nenum), dim3(256), lds, st, IN, enum_list.ptr, dbpert.ptr,
+                                       dcombos.ptr, dcombo_off.ptr, dcombo_cnt.ptr, gk[gcur].ptr, barena.ptr,
+                                       btcap.ptr, counters.ptr + 8);
+                });
+                HIPCHK(hipGetLastError());
+            }
+        }
+        prof.mark(kPhEnum, st);
+
+        // ---- measurement update
+        out.reserve(nslots);
+        gy.reserve((size_t)nslots * (2 + 2 * d));
+        keys.reserve(nslots);
+        TermPool OUT = out.view();
+        HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
+                                             slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr,
+                                             keys.ptr, counters.ptr + 1));
+        HIPCHK(hipGetLastError());
+        prof.mark(kPhMu, st);
+
+        // ---- moments, summed in the reference's order
+        const int NC = 1 + d + d * d;
+        mom_part.reserve((size_t)T * NC * 2);
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_chain<DD>, dim3(T), dim3(128), 0, st, gy.ptr, out.flags.ptr,
+                                             slot_base.ptr, din_off.ptr, sr, T, mom_part.ptr));
+        HIPCHK(hipGetLastError());
+        std::vector<double> part((size_t)T * NC * 2);
+        HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        int mu_err = 0;
+        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        // per-thread chunks added in thread order (cauchy_estimator.hpp:886-895)
+        std::vector<double> mo(NC * 2, 0.0);
+        for (int t = 0; t < T; t++)
+            for (int c = 0; c < NC; c++) {
+                mo[2 * c] += part[((size_t)t * NC + c) * 2];
+                mo[2 * c + 1] += part[((size_t)t * NC + c) * 2 + 1];
+            }
+        prof.mark(kPhMom, st);
+        if (mu_err) throw std::runtime_error("two H-orthogonal hyperplanes coalign negatively (reference exits here)");
+        fz = cmake(mo[0], mo[1]);
+        for (int j = 0; j < d; j++) mean[j] = cmake(mo[2 + 2 * j], mo[3 + 2 * j]);
+        for (int j = 0; j < d * d; j++) cov[j] = cmake(mo[2 + 2 * d + 2 * j], mo[3 + 2 * d + 2 * j]);
+        if (first) {
+            // compute_moments(true), cauchy_estimator.hpp:524-602 (no numeric checks at step 1)
+            gscale = kInvTwoPi / fz.re;
+            const cplx ifz = cmake(0, fz.re);
+            for (int i = 0; i < d; i++) mean[i] = cdiv(mean[i], ifz);
+            for (int i = 0; i < d; i++)
+                for (int j = 0; j < d; j++) cov[i * d + j] = csub(cdiv(cov[i * d + j], fz), cmul(mean[i], mean[j]));
+        } else {
+            finalize_and_check();
+        }
+        diag_mom.assign(6 + 2 * d + 2 * d * d, 0.0);
+        diag_cnt.assign(4 + 2 * sr, 0);
+        diag_mom[2] = fz.re;
+        diag_mom[3] = fz.im;
+        for (int i = 0; i < d; i++) {
+            diag_mom[6 + 2 * i] = mean[i].re;
+            diag_mom[7 + 2 * i] = mean[i].im;
+        }
+        for (int i = 0; i < d * d; i++) {
+            diag_mom[6 + 2 * d + 2 * i] = cov[i].re;
+            diag_mom[7 + 2 * d + 2 * i] = cov[i].im;
+        }
+
+        if (skip) {
+            HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
+            hipLaunchKernelGGL(k_skip_counts, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, OUT, nslots, hist.ptr);
+            std::vector<int> h(sr + 1);
+            HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            Nt = 0;
+            for (int m = 0; m < sr; m++) {
+                diag_cnt[4 + m] = h[m];
+                diag_cnt[4 + sr + m] = h[m];
+                Nt += h[m];
+            }
+            diag_cnt[1] = Nt;
+        } else {
+            // ---- gather into shape buckets in the reference's term order
+            iota.reserve(nslots);
+            perm.reserve(nslots);
+            skeys.reserve(nslots);
+            hipLaunchKernelGGL(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+            cub([&](void* t, size_t& b) {
+                return hipcub::DeviceRadixSort::SortPairs(t, b, keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0,
+                                                          32, st);
+            });
+            HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
+            hipLaunchKernelGGL(k_shape_hist, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, OUT, nslots, hist.ptr);
+            std::vector<int> cnt(sr + 1);
+            HIPCHK(hipMemcpyAsync(cnt.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            prof.mark(kPhGather, st);
+            int nt_mu = 0;
+            for (int m = 0; m < sr; m++) {
+                diag_cnt[4 + m] = cnt[m];
+                nt_mu += cnt[m];
+            }
+            diag_cnt[1] = nt_mu;
+            if (first) {
+                first_gtables(nt_mu);
+            } else {
+                reduce_and_flatten(cnt);
+            }
+            for (int m = 0; m < sr; m++) diag_cnt[4 + sr + m] = in_count[m];
+        }
+        {
+            const int guard = read1(counters.ptr + 8);
+            if (guard) {
+                char msg[160];
+                snprintf(msg, sizeof(msg), "device bound guard tripped (flags 0x%x) at update %d", guard, master_step);
+                throw std::runtime_error(msg);
+            }
+        }
+        if (!skip) fz = cmake(1, 0);
+        if (first) {
+            last_mean = mean;
+            last_cov = cov;
+            last_fz = fz;
+        }
+        master_step++;
+        prof.end(st);
+        diag_mom[0] = fz.re;
+        diag_mom[1] = fz.im;
+        diag_mom[4] = last_fz.re;
+        diag_mom[5] = last_fz.im;
+        diag_cnt[0] = errs;
+        diag_cnt[2] = Nt;
+        diag_cnt[3] = threads_used;
+        have_diag = true;
+        return errs;
+    }
+
+    void widen(const int* src, int64_t* dst, int n);
+
+    // g-tables of the first update's d+1 terms, which become the parents.
+    void first_gtables(int n) {
+        const int cells = 1 << (d - 1);
+        const int nx = gcur ^ 1;
+        gk[nx].reserve((size_t)n * cells);
+        gv[nx].reserve((size_t)n * cells);
+        in.reserve(n);
+        TermPool OUT = out.view();
+        TermPool NEXT = in.view();
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_gtable_first<DD>, dim3(1), dim3(64), 0, st, OUT, n, perm.ptr, gscale,
+                                             gk[nx].ptr, gv[nx].ptr, NEXT));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        gcur = nx;
+        in_count.assign(sr, 0);
+        in_count[d] = n;
+        Nt = n;
+    }
+
+    // Fast term reduction and g-tables for every shape bucket
+    // (fast_term_reduction_and_create_gtables, cauchy_estimator.hpp:981-1177).
+    void reduce_and_flatten(const std::vector<int>& cnt) {
+        TermPool OUT = out.view();
+        std::vector<int> off(sr + 1, 0);
+        for (int m = 0; m < sr; m++) off[m + 1] = off[m] + cnt[m];
+        const int total = off[sr];
+        grp_root.reserve(total);
+        grp_moff.reserve(total);
+        grp_mcnt.reserve(total);
+        grp_m.reserve(total);
+        mem_slots.reserve(total);
+        std::vector<int> gbase_of(sr + 1, 0);
+        int gbase = 0, mbase = 0;
+        for (int m = 0; m < sr; m++) {
+            gbase_of[m] = gbase;
+            const int N = cnt[m];
+            if (N == 0) continue;
+            const int base = off[m];
+            reduce_bucket(OUT, m, base, N, gbase, mbase);
+        }
+        gbase_of[sr] = gbase;
+        const int ng = gbase;
+        prof.mark(kPhFtr, st);
+
+        // ---- group B-tables
+        ub64.reserve(ng);
+        b2off.reserve(ng);
+        b2n.reserve(ng);
+        b2n64.reserve(ng);
+        goff.reserve(ng);
+        hipLaunchKernelGGL(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
+                           (int*)b2n.ptr);
+        widen(b2n.ptr, ub64.ptr, ng);
+        scan_i64_excl(ub64.ptr, b2off.ptr, ng);
+        const int64_t ub_total = read1(b2off.ptr + ng - 1) + read1(ub64.ptr + ng - 1);
+        b2arena.reserve((size_t)ub_total + 1);
+        for (int m = 1; m < sr; m++) {
+            const int ngb = gbase_of[m + 1] - gbase_of[m];
+            if (ngb == 0) continue;
+            const int words = m >= 5 ? (1 << (m - 5)) : 1;
+            int wpb = 4;
+            while (wpb > 1 && (size_t)wpb * words * 4 > 65536) wpb--;
+            const size_t lds = (size_t)wpb * words * 4;
+            HIPCHK(hipFuncSetAttribute((const void*)k_group_btab, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)std::max<size_t>(lds, 4)));
+            hipLaunchKernelGGL(k_group_btab, dim3(ceil_div(ngb, wpb)), dim3(64 * wpb), lds, st, OUT, gbase_of[m], ngb,
+                               m, d, grp_root.ptr, barena.ptr, b2off.ptr, ub64.ptr, b2arena.ptr, b2n.ptr, words,
+                               counters.ptr + 8);
+            HIPCHK(hipGetLastError());
+        }
+        widen(b2n.ptr, b2n64.ptr, ng);
+        scan_i64_excl(b2n64.ptr, goff.ptr, ng);
+        const int64_t cells_total = read1(goff.ptr + ng - 1) + read1(b2n64.ptr + ng - 1);
+        prof.mark(kPhGBtab, st);
+
+        // ---- group g-tables
+        const int nx = gcur ^ 1;
+        gk[nx].reserve((size_t)cells_total + 1);
+        gv[nx].reserve((size_t)cells_total + 1);
+        scratch.reserve((size_t)cells_total + 1);
+        surv.reserve(ng);
+        gxor.reserve(ng);
+        sort_list.reserve(ng);
+        HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
+        hipLaunchKernelGGL(k_group_eval, dim3(ng), dim3(64), 0, st, OUT, ng, d, grp_root.ptr, grp_moff.ptr,
+                           grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr, b2n.ptr, goff.ptr,
+                           gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr, gscale, surv.ptr, gxor.ptr,
+                           sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+        HIPCHK(hipGetLastError());
+        prof.geval_launches++;
+        const int nsort = read1(counters.ptr + 2);
+        const int rare = read1(counters.ptr + 3);
+        if (rare) fprintf(stderr, "mce: a parent term joined a reduction group (flag %d); treated as a new child\n", rare);
+        if (nsort > 0) {
+            tmpk.reserve((size_t)cells_total + 1);
+            tmpv.reserve((size_t)cells_total + 1);
+            const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
+            const size_t lds = (size_t)words * 4;
+            HIPCHK(hipFuncSetAttribute((const void*)k_group_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
+            hipLaunchKernelGGL(k_group_sort, dim3(nsort), dim3(256), lds, st, nsort, sort_list.ptr, grp_m.ptr, b2n.ptr,
+                               goff.ptr, gk[nx].ptr, gv[nx].ptr, tmpk.ptr, tmpv.ptr);
+            HIPCHK(hipGetLastError());
+        }
+        prof.mark(kPhGEval, st);
+
+        // ---- survivors become the next parents (shape ascending, root index ascending)
+        sflag.reserve(ng);
+        spos.reserve(ng);
+        hipLaunchKernelGGL(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
+        scan_i32_excl(sflag.ptr, spos.ptr, ng);
+        const int nsurv = read1(spos.ptr + ng - 1) + read1(sflag.ptr + ng - 1);
+        in.reserve(std::max(nsurv, 1));
+        TermPool NEXT = in.view();
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
+                                             surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
+        hipLaunchKernelGGL(k_surv_hist, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, grp_m.ptr, hist.ptr);
+        std::vector<int> h(sr + 1);
+        HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        prof.mark(kPhCompact, st);
+        in_count.assign(sr, 0);
+        Nt = 0;
+        for (int m = 0; m < sr; m++) {
+            in_count[m] = h[m];
+            Nt += h[m];
+        }
+        gcur = nx;
+    }
+
+    // Fast term reduction of one shape bucket; appends its reduction groups.
+    void reduce_bucket(const TermPool& OUT, int m, int base, int N, int& gbase, int& mbase) {
+        const int a0 = g_search_order[0];
+        fkeys.reserve(N);
+        fskeys.reserve(N);
+        fidx.reserve(N);
+        fsidx.reserve(N);
+        fvals.reserve(N);
+        ecnt.reserve(N);
+        eoff.reserve(N);
+        hipLaunchKernelGGL(k_ftr_keys, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr, base, N, d, a0,
+                           fkeys.ptr, fidx.ptr);
+        cub([&](void* t, size_t& b) {
+            return hipcub::DeviceRadixSort::SortPairs(t, b, fkeys.ptr, fskeys.ptr, fidx.ptr, fsidx.ptr, N, 0, 64, st);
+        });
+        hipLaunchKernelGGL(k_ftr_vals, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr, base, N, d, a0,
+                           fsidx.ptr, fvals.ptr);
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr,
+                                             base, N, m, dconst.ptr, fvals.ptr, fsidx.ptr, ecnt.ptr, eoff.ptr,
+                                             edges.ptr, 0));
+        HIPCHK(hipGetLastError());
+        scan_i32_excl(ecnt.ptr, eoff.ptr, N);
+        const int E = read1(eoff.ptr + N - 1) + read1(ecnt.ptr + N - 1);
+        const int nchunk = (N > 10000) ? 8 : 1;
+        const int chunk = N / 8;
+        Fc.reserve((size_t)nchunk * N);
+        hipLaunchKernelGGL(k_ftr_finit, dim3(ceil_div((int64_t)nchunk * N, 256)), dim3(256), 0, st, Fc.ptr,
+                           (int64_t)nchunk * N, N);
+        if (E > 0) {
+            edges.reserve(E);
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT,
+                                                 perm.ptr, base, N, m, dconst.ptr, fvals.ptr, fsidx.ptr, ecnt.ptr,
+                                                 eoff.ptr, edges.ptr, 1));
+            indeg.reserve(N);
+            inoff.reserve(N);
+            cursor.reserve(N);
+            HIPCHK(hipMemsetAsync(indeg.ptr, 0, N * sizeof(int), st));
+            hipLaunchKernelGGL(k_ftr_indeg, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr, edges.ptr,
+                               nchunk, chunk, indeg.ptr);
+            scan_i32_excl(indeg.ptr, inoff.ptr, N);
+            const int Ein = read1(inoff.ptr + N - 1) + read1(indeg.ptr + N - 1);
+            DBuf<int>& inedges = memoff;  // reused as in-edge storage
+            inedges.reserve(std::max(Ein, 1));
+            HIPCHK(hipMemsetAsync(cursor.ptr, 0, N * sizeof(int), st));
+            hipLaunchKernelGGL(k_ftr_infill, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr,
+                               edges.ptr, nchunk, chunk, inoff.ptr, cursor.ptr, inedges.ptr);
+            st0.reserve(N);
+            st1.reserve(N);
+            HIPCHK(hipMemsetAsync(st0.ptr, kUnknown, N, st));
+            uint8_t* sa = st0.ptr;
+            uint8_t* sb = st1.ptr;
+            for (int round = 0; round <= N + 1; round++) {
+                HIPCHK(hipMemsetAsync(counters.ptr + 4, 0, sizeof(int), st));
+                hipLaunchKernelGGL(k_ftr_round, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, inoff.ptr, indeg.ptr,
+                                   inedges.ptr, sa, sb, counters.ptr + 4);
+                std::swap(sa, sb);
+                if (read1(counters.ptr + 4) == 0) break;
+            }
+            hipLaunchKernelGGL(k_ftr_claim, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr, edges.ptr,
+                               sa, nchunk, chunk, Fc.ptr);
+            if (nchunk == 8) {
+                for (int lvl = 0; lvl < 3; lvl++) {
+                    const int pairs = 8 >> (lvl + 1);
+                    hipLaunchKernelGGL(k_ftr_merge, dim3(ceil_div(N, 256), pairs), dim3(256), 0, st, N, Fc.ptr, lvl,
+                                       chunk, 8);
+                }
+            }
+            HIPCHK(hipGetLastError());
+        }
+        // reduction groups: roots ascending, members ascending per root
+        isroot.reserve(N);
+        memkey.reserve(N);
+        memval.reserve(N);
+        smkey.reserve(N);
+        smval.reserve(N);
+        roots.reserve(N);
+        jiota.reserve(N);
+        memcnt.reserve(N);
+        hipLaunchKernelGGL(k_ftr_rootflags, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, Fc.ptr, isroot.ptr,
+                           memkey.ptr, memval.ptr);
+        hipLaunchKernelGGL(k_iota, dim3(ceil_div(N, 256)), dim3(256), 0, st, jiota.ptr, N);
+        cub([&](void* t, size_t& b) {
+            return hipcub::DeviceSelect::Flagged(t, b, jiota.ptr, isroot.ptr, roots.ptr, counters.ptr + 5, N, st);
+        });
+        const int nroots = read1(counters.ptr + 5);
+        const int nm = N - nroots;
+        if (nm > 0) {
+            cub([&](void* t, size_t& b) {
+                return hipcub::DeviceRadixSort::SortPairs(t, b, memkey.ptr, smkey.ptr, memval.ptr, smval.ptr, N, 0, 32,
+                                                          st);
+            });
+        }
+        HIPCHK(hipMemsetAsync(memcnt.ptr, 0, N * sizeof(int), st));
+        if (nm > 0)
+            hipLaunchKernelGGL(k_ftr_memcount, dim3(ceil_div(nm, 256)), dim3(256), 0, st, nm, smkey.ptr, memcnt.ptr);
+        DBuf<int>& moff = cursor;  // member offsets per root (local)
+        moff.reserve(N);
+        scan_i32_excl(memcnt.ptr, moff.ptr, N);
+        if (nm > 0)
+            hipLaunchKernelGGL(k_ftr_memslots, dim3(ceil_div(nm, 256)), dim3(256), 0, st, nm, smval.ptr, perm.ptr,
+                               base, mbase, mem_slots.ptr);
+        hipLaunchKernelGGL(k_ftr_groups, dim3(ceil_div(nroots, 256)), dim3(256), 0, st, nroots, roots.ptr, perm.ptr,
+                           base, gbase, mbase, m, memcnt.ptr, moff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr,
+                           grp_m.ptr);
+        HIPCHK(hipGetLastError());
+        gbase += nroots;
+        mbase += nm;
+    }
+
+    // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).
+    void deterministic_transform(const double* Trans, const double* bias);
+
+    // Host copy of parent term `idx` of bucket `shape`.
+    int term(int shape, int idx, int* info, double* A, double* p_, double* b, unsigned int* keys_, double* g) {
+        if (shape < 0 || shape >= sr || idx < 0 || idx >= in_count[shape]) return -1;
+        int o = 0;
+        for (int m = 0; m < shape; m++) o += in_count[m];
+        o += idx;
+        int mm;
+        int gn;
+        int64_t go;
+        HIPCHK(hipMemcpy(&mm, in.m.ptr + o, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&gn, in.gp_n.ptr + o, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&go, in.gp_off.ptr + o, sizeof(int64_t), hipMemcpyDeviceToHost));
+        info[0] = mm;
+        info[1] = gn;
+        info[2] = mm;
+        if (A) HIPCHK(hipMemcpy(A, in.A.ptr + (int64_t)o * MM * d, mm * d * sizeof(double), hipMemcpyDeviceToHost));
+        if (p_) HIPCHK(hipMemcpy(p_, in.p.ptr + (int64_t)o * MM, mm * sizeof(double), hipMemcpyDeviceToHost));
+        if (b) HIPCHK(hipMemcpy(b, in.b.ptr + (int64_t)o * d, d * sizeof(double), hipMemcpyDeviceToHost));
+        if (keys_ && g && gn > 0) {
+            HIPCHK(hipMemcpy(keys_, gk[gcur].ptr + go, gn * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(g, gv[gcur].ptr + go, gn * sizeof(cplx), hipMemcpyDeviceToHost));
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+void Engine::widen(const int* src, int64_t* dst, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_widen, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, n);
+}
+
+void Engine::deterministic_transform(const double* Trans, const double* bias) {
+    DBuf<double> dT;
+    dT.reserve(d * d + d);
+    HIPCHK(hipMemcpy(dT.ptr, Trans, d * d * sizeof(double), hipMemcpyHostToDevice));
+    const bool shift = !skip && bias != nullptr;
+    if (shift) HIPCHK(hipMemcpy(dT.ptr + d * d, bias, d * sizeof(double), hipMemcpyHostToDevice));
+    TermPool IN = in.view();
+    MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_det_tp<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dT.ptr,
+                                         shift ? dT.ptr + d * d : nullptr));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+// ======================================================================== C ABI
+namespace {
+
+struct Handle {
+    CauchyDynamicsUpdateContainer duc;
+    Engine* eng = nullptr;
+    void (*cb)(CauchyDynamicsUpdateContainer*) = nullptr;
+    std::vector<double> Phi, Gamma, B, H, beta, gamma, x;
+};
+
+void set_err(const char* where, const std::exception& e) {
+    g_last_error = std::string(where) + ": " + e.what();
+    fprintf(stderr, "[cauchy_mi355x] %s\n", g_last_error.c_str());
+}
+
+template <class T>
+T* mallocd(size_t n) {
+    T* p = (T*)malloc(std::max<size_t>(n, 1) * sizeof(T));
+    if (!p) {
+        fprintf(stderr, "Pointer allocated by malloc has returned with NULL, indicating FAILURE!\n");
+        exit(1);
+    }
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* pycauchy_initialize_lti(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                              int size_b0, double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B,
+                              int size_B, double* beta, int size_beta, double* H, int size_H, double* gamma,
+                              int size_gamma, double dt, int init_step, bool debug_print) {
+    (void)size_A0;
+    (void)size_p0;
+    (void)debug_print;
+    try {
+        const int n = size_b0;
+        const int pncc = size_beta;
+        const int cmcc = n > 0 ? size_B / n : 0;
+        const int p = size_gamma;
+        Handle* h = new Handle;
+        h->Phi.assign(Phi, Phi + size_Phi);
+        h->Gamma.assign(Gamma, Gamma + size_Gamma);
+        h->B.assign(B, B + size_B);
+        h->H.assign(H, H + size_H);
+        h->beta.assign(beta, beta + size_beta);
+        h->gamma.assign(gamma, gamma + size_gamma);
+        h->x.assign(n, 0.0);
+        h->Phi.resize(std::max(n * n, 1));
+        h->Gamma.resize(std::max(n * pncc, 1));
+        h->B.resize(std::max(n * cmcc, 1));
+        h->H.resize(std::max(p * n, 1));
+        h->beta.resize(std::max(pncc, 1));
+        h->gamma.resize(std::max(p, 1));
+        CauchyDynamicsUpdateContainer& duc = h->duc;
+        memset(&duc, 0, sizeof(duc));
+        duc.x = h->x.data();
+        duc.u = nullptr;
+        duc.dt = dt;
+        duc.step = init_step;
+        duc.n = n;
+        duc.cmcc = cmcc;
+        duc.pncc = pncc;
+        duc.p = p;
+        duc.Phi = h->Phi.data();
+        duc.Gamma = h->Gamma.data();
+        duc.B = h->B.data();
+        duc.H = h->H.data();
+        duc.beta = h->beta.data();
+        duc.gamma = h->gamma.data();
+        h->eng = new Engine(num_steps, n, cmcc, pncc, p, A0, p0, b0);
+        return h;
+    } catch (const std::exception& e) {
+        set_err("pycauchy_initialize_lti", e);
+        return nullptr;
+    }
+}
+
+void* pycauchy_initialize_ltv(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                              int size_b0, double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B,
+                              int size_B, double* beta, int size_beta, double* H, int size_H, double* gamma,
+                              int size_gamma, void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                              double dt, int init_step, bool debug_print) {
+    Handle* h = (Handle*)pycauchy_initialize_lti(num_steps, A0, size_A0, p0, size_p0, b0, size_b0, Phi, size_Phi,
+                                                 Gamma, size_Gamma, B, size_B, beta, size_beta, H, size_H, gamma,
+                                                 size_gamma, dt, init_step, debug_print);
+    if (h) h->cb = f_dyn_update_callback;
+    return h;
+}
+
+void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, double* controls, int size_controls,
+                               double** out_Phi, int* size_out_Phi, double** out_Gamma, int* size_out_Gamma,
+                               double** out_B, int* size_out_B, double** out_H, int* size_out_H, double** out_beta,
+                               int* size_out_beta, double** out_gamma, int* size_out_gamma, double** out_fz,
+                               int* size_out_fz, double** out_xhat, int* size_out_xhat, double** out_Phat,
+                               int* size_out_Phat, double** out_cerr_fz, int* size_out_cerr_fz,
+                               double** out_cerr_xhat, int* size_out_cerr_xhat, double** out_cerr_Phat,
+                               int* size_out_cerr_Phat, int** out_err_code, int* size_out_err_code) {
+    (void)size_controls;
+    Handle* h = (Handle*)handle;
+    Engine* e = h->eng;
+    CauchyDynamicsUpdateContainer& duc = h->duc;
+    duc.u = controls;
+    const int p = duc.p, n = duc.n, cmcc = duc.cmcc, pncc = duc.pncc;
+    const int nm = size_msmts;
+    const bool is_ltv = h->cb != nullptr;
+    // dynamics outputs (pycauchy.hpp:379-419)
+    if (!is_ltv) {
+        *size_out_Phi = *size_out_Gamma = *size_out_B = *size_out_H = *size_out_beta = *size_out_gamma = 0;
+        *out_Phi = mallocd<double>(0);
+        *out_Gamma = mallocd<double>(0);
+        *out_B = mallocd<double>(0);
+        *out_H = mallocd<double>(0);
+        *out_beta = mallocd<double>(0);
+        *out_gamma = mallocd<double>(0);
+    } else {
+        *size_out_Phi = n * n;
+        *out_Phi = mallocd<double>(n * n);
+        *size_out_Gamma = n * pncc;
+        *out_Gamma = mallocd<double>(n * pncc);
+        *size_out_B = n * cmcc;
+        *out_B = mallocd<double>(n * cmcc);
+        *size_out_H = nm * n;
+        *out_H = mallocd<double>(nm * n);
+        *size_out_beta = pncc;
+        *out_beta = mallocd<double>(pncc);
+        *size_out_gamma = nm;
+        *out_gamma = mallocd<double>(nm);
+    }
+    *size_out_fz = nm;
+    *out_fz = mallocd<double>(nm);
+    *size_out_xhat = nm * n;
+    *out_xhat = mallocd<double>(nm * n);
+    *size_out_Phat = nm * n * n;
+    *out_Phat = mallocd<double>(nm * n * n);
+    *size_out_cerr_fz = nm;
+    *out_cerr_fz = mallocd<double>(nm);
+    *size_out_cerr_xhat = nm;
+    *out_cerr_xhat = mallocd<double>(nm);
+    *size_out_cerr_Phat = nm;
+    *out_cerr_Phat = mallocd<double>(nm);
+    *size_out_err_code = nm;
+    *out_err_code = mallocd<int>(nm);
+    const int start = p - nm;
+    if (is_ltv) {
+        for (int i = 0; i < n; i++) duc.x[i] = e->mean[i].re;
+        h->cb(&duc);
+        memcpy(*out_Phi, duc.Phi, n * n * sizeof(double));
+        memcpy(*out_Gamma, duc.Gamma, n * pncc * sizeof(double));
+        memcpy(*out_B, duc.B, n * cmcc * sizeof(double));
+        memcpy(*out_H, duc.H + start * n, nm * n * sizeof(double));
+        memcpy(*out_beta, duc.beta, pncc * sizeof(double));
+        memcpy(*out_gamma, duc.gamma + start, nm * sizeof(double));
+    }
+    for (int i = 0; i < nm; i++) {
+        const int idx = start + i;
+        try {
+            e->update(msmts[i], duc.Phi, duc.Gamma, duc.beta, duc.H + idx * n, duc.gamma[idx], duc.B, duc.u);
+        } catch (const std::exception& ex) {
+            set_err("pycauchy_single_step_ltiv", ex);
+        }
+        (*out_fz)[i] = e->fz.re;
+        for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
+        for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
+        (*out_cerr_fz)[i] = e->fz.im;
+        double mx = fabs(e->mean[0].im);
+        for (int j = 1; j < n; j++) mx = std::max(mx, fabs(e->mean[j].im));
+        (*out_cerr_xhat)[i] = mx;
+        double mp = fabs(e->cov[0].im);
+        for (int j = 1; j < n * n; j++) mp = std::max(mp, fabs(e->cov[j].im));
+        (*out_cerr_Phat)[i] = mp;
+        (*out_err_code)[i] = e->errs;
+    }
+}
+
+void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                                int size_b0, double* xbar, int size_xbar) {
+    Handle* h = (Handle*)handle;
+    Engine* e = h->eng;
+    const int n = e->d;
+    if (size_A0 > 0) memcpy(e->A0.data(), A0, n * n * sizeof(double));
+    if (size_p0 > 0) memcpy(e->p0.data(), p0, n * sizeof(double));
+    if (size_b0 > 0) memcpy(e->b0.data(), b0, n * sizeof(double));
+    if (size_xbar > 0) memcpy(h->duc.x, xbar, n * sizeof(double));
+    try {
+        e->reset_terms();
+    } catch (const std::exception& ex) {
+        set_err("pycauchy_single_step_reset", ex);
+    }
+}
+
+void pycauchy_single_step_set_master_step(void* handle, int step) { ((Handle*)handle)->eng->master_step = step; }
+
+void pycauchy_single_step_set_window_number(void* handle, int win_num) { ((Handle*)handle)->eng->win_num = win_num; }
+
+int pycauchy_single_step_get_number_of_terms(void* handle) { return ((Handle*)handle)->eng->Nt; }
+
+void pycauchy_single_step_deterministic_transform(void* handle, double* Trans, int size_Trans, double* bias,
+                                                  int size_bias) {
+    (void)size_Trans;
+    try {
+        ((Handle*)handle)->eng->deterministic_transform(Trans, size_bias > 0 ? bias : nullptr);
+    } catch (const std::exception& ex) {
+        set_err("pycauchy_single_step_deterministic_transform", ex);
+    }
+}
+
+void* pycauchy_single_step_get_duc(void* handle) { return &((Handle*)handle)->duc; }
+
+void pycauchy_single_step_shutdown(void* handle) {
+    Handle* h = (Handle*)handle;
+    if (!h) return;
+    delete h->eng;
+    delete h;
+}
+
+int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering) {
+    if (size_ordering > 12) {
+        fprintf(stderr, "[ERROR pycauchy_set_tr_search_idxs_ordering:] Cannot run estimation problems over 12 dimensions currently!\n");
+        return 1;
+    }
+    for (int i = 0; i < size_ordering; i++) g_search_order[i] = ordering[i];
+    return 0;
+}
+
+void* mce_estimator_create(double* A0, double* p0, double* b0, int steps, int d, int cmcc, int pncc, int p,
+                           bool print_basic_info) {
+    (void)print_basic_info;
+    try {
+        Handle* h = new Handle;
+        h->x.assign(d, 0.0);
+        memset(&h->duc, 0, sizeof(h->duc));
+        h->duc.x = h->x.data();
+        h->duc.n = d;
+        h->duc.cmcc = cmcc;
+        h->duc.pncc = pncc;
+        h->duc.p = p;
+        h->eng = new Engine(steps, d, cmcc, pncc, p, A0, p0, b0);
+        return h;
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_create", ex);
+        return nullptr;
+    }
+}
+
+int mce_estimator_step(void* handle, double msmt, double* Phi, double* Gamma, double* beta, double* H,
+                       double gamma, double* B, double* u) {
+    Engine* e = ((Handle*)handle)->eng;
+    if ((e->errs & (1 << 9)) || e->master_step == e->n_mu) return -1;
+    try {
+        return e->update(msmt, Phi, Gamma, beta, H, gamma, B, u);
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_step", ex);
+        return -2;
+    }
+}
+
+int mce_estimator_moments(void* handle, double* fz, double* mean, double* cov) {
+    Engine* e = ((Handle*)handle)->eng;
+    fz[0] = e->fz.re;
+    fz[1] = e->fz.im;
+    for (int i = 0; i < e->d; i++) {
+        mean[2 * i] = e->mean[i].re;
+        mean[2 * i + 1] = e->mean[i].im;
+    }
+    for (int i = 0; i < e->d * e->d; i++) {
+        cov[2 * i] = e->cov[i].re;
+        cov[2 * i + 1] = e->cov[i].im;
+    }
+    return e->Nt;
+}
+
+void mce_estimator_reset(void* handle) {
+    try {
+        ((Handle*)handle)->eng->reset_terms();
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_reset", ex);
+    }
+}
+
+void mce_estimator_destroy(void* handle) { pycauchy_single_step_shutdown(handle); }
+
+int mce_last_update(void* handle, double* out_mom, int* out_cnt) {
+    Engine* e = ((Handle*)handle)->eng;
+    if (!e->have_diag) return -1;
+    memcpy(out_mom, e->diag_mom.data(), e->diag_mom.size() * sizeof(double));
+    memcpy(out_cnt, e->diag_cnt.data(), e->diag_cnt.size() * sizeof(int));
+    return e->errs;
+}
+
+int mce_shape_range(void* handle) { return ((Handle*)handle)->eng->sr; }
+
+int mce_term(void* handle, int shape, int idx, int* info, double* A, double* p, double* b, unsigned int* keys,
+             double* gvals) {
+    try {
+        return ((Handle*)handle)->eng->term(shape, idx, info, A, p, b, keys, gvals);
+    } catch (const std::exception& ex) {
+        set_err("mce_term", ex);
+        return -1;
+    }
+}
+
+void mce_set_random_draws(void* handle, const double* root_point, const double* b_pert_8xmax) {
+    Engine* e = ((Handle*)handle)->eng;
+    memcpy(e->root.data(), root_point, e->d * sizeof(double));
+    memcpy(e->bpert.data(), b_pert_8xmax, 8 * e->MM * sizeof(double));
+    e->upload_draws();
+}
+
+void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax) {
+    Engine* e = ((Handle*)handle)->eng;
+    memcpy(root_point, e->root.data(), e->d * sizeof(double));
+    memcpy(b_pert_8xmax, e->bpert.data(), 8 * e->MM * sizeof(double));
+}
+
+void mce_profile_enable(void* handle, int on) { ((Handle*)handle)->eng->prof.on = on != 0; }
+
+void mce_profile_read(void* handle, double* out) {
+    Profiler& p = ((Handle*)handle)->eng->prof;
+    for (int i = 0; i < kNumPhases; i++) out[i] = p.ms[i];
+    out[kNumPhases] = p.total;
+    out[kNumPhases + 1] = (double)p.geval_launches;
+    out[kNumPhases + 2] = p.geval_bytes;
+}
+
+void mce_profile_reset(void* handle) {
+    Profiler& p = ((Handle*)handle)->eng->prof;
+    for (int i = 0; i < kNumPhases; i++) p.ms[i] = 0;
+    p.total = 0;
+    p.geval_launches = 0;
+    p.geval_bytes = 0;
+}
+
+const char* mce_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

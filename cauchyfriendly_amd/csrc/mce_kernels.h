@@ -1,0 +1,1478 @@
+// HIP kernels of the MCE term-propagation step (gfx950).
+//
+// Per scalar measurement update the host drives, in order:
+//   k_tp               time propagation + Gamma coalignment, one thread per parent
+//   k_btab_simple      B-tables that need no enumeration (copy / identity / empty)
+//   k_btab_enum        time-propagation B-table by vertex enumeration, one block per parent
+//   k_mu               measurement update, one thread per output slot (parent x child)
+//   k_mom_partial/final  deterministic FP64 tree reduction of fz / mean / covariance
+//   k_ftr_*            fast term reduction per shape bucket
+//   k_group_btab       B-table of each reduction group's root, one wavefront per group
+//   k_group_eval       g-tables of each group: root, replacement chain, member sums
+//   k_group_sort       key order for groups whose root was replaced
+//   k_compact          survivors become the next step's parents
+#pragma once
+#include "mce_device.h"
+#include "mce_pool.h"
+
+namespace mce {
+
+constexpr int kWave = 64;
+
+enum BtabMode : uint8_t { kModeNone = 0, kModeCopy = 1, kModeIdent = 2, kModeTrivial = 3, kModeEmpty = 4, kModeEnum = 5 };
+
+// Per-update constants (one small device buffer).
+struct StepConst {
+    double Phi[kMaxD * kMaxD];
+    double H[kMaxD];
+    double G[kMaxM * kMaxD];   // pre-coaligned Gamma rows
+    double beta[kMaxM];
+    double Bu[kMaxD];          // B u, computed once per update
+    double root[kMaxD];
+    double msmt, gamma, gscale;
+    int d, ng, cmcc, with_tp, skip, first, T, threaded;
+    int order[kMaxD];          // fast-term-reduction axis order
+};
+
+// ------------------------------------------------------------------ small dense linear algebra
+// Column-sum one-norm, cauchy_linalg.hpp:1301-1314.
+template <int D>
+__device__ inline double dev_one_norm(const double* A) {
+    double best = -1;
+    for (int c = 0; c < D; c++) {
+        double s = 0;
+        for (int r = 0; r < D; r++) s += fabs(A[r * D + c]);
+        if (s > best) best = s;
+    }
+    return best;
+}
+
+// In-place partial-pivot LU, cauchy_linalg.hpp:1180-1232.
+template <int D>
+__device__ inline bool dev_lu(double* A, int* piv) {
+    for (int j = 0; j < D; ++j) {
+        double pv = kPluEps;
+        int pi = -1;
+        for (int i = j; i < D; ++i)
+            if (fabs(A[i * D + j]) > fabs(pv)) {
+                pv = A[i * D + j];
+                pi = i;
+            }
+        if (pi == -1) return false;
+        if (pi != j)
+            for (int k = 0; k < D; k++) {
+                double t = A[j * D + k];
+                A[j * D + k] = A[pi * D + k];
+                A[pi * D + k] = t;
+            }
+        piv[j] = pi;
+        for (int k = j + 1; k < D; ++k) {
+            A[k * D + j] /= A[j * D + j];
+            const double t = A[k * D + j];
+            for (int q = j + 1; q < D; q++) A[k * D + q] -= t * A[j * D + q];
+        }
+    }
+    return true;
+}
+
+template <int D>
+__device__ inline void dev_fwd(const double* LU, double* b) {
+    for (int i = 0; i < D; i++) {
+        double s = b[i];
+        for (int j = 0; j < i; j++) s -= LU[i * D + j] * b[j];
+        b[i] = s;
+    }
+}
+template <int D>
+__device__ inline void dev_bwd(const double* LU, double* b) {
+    for (int i = D - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = D - 1; j > i; j--) s -= LU[i * D + j] * b[j];
+        b[i] = s / LU[i * D + i];
+    }
+}
+template <int D>
+__device__ inline void dev_pivot_inverse(const int* piv, int* pt) {
+    int preg[D];
+    for (int i = 0; i < D; i++) preg[i] = i;
+    for (int i = 0; i < D; i++) {
+        const int t = preg[i];
+        preg[i] = preg[piv[i]];
+        preg[piv[i]] = t;
+    }
+    for (int i = 0; i < D; i++) pt[preg[i]] = i;
+}
+
+// One-norm condition number via explicit inverse, cauchy_linalg.hpp:1337-1392.
+template <int D>
+__device__ inline double dev_cond1(double* A, int* piv) {
+    const double na = dev_one_norm<D>(A);
+    if (!dev_lu<D>(A, piv)) return 1.7976931348623157e308;
+    double W[D * D];
+    for (int i = 0; i < D * D; i++) W[i] = 0;
+    int pt[D];
+    dev_pivot_inverse<D>(piv, pt);
+    for (int i = 0; i < D; i++) {
+        W[i * D + pt[i]] = 1;
+        dev_fwd<D>(A, W + i * D);
+        dev_bwd<D>(A, W + i * D);
+    }
+    // one-norm of the transpose = largest row sum of W
+    double best = -1;
+    for (int c = 0; c < D; c++) {
+        double s = 0;
+        for (int r = 0; r < D; r++) s += fabs(W[c * D + r]);
+        if (s > best) best = s;
+    }
+    return na * best;
+}
+
+template <int D>
+__device__ inline void dev_lu_solve(const double* LU, const int* piv, const double* b, double* x) {
+    int pt[D];
+    dev_pivot_inverse<D>(piv, pt);
+    for (int i = 0; i < D; i++) x[pt[i]] = b[i];
+    dev_fwd<D>(LU, x);
+    dev_bwd<D>(LU, x);
+}
+
+template <int D>
+__device__ inline double dev_dot(const double* x, const double* y) {
+    double z = 0.0;
+    for (int i = 0; i < D; i++) z += x[i] * y[i];
+    return z;
+}
+
+// ------------------------------------------------------------------ wave helpers
+__device__ inline int wave_lane() { return threadIdx.x & (kWave - 1); }
+
+__device__ inline int wave_excl_scan(int v, int* total) {
+    const int lane = wave_lane();
+    int x = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        int y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, kWave - 1, kWave);
+    return x - v;
+}
+
+__device__ inline bool wave_all(bool pred) { return __all(pred); }
+
+// ------------------------------------------------------------------ time propagation
+// cauchy_term.hpp:435-531 (time_prop, normalize_hps, tp_coalign), with the B-table
+// mode decision of cauchy_estimator.hpp:678-708 (serial) / 1614-1637 (threaded).
+template <int D>
+__global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const int* __restrict__ in_off,
+                     const int* __restrict__ cen_half, int* __restrict__ slots, int* __restrict__ btcap,
+                     int* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int MM = P.MM;
+    int m = P.m[i];
+    const int phc = m;
+    P.phc[i] = phc;
+    double* A = P.A + (int64_t)i * MM * D;
+    double* p = P.p + (int64_t)i * MM;
+    double* b = P.b + (int64_t)i * D;
+    if (sc->with_tp) {
+        double t[D];
+        for (int r = 0; r < m; r++) {
+            for (int k = 0; k < D; k++) t[k] = A[r * D + k];
+            for (int j = 0; j < D; j++) {
+                double s = 0.0;
+                for (int k = 0; k < D; k++) s += t[k] * sc->Phi[k + j * D];
+                A[r * D + j] = s;
+            }
+        }
+        for (int k = 0; k < D; k++) t[k] = b[k];
+        for (int r = 0; r < D; r++) {
+            double s = 0.0;
+            for (int j = 0; j < D; j++) s += sc->Phi[r * D + j] * t[j];
+            b[r] = s;
+        }
+        if (sc->cmcc > 0)
+            for (int r = 0; r < D; r++) b[r] += 1.0 * sc->Bu[r];
+        for (int r = 0; r < m; r++) {
+            double n1 = 0;
+            for (int k = 0; k < D; k++) n1 += fabs(A[r * D + k]);
+            p[r] *= n1;
+            for (int k = 0; k < D; k++) A[r * D + k] /= n1;
+        }
+        uint32_t F = (sc->ng >= 32) ? 0xffffffffu : ((1u << sc->ng) - 1u);
+        for (int j = 0; j < sc->ng; j++)
+            for (int k = 0; k < m; k++) {
+                if (!((F >> j) & 1u)) continue;
+                bool pos, neg;
+                coalign_gates(&sc->G[j * D], &A[k * D], D, pos, neg);
+                if (pos || neg) {
+                    F &= ~(1u << j);
+                    p[k] += sc->beta[j];
+                }
+            }
+        for (int j = 0; j < sc->ng; j++)
+            if ((F >> j) & 1u) {
+                if (m >= MM) {
+                    atomicOr(err, 4);
+                    break;
+                }
+                for (int k = 0; k < D; k++) A[m * D + k] = sc->G[j * D + k];
+                p[m] = sc->beta[j];
+                m++;
+            }
+        P.m[i] = m;
+    }
+    int tid = 0;
+    if (sc->threaded) tid = (i - in_off[phc]) % sc->T;
+    P.tid[i] = tid;
+    uint8_t mode;
+    int cap;
+    const int gn = P.gp_n[i];
+    if (sc->skip) {
+        mode = kModeNone;
+        cap = 0;
+    } else if (!sc->with_tp) {
+        mode = kModeCopy;
+        cap = gn;
+    } else if (sc->threaded && m <= D) {
+        mode = kModeIdent;
+        cap = gn;
+    } else if (m == phc) {
+        mode = kModeCopy;
+        cap = gn;
+    } else if (m < D) {
+        mode = kModeTrivial;
+        cap = max(1 << (m - 1), gn);
+    } else if (m == D) {
+        mode = kModeEmpty;
+        cap = 0;
+    } else {
+        mode = kModeEnum;
+        cap = cen_half[m];
+    }
+    P.mode[i] = mode;
+    btcap[i] = cap;
+    slots[i] = m + 1;
+}
+
+// B-tables that need no enumeration; enumeration jobs are appended to a list.
+__global__ void k_btab_simple(TermPool P, int n, const int64_t* __restrict__ bt_base, const uint32_t* __restrict__ gkeys,
+                              uint32_t* __restrict__ Barena, int* __restrict__ enum_list, int* __restrict__ enum_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t mode = P.mode[i];
+    const int64_t off = bt_base[i];
+    P.bt_off[i] = off;
+    P.bt_sgn[i] = 0;
+    const int gn = P.gp_n[i];
+    uint32_t* B = Barena + off;
+    switch (mode) {
+        case kModeCopy: {
+            const uint32_t* src = gkeys + P.gp_off[i];
+            for (int k = 0; k < gn; k++) B[k] = src[k];
+            P.bt_n[i] = gn;
+        } break;
+        case kModeIdent:
+            for (int k = 0; k < gn; k++) B[k] = (uint32_t)k;
+            P.bt_n[i] = gn;
+            break;
+        case kModeTrivial: {
+            const int m = P.m[i];
+            const int nn = max(1 << (m - 1), gn);
+            for (int k = 0; k < nn; k++) B[k] = (uint32_t)k;
+            P.bt_n[i] = gn;
+        } break;
+        case kModeEnum: {
+            const int slot = atomicAdd(enum_count, 1);
+            enum_list[slot] = i;
+            P.bt_n[i] = 0;
+        } break;
+        default:
+            P.bt_n[i] = 0;
+            break;
+    }
+}
+
+// Cells of the time-propagated arrangement by vertex enumeration,
+// cell_enumeration.hpp:676-855.  One block per parent; the sign vectors seen are
+// an LDS bitset of 2^m bits, so the output comes out sorted and de-duplicated.
+template <int D>
+__global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __restrict__ list,
+                                                   const double* __restrict__ bpert, const uint8_t* __restrict__ combos,
+                                                   const int64_t* __restrict__ combo_off, const int* __restrict__ combo_cnt,
+                                                   const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ Barena,
+                                                   const int* __restrict__ btcap, int* __restrict__ err) {
+    extern __shared__ uint32_t bits[];
+    __shared__ int scan_buf[256];
+    const int i = list[blockIdx.x];
+    const int MM = P.MM;
+    const int m = P.m[i];
+    const int phc = P.phc[i];
+    const int W = m >= 5 ? (1 << (m - 5)) : 1;
+    for (int w = threadIdx.x; w < W; w += blockDim.x) bits[w] = 0;
+    __syncthreads();
+    const double* A = P.A + (int64_t)i * MM * D;
+    const double* bp = bpert + (int64_t)P.tid[i] * MM;
+    const uint8_t* cmb = combos + combo_off[m];
+    const int nc = combo_cnt[m];
+    for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
+        const uint8_t* cb = cmb + ci * D;
+        double Ac[D * D], bc[D], v[D];
+        int piv[D];
+        for (int j = 0; j < D; j++) {
+            for (int k = 0; k < D; k++) Ac[j * D + k] = A[cb[j] * D + k];
+            bc[j] = bp[cb[j]];
+        }
+        if (dev_cond1<D>(Ac, piv) > kCondEps) continue;
+        dev_lu_solve<D>(Ac, piv, bc, v);
+        uint32_t niv = 0;
+        int cj = 0;
+        for (int ac = 0; ac < m; ac++) {
+            if (cj < D && cb[cj] == ac) {
+                cj++;
+                continue;
+            }
+            if ((dev_dot<D>(A + ac * D, v) - bp[ac]) < 0) niv |= (1u << ac);
+        }
+        for (int s = 0; s < (1 << D); s++) {
+            uint32_t sv = niv;
+            for (int k = 0; k < D; k++)
+                if ((s >> k) & 1) sv |= (1u << cb[k]);
+            atomicOr(&bits[sv >> 5], 1u << (sv & 31));
+        }
+    }
+    __syncthreads();
+    // keep sign vectors whose parent part is a cell of the parent arrangement
+    const uint32_t* gk = gkeys + P.gp_off[i];
+    const int gn = P.gp_n[i];
+    const uint32_t top_phc = 1u << (phc - 1), rev_phc = (phc >= 32) ? 0xffffffffu : ((1u << phc) - 1u);
+    const uint32_t valid_mask = (m >= 5) ? 0xffffffffu : ((1u << (1 << m)) - 1u);
+    for (int w = threadIdx.x; w < W; w += blockDim.x) {
+        uint32_t x = bits[w] & valid_mask, keep = 0;
+        while (x) {
+            const int bt = __ffs(x) - 1;
+            x &= x - 1;
+            uint32_t psv = ((uint32_t)(w * 32 + bt)) & rev_phc;
+            if (psv & top_phc) psv ^= rev_phc;
+            if (bsearch_u32(gk, psv, gn) >= 0) keep |= (1u << bt);
+        }
+        bits[w] = keep;
+    }
+    __syncthreads();
+    // antipodal pairs; the half-storage representative has bit m-1 clear
+    const uint32_t rev_m = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    const int nlow = 1 << (m - 1);  // candidate keys [0, 2^(m-1))
+    const int WL = (nlow + 31) >> 5;
+    const int per = (WL + blockDim.x - 1) / blockDim.x;
+    const int w0 = threadIdx.x * per, w1 = min(WL, w0 + per);
+    int cnt = 0;
+    for (int w = w0; w < w1; w++) {
+        uint32_t x = bits[w];
+        if (nlow < 32) x &= (1u << nlow) - 1u;
+        while (x) {
+            const int bt = __ffs(x) - 1;
+            x &= x - 1;
+            const uint32_t sv = (uint32_t)(w * 32 + bt);
+            const uint32_t pr = sv ^ rev_m;
+            if ((bits[pr >> 5] >> (pr & 31)) & 1u) cnt++;
+        }
+    }
+    scan_buf[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        int v = (threadIdx.x >= (unsigned)o) ? scan_buf[threadIdx.x - o] : 0;
+        __syncthreads();
+        scan_buf[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int pos = scan_buf[threadIdx.x] - cnt;
+    const int total = scan_buf[blockDim.x - 1];
+    uint32_t* out = Barena + P.bt_off[i];
+    const int cap = btcap[i];
+    for (int w = w0; w < w1; w++) {
+        uint32_t x = bits[w];
+        if (nlow < 32) x &= (1u << nlow) - 1u;
+        while (x) {
+            const int bt = __ffs(x) - 1;
+            x &= x - 1;
+            const uint32_t sv = (uint32_t)(w * 32 + bt);
+            const uint32_t pr = sv ^ rev_m;
+            if ((bits[pr >> 5] >> (pr & 31)) & 1u) {
+                if (pos < cap) out[pos] = sv;
+                pos++;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        P.bt_n[i] = min(total, cap);
+        if (total > cap) atomicOr(err, 8);
+    }
+}
+
+// ------------------------------------------------------------------ g evaluation helpers
+// g value and y_ei of one term at the estimator's root point, cauchy_term.hpp:312-401.
+template <int D>
+__device__ inline cplx dev_eval_g_yei(const double* A, const double* p, const double* b, double c, double dv, int m,
+                                      uint32_t hf, int phc, int z, int lhp, const uint32_t* gk, const cplx* gv,
+                                      int gn, const double* root, bool first, double* ty) {
+    for (int k = 0; k < D; k++) ty[k] = 0.0;
+    double ygi = 0.0;
+    uint32_t neg = 0;
+    for (int l = 0; l < m; l++) {
+        const double s = dev_dot<D>(A + l * D, root) > 0 ? 1 : -1;
+        if (s < 0) neg |= (1u << l);
+        const double scl = p[l] * s;
+        for (int k = 0; k < D; k++) ty[k] += scl * A[l * D + k];
+        if (!((hf >> l) & 1u)) ygi += p[l] * s;
+    }
+    cplx gp, gm;
+    if (first) {
+        gp = cmake(1, 0);
+        gm = cmake(1, 0);
+    } else {
+        const int top = 1 << (phc - 1), rev = (phc >= 32) ? -1 : ((1 << phc) - 1);
+        int lp = 0, lmv = 0, k = 0;
+        for (int l = 0; l < m; l++) {
+            if (k < phc) {
+                if (k == z) {
+                    lmv |= (1 << k);
+                    k++;
+                    if (k == phc) continue;
+                }
+                if ((neg >> l) & 1u) {
+                    lp |= (1 << k);
+                    lmv |= (1 << k);
+                }
+                k++;
+            }
+        }
+        gp = lookup_num(lp ^ lhp, top, rev, gk, gv, gn);
+        gm = lookup_num(lmv ^ lhp, top, rev, gk, gv, gn);
+    }
+    cplx g = csub(cdiv(gp, cmake(ygi + dv, c)), cdiv(gm, cmake(ygi - dv, c)));
+    return cscale(g, kInvTwoPi);
+}
+
+// Measurement-update coalignment in place, cauchy_term.hpp:533-745.  Returns the
+// new shape; sets *cm when hyperplanes were merged (c_map kept).
+template <int D>
+__device__ inline int dev_mu_coalign(double* A, double* p, double* q, uint8_t* cmap, int8_t* csmap, uint32_t* hfp,
+                                     int m, int* err) {
+    for (int r = 0; r < m; r++) {
+        q[r] = p[r];
+        double n1 = 0;
+        for (int k = 0; k < D; k++) n1 += fabs(A[r * D + k]);
+        p[r] *= n1;
+        for (int k = 0; k < D; k++) A[r * D + k] /= n1;
+        cmap[r] = 255;
+        csmap[r] = 1;
+    }
+    const uint32_t hf = *hfp;
+    uint32_t hu = hf;
+    uint32_t F = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    int u = 0;
+    for (int j = 0; j < m - 1; j++) {
+        if (!((F >> j) & 1u)) continue;
+        cmap[j] = (uint8_t)u;
+        for (int k = j + 1; k < m; k++) {
+            if (!((F >> k) & 1u)) continue;
+            bool pos, neg;
+            coalign_gates(A + j * D, A + k * D, D, pos, neg);
+            if (pos) {
+                if (hf) {
+                    const bool hj = (hu >> j) & 1u, hk = (hu >> k) & 1u;
+                    if (!hj && !hk) q[j] += q[k];
+                    else if (hj && !hk) {
+                        q[j] = q[k];
+                        hu &= ~(1u << j);
+                    } else if (!hj && hk) {
+                        hu &= ~(1u << k);
+                    } else {
+                        hu &= ~(1u << k);
+                        q[j] += q[k];
+                    }
+                } else {
+                    q[j] += q[k];
+                }
+                F &= ~(1u << k);
+                p[j] += p[k];
+                cmap[k] = (uint8_t)u;
+                csmap[k] = 1;
+            }
+            if (neg) {
+                if (hf) {
+                    const bool hj = (hu >> j) & 1u, hk = (hu >> k) & 1u;
+                    if (!hj && !hk) q[j] -= q[k];
+                    else if (hj && !hk) {
+                        q[j] = -q[k];
+                        hu &= ~(1u << j);
+                    } else if (!hj && hk) {
+                        hu &= ~(1u << k);
+                    } else {
+                        atomicOr(err, 1);  // the reference exits here (cauchy_term.hpp:676-685)
+                    }
+                } else {
+                    q[j] -= q[k];
+                }
+                F &= ~(1u << k);
+                p[j] += p[k];
+                cmap[k] = (uint8_t)u;
+                csmap[k] = -1;
+            }
+        }
+        u += 1;
+    }
+    if ((F >> (m - 1)) & 1u) cmap[m - 1] = (uint8_t)u;
+    const int ns = __popc(F);
+    if (ns != m) {
+        int w = 1;
+        for (int j = 1; j < m; j++)
+            if ((F >> j) & 1u) {
+                if (w < j) {
+                    for (int k = 0; k < D; k++) A[w * D + k] = A[j * D + k];
+                    p[w] = p[j];
+                    q[w] = q[j];
+                }
+                w++;
+            }
+        if (hf) {
+            uint32_t nh = 0;
+            int cc = 0;
+            for (int j = 0; j < m; j++)
+                if ((F >> j) & 1u) nh |= (((hu >> j) & 1u) << cc++);
+            *hfp = nh;
+        }
+    }
+    return ns;
+}
+
+// ------------------------------------------------------------------ measurement update
+// cauchy_term.hpp:84-310 (msmt_update) + cache_moments inputs + MU coalignment,
+// one thread per output slot.  Slot t < m of parent P is its child t; slot m is
+// the parent itself.  Writes the gather key (shape, reference thread, is_child)
+// that reproduces the reference's term order (cauchy_estimator.hpp:898-937, 1683-1704).
+template <int D>
+__global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT, int nslots,
+                     const StepConst* __restrict__ sc, const uint32_t* __restrict__ gk_prev,
+                     const cplx* __restrict__ gv_prev, double* __restrict__ gy, uint32_t* __restrict__ keys_out,
+                     int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slot_base[mid] <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    const int P = lo;
+    const int t = s - slot_base[P];
+    const int MM = IN.MM;
+    const int m = IN.m[P];
+    const double* A = IN.A + (int64_t)P * MM * D;
+    const double* p = IN.p + (int64_t)P * MM;
+    const double* b = IN.b + (int64_t)P * D;
+    const double* H = sc->H;
+    uint32_t integ = 0, negs = 0;
+    for (int l = 0; l < m; l++) {
+        const double h = dev_dot<D>(H, A + l * D);
+        if (!(fabs(h) < kMuEps)) {
+            integ |= (1u << l);
+            if (!(h > 0)) negs |= (1u << l);
+        }
+    }
+    double* g_out = gy + (int64_t)s * (2 + 2 * D);
+    if (t < m && !((integ >> t) & 1u)) {
+        OUT.flags[s] = 0;
+        keys_out[s] = 0xffffffffu;
+        for (int k = 0; k < 2 + 2 * D; k++) g_out[k] = 0.0;
+        return;
+    }
+    const double zeta = sc->msmt - dev_dot<D>(H, b);
+    double mut[D];
+    double rho_t;
+    if (t == m) {
+        for (int k = 0; k < D; k++) mut[k] = 0.0;
+        rho_t = sc->gamma;
+    } else {
+        const double h = dev_dot<D>(H, A + t * D);
+        const double r = 1.0 / h;
+        for (int k = 0; k < D; k++) mut[k] = A[t * D + k] * r;
+        rho_t = p[t] * fabs(h);
+    }
+    double* oA = OUT.A + (int64_t)s * MM * D;
+    double* op = OUT.p + (int64_t)s * MM;
+    double* oq = OUT.q + (int64_t)s * MM;
+    double* ob = OUT.b + (int64_t)s * D;
+    for (int k = 0; k < D; k++) ob[k] = b[k] + zeta * mut[k];
+    int lo_row = 0;
+    uint32_t hf = 0;
+    for (int ll = 0; ll <= m; ll++) {
+        if (ll == t) continue;
+        double rho;
+        if (ll == m) {
+            rho = sc->gamma;
+            for (int k = 0; k < D; k++) oA[lo_row * D + k] = 0.0 - mut[k];
+        } else {
+            const double h = dev_dot<D>(H, A + ll * D);
+            if (fabs(h) < kMuEps) {
+                rho = p[ll];
+                for (int k = 0; k < D; k++) oA[lo_row * D + k] = A[ll * D + k];
+                hf |= (1u << lo_row);
+            } else {
+                const double r = 1.0 / h;
+                rho = p[ll] * fabs(h);
+                for (int k = 0; k < D; k++) oA[lo_row * D + k] = (A[ll * D + k] * r) - mut[k];
+            }
+        }
+        op[lo_row] = rho;
+        lo_row++;
+    }
+    const int phc = IN.phc[P];
+    OUT.m[s] = m;
+    OUT.c[s] = zeta;
+    OUT.dv[s] = rho_t;
+    OUT.phc[s] = phc;
+    OUT.pbc[s] = m;
+    OUT.z[s] = t;
+    OUT.gp_off[s] = IN.gp_off[P];
+    OUT.gp_n[s] = IN.gp_n[P];
+    const uint32_t enc = negs;
+    uint32_t lhp = enc;
+    if (phc < m) lhp &= (1u << phc) - 1u;
+    uint32_t sg = enc;
+    const uint32_t rev_m = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    if (sg & (1u << (m - 1))) sg ^= rev_m;
+    OUT.lhp[s] = (int)lhp;
+    OUT.bt_off[s] = IN.bt_off[P];
+    OUT.bt_n[s] = IN.bt_n[P];
+    OUT.bt_sgn[s] = sc->skip ? 0 : (int)sg;
+    OUT.tid[s] = IN.tid[P];
+    // moments: g and y_ei of this term (cache_moments, cauchy_estimator.hpp:307-338)
+    {
+        double ty[D];
+        const int64_t go = IN.gp_off[P];
+        const cplx g = dev_eval_g_yei<D>(oA, op, ob, zeta, rho_t, m, hf, phc, t, (int)lhp, gk_prev + go, gv_prev + go,
+                                         IN.gp_n[P], sc->root, sc->first != 0, ty);
+        g_out[0] = g.re;
+        g_out[1] = g.im;
+        for (int k = 0; k < D; k++) {
+            g_out[2 + 2 * k] = -ty[k];
+            g_out[3 + 2 * k] = ob[k];
+        }
+    }
+    int mnew = m;
+    uint8_t fl = kValid | ((t != m) ? kIsNew : 0);
+    if (!sc->skip && !sc->first) {
+        uint8_t* cm = OUT.cmap + (int64_t)s * MM;
+        int8_t* cs = OUT.csmap + (int64_t)s * MM;
+        if (t == m) {
+            for (int r = 0; r < m; r++) {
+                oq[r] = op[r];
+                double n1 = 0;
+                for (int k = 0; k < D; k++) n1 += fabs(oA[r * D + k]);
+                op[r] *= n1;
+                for (int k = 0; k < D; k++) oA[r * D + k] /= n1;
+            }
+        } else {
+            mnew = dev_mu_coalign<D>(oA, op, oq, cm, cs, &hf, m, err);
+            if (mnew < m) fl |= kHasCmap;
+        }
+    }
+    OUT.hflag[s] = hf;
+    OUT.flags[s] = fl;
+    OUT.m[s] = mnew;
+    keys_out[s] = (uint32_t)(((mnew * sc->T) + IN.tid[P]) * 2 + ((t != m) ? 1 : 0));
+}
+
+// ------------------------------------------------------------------ moments
+// Per-block partial sums of fz, mean, covariance over 256 slots, summed in slot order.
+template <int D>
+__global__ __launch_bounds__(256) void k_mom_partial(const double* __restrict__ gy, int nslots,
+                                                     const uint8_t* __restrict__ flags, double* __restrict__ part) {
+    constexpr int NS = 2 + 2 * D;
+    constexpr int NC = 1 + D + D * D;
+    __shared__ double st[256 * NS];
+    __shared__ int valid[256];
+    const int base = blockIdx.x * 256;
+    const int s = base + threadIdx.x;
+    valid[threadIdx.x] = (s < nslots) && (flags[s] & kValid);
+    for (int k = 0; k < NS; k++) st[threadIdx.x * NS + k] = (s < nslots) ? gy[(int64_t)s * NS + k] : 0.0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < NC; c += blockDim.x) {
+        cplx acc = cmake(0, 0);
+        for (int r = 0; r < 256; r++) {
+            if (!valid[r]) continue;
+            const double* e = st + r * NS;
+            const cplx g = cmake(e[0], e[1]);
+            if (c == 0) {
+                acc = cadd(acc, g);
+            } else if (c <= D) {
+                const cplx y = cmake(e[2 + 2 * (c - 1)], e[3 + 2 * (c - 1)]);
+                acc = cadd(acc, cmul(g, y));
+            } else {
+                const int jk = c - 1 - D, j = jk / D, k = jk % D;
+                const cplx yj = cmake(e[2 + 2 * j], e[3 + 2 * j]);
+                const cplx yk = cmake(e[2 + 2 * k], e[3 + 2 * k]);
+                acc = csub(acc, cmul(cmul(g, yj), yk));
+            }
+        }
+        part[((int64_t)blockIdx.x * NC + c) * 2] = acc.re;
+        part[((int64_t)blockIdx.x * NC + c) * 2 + 1] = acc.im;
+    }
+}
+
+template <int D>
+__global__ void k_mom_final(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+    constexpr int NC = 1 + D + D * D;
+    for (int c = threadIdx.x; c < NC; c += blockDim.x) {
+        double re = 0, im = 0;
+        for (int b = 0; b < nblk; b++) {
+            re += part[((int64_t)b * NC + c) * 2];
+            im += part[((int64_t)b * NC + c) * 2 + 1];
+        }
+        out[2 * c] = re;
+        out[2 * c + 1] = im;
+    }
+}
+
+// Moment sums in the reference's exact order: one chain per reference thread t
+// (parents i = t, t+T, ... of every shape bucket, cauchy_estimator.hpp:1575-1604),
+// each parent's own term before its children (cache_moments, cauchy_estimator.hpp:307-338
+// and 1436-1468), so the partial sums, and hence fz, x_hat, P_hat and the g-table
+// scale 1/(2 pi Re fz), are bit-identical to the reference.  One block per chain,
+// one lane per moment component.
+template <int D>
+__global__ __launch_bounds__(128) void k_mom_chain(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
+                                                   const int* __restrict__ slot_base, const int* __restrict__ in_off,
+                                                   int sr, int T, double* __restrict__ part) {
+    constexpr int NS = 2 + 2 * D;
+    constexpr int NC = 1 + D + D * D;
+    const int t = blockIdx.x;
+    const int c = threadIdx.x;
+    if (c >= NC) return;
+    int kind, j = 0, k = 0;
+    if (c == 0) kind = 0;
+    else if (c <= D) {
+        kind = 1;
+        j = c - 1;
+    } else {
+        kind = 2;
+        j = (c - 1 - D) / D;
+        k = (c - 1 - D) % D;
+    }
+    cplx acc = cmake(0, 0);
+    for (int m = 0; m < sr; m++) {
+        for (int i = in_off[m] + t; i < in_off[m + 1]; i += T) {
+            const int s0 = slot_base[i];
+            const int mp = slot_base[i + 1] - s0 - 1;
+            for (int q = -1; q < mp; q++) {
+                const int s = (q < 0) ? s0 + mp : s0 + q;
+                if (q >= 0 && !(flags[s] & kValid)) continue;
+                const double* e = gy + (int64_t)s * NS;
+                const cplx g = cmake(e[0], e[1]);
+                if (kind == 0) {
+                    acc = cadd(acc, g);
+                } else if (kind == 1) {
+                    acc = cadd(acc, cmul(g, cmake(e[2 + 2 * j], e[3 + 2 * j])));
+                } else {
+                    const cplx gy_j = cmul(g, cmake(e[2 + 2 * j], e[3 + 2 * j]));
+                    acc = csub(acc, cmul(gy_j, cmake(e[2 + 2 * k], e[3 + 2 * k])));
+                }
+            }
+        }
+    }
+    part[((int64_t)t * NC + c) * 2] = acc.re;
+    part[((int64_t)t * NC + c) * 2 + 1] = acc.im;
+}
+
+// ------------------------------------------------------------------ gather
+__global__ void k_iota(uint32_t* __restrict__ v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+__global__ void k_shape_hist(const TermPool OUT, int n, int* __restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (OUT.flags[i] & kValid) atomicAdd(&hist[OUT.m[i]], 1);
+}
+
+// ------------------------------------------------------------------ fast term reduction
+// Bucket-local index r -> slot perm[base + r].
+__global__ void k_ftr_keys(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int d, int axis,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t s = perm[base + r];
+    keys[r] = f64_radix_key(OUT.b[(int64_t)s * d + axis]);
+    idx[r] = (uint32_t)r;
+}
+
+__global__ void k_ftr_vals(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int d, int axis,
+                           const uint32_t* __restrict__ sidx, double* __restrict__ vals) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    vals[r] = OUT.b[(int64_t)perm[base + sidx[r]] * d + axis];
+}
+
+__device__ inline int upper_bound_f64(const double* v, int n, double x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] > x) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Candidate pairs (i, j > i) of the greedy reduction, term_reduction.hpp:159-276:
+// b_j inside (b_i - ep, b_i + ep] on every axis (the window ftr_binary_search
+// selects, term_reduction.hpp:89-157; a window must hold two points), then the p
+// check and the element-wise check over the first m flattened entries of A.
+// pass 0 counts, pass 1 writes the out-edge list.
+template <int D>
+__global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int m,
+                            const StepConst* __restrict__ sc, const double* __restrict__ vals,
+                            const uint32_t* __restrict__ sidx, int* __restrict__ cnt, const int* __restrict__ eoff,
+                            int* __restrict__ edges, int pass) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ep = kReductionEps;
+    const int MM = OUT.MM;
+    const int64_t si = perm[base + i];
+    const double* bi = OUT.b + si * D;
+    const double* pi = OUT.p + si * MM;
+    const double* Ai = OUT.A + si * MM * D;
+    const int a0 = sc->order[0];
+    const int r0 = upper_bound_f64(vals, n, bi[a0] - ep);
+    const int r1 = upper_bound_f64(vals, n, bi[a0] + ep);
+    int c = 0;
+    int w = pass ? eoff[i] : 0;
+    if (r1 - r0 >= 2) {
+        for (int r = r0; r < r1; r++) {
+            const int j = (int)sidx[r];
+            if (j <= i) continue;
+            const int64_t sj = perm[base + j];
+            const double* bj = OUT.b + sj * D;
+            bool ok = true;
+            for (int a = 1; a < D && ok; a++) {
+                const int ax = sc->order[a];
+                const double l2 = bi[ax] - ep, h2 = bi[ax] + ep;
+                const double v = bj[ax];
+                if (!(v > l2 && v <= h2)) ok = false;
+            }
+            if (!ok) continue;
+            const double* pj = OUT.p + sj * MM;
+            for (int k = 0; k < m; k++)
+                if (fabs(pj[k] - pi[k]) > ep) {
+                    ok = false;
+                    break;
+                }
+            if (!ok) continue;
+            const double* Aj = OUT.A + sj * MM * D;
+            for (int k = 0; k < m; k++) {
+                const double x = Ai[k], y = Aj[k];
+                if (!((fabs(x - y) < ep) || (fabs(x + y) < ep))) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (!ok) continue;
+            if (pass) edges[w++] = j;
+            c++;
+        }
+    }
+    if (!pass) cnt[i] = c;
+}
+
+__device__ inline int ftr_chunk(int i, int nchunk, int chunk) {
+    if (nchunk == 1) return 0;
+    const int c = i / chunk;
+    return c >= nchunk ? nchunk - 1 : c;
+}
+
+// In-degree over same-chunk edges.
+__global__ void k_ftr_indeg(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
+                            const int* __restrict__ edges, int nchunk, int chunk, int* __restrict__ indeg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ci = ftr_chunk(i, nchunk, chunk);
+    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) {
+        const int j = edges[e];
+        if (ftr_chunk(j, nchunk, chunk) == ci) atomicAdd(&indeg[j], 1);
+    }
+}
+
+__global__ void k_ftr_infill(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
+                             const int* __restrict__ edges, int nchunk, int chunk, const int* __restrict__ inoff,
+                             int* __restrict__ cursor, int* __restrict__ inedges) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ci = ftr_chunk(i, nchunk, chunk);
+    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) {
+        const int j = edges[e];
+        if (ftr_chunk(j, nchunk, chunk) == ci) {
+            const int slot = atomicAdd(&cursor[j], 1);
+            inedges[inoff[j] + slot] = i;
+        }
+    }
+}
+
+enum { kUnknown = 0, kRoot = 1, kClaimed = 2 };
+
+// One round of root resolution: a term is a root of its chunk's greedy pass when
+// no same-chunk root before it claims it.
+__global__ void k_ftr_round(int n, const int* __restrict__ inoff, const int* __restrict__ indeg,
+                            const int* __restrict__ inedges, const uint8_t* __restrict__ st_in,
+                            uint8_t* __restrict__ st_out, int* __restrict__ unresolved) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint8_t s = st_in[j];
+    if (s == kUnknown) {
+        bool any_root = false, all_claimed = true;
+        for (int e = inoff[j]; e < inoff[j] + indeg[j]; e++) {
+            const uint8_t si = st_in[inedges[e]];
+            if (si == kRoot) {
+                any_root = true;
+                break;
+            }
+            if (si != kClaimed) all_claimed = false;
+        }
+        s = any_root ? kClaimed : (all_claimed ? kRoot : kUnknown);
+        if (s == kUnknown) atomicAdd(unresolved, 1);
+    }
+    st_out[j] = s;
+}
+
+// F_c[j] = min root of chunk c claiming j (identity when unclaimed).
+__global__ void k_ftr_finit(int* __restrict__ F, int64_t total, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) F[i] = (int)(i % n);
+}
+
+__global__ void k_ftr_claim(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
+                            const int* __restrict__ edges, const uint8_t* __restrict__ st, int nchunk, int chunk,
+                            int* __restrict__ F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || st[i] != kRoot) return;
+    int* Fc = F + (int64_t)ftr_chunk(i, nchunk, chunk) * n;
+    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) atomicMin(&Fc[edges[e]], i);
+}
+
+// One level of the flag-array merge tree, term_reduction.hpp:592-649.  Every
+// value of a flag array is a fixed point of it, so the level is data parallel.
+__global__ void k_ftr_merge(int n, int* __restrict__ F, int level, int chunk, int nchunk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pairs = nchunk >> (level + 1);
+    const int k = blockIdx.y;
+    if (k >= pairs) return;
+    const int rt = k * (1 << (level + 1));
+    const int cm = rt + (1 << level);
+    const int start = cm * chunk;
+    const int idx = start + i;
+    if (idx >= n) return;
+    int* F1 = F + (int64_t)rt * n;
+    const int* F2 = F + (int64_t)cm * n;
+    const int f2 = F2[idx];
+    if (f2 != -1 && f2 != idx && F1[idx] == idx) F1[idx] = (F1[f2] == f2) ? f2 : F1[f2];
+}
+
+__global__ void k_ftr_rootflags(int n, const int* __restrict__ F, uint8_t* __restrict__ isroot,
+                                uint32_t* __restrict__ memkey, uint32_t* __restrict__ memval) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int f = F[j];
+    isroot[j] = (f == j);
+    memkey[j] = (f == j) ? 0xffffffffu : (uint32_t)f;
+    memval[j] = (uint32_t)j;
+}
+
+// Group table of one bucket: roots (ascending) and their member counts.
+__global__ void k_ftr_groups(int nroots, const uint32_t* __restrict__ roots, const uint32_t* __restrict__ perm,
+                             int base, int gbase, int mbase, int m, const int* __restrict__ memcnt_by_root,
+                             const int* __restrict__ memoff_by_root, int* __restrict__ grp_root,
+                             int* __restrict__ grp_moff, int* __restrict__ grp_mcnt, int* __restrict__ grp_m) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nroots) return;
+    const uint32_t r = roots[g];
+    grp_root[gbase + g] = (int)perm[base + r];
+    grp_moff[gbase + g] = mbase + memoff_by_root[r];
+    grp_mcnt[gbase + g] = memcnt_by_root[r];
+    grp_m[gbase + g] = m;
+}
+
+__global__ void k_ftr_memcount(int nm, const uint32_t* __restrict__ skey, int* __restrict__ cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    atomicAdd(&cnt[skey[k]], 1);
+}
+
+__global__ void k_ftr_memslots(int nm, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ perm, int base,
+                               int mbase, int* __restrict__ mem_slots) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    mem_slots[mbase + k] = (int)perm[base + sval[k]];
+}
+
+// ------------------------------------------------------------------ group B-tables
+// Upper bound of each group's B-table size.
+__global__ void k_group_ub(const TermPool OUT, int ng, const int* __restrict__ grp_root, const int* __restrict__ grp_m,
+                           int d, int* __restrict__ ub) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int s = grp_root[g];
+    const int m = grp_m[g];
+    int u = OUT.bt_n[s];
+    if ((OUT.flags[s] & kIsNew) && m <= d) u = max(u, 1 << (m - 1));
+    ub[g] = max(u, 1);
+}
+
+// B-table of a group root: new children pair the parent's post-update cells
+// across their defining hyperplane and, when coaligned, project through c_map
+// (make_new_child_btable, cell_enumeration.hpp:206-371); parents keep their own
+// post-update cells.  One wavefront per group, an LDS bitset of 2^m bits per
+// wavefront, sorted output.
+__global__ void k_group_btab(const TermPool OUT, int g0, int ngb, int m, int d, const int* __restrict__ grp_root,
+                             const uint32_t* __restrict__ Barena, const int64_t* __restrict__ b2_off,
+                             const int64_t* __restrict__ ub, uint32_t* __restrict__ B2, int* __restrict__ b2_n,
+                             int words_per_wave, int* __restrict__ err) {
+    extern __shared__ uint32_t sh[];
+    const int wv = threadIdx.x / kWave;
+    const int lane = wave_lane();
+    const int g = g0 + blockIdx.x * (blockDim.x / kWave) + wv;
+    if (g >= g0 + ngb) return;
+    uint32_t* bits = sh + wv * words_per_wave;
+    const int s = grp_root[g];
+    uint32_t* out = B2 + b2_off[g];
+    const bool is_new = OUT.flags[s] & kIsNew;
+    if (is_new && m <= d) {
+        const int nn = (int)min((int64_t)(1 << (m - 1)), ub[g]);
+        for (int k = lane; k < nn; k += kWave) out[k] = (uint32_t)k;
+        if (lane == 0) b2_n[g] = nn;
+        return;
+    }
+    const int W = m >= 5 ? (1 << (m - 5)) : 1;
+    for (int w = lane; w < W; w += kWave) bits[w] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t* Bt = Barena + OUT.bt_off[s];
+    const int nb = OUT.bt_n[s];
+    const uint32_t sg = (uint32_t)OUT.bt_sgn[s];
+    if (is_new) {
+        const int pbc = OUT.pbc[s], z = OUT.z[s];
+        const bool coal = m < pbc;
+        int bitpos[kMaxM];
+        int ncoal = 0;
+        if (coal) {
+            const uint8_t* cm = OUT.cmap + (int64_t)s * OUT.MM;
+            uint32_t seen = 0;
+            for (int j = 0; j < pbc; j++) {
+                const int ci = cm[j];
+                if (!((seen >> ci) & 1u)) {
+                    seen |= (1u << ci);
+                    bitpos[ncoal++] = j;
+                }
+            }
+        }
+        const uint32_t mask_z = 1u << z, top = 1u << (pbc - 1);
+        const uint32_t rev = (pbc >= 32) ? 0xffffffffu : ((1u << pbc) - 1u);
+        const uint32_t low = mask_z - 1u;
+        for (int j = lane; j < nb; j += kWave) {
+            const uint32_t bb = Bt[j] ^ sg;
+            uint32_t qv = bb ^ mask_z;
+            if (qv & top) qv ^= rev;
+            if (bsearch_u32(Bt, qv ^ sg, nb) < 0) continue;
+            const uint32_t zb = (bb & mask_z) >> z;
+            const uint32_t c1 = ((bb >> (z + 1)) << z) | (bb & low) | (zb << (pbc - 1));
+            const uint32_t c2 = c1 ^ top;
+            uint32_t cs[2] = {(c1 & top) ? (c1 ^ rev) : c1, (c2 & top) ? (c2 ^ rev) : c2};
+            for (int e = 0; e < 2; e++) {
+                uint32_t key = cs[e];
+                if (coal) {
+                    uint32_t bc = 0;
+                    for (int l = 0; l < ncoal; l++)
+                        if ((key >> bitpos[l]) & 1u) bc |= (1u << l);
+                    key = bc;
+                }
+                if (key >= (1u << m)) {
+                    atomicOr(err, 16);
+                    continue;
+                }
+                atomicOr(&bits[key >> 5], 1u << (key & 31));
+            }
+        }
+    } else {
+        for (int j = lane; j < nb; j += kWave) {
+            const uint32_t key = Bt[j] ^ sg;
+            if (key >= (1u << m)) {
+                atomicOr(err, 16);
+                continue;
+            }
+            atomicOr(&bits[key >> 5], 1u << (key & 31));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int per = (W + kWave - 1) / kWave;
+    const int w0 = lane * per, w1 = min(W, w0 + per);
+    int cnt = 0;
+    for (int w = w0; w < w1; w++) cnt += __popc(bits[w]);
+    int total;
+    int pos = wave_excl_scan(cnt, &total);
+    const int64_t cap = ub[g];
+    for (int w = w0; w < w1; w++) {
+        uint32_t x = bits[w];
+        while (x) {
+            const int bt = __ffs(x) - 1;
+            x &= x - 1;
+            if (pos < cap) out[pos] = (uint32_t)(w * 32 + bt);
+            pos++;
+        }
+    }
+    if (lane == 0) {
+        b2_n[g] = (int)min((int64_t)total, cap);
+        if (total > cap) atomicOr(err, 32);
+    }
+}
+
+// ------------------------------------------------------------------ group g-tables
+// g value of one term at one cell (make_gtable loop body, flattening.hpp:129-248).
+__device__ inline cplx dev_cell_g(const TermPool& T, int64_t s, uint32_t key, int m, const uint32_t* gk,
+                                  const cplx* gv, double gscale) {
+    const int MM = T.MM;
+    const double* q = T.q + s * MM;
+    const uint32_t hf = T.hflag[s];
+    const int phc = T.phc[s];
+    const int z = T.z[s];
+    const uint8_t fl = T.flags[s];
+    double ygi = 0;
+    for (int k = 0; k < m; k++) {
+        const double sb = ((key >> k) & 1u) ? -1 : 1;
+        if (!((hf >> k) & 1u)) ygi += q[k] * sb;
+    }
+    const int rev_phc = (phc >= 32) ? -1 : ((1 << phc) - 1);
+    const int top_phc = 1 << (phc - 1);
+    int lp = 0, lmv = 0;
+    if (!(fl & kIsNew)) {
+        lp = (int)key & rev_phc;
+        lmv = lp;
+    } else {
+        const bool hc = fl & kHasCmap;
+        const uint8_t* cm = T.cmap + s * MM;
+        const int8_t* cs = T.csmap + s * MM;
+        int k = 0, l = 0;
+        while (k < phc) {
+            if (k == z) {
+                lmv |= (1 << k);
+                k++;
+                if (k == phc) break;
+            }
+            int bv;
+            if (hc) bv = (((key >> cm[l]) & 1u) ? -1 : 1) * (int)cs[l];
+            else bv = ((key >> l) & 1u) ? -1 : 1;
+            if (bv < 0) {
+                lp |= (1 << k);
+                lmv |= (1 << k);
+            }
+            k++;
+            l++;
+        }
+    }
+    const int64_t go = T.gp_off[s];
+    const int gn = T.gp_n[s];
+    const int lhp = T.lhp[s];
+    const cplx a = lookup_num(lp ^ lhp, top_phc, rev_phc, gk + go, gv + go, gn);
+    const cplx bm = lookup_num(lmv ^ lhp, top_phc, rev_phc, gk + go, gv + go, gn);
+    const double c = T.c[s], dv = T.dv[s];
+    const cplx v = csub(cdiv(a, cmake(ygi + dv, c)), cdiv(bm, cmake(ygi - dv, c)));
+    return cscale(v, gscale);
+}
+
+__device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
+    const double* p = T.p + s * T.MM;
+    double ps = 0;
+    for (int k = 0; k < m; k++) ps += p[k];
+    return ps * ps;
+}
+
+// Flattening of one reduction group, make_gtables (flattening.hpp:351-564):
+// the root's g-table; if the root is negligible (term approximation), the
+// members in order re-orient the group B-table until one is not (root
+// replacement); then every later non-negligible member's g-table is added into
+// the root's in member order (add_gtables, flattening.hpp:259-315).
+// One wavefront per group; lanes own cells.
+__global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, const int* __restrict__ grp_root,
+                                                  const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
+                                                  const int* __restrict__ grp_m, const int* __restrict__ mem_slots,
+                                                  const uint32_t* __restrict__ B2, const int64_t* __restrict__ b2_off,
+                                                  const int* __restrict__ b2_n, const int64_t* __restrict__ g_off,
+                                                  const uint32_t* __restrict__ gk_prev, const cplx* __restrict__ gv_prev,
+                                                  uint32_t* __restrict__ gk_out, cplx* __restrict__ gv_out,
+                                                  cplx* __restrict__ scratch, double gscale, int* __restrict__ surv,
+                                                  int* __restrict__ gxor, int* __restrict__ sort_list,
+                                                  int* __restrict__ sort_count, int* __restrict__ rare) {
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int lane = threadIdx.x;
+    const int m = grp_m[g];
+    const int MM = OUT.MM;
+    const uint32_t* B = B2 + b2_off[g];
+    const int n = b2_n[g];
+    uint32_t* ok = gk_out + g_off[g];
+    cplx* ov = gv_out + g_off[g];
+    cplx* sc = scratch + g_off[g];
+    const int* mem = mem_slots + grp_moff[g];
+    const int nmem = grp_mcnt[g];
+    const uint32_t top = 1u << (m - 1);
+    const uint32_t rev = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    int r = grp_root[g];
+    uint32_t X = 0;
+    // root over its own B-table
+    {
+        const double ps = dev_psum2(OUT, r, m);
+        bool small = true;
+        for (int i = lane; i < n; i += kWave) {
+            const cplx v = dev_cell_g(OUT, r, B[i], m, gk_prev, gv_prev, gscale);
+            ov[i] = v;
+            if ((ps * hypot(v.re, v.im)) > kApproxEps) small = false;
+        }
+        const bool negl = wave_all(small);
+        int k = 0;
+        if (negl) {
+            // replacement chain (flattening.hpp:412-489)
+            int lfr = r;
+            r = -1;
+            while (k < nmem) {
+                const int sk = mem[k++];
+                if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 1);
+                int sig = orientation_mask(OUT.A + (int64_t)lfr * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
+                if ((uint32_t)sig & top) sig ^= (int)rev;
+                X ^= (uint32_t)sig;
+                const double psk = dev_psum2(OUT, sk, m);
+                bool sm = true;
+                for (int i = lane; i < n; i += kWave) {
+                    const cplx v = dev_cell_g(OUT, sk, B[i] ^ X, m, gk_prev, gv_prev, gscale);
+                    ov[i] = v;
+                    if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
+                }
+                if (!wave_all(sm)) {
+                    r = sk;
+                    break;
+                }
+                lfr = sk;
+            }
+            if (r < 0) {
+                if (lane == 0) surv[g] = -1;
+                return;
+            }
+        }
+        // later members are added into the root's table
+        const double* Ar = OUT.A + (int64_t)r * MM * d;
+        for (; k < nmem; k++) {
+            const int sk = mem[k];
+            if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
+            const int sig = orientation_mask(Ar, OUT.A + (int64_t)sk * MM * d, m, d);
+            const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
+            const double psk = dev_psum2(OUT, sk, m);
+            bool sm = true;
+            for (int i = lane; i < n; i += kWave) {
+                const cplx v = dev_cell_g(OUT, sk, B[i] ^ X ^ sh, m, gk_prev, gv_prev, gscale);
+                sc[i] = v;
+                if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
+            }
+            if (wave_all(sm)) continue;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __syncthreads();
+            for (int i = lane; i < n; i += kWave) {
+                const uint32_t bi = B[i] ^ X;
+                uint32_t x = bi ^ (uint32_t)sig;
+                bool cj = false;
+                if (x & top) {
+                    cj = true;
+                    x ^= rev;
+                }
+                cplx v;
+                if (x == (bi ^ sh)) {
+                    v = sc[i];
+                } else {
+                    const int ii = bsearch_u32(B, x ^ sh ^ X, n);
+                    if (ii < 0) continue;
+                    v = sc[ii];
+                }
+                const cplx cur = ov[i];
+                ov[i] = cadd(cur, cj ? cconj(v) : v);
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < n; i += kWave) ok[i] = B[i] ^ X;
+    if (lane == 0) {
+        surv[g] = r;
+        gxor[g] = (int)X;
+        if (X != 0) {
+            const int slot = atomicAdd(sort_count, 1);
+            sort_list[slot] = g;
+        }
+    }
+}
+
+// Restores ascending key order of a group table whose keys were re-oriented by
+// a root replacement.  One block per listed group; rank from an LDS bitset.
+__global__ void k_group_sort(int nlist, const int* __restrict__ list, const int* __restrict__ grp_m,
+                             const int* __restrict__ b2_n, const int64_t* __restrict__ g_off,
+                             uint32_t* __restrict__ gk, cplx* __restrict__ gv, uint32_t* __restrict__ tmpk,
+                             cplx* __restrict__ tmpv) {
+    extern __shared__ uint32_t bits[];
+    __shared__ int wbase[1024];
+    const int g = list[blockIdx.x];
+    const int m = grp_m[g];
+    const int n = b2_n[g];
+    const int W = m >= 5 ? (1 << (m - 5)) : 1;
+    uint32_t* k = gk + g_off[g];
+    cplx* v = gv + g_off[g];
+    uint32_t* tk = tmpk + g_off[g];
+    cplx* tv = tmpv + g_off[g];
+    for (int w = threadIdx.x; w < W; w += blockDim.x) bits[w] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        tk[i] = k[i];
+        tv[i] = v[i];
+        atomicOr(&bits[k[i] >> 5], 1u << (k[i] & 31));
+    }
+    __syncthreads();
+    // prefix popcounts over words, in chunks of blockDim
+    const int per = (W + blockDim.x - 1) / blockDim.x;
+    const int w0 = threadIdx.x * per, w1 = min(W, w0 + per);
+    int c = 0;
+    for (int w = w0; w < w1; w++) c += __popc(bits[w]);
+    wbase[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        int t = (threadIdx.x >= (unsigned)o) ? wbase[threadIdx.x - o] : 0;
+        __syncthreads();
+        wbase[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const int tb = wbase[threadIdx.x] - c;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t key = tk[i];
+        const int w = key >> 5;
+        const int owner = w / per;
+        int pos = (owner > 0 ? wbase[owner - 1] : 0);
+        for (int ww = owner * per; ww < w; ww++) pos += __popc(bits[ww]);
+        pos += __popc(bits[w] & ((1u << (key & 31)) - 1u));
+        k[pos] = key;
+        v[pos] = tv[i];
+    }
+    (void)tb;
+}
+
+// ------------------------------------------------------------------ compaction
+template <int D>
+__global__ void k_compact(const TermPool OUT, int ng, const int* __restrict__ surv, const int* __restrict__ pos,
+                          const int* __restrict__ grp_m, const int64_t* __restrict__ g_off,
+                          const int* __restrict__ b2_n, TermPool NEXT) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int r = surv[g];
+    if (r < 0) return;
+    const int o = pos[g];
+    const int m = grp_m[g];
+    const int MM = OUT.MM;
+    const double* A = OUT.A + (int64_t)r * MM * D;
+    const double* p = OUT.p + (int64_t)r * MM;
+    const double* b = OUT.b + (int64_t)r * D;
+    double* nA = NEXT.A + (int64_t)o * MM * D;
+    double* np_ = NEXT.p + (int64_t)o * MM;
+    double* nb = NEXT.b + (int64_t)o * D;
+    for (int k = 0; k < m * D; k++) nA[k] = A[k];
+    for (int k = 0; k < m; k++) np_[k] = p[k];
+    for (int k = 0; k < D; k++) nb[k] = b[k];
+    NEXT.m[o] = m;
+    NEXT.gp_off[o] = g_off[g];
+    NEXT.gp_n[o] = b2_n[g];
+}
+
+// g-tables of the first update's terms (make_gtable_first, flattening.hpp:14-67).
+template <int D>
+__global__ void k_gtable_first(const TermPool OUT, int n, const uint32_t* __restrict__ perm, double gscale,
+                               uint32_t* __restrict__ gk, cplx* __restrict__ gv, TermPool NEXT) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t s = perm[r];
+    const int cells = 1 << (D - 1);
+    const int MM = OUT.MM;
+    const double* p = OUT.p + s * MM;
+    const uint32_t hf = OUT.hflag[s];
+    const double c = OUT.c[s], dv = OUT.dv[s];
+    const int64_t off = (int64_t)r * cells;
+    for (int j = 0; j < cells; j++) {
+        double ygi = 0;
+        for (int k = 0; k < D; k++)
+            if (!((hf >> k) & 1u)) {
+                const double sb = ((j >> k) & 1) == 0 ? 1 : -1;
+                ygi += p[k] * sb;
+            }
+        cplx v = csub(cdiv(cmake(1.0, 0.0), cmake(ygi + dv, c)), cdiv(cmake(1.0, 0.0), cmake(ygi - dv, c)));
+        gk[off + j] = (uint32_t)j;
+        gv[off + j] = cscale(v, gscale);
+    }
+    const double* A = OUT.A + s * MM * D;
+    const double* b = OUT.b + s * D;
+    for (int k = 0; k < D * D; k++) NEXT.A[(int64_t)r * MM * D + k] = A[k];
+    for (int k = 0; k < D; k++) NEXT.p[(int64_t)r * MM + k] = p[k];
+    for (int k = 0; k < D; k++) NEXT.b[(int64_t)r * D + k] = b[k];
+    NEXT.m[r] = D;
+    NEXT.gp_off[r] = off;
+    NEXT.gp_n[r] = cells;
+}
+
+// Per-shape counts of the terms a last update leaves (no reduction follows).
+__global__ void k_skip_counts(const TermPool OUT, int n, int* __restrict__ hist) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (OUT.flags[s] & kValid) atomicAdd(&hist[OUT.m[s]], 1);
+}
+
+// ------------------------------------------------------------------ small helpers
+__global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (int64_t)src[i];
+}
+__global__ void k_surv_flags(int n, const int* __restrict__ surv, int* __restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = surv[i] >= 0 ? 1 : 0;
+}
+__global__ void k_surv_hist(int n, const int* __restrict__ surv, const int* __restrict__ grp_m, int* __restrict__ h) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && surv[i] >= 0) atomicAdd(&h[grp_m[i]], 1);
+}
+template <int D>
+__global__ void k_det_tp(TermPool P, int n, const double* __restrict__ Tr, const double* __restrict__ bias) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = P.m[i];
+    double* A = P.A + (int64_t)i * P.MM * D;
+    double* b = P.b + (int64_t)i * D;
+    double t[D];
+    for (int r = 0; r < m; r++) {
+        for (int k = 0; k < D; k++) t[k] = A[r * D + k];
+        for (int j = 0; j < D; j++) {
+            double s = 0.0;
+            for (int k = 0; k < D; k++) s += t[k] * Tr[k + j * D];
+            A[r * D + j] = s;
+        }
+    }
+    for (int k = 0; k < D; k++) t[k] = b[k];
+    for (int r = 0; r < D; r++) {
+        double s = 0.0;
+        for (int j = 0; j < D; j++) s += Tr[r * D + j] * t[j];
+        b[r] = s;
+    }
+    if (bias)
+        for (int r = 0; r < D; r++) b[r] += bias[r];
+}
+
+
+}  // namespace mce

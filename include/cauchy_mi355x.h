@@ -1,0 +1,166 @@
+/*
+ * cauchy_mi355x.h -- C ABI of the MI355X (gfx950) Multivariate Cauchy Estimator
+ * term-propagation engine.
+ *
+ * The pycauchy_* entry points keep the signatures, argument meaning, output
+ * ownership (outputs are malloc'd here and released by the caller with free())
+ * and error behaviour of the reference's SWIG boundary,
+ * scripts/swig/cauchy/pycauchy.hpp (declared for SWIG in pycauchy.i:80-423),
+ * so the reference's Python module, MATLAB mex files and C++ drivers can bind
+ * this library instead.  Every term of the estimator lives in GPU memory and
+ * every measurement update runs as HIP kernels; the host only sequences them.
+ *
+ * The mce_* entry points have no counterpart in the reference; they expose the
+ * per-update diagnostics (pre-reduction moments, term counts per shape) and the
+ * term tables that the parity tests compare against the reference.
+ */
+#ifndef CAUCHY_MI355X_H
+#define CAUCHY_MI355X_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as CauchyDynamicsUpdateContainer, include/dynamic_models.hpp:62-80,
+ * mirrored by ctypes in scripts/swig/cauchy/cauchy_estimator.py:40-58. */
+typedef struct CauchyDynamicsUpdateContainer {
+    double* x;
+    double* u;
+    double dt;
+    int step;
+    int n;
+    int cmcc;
+    int pncc;
+    int p;
+    double* Phi;
+    double* Gamma;
+    double* B;
+    double* H;
+    double* beta;
+    double* gamma;
+    bool is_xbar_set_for_ece;
+    void* other_stuff;
+} CauchyDynamicsUpdateContainer;
+
+/* Replaces pycauchy_initialize_lti, scripts/swig/cauchy/pycauchy.hpp:295-328.
+ * Sizes follow the reference: n = size_b0, pncc = size_beta, cmcc = size_B / n,
+ * p = size_gamma.  Matrices are row-major.  Returns NULL on failure (the
+ * reference exits; this library prints the reason and returns NULL). */
+void* pycauchy_initialize_lti(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                              int size_b0, double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B,
+                              int size_B, double* beta, int size_beta, double* H, int size_H, double* gamma,
+                              int size_gamma, double dt, int init_step, bool debug_print);
+
+/* Replaces pycauchy_initialize_ltv, scripts/swig/cauchy/pycauchy.hpp:330-370.  The
+ * callback is invoked on the calling thread before every time step, as in the reference. */
+void* pycauchy_initialize_ltv(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                              int size_b0, double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B,
+                              int size_B, double* beta, int size_beta, double* H, int size_H, double* gamma,
+                              int size_gamma, void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                              double dt, int init_step, bool debug_print);
+
+/* Replaces pycauchy_single_step_ltiv, scripts/swig/cauchy/pycauchy.hpp:461-543.
+ * Processes size_msmts scalar measurement updates (rows p-size_msmts .. p-1 of H).
+ * All 13 outputs are malloc'd by the library. */
+void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, double* controls, int size_controls,
+                               double** out_Phi, int* size_out_Phi, double** out_Gamma, int* size_out_Gamma,
+                               double** out_B, int* size_out_B, double** out_H, int* size_out_H, double** out_beta,
+                               int* size_out_beta, double** out_gamma, int* size_out_gamma, double** out_fz,
+                               int* size_out_fz, double** out_xhat, int* size_out_xhat, double** out_Phat,
+                               int* size_out_Phat, double** out_cerr_fz, int* size_out_cerr_fz,
+                               double** out_cerr_xhat, int* size_out_cerr_xhat, double** out_cerr_Phat,
+                               int* size_out_cerr_Phat, int** out_err_code, int* size_out_err_code);
+
+/* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
+void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                                int size_b0, double* xbar, int size_xbar);
+
+/* Replaces pycauchy_single_step_set_master_step, scripts/swig/cauchy/pycauchy.hpp:767-777. */
+void pycauchy_single_step_set_master_step(void* handle, int step);
+
+/* Replaces pycauchy_single_step_set_window_number, scripts/swig/cauchy/pycauchy.hpp:779-789. */
+void pycauchy_single_step_set_window_number(void* handle, int win_num);
+
+/* Replaces pycauchy_single_step_get_number_of_terms, scripts/swig/cauchy/pycauchy.hpp:666-671. */
+int pycauchy_single_step_get_number_of_terms(void* handle);
+
+/* Replaces pycauchy_single_step_deterministic_transform, scripts/swig/cauchy/pycauchy.hpp:654-664:
+ * A <- A Trans^T, b <- Trans b + bias for every term (cauchy_estimator.hpp:1312-1355). */
+void pycauchy_single_step_deterministic_transform(void* handle, double* Trans, int size_Trans, double* bias,
+                                                  int size_bias);
+
+/* Replaces pycauchy_single_step_get_duc, scripts/swig/cauchy/pycauchy.hpp:791-795. */
+void* pycauchy_single_step_get_duc(void* handle);
+
+/* Replaces pycauchy_single_step_shutdown, scripts/swig/cauchy/pycauchy.hpp:752-765. */
+void pycauchy_single_step_shutdown(void* handle);
+
+/* Replaces pycauchy_set_tr_search_idxs_ordering, scripts/swig/cauchy/pycauchy.hpp:994-1006.
+ * Returns 1 on error (more than 12 entries), 0 otherwise. */
+int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering);
+
+/* ---- the C++ CauchyEstimator interface (include/cauchy_estimator.hpp:47-1432) --------- */
+
+/* Replaces the CauchyEstimator constructor, include/cauchy_estimator.hpp:87-185: steps time
+ * steps of p scalar updates each; the root point and the cell enumerator's perturbations are
+ * drawn from glibc rand() in the reference's order.  Returns NULL on failure. */
+void* mce_estimator_create(double* A0, double* p0, double* b0, int steps, int d, int cmcc, int pncc, int p,
+                           bool print_basic_info);
+
+/* Replaces CauchyEstimator::step, include/cauchy_estimator.hpp:1211-1245: one scalar update
+ * with row H (d) and scale gamma; Phi (d x d), Gamma (d x pncc), beta (pncc), B (d x cmcc), u
+ * (cmcc) are used when the update starts a time step.  Returns numeric_moment_errors; -1 when
+ * the estimator refuses to step (window full or fz negative). */
+int mce_estimator_step(void* handle, double msmt, double* Phi, double* Gamma, double* beta, double* H,
+                       double gamma, double* B, double* u);
+
+/* Public members fz, conditional_mean, conditional_variance as (re, im) pairs, plus Nt. */
+int mce_estimator_moments(void* handle, double* fz, double* mean, double* cov);
+
+/* Replaces CauchyEstimator::reset, include/cauchy_estimator.hpp:1247-1300. */
+void mce_estimator_reset(void* handle);
+
+/* Releases an estimator made by mce_estimator_create (same as pycauchy_single_step_shutdown). */
+void mce_estimator_destroy(void* handle);
+
+/* ---- diagnostics (no reference counterpart) ---------------------------------------- */
+
+/* Diagnostics of the last scalar update, in the layout of oracle/ref_harness.cpp ref_step:
+ *   out_mom (6 + 2d + 2d^2): fz public (re,im), fz before reduction (re,im), last_fz (re,im),
+ *                            mean (re,im) x d, covariance (re,im) x d^2
+ *   out_cnt (4 + 2*shape_range): err code, terms after MU/MUC, terms after reduction,
+ *                            reference thread count, per-shape counts after MU/MUC, after reduction.
+ * Returns the error code, or -1 before any update. */
+int mce_last_update(void* handle, double* out_mom, int* out_cnt);
+
+/* Shape range (largest hyperplane count + 1) of the estimator. */
+int mce_shape_range(void* handle);
+
+/* Copies term `idx` of shape bucket `shape` (after an update that ran term reduction):
+ * info = [m, g-table size, m]; A (m x d), p (m), b (d), keys / g (re,im interleaved) of its g-table.
+ * Returns 0, or -1 when there is no such term. */
+int mce_term(void* handle, int shape, int idx, int* info, double* A, double* p, double* b, unsigned int* keys,
+             double* gvals);
+
+/* Replaces the construction-time random draws: root point (d) and the 8 x max_shape
+ * perturbation vectors of the cell enumerator. */
+void mce_set_random_draws(void* handle, const double* root_point, const double* b_pert_8xmax);
+void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax);
+
+/* Accumulated device time (ms, HIP events on the engine's stream) per phase since the last
+ * reset: [time-prop, B-table enumeration, measurement update, moments, gather, reduction,
+ * group B-tables, group g-tables, compaction, total] and the launch count of the group
+ * g-table kernel, plus its algorithmic bytes.  Timing is only recorded after mce_profile_enable(h, 1). */
+void mce_profile_enable(void* handle, int on);
+void mce_profile_read(void* handle, double* out_ms12);
+void mce_profile_reset(void* handle);
+
+/* Last error message ("" when none). */
+const char* mce_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAUCHY_MI355X_H */

@@ -305,7 +305,7 @@ struct Est {
     std::vector<cx> mean, cov, last_mean, last_cov;
     double gscale = 0;
     int threads_used = 1;
-    long long counters[3] = {0, 0, 0};
+    long long counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> skip_counts_cache;
 };
 
@@ -1281,10 +1281,15 @@ void ftr_and_gtables(Est& E) {
                 int np = cj->cells;
                 cj->B = std::make_shared<std::vector<int>>();
                 child_btable(*cj, d, *parentB, np);
+                for (int q = 0; q < cj->cells; q++)
+                    if ((*cj->B)[q] & (1 << (cj->m - 1))) { E.counters[5]++; break; }
             }
+            E.counters[6] += members[j].size();
+            if (members[j].size() > (size_t)E.counters[7]) E.counters[7] = members[j].size();
             if (make_gtable(*cj, E.gscale)) rt = -1;
             const std::vector<int>& mem = members[j];
             size_t k = 0;
+            if (rt == -1 && !mem.empty()) E.counters[3]++;
             if (rt == -1) {
                 E.counters[0]++;
                 int ncg = cj->cells;
@@ -1306,6 +1311,7 @@ void ftr_and_gtables(Est& E) {
                         }
                     }
                     if (!make_gtable(*ck, E.gscale)) {
+                        E.counters[4]++;
                         rt = cp;
                         cj = ck;
                         break;
@@ -1516,7 +1522,7 @@ int oracle_set_tr_search_idxs_ordering(const int* ordering, int n) {
 
 void oracle_path_counters(void* h, long long* out3) {
     Est& E = *(Est*)h;
-    for (int i = 0; i < 3; i++) out3[i] = E.counters[i];
+    for (int i = 0; i < 8; i++) out3[i] = E.counters[i];
 }
 
 }  // extern "C"

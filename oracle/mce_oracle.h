@@ -34,7 +34,9 @@ int oracle_term(void* h, int shape, int idx, int* info, double* A, double* p, do
 int oracle_set_tr_search_idxs_ordering(const int* ordering, int n);
 /* Overrides the construction-time random draws (root point, 8 x max_shape b_pert). */
 void oracle_set_random_draws(void* h, const double* root_point, const double* b_pert_8xmax);
-/* Counters of rare paths seen so far: [root replacements, old-term members, cell-count mismatches]. */
+/* Counters of rare flattening paths seen so far: [negligible roots, old-term members, cell-count
+   mismatches, negligible roots with members, replacement roots found, child B-tables holding a key
+   with its top bit set, total members, largest member list]. */
 void oracle_path_counters(void* h, long long* out3);
 
 #ifdef __cplusplus
