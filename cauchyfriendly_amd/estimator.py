@@ -119,15 +119,17 @@ class GpuEstimator:
 
 
 def run_system(sysobj, num_steps=None, seed=None, collect_terms=False):
-    """Runs one GPU estimator over a System's record (same contract as oracle.driver.run_system)."""
+    """Runs one GPU estimator over a System's measurement record (num_steps time steps)."""
     steps = sysobj.num_steps if num_steps is None else num_steps
     est = GpuEstimator(steps, sysobj.n, sysobj.pncc, sysobj.p, sysobj.A0, sysobj.p0, sysobj.b0,
-                       seed=sysobj.srand_seed if seed is None else seed)
+                       cmcc=sysobj.cmcc, seed=sysobj.srand_seed if seed is None else seed)
     recs = []
     Gt = np.ascontiguousarray(sysobj.Gamma.reshape(-1))
+    B = None if sysobj.B is None else np.ascontiguousarray(sysobj.B.reshape(-1))
     for k in range(steps):
         Phi = sysobj.Phi(k)
         for i in range(sysobj.p):
-            r = est.step(sysobj.zs[k, i], Phi.reshape(-1), Gt, sysobj.beta, sysobj.H[i], sysobj.gamma[i])
+            r = est.step(sysobj.zs[k, i], Phi.reshape(-1), Gt, sysobj.beta, sysobj.H[i], sysobj.gamma[i],
+                         B, sysobj.u(k))
             recs.append(r)
     return est, recs

@@ -36,14 +36,36 @@ class System:
     zs: np.ndarray              # (num_steps, p) measurements
     xs: np.ndarray              # (num_steps, d) true states
     srand_seed: int             # glibc srand seed used before constructing an estimator
-    B: Optional[np.ndarray] = None
+    B: Optional[np.ndarray] = None    # d x cmcc control matrix
+    us: Optional[np.ndarray] = None   # (num_steps, cmcc) controls
 
     @property
     def num_steps(self) -> int:
         return int(self.zs.shape[0])
 
+    @property
+    def cmcc(self) -> int:
+        return 0 if self.B is None else int(self.B.shape[1])
+
     def Phi(self, k: int) -> np.ndarray:
         return self.Phis[min(k, len(self.Phis) - 1)]
+
+    def u(self, k: int):
+        return None if self.us is None else self.us[k]
+
+    def to_dict(self) -> dict:
+        f = lambda a: None if a is None else np.asarray(a, dtype=np.float64).tolist()
+        return {"name": self.name, "n": self.n, "pncc": self.pncc, "p": self.p, "A0": f(self.A0), "p0": f(self.p0),
+                "b0": f(self.b0), "Phis": [f(P) for P in self.Phis], "Gamma": f(self.Gamma), "beta": f(self.beta),
+                "H": f(self.H), "gamma": f(self.gamma), "zs": f(self.zs), "xs": f(self.xs),
+                "srand_seed": self.srand_seed, "B": f(self.B), "us": f(self.us)}
+
+    @staticmethod
+    def from_dict(o: dict) -> "System":
+        a = lambda v: None if v is None else np.asarray(v, dtype=np.float64)
+        return System(o["name"], int(o["n"]), int(o["pncc"]), int(o["p"]), a(o["A0"]), a(o["p0"]), a(o["b0"]),
+                      [a(P) for P in o["Phis"]], a(o["Gamma"]), a(o["beta"]), a(o["H"]), a(o["gamma"]), a(o["zs"]),
+                      a(o["xs"]), int(o["srand_seed"]), a(o.get("B")), a(o.get("us")))
 
 
 def _orthogonal(rng: np.random.Generator, n: int) -> np.ndarray:
@@ -142,4 +164,75 @@ def batch_systems(count: int = 512, seed_offset: int = 2, outlier_prob: float = 
                        outlier_prob=outlier_prob, outlier_scale=outlier_scale,
                        name=f"batch{j}_d{d}_w{w}")
         out.append(s)
+    return out
+
+
+def explicit_system(name, Phi, Gamma, H, beta, gamma, A0, p0, b0, zs, srand_seed=1, B=None, us=None):
+    """A system given by explicit matrices and a measurement record (zs: one row per time step)."""
+    n = len(b0)
+    Gamma = np.asarray(Gamma, dtype=np.float64).reshape(n, -1)
+    H = np.asarray(H, dtype=np.float64).reshape(-1, n)
+    zs = np.asarray(zs, dtype=np.float64).reshape(-1, H.shape[0])
+    return System(name, n, Gamma.shape[1], H.shape[0], np.asarray(A0, dtype=np.float64).reshape(n, n),
+                  np.asarray(p0, dtype=np.float64), np.asarray(b0, dtype=np.float64),
+                  [np.asarray(Phi, dtype=np.float64).reshape(n, n)], Gamma, np.asarray(beta, dtype=np.float64),
+                  H, np.asarray(gamma, dtype=np.float64), zs, np.zeros((zs.shape[0], n)), srand_seed,
+                  None if B is None else np.asarray(B, dtype=np.float64).reshape(n, -1),
+                  None if us is None else np.asarray(us, dtype=np.float64).reshape(zs.shape[0], -1))
+
+
+def edge_systems() -> List[System]:
+    """Systems that drive the estimator's special paths."""
+    rng = np.random.default_rng(BASE_SEED + 7)
+    out = []
+    # H orthogonal to Gamma and to two initial hyperplanes: non-integrable rows at every update
+    Phi = 0.9 * _orthogonal(rng, 3)
+    zs = 0.3 * rng.standard_cauchy((7, 1))
+    out.append(explicit_system("horthog3", Phi, [0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.1], [0.2], np.eye(3),
+                               [0.1, 0.08, 0.05], np.zeros(3), zs, srand_seed=11))
+    # Gamma parallel to an initial hyperplane: time-propagation coalignment on the first propagation
+    zs = 0.3 * rng.standard_cauchy((7, 1))
+    out.append(explicit_system("tpcoalign3", np.eye(3), [1.0, 0.0, 0.0], [0.7, -0.4, 0.3], [0.1], [0.2],
+                               np.eye(3), [0.1, 0.08, 0.05], np.zeros(3), zs, srand_seed=12))
+    # controls B u
+    Phi = 0.95 * _orthogonal(rng, 3)
+    zs = 0.3 * rng.standard_cauchy((6, 1))
+    us = rng.standard_normal((6, 1))
+    out.append(explicit_system("controls3", Phi, rng.standard_normal(3), rng.standard_normal(3), [0.1], [0.2],
+                               np.eye(3), [0.1, 0.1, 0.1], np.zeros(3), zs, srand_seed=13,
+                               B=[[0.5], [-0.2], [0.1]], us=us))
+    # 50x impulsive measurement outliers
+    s = lti_system(d=3, steps=7, seed_offset=40, outlier_prob=0.3, outlier_scale=50.0, name="outliers3")
+    out.append(s)
+    # scalar state (every hyperplane coaligns)
+    zs = 0.3 * rng.standard_cauchy((10, 1))
+    out.append(explicit_system("scalar1", [0.9], [0.4], [2.0], [0.1], [0.2], [1.0], [0.1], [0.0], zs, srand_seed=14))
+    # 7 states, two measurements per step
+    s7 = ltv_system(d=7, steps=4, seed_offset=41, pncc=1, p=2, beta=1e-3, gamma=5e-2, name="ltv7_p2")
+    out.append(s7)
+    return out
+
+
+def reference_demo_systems() -> List[System]:
+    """Input vectors held by the reference's demo drivers (src/cauchy_estimator.cpp:53-76, 93-119,
+    160-196); the reference constructs these estimators without calling srand (glibc seed 1)."""
+    out = []
+    out.append(explicit_system(
+        "ref_2state", [0.9, 0.1, -0.2, 1.1], [1, 0.3], [1.0, 1.0], [0.1], [0.2], np.eye(2), [0.10, 0.05],
+        np.zeros(2), [0.0338, 0.2049, -2.3543, -0.6042, -0.2662, 0.1307, -0.2250, 0.1951, -0.2191, 0.0996],
+        srand_seed=1))
+    out.append(explicit_system(
+        "ref_3state", [1.4, -0.6, -1.0, -0.2, 1.0, 0.5, 0.6, -0.6, -0.2], [.1, 0.3, -0.2], [1.0, 0.5, 0.2],
+        [0.1], [0.2], np.eye(3), [0.10, 0.08, 0.05], np.zeros(3),
+        [-1.2172011200334241, -0.35943271347277583, -0.52353301003957098, 0.5855389648301792,
+         -0.8048243525901404, 0.34053610027255954, 1.0580483915838776, -0.55152999529515989,
+         -0.72879029737003309, -0.82415138330170357, 0.0], srand_seed=1))
+    out.append(explicit_system(
+        "ref_3state_3msmts_horth", [1.4, -0.6, -1.0, -0.2, 1.0, 0.5, 0.6, -0.6, -0.2], [.1, 0.3, -0.2],
+        [1, 0, 0, 0, 1, 0, 0, 0, 1], [0.1], [0.2, 0.15, 0.10], np.eye(3), [0.1, 0.08, 0.05], np.zeros(3),
+        [0.10943250903225685, 0.32131358116921616, -0.39352816664526724,
+         0.76258687662854907, -0.25344840215960657, 0.1578820974338809,
+         0.52543601367678883, -0.67309502187832315, -0.37267005411252474,
+         2.7335350536863903, -0.3754139600950176, 0.6986657326616188,
+         0.52558307773279223, 0.82802377147093642, 0.98211422248186553], srand_seed=1))
     return out

@@ -80,6 +80,8 @@ class RefEstimator:
     pre-FTR moments and the term counts after MU/MUC and after FTR."""
 
     def __init__(self, num_steps, d, pncc, p, A0, p0, b0, cmcc=0, seed=None, kind="ref"):
+        if kind == "ref" and not available("ref"):
+            raise RuntimeError("reference build oracle/_ref/libcauchy_ref.so is not present")
         self.kind = kind
         L = lib(kind)
         self.d, self.pncc, self.p, self.cmcc = d, pncc, p, cmcc
@@ -182,13 +184,15 @@ def run_system(sysobj, num_steps=None, seed=None, collect_terms=False, kind="ref
     """Runs one estimator over a System's measurement record (num_steps time steps)."""
     steps = sysobj.num_steps if num_steps is None else num_steps
     est = RefEstimator(steps, sysobj.n, sysobj.pncc, sysobj.p, sysobj.A0, sysobj.p0, sysobj.b0,
-                       seed=sysobj.srand_seed if seed is None else seed, kind=kind)
+                       cmcc=sysobj.cmcc, seed=sysobj.srand_seed if seed is None else seed, kind=kind)
     recs = []
     Gt = np.ascontiguousarray(sysobj.Gamma.reshape(-1))
+    B = None if sysobj.B is None else np.ascontiguousarray(sysobj.B.reshape(-1))
     for k in range(steps):
         Phi = sysobj.Phi(k)
         for i in range(sysobj.p):
-            r = est.step(sysobj.zs[k, i], Phi.reshape(-1), Gt, sysobj.beta, sysobj.H[i], sysobj.gamma[i])
+            r = est.step(sysobj.zs[k, i], Phi.reshape(-1), Gt, sysobj.beta, sysobj.H[i], sysobj.gamma[i],
+                         B, sysobj.u(k))
             if collect_terms and r["nt_ftr"] > 0 and not (k == steps - 1 and i == sysobj.p - 1):
                 r["terms"] = est.terms()
             recs.append(r)
