@@ -187,6 +187,8 @@ struct Profiler {
     double total = 0;
     long long geval_launches = 0;
     double geval_bytes = 0;
+    double geval_ms = 0;
+    hipEvent_t k0 = nullptr, k1 = nullptr;
     std::vector<std::pair<int, hipEvent_t>> marks;
     hipEvent_t start_ev = nullptr;
     void begin(hipStream_t s) {
@@ -240,6 +242,13 @@ struct Engine {
     int threads_used = 1;
 
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;  // moment chains, overlapped with the reduction phase
+    hipEvent_t ev_mu = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
+    DBuf<uint32_t> chain_key, chain_skey, chain_perm;
+    DBuf<int> chain_off;
+    int cur_T = 1;
+    bool moments_pending = false;
+    int max_gp_cells = 1;  // largest parent g-table of the current parents
     PoolStore in, out;
     DBuf<uint32_t> gk[2];
     DBuf<cplx> gv[2];
@@ -267,6 +276,7 @@ struct Engine {
     DBuf<int> grp_root, grp_moff, grp_mcnt, grp_m, mem_slots, b2n, surv, gxor, sort_list, spos, sflag;
     DBuf<int64_t> ub64, b2off, goff, b2n64;
     DBuf<uint8_t> cub_tmp;
+    DBuf<unsigned long long> bytes_acc;
     Profiler prof;
 
     Engine(int num_steps, int d_, int cmcc_, int pncc_, int p_, const double* A0_, const double* p0_,
@@ -295,6 +305,10 @@ struct Engine {
         fz = cmake(0, 0);
         last_fz = cmake(0, 0);
         HIPCHK(hipStreamCreate(&st));
+        HIPCHK(hipStreamCreate(&st2));
+        HIPCHK(hipEventCreateWithFlags(&ev_mu, hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&ev_c0));
+        HIPCHK(hipEventCreate(&ev_c1));
         in.MM = out.MM = MM;
         in.d = out.d = d;
         // combinations in lexicographic order per shape (cell_enumeration.hpp:71-92)
@@ -337,6 +351,10 @@ struct Engine {
 
     ~Engine() {
         if (st) (void)hipStreamDestroy(st);
+        if (st2) (void)hipStreamDestroy(st2);
+        if (ev_mu) (void)hipEventDestroy(ev_mu);
+        if (ev_c0) (void)hipEventDestroy(ev_c0);
+        if (ev_c1) (void)hipEventDestroy(ev_c1);
     }
 
     void upload_draws() {
@@ -528,6 +546,56 @@ struct Engine {
         cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
     }
 
+    // Joins the moment chains, adds the per-thread chunks in thread order
+    // (cauchy_estimator.hpp:886-895) and finalises the moments.
+    void collect_moments(bool first) {
+        if (!moments_pending) return;
+        const int NC = 1 + d + d * d;
+        const int T = cur_T;
+        std::vector<double> part((size_t)T * NC * 2);
+        HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st2));
+        int mu_err = 0;
+        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st2));
+        HIPCHK(hipStreamSynchronize(st));
+        moments_pending = false;
+        if (prof.on) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev_c0, ev_c1));
+            prof.ms[kPhMom] += t;
+        }
+        if (mu_err) throw std::runtime_error("two H-orthogonal hyperplanes coalign negatively (reference exits here)");
+        std::vector<double> mo(NC * 2, 0.0);
+        for (int t = 0; t < T; t++)
+            for (int c = 0; c < NC; c++) {
+                mo[2 * c] += part[((size_t)t * NC + c) * 2];
+                mo[2 * c + 1] += part[((size_t)t * NC + c) * 2 + 1];
+            }
+        fz = cmake(mo[0], mo[1]);
+        for (int j = 0; j < d; j++) mean[j] = cmake(mo[2 + 2 * j], mo[3 + 2 * j]);
+        for (int j = 0; j < d * d; j++) cov[j] = cmake(mo[2 + 2 * d + 2 * j], mo[3 + 2 * d + 2 * j]);
+        if (first) {
+            // compute_moments(true), cauchy_estimator.hpp:524-602 (no numeric checks at step 1)
+            gscale = kInvTwoPi / fz.re;
+            const cplx ifz = cmake(0, fz.re);
+            for (int i = 0; i < d; i++) mean[i] = cdiv(mean[i], ifz);
+            for (int i = 0; i < d; i++)
+                for (int j = 0; j < d; j++) cov[i * d + j] = csub(cdiv(cov[i * d + j], fz), cmul(mean[i], mean[j]));
+        } else {
+            finalize_and_check();
+        }
+        diag_mom[2] = fz.re;
+        diag_mom[3] = fz.im;
+        for (int i = 0; i < d; i++) {
+            diag_mom[6 + 2 * i] = mean[i].re;
+            diag_mom[7 + 2 * i] = mean[i].im;
+        }
+        for (int i = 0; i < d * d; i++) {
+            diag_mom[6 + 2 * d + 2 * i] = cov[i].re;
+            diag_mom[7 + 2 * d + 2 * i] = cov[i].im;
+        }
+    }
+
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
@@ -623,63 +691,45 @@ struct Engine {
         out.reserve(nslots);
         gy.reserve((size_t)nslots * (2 + 2 * d));
         keys.reserve(nslots);
+        chain_key.reserve(nslots);
+        chain_skey.reserve(nslots);
+        chain_perm.reserve(nslots);
+        iota.reserve(nslots);
+        chain_off.reserve(16);
+        mom_part.reserve((size_t)T * (1 + d + d * d) * 2);
         TermPool OUT = out.view();
         HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                              slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr,
-                                             keys.ptr, counters.ptr + 1));
+                                             keys.ptr, chain_key.ptr, counters.ptr + 1));
         HIPCHK(hipGetLastError());
         prof.mark(kPhMu, st);
 
-        // ---- moments, summed in the reference's order
-        const int NC = 1 + d + d * d;
-        mom_part.reserve((size_t)T * NC * 2);
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_chain<DD>, dim3(T), dim3(128), 0, st, gy.ptr, out.flags.ptr,
-                                             slot_base.ptr, din_off.ptr, sr, T, mom_part.ptr));
+        // ---- moments: slots in reference-thread chain order, then the chains on st2
+        hipLaunchKernelGGL(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+        cub([&](void* t_, size_t& b_) {
+            return hipcub::DeviceRadixSort::SortPairs(t_, b_, chain_key.ptr, chain_skey.ptr, iota.ptr, chain_perm.ptr,
+                                                      nslots, 0, 4, st);
+        });
+        hipLaunchKernelGGL(k_chain_off, dim3(1), dim3(16), 0, st, chain_skey.ptr, nslots, T, chain_off.ptr);
+        HIPCHK(hipEventRecord(ev_mu, st));
+        HIPCHK(hipStreamWaitEvent(st2, ev_mu, 0));
+        if (prof.on) HIPCHK(hipEventRecord(ev_c0, st2));
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_chain<DD>, dim3(T, ceil_div(1 + d + d * d, kChainComps)), dim3(256),
+                                             0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr,
+                                             mom_part.ptr));
         HIPCHK(hipGetLastError());
-        std::vector<double> part((size_t)T * NC * 2);
-        HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        int mu_err = 0;
-        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        // per-thread chunks added in thread order (cauchy_estimator.hpp:886-895)
-        std::vector<double> mo(NC * 2, 0.0);
-        for (int t = 0; t < T; t++)
-            for (int c = 0; c < NC; c++) {
-                mo[2 * c] += part[((size_t)t * NC + c) * 2];
-                mo[2 * c + 1] += part[((size_t)t * NC + c) * 2 + 1];
-            }
-        prof.mark(kPhMom, st);
-        if (mu_err) throw std::runtime_error("two H-orthogonal hyperplanes coalign negatively (reference exits here)");
-        fz = cmake(mo[0], mo[1]);
-        for (int j = 0; j < d; j++) mean[j] = cmake(mo[2 + 2 * j], mo[3 + 2 * j]);
-        for (int j = 0; j < d * d; j++) cov[j] = cmake(mo[2 + 2 * d + 2 * j], mo[3 + 2 * d + 2 * j]);
-        if (first) {
-            // compute_moments(true), cauchy_estimator.hpp:524-602 (no numeric checks at step 1)
-            gscale = kInvTwoPi / fz.re;
-            const cplx ifz = cmake(0, fz.re);
-            for (int i = 0; i < d; i++) mean[i] = cdiv(mean[i], ifz);
-            for (int i = 0; i < d; i++)
-                for (int j = 0; j < d; j++) cov[i * d + j] = csub(cdiv(cov[i * d + j], fz), cmul(mean[i], mean[j]));
-        } else {
-            finalize_and_check();
-        }
+        if (prof.on) HIPCHK(hipEventRecord(ev_c1, st2));
+        cur_T = T;
+        moments_pending = true;
         diag_mom.assign(6 + 2 * d + 2 * d * d, 0.0);
         diag_cnt.assign(4 + 2 * sr, 0);
-        diag_mom[2] = fz.re;
-        diag_mom[3] = fz.im;
-        for (int i = 0; i < d; i++) {
-            diag_mom[6 + 2 * i] = mean[i].re;
-            diag_mom[7 + 2 * i] = mean[i].im;
-        }
-        for (int i = 0; i < d * d; i++) {
-            diag_mom[6 + 2 * d + 2 * i] = cov[i].re;
-            diag_mom[7 + 2 * d + 2 * i] = cov[i].im;
-        }
+        if (first || skip) collect_moments(first);
 
         if (skip) {
             HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
-            hipLaunchKernelGGL(k_skip_counts, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, OUT, nslots, hist.ptr);
+            hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+                               out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> h(sr + 1);
             HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -701,7 +751,8 @@ struct Engine {
                                                           32, st);
             });
             HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
-            hipLaunchKernelGGL(k_shape_hist, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, OUT, nslots, hist.ptr);
+            hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+                               out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> cnt(sr + 1);
             HIPCHK(hipMemcpyAsync(cnt.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -765,6 +816,7 @@ struct Engine {
         in_count.assign(sr, 0);
         in_count[d] = n;
         Nt = n;
+        max_gp_cells = cells;
     }
 
     // Fast term reduction and g-tables for every shape bucket
@@ -823,7 +875,8 @@ struct Engine {
         const int64_t cells_total = read1(goff.ptr + ng - 1) + read1(b2n64.ptr + ng - 1);
         prof.mark(kPhGBtab, st);
 
-        // ---- group g-tables
+        // ---- group g-tables need the update's g scale 1/(2 pi Re fz)
+        collect_moments(false);
         const int nx = gcur ^ 1;
         gk[nx].reserve((size_t)cells_total + 1);
         gv[nx].reserve((size_t)cells_total + 1);
@@ -832,12 +885,43 @@ struct Engine {
         gxor.reserve(ng);
         sort_list.reserve(ng);
         HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
-        hipLaunchKernelGGL(k_group_eval, dim3(ng), dim3(64), 0, st, OUT, ng, d, grp_root.ptr, grp_moff.ptr,
-                           grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr, b2n.ptr, goff.ptr,
-                           gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr, gscale, surv.ptr, gxor.ptr,
-                           sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+        if (prof.on) {
+            if (!prof.k0) {
+                HIPCHK(hipEventCreate(&prof.k0));
+                HIPCHK(hipEventCreate(&prof.k1));
+            }
+            HIPCHK(hipEventRecord(prof.k0, st));
+        }
+        {
+            const size_t lds = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
+            if (lds <= 48 * 1024) {
+                HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_group_eval<true>, dim3(ng), dim3(64), lds, st, OUT, ng, d, grp_root.ptr,
+                                   grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,
+                                   b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,
+                                   gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+            } else {
+                hipLaunchKernelGGL(k_group_eval<false>, dim3(ng), dim3(64), 0, st, OUT, ng, d, grp_root.ptr,
+                                   grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,
+                                   b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,
+                                   gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+            }
+        }
         HIPCHK(hipGetLastError());
-        prof.geval_launches++;
+        if (prof.on) {
+            HIPCHK(hipEventRecord(prof.k1, st));
+            bytes_acc.reserve(1);
+            HIPCHK(hipMemsetAsync(bytes_acc.ptr, 0, sizeof(unsigned long long), st));
+            hipLaunchKernelGGL(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
+                               grp_mcnt.ptr, b2n.ptr, bytes_acc.ptr);
+            const unsigned long long bytes = read1(bytes_acc.ptr);
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, prof.k0, prof.k1));
+            prof.geval_ms += t;
+            prof.geval_bytes += (double)bytes;
+            prof.geval_launches++;
+        }
         const int nsort = read1(counters.ptr + 2);
         const int rare = read1(counters.ptr + 3);
         if (rare) fprintf(stderr, "mce: a parent term joined a reduction group (flag %d); treated as a new child\n", rare);
@@ -866,10 +950,17 @@ struct Engine {
                                              surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
-        hipLaunchKernelGGL(k_surv_hist, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, grp_m.ptr, hist.ptr);
+        hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
+                           (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist.ptr);
         std::vector<int> h(sr + 1);
         HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        {
+            DBuf<int>& mx = memcnt;  // scalar scratch
+            mx.reserve(1);
+            cub([&](void* t_, size_t& b_) { return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, mx.ptr, ng, st); });
+            max_gp_cells = read1(mx.ptr);
+        }
         prof.mark(kPhCompact, st);
         in_count.assign(sr, 0);
         Nt = 0;
@@ -1361,6 +1452,7 @@ void mce_profile_read(void* handle, double* out) {
     out[kNumPhases] = p.total;
     out[kNumPhases + 1] = (double)p.geval_launches;
     out[kNumPhases + 2] = p.geval_bytes;
+    out[kNumPhases + 3] = p.geval_ms;
 }
 
 void mce_profile_reset(void* handle) {
@@ -1369,6 +1461,7 @@ void mce_profile_reset(void* handle) {
     p.total = 0;
     p.geval_launches = 0;
     p.geval_bytes = 0;
+    p.geval_ms = 0;
 }
 
 const char* mce_last_error(void) { return g_last_error.c_str(); }

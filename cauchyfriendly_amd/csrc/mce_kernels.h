@@ -555,7 +555,7 @@ template <int D>
 __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT, int nslots,
                      const StepConst* __restrict__ sc, const uint32_t* __restrict__ gk_prev,
                      const cplx* __restrict__ gv_prev, double* __restrict__ gy, uint32_t* __restrict__ keys_out,
-                     int* __restrict__ err) {
+                     uint32_t* __restrict__ chain_key, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int lo = 0, hi = np - 1;
@@ -565,9 +565,11 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
         else hi = mid - 1;
     }
     const int P = lo;
-    const int t = s - slot_base[P];
     const int MM = IN.MM;
     const int m = IN.m[P];
+    // slot 0 of a parent holds the parent itself (child m), slots 1..m its children 0..m-1
+    const int ts = s - slot_base[P];
+    const int t = (ts == 0) ? m : ts - 1;
     const double* A = IN.A + (int64_t)P * MM * D;
     const double* p = IN.p + (int64_t)P * MM;
     const double* b = IN.b + (int64_t)P * D;
@@ -584,9 +586,11 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
     if (t < m && !((integ >> t) & 1u)) {
         OUT.flags[s] = 0;
         keys_out[s] = 0xffffffffu;
+        chain_key[s] = 15u;
         for (int k = 0; k < 2 + 2 * D; k++) g_out[k] = 0.0;
         return;
     }
+    chain_key[s] = (uint32_t)IN.tid[P];
     const double zeta = sc->msmt - dev_dot<D>(H, b);
     double mut[D];
     double rho_t;
@@ -735,54 +739,120 @@ __global__ void k_mom_final(const double* __restrict__ part, int nblk, double* _
     }
 }
 
-// Moment sums in the reference's exact order: one chain per reference thread t
-// (parents i = t, t+T, ... of every shape bucket, cauchy_estimator.hpp:1575-1604),
-// each parent's own term before its children (cache_moments, cauchy_estimator.hpp:307-338
-// and 1436-1468), so the partial sums, and hence fz, x_hat, P_hat and the g-table
-// scale 1/(2 pi Re fz), are bit-identical to the reference.  One block per chain,
-// one lane per moment component.
+// Moment sums in the reference's exact order.  Reference thread t owns parents
+// i = t, t+T, ... of every shape bucket (cauchy_estimator.hpp:1575-1604) and adds,
+// per parent, the parent's own term before its children (cache_moments,
+// cauchy_estimator.hpp:307-338 and 1436-1468).  `chain` lists the output slots in
+// exactly that order (slots are parent-first), so chain t is one contiguous range.
+// One block per chain: tiles of 256 slot records are staged in LDS by all threads,
+// then lane c adds component c of every record in order, so the partial sums --
+// and hence fz, x_hat, P_hat and the g-table scale 1/(2 pi Re fz) -- are
+// bit-identical to the reference.
+constexpr int kChainComps = 16;  // moment components per chain block
+
 template <int D>
-__global__ __launch_bounds__(128) void k_mom_chain(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
-                                                   const int* __restrict__ slot_base, const int* __restrict__ in_off,
-                                                   int sr, int T, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_mom_chain(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
+                                                   const uint32_t* __restrict__ chain,
+                                                   const int* __restrict__ chain_off, double* __restrict__ part) {
     constexpr int NS = 2 + 2 * D;
     constexpr int NC = 1 + D + D * D;
+    constexpr int TILE = 256;
+    constexpr int PER = (TILE * NS + 255) / 256;  // record doubles per thread per tile
+    __shared__ double st[TILE * NS];
+    __shared__ double xr[kChainComps * TILE];
+    __shared__ double xi[kChainComps * TILE];
+    __shared__ uint8_t val[TILE];
     const int t = blockIdx.x;
-    const int c = threadIdx.x;
-    if (c >= NC) return;
-    int kind, j = 0, k = 0;
-    if (c == 0) kind = 0;
-    else if (c <= D) {
-        kind = 1;
-        j = c - 1;
-    } else {
-        kind = 2;
-        j = (c - 1 - D) / D;
-        k = (c - 1 - D) % D;
-    }
-    cplx acc = cmake(0, 0);
-    for (int m = 0; m < sr; m++) {
-        for (int i = in_off[m] + t; i < in_off[m + 1]; i += T) {
-            const int s0 = slot_base[i];
-            const int mp = slot_base[i + 1] - s0 - 1;
-            for (int q = -1; q < mp; q++) {
-                const int s = (q < 0) ? s0 + mp : s0 + q;
-                if (q >= 0 && !(flags[s] & kValid)) continue;
-                const double* e = gy + (int64_t)s * NS;
-                const cplx g = cmake(e[0], e[1]);
-                if (kind == 0) {
-                    acc = cadd(acc, g);
-                } else if (kind == 1) {
-                    acc = cadd(acc, cmul(g, cmake(e[2 + 2 * j], e[3 + 2 * j])));
+    const int c0 = blockIdx.y * kChainComps;
+    const int beg = chain_off[t], end = chain_off[t + 1];
+    const int tx = threadIdx.x;
+    double acc_re = 0.0, acc_im = 0.0;
+    // register prefetch of one tile
+    double pre[PER];
+    uint8_t pre_v = 0;
+    auto fetch = [&](int base) {
+        const int n = min(TILE, end - base);
+        for (int q = 0; q < PER; q++) {
+            const int e = tx + q * 256;
+            const int r = e / NS, w = e - r * NS;
+            pre[q] = (r < n) ? gy[(int64_t)chain[base + r] * NS + w] : 0.0;
+        }
+        pre_v = (tx < n) ? (flags[chain[base + tx]] & kValid) : 0;
+    };
+    if (beg < end) fetch(beg);
+    for (int base = beg; base < end; base += TILE) {
+        const int n = min(TILE, end - base);
+        for (int q = 0; q < PER; q++) {
+            const int e = tx + q * 256;
+            if (e < TILE * NS) st[e] = pre[q];
+        }
+        val[tx] = pre_v;
+        __syncthreads();
+        if (base + TILE < end) fetch(base + TILE);  // in flight during this tile's work
+        // products with the reference's operations: g, g*y_j and the negated (g*y_j)*y_k
+        // (the reference subtracts it; a - b == a + (-b) exactly)
+        if (tx < n) {
+            const double* rec = st + tx * NS;
+            const cplx g = cmake(rec[0], rec[1]);
+            for (int q = 0; q < kChainComps; q++) {
+                const int cc = c0 + q;
+                if (cc >= NC) break;
+                cplx x;
+                if (cc == 0) {
+                    x = g;
+                } else if (cc <= D) {
+                    x = cmul(g, cmake(rec[2 + 2 * (cc - 1)], rec[3 + 2 * (cc - 1)]));
                 } else {
-                    const cplx gy_j = cmul(g, cmake(e[2 + 2 * j], e[3 + 2 * j]));
-                    acc = csub(acc, cmul(gy_j, cmake(e[2 + 2 * k], e[3 + 2 * k])));
+                    const int j = (cc - 1 - D) / D, kk = (cc - 1 - D) % D;
+                    const cplx gyj = cmul(g, cmake(rec[2 + 2 * j], rec[3 + 2 * j]));
+                    const cplx p = cmul(gyj, cmake(rec[2 + 2 * kk], rec[3 + 2 * kk]));
+                    x = cmake(-p.re, -p.im);
                 }
+                xr[q * TILE + tx] = x.re;
+                xi[q * TILE + tx] = x.im;
             }
         }
+        __syncthreads();
+        // the in-order add chain, one lane per component
+        if (tx < kChainComps && c0 + tx < NC) {
+            const double* pr = xr + tx * TILE;
+            const double* pi = xi + tx * TILE;
+            for (int r = 0; r < n; r++) {
+                const double a = pr[r], b = pi[r];
+                const bool v = val[r] != 0;
+                acc_re = v ? acc_re + a : acc_re;
+                acc_im = v ? acc_im + b : acc_im;
+            }
+        }
+        __syncthreads();
     }
-    part[((int64_t)t * NC + c) * 2] = acc.re;
-    part[((int64_t)t * NC + c) * 2 + 1] = acc.im;
+    if (tx < kChainComps && c0 + tx < NC) {
+        part[((int64_t)t * NC + c0 + tx) * 2] = acc_re;
+        part[((int64_t)t * NC + c0 + tx) * 2 + 1] = acc_im;
+    }
+}
+
+// Histogram of small integer values privatised in LDS (nbins <= 64).  mode 0: valid
+// slots by shape; mode 1: surviving groups by shape; mode 2: chain keys.
+__global__ void k_hist(int n, int nbins, int mode, const uint8_t* __restrict__ flags, const int* __restrict__ vals,
+                       const int* __restrict__ surv, const uint32_t* __restrict__ ukeys, int* __restrict__ hist) {
+    __shared__ int h[64];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int v = -1;
+        if (mode == 0) {
+            if (flags[i] & kValid) v = vals[i];
+        } else if (mode == 1) {
+            if (surv[i] >= 0) v = vals[i];
+        } else {
+            v = (int)ukeys[i];
+        }
+        if (v >= 0 && v < nbins) atomicAdd(&h[v], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
 // ------------------------------------------------------------------ gather
@@ -1136,39 +1206,43 @@ __global__ void k_group_btab(const TermPool OUT, int g0, int ngb, int m, int d, 
 }
 
 // ------------------------------------------------------------------ group g-tables
+// Per-term data of one g-table evaluation, staged once per evaluated term: its
+// q, coalignment maps and its parent g-table keys live in LDS, the rest in registers.
+struct EvalTerm {
+    const uint32_t* pkeys;  // parent g-table keys (LDS copy or global)
+    const cplx* pvals;      // parent g-table values (global)
+    int pn;                 // parent g-table size
+    int m, phc, z, lhp;
+    uint32_t hf;
+    bool is_new, has_cmap;
+    double c, dv, gscale;
+};
+
 // g value of one term at one cell (make_gtable loop body, flattening.hpp:129-248).
-__device__ inline cplx dev_cell_g(const TermPool& T, int64_t s, uint32_t key, int m, const uint32_t* gk,
-                                  const cplx* gv, double gscale) {
-    const int MM = T.MM;
-    const double* q = T.q + s * MM;
-    const uint32_t hf = T.hflag[s];
-    const int phc = T.phc[s];
-    const int z = T.z[s];
-    const uint8_t fl = T.flags[s];
+__device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8_t* cm, const int8_t* cs,
+                                 uint32_t key) {
     double ygi = 0;
-    for (int k = 0; k < m; k++) {
+    for (int k = 0; k < T.m; k++) {
         const double sb = ((key >> k) & 1u) ? -1 : 1;
-        if (!((hf >> k) & 1u)) ygi += q[k] * sb;
+        if (!((T.hf >> k) & 1u)) ygi += q[k] * sb;
     }
+    const int phc = T.phc;
     const int rev_phc = (phc >= 32) ? -1 : ((1 << phc) - 1);
     const int top_phc = 1 << (phc - 1);
     int lp = 0, lmv = 0;
-    if (!(fl & kIsNew)) {
+    if (!T.is_new) {
         lp = (int)key & rev_phc;
         lmv = lp;
     } else {
-        const bool hc = fl & kHasCmap;
-        const uint8_t* cm = T.cmap + s * MM;
-        const int8_t* cs = T.csmap + s * MM;
         int k = 0, l = 0;
         while (k < phc) {
-            if (k == z) {
+            if (k == T.z) {
                 lmv |= (1 << k);
                 k++;
                 if (k == phc) break;
             }
             int bv;
-            if (hc) bv = (((key >> cm[l]) & 1u) ? -1 : 1) * (int)cs[l];
+            if (T.has_cmap) bv = (((key >> cm[l]) & 1u) ? -1 : 1) * (int)cs[l];
             else bv = ((key >> l) & 1u) ? -1 : 1;
             if (bv < 0) {
                 lp |= (1 << k);
@@ -1178,14 +1252,10 @@ __device__ inline cplx dev_cell_g(const TermPool& T, int64_t s, uint32_t key, in
             l++;
         }
     }
-    const int64_t go = T.gp_off[s];
-    const int gn = T.gp_n[s];
-    const int lhp = T.lhp[s];
-    const cplx a = lookup_num(lp ^ lhp, top_phc, rev_phc, gk + go, gv + go, gn);
-    const cplx bm = lookup_num(lmv ^ lhp, top_phc, rev_phc, gk + go, gv + go, gn);
-    const double c = T.c[s], dv = T.dv[s];
-    const cplx v = csub(cdiv(a, cmake(ygi + dv, c)), cdiv(bm, cmake(ygi - dv, c)));
-    return cscale(v, gscale);
+    const cplx a = lookup_num(lp ^ T.lhp, top_phc, rev_phc, T.pkeys, T.pvals, T.pn);
+    const cplx bm = lookup_num(lmv ^ T.lhp, top_phc, rev_phc, T.pkeys, T.pvals, T.pn);
+    const cplx v = csub(cdiv(a, cmake(ygi + T.dv, T.c)), cdiv(bm, cmake(ygi - T.dv, T.c)));
+    return cscale(v, T.gscale);
 }
 
 __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
@@ -1195,12 +1265,53 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
     return ps * ps;
 }
 
+// Stages term s for evaluation (one wavefront, blockDim == 64).
+template <bool LDS_KEYS>
+__device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const uint32_t* gk_prev,
+                                      const cplx* gv_prev, double gscale, uint32_t* lds_keys, double* q_s,
+                                      uint8_t* cm_s, int8_t* cs_s) {
+    EvalTerm T;
+    const int64_t go = O.gp_off[s];
+    T.pn = O.gp_n[s];
+    T.pvals = gv_prev + go;
+    T.m = m;
+    T.phc = O.phc[s];
+    T.z = O.z[s];
+    T.lhp = O.lhp[s];
+    T.hf = O.hflag[s];
+    const uint8_t fl = O.flags[s];
+    T.is_new = fl & kIsNew;
+    T.has_cmap = fl & kHasCmap;
+    T.c = O.c[s];
+    T.dv = O.dv[s];
+    T.gscale = gscale;
+    __syncthreads();
+    const int lane = threadIdx.x;
+    if (LDS_KEYS) {
+        const uint32_t* src = gk_prev + go;
+        for (int i = lane; i < T.pn; i += kWave) lds_keys[i] = src[i];
+        T.pkeys = lds_keys;
+    } else {
+        T.pkeys = gk_prev + go;
+    }
+    const int MM = O.MM;
+    if (lane < m) q_s[lane] = O.q[s * MM + lane];
+    const int pbc = O.pbc[s];
+    if (T.has_cmap && lane < pbc) {
+        cm_s[lane] = O.cmap[s * MM + lane];
+        cs_s[lane] = O.csmap[s * MM + lane];
+    }
+    __syncthreads();
+    return T;
+}
+
 // Flattening of one reduction group, make_gtables (flattening.hpp:351-564):
 // the root's g-table; if the root is negligible (term approximation), the
 // members in order re-orient the group B-table until one is not (root
 // replacement); then every later non-negligible member's g-table is added into
 // the root's in member order (add_gtables, flattening.hpp:259-315).
 // One wavefront per group; lanes own cells.
+template <bool LDS_KEYS>
 __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, const int* __restrict__ grp_root,
                                                   const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
                                                   const int* __restrict__ grp_m, const int* __restrict__ mem_slots,
@@ -1211,6 +1322,10 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                                                   cplx* __restrict__ scratch, double gscale, int* __restrict__ surv,
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
                                                   int* __restrict__ sort_count, int* __restrict__ rare) {
+    extern __shared__ uint32_t lds_keys[];
+    __shared__ double q_s[kMaxM];
+    __shared__ uint8_t cm_s[kMaxM];
+    __shared__ int8_t cs_s[kMaxM];
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int lane = threadIdx.x;
@@ -1227,12 +1342,12 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     const uint32_t rev = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
     int r = grp_root[g];
     uint32_t X = 0;
-    // root over its own B-table
     {
+        EvalTerm T = stage_term<LDS_KEYS>(OUT, r, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
         const double ps = dev_psum2(OUT, r, m);
         bool small = true;
         for (int i = lane; i < n; i += kWave) {
-            const cplx v = dev_cell_g(OUT, r, B[i], m, gk_prev, gv_prev, gscale);
+            const cplx v = eval_cell(T, q_s, cm_s, cs_s, B[i]);
             ov[i] = v;
             if ((ps * hypot(v.re, v.im)) > kApproxEps) small = false;
         }
@@ -1248,10 +1363,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 int sig = orientation_mask(OUT.A + (int64_t)lfr * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
                 if ((uint32_t)sig & top) sig ^= (int)rev;
                 X ^= (uint32_t)sig;
+                EvalTerm Tk = stage_term<LDS_KEYS>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
                 for (int i = lane; i < n; i += kWave) {
-                    const cplx v = dev_cell_g(OUT, sk, B[i] ^ X, m, gk_prev, gv_prev, gscale);
+                    const cplx v = eval_cell(Tk, q_s, cm_s, cs_s, B[i] ^ X);
                     ov[i] = v;
                     if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
                 }
@@ -1273,10 +1389,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
             const int sig = orientation_mask(Ar, OUT.A + (int64_t)sk * MM * d, m, d);
             const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
+            EvalTerm Tk = stage_term<LDS_KEYS>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
             const double psk = dev_psum2(OUT, sk, m);
             bool sm = true;
             for (int i = lane; i < n; i += kWave) {
-                const cplx v = dev_cell_g(OUT, sk, B[i] ^ X ^ sh, m, gk_prev, gv_prev, gscale);
+                const cplx v = eval_cell(Tk, q_s, cm_s, cs_s, B[i] ^ X ^ sh);
                 sc[i] = v;
                 if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
             }
@@ -1433,6 +1550,34 @@ __global__ void k_skip_counts(const TermPool OUT, int n, int* __restrict__ hist)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     if (OUT.flags[s] & kValid) atomicAdd(&hist[OUT.m[s]], 1);
+}
+
+// Algorithmic bytes of k_group_eval: per group, the B-table keys (4 B/cell) and the
+// output table (key + complex value, 20 B/cell); per evaluated term (root and every
+// member), its record (q, p, A rows, scalars, coalignment maps) and, per cell, the two
+// parent g-table entries its numerators need (2 x 20 B).
+__global__ void k_geval_bytes(int ng, int d, const int* __restrict__ grp_m, const int* __restrict__ grp_mcnt,
+                              const int* __restrict__ b2_n, unsigned long long* __restrict__ acc) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const unsigned long long n = (unsigned long long)b2_n[g];
+    const unsigned long long m = (unsigned long long)grp_m[g];
+    const unsigned long long rec = m * (16ull + 8ull * d) + 2ull * m + 52ull;
+    const unsigned long long evals = 1ull + (unsigned long long)grp_mcnt[g];
+    atomicAdd(acc, 24ull * n + evals * (40ull * n + rec));
+}
+
+// Start of each reference thread's chain in the sorted chain keys (T+1 entries).
+__global__ void k_chain_off(const uint32_t* __restrict__ skeys, int n, int T, int* __restrict__ off) {
+    const int t = threadIdx.x;
+    if (t > T) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (skeys[mid] < (uint32_t)t) lo = mid + 1;
+        else hi = mid;
+    }
+    off[t] = lo;
 }
 
 // ------------------------------------------------------------------ small helpers

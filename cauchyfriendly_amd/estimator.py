@@ -113,9 +113,12 @@ class GpuEstimator:
             L.mce_profile_enable(self.h, int(bool(enable)))
         if reset:
             L.mce_profile_reset(self.h)
-        out = np.zeros(12)
+        out = np.zeros(13)
         L.mce_profile_read(self.h, out.ctypes.data_as(_dp))
         return out
+
+    def reset(self):
+        _native.lib().mce_estimator_reset(self.h)
 
 
 def run_system(sysobj, num_steps=None, seed=None, collect_terms=False):

@@ -149,10 +149,11 @@ int mce_term(void* handle, int shape, int idx, int* info, double* A, double* p, 
 void mce_set_random_draws(void* handle, const double* root_point, const double* b_pert_8xmax);
 void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax);
 
-/* Accumulated device time (ms, HIP events on the engine's stream) per phase since the last
- * reset: [time-prop, B-table enumeration, measurement update, moments, gather, reduction,
- * group B-tables, group g-tables, compaction, total] and the launch count of the group
- * g-table kernel, plus its algorithmic bytes.  Timing is only recorded after mce_profile_enable(h, 1). */
+/* Accumulated device time (ms, HIP events on the engine's stream) since the last reset:
+ * out[0..8] per phase [time-prop, B-table enumeration, measurement update, moments, gather,
+ * reduction, group B-tables, group g-tables, compaction], out[9] their total, out[10] the
+ * launch count of the group g-table kernel, out[11] its algorithmic bytes, out[12] its
+ * kernel time.  Recorded only after mce_profile_enable(h, 1); out must hold 13 doubles. */
 void mce_profile_enable(void* handle, int on);
 void mce_profile_read(void* handle, double* out_ms12);
 void mce_profile_reset(void* handle);
