@@ -271,7 +271,9 @@ struct Engine {
     DBuf<int> ecnt, eoff, edges, indeg, inoff, cursor, Fc;
     DBuf<uint8_t> st0, st1, isroot;
     DBuf<uint32_t> memkey, memval, smkey, smval, roots, jiota;
-    DBuf<int> memcnt, memoff;
+    DBuf<int> memcnt, memoff, inedges, dboff, gbase_dev;
+    DBuf<uint32_t> bkey, bskey;
+    std::vector<int> h_boff;
     // groups (all buckets of an update)
     DBuf<int> grp_root, grp_moff, grp_mcnt, grp_m, mem_slots, b2n, surv, gxor, sort_list, spos, sflag;
     DBuf<int64_t> ub64, b2off, goff, b2n64;
@@ -831,17 +833,8 @@ struct Engine {
         grp_mcnt.reserve(total);
         grp_m.reserve(total);
         mem_slots.reserve(total);
-        std::vector<int> gbase_of(sr + 1, 0);
-        int gbase = 0, mbase = 0;
-        for (int m = 0; m < sr; m++) {
-            gbase_of[m] = gbase;
-            const int N = cnt[m];
-            if (N == 0) continue;
-            const int base = off[m];
-            reduce_bucket(OUT, m, base, N, gbase, mbase);
-        }
-        gbase_of[sr] = gbase;
-        const int ng = gbase;
+        std::vector<int> gbase_of;
+        const int ng = reduce_all(OUT, off, gbase_of);
         prof.mark(kPhFtr, st);
 
         // ---- group B-tables
@@ -956,7 +949,7 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         {
-            DBuf<int>& mx = memcnt;  // scalar scratch
+            DBuf<int>& mx = gbase_dev;  // scalar scratch
             mx.reserve(1);
             cub([&](void* t_, size_t& b_) { return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, mx.ptr, ng, st); });
             max_gp_cells = read1(mx.ptr);
@@ -971,76 +964,91 @@ struct Engine {
         gcur = nx;
     }
 
-    // Fast term reduction of one shape bucket; appends its reduction groups.
-    void reduce_bucket(const TermPool& OUT, int m, int base, int N, int& gbase, int& mbase) {
+    // Fast term reduction of every shape bucket at once.  off[m] is the first gathered
+    // position of shape m; fills the reduction groups (shape ascending, roots ascending,
+    // members ascending per root) and gbase_of[m], the first group of shape m.
+    int reduce_all(const TermPool& OUT, const std::vector<int>& off, std::vector<int>& gbase_of) {
+        const int N = off[sr];
+        gbase_of.assign(sr + 1, 0);
+        if (N == 0) return 0;
         const int a0 = g_search_order[0];
+        h_boff = off;
+        dboff.reserve(sr + 1);
+        HIPCHK(hipMemcpyAsync(dboff.ptr, h_boff.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+        const int* boff = dboff.ptr;
         fkeys.reserve(N);
         fskeys.reserve(N);
         fidx.reserve(N);
         fsidx.reserve(N);
+        bkey.reserve(N);
+        bskey.reserve(N);
         fvals.reserve(N);
         ecnt.reserve(N);
         eoff.reserve(N);
-        hipLaunchKernelGGL(k_ftr_keys, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr, base, N, d, a0,
-                           fkeys.ptr, fidx.ptr);
+        const dim3 g1(ceil_div(N, 256)), b1(256);
+        // value order on the first search axis, then (stable) bucket order
+        hipLaunchKernelGGL(k_ftr_keys, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, fkeys.ptr, fidx.ptr);
         cub([&](void* t, size_t& b) {
             return hipcub::DeviceRadixSort::SortPairs(t, b, fkeys.ptr, fskeys.ptr, fidx.ptr, fsidx.ptr, N, 0, 64, st);
         });
-        hipLaunchKernelGGL(k_ftr_vals, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr, base, N, d, a0,
-                           fsidx.ptr, fvals.ptr);
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT, perm.ptr,
-                                             base, N, m, dconst.ptr, fvals.ptr, fsidx.ptr, ecnt.ptr, eoff.ptr,
-                                             edges.ptr, 0));
+        hipLaunchKernelGGL(k_ftr_bkeys, g1, b1, 0, st, N, boff, sr, fsidx.ptr, bkey.ptr);
+        int bbits = 1;
+        while ((1 << bbits) < sr) bbits++;
+        cub([&](void* t, size_t& b) {
+            return hipcub::DeviceRadixSort::SortPairs(t, b, bkey.ptr, bskey.ptr, fsidx.ptr, fidx.ptr, N, 0, bbits, st);
+        });
+        const uint32_t* sidx = fidx.ptr;
+        hipLaunchKernelGGL(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, dconst.ptr,
+                                             fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
         HIPCHK(hipGetLastError());
         scan_i32_excl(ecnt.ptr, eoff.ptr, N);
         const int E = read1(eoff.ptr + N - 1) + read1(ecnt.ptr + N - 1);
-        const int nchunk = (N > 10000) ? 8 : 1;
-        const int chunk = N / 8;
-        Fc.reserve((size_t)nchunk * N);
-        hipLaunchKernelGGL(k_ftr_finit, dim3(ceil_div((int64_t)nchunk * N, 256)), dim3(256), 0, st, Fc.ptr,
-                           (int64_t)nchunk * N, N);
+        Fc.reserve((size_t)8 * N);
+        hipLaunchKernelGGL(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
         if (E > 0) {
             edges.reserve(E);
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, dim3(ceil_div(N, 256)), dim3(256), 0, st, OUT,
-                                                 perm.ptr, base, N, m, dconst.ptr, fvals.ptr, fsidx.ptr, ecnt.ptr,
-                                                 eoff.ptr, edges.ptr, 1));
+            inedges.reserve(E);
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+                                                 dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
             indeg.reserve(N);
             inoff.reserve(N);
             cursor.reserve(N);
             HIPCHK(hipMemsetAsync(indeg.ptr, 0, N * sizeof(int), st));
-            hipLaunchKernelGGL(k_ftr_indeg, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr, edges.ptr,
-                               nchunk, chunk, indeg.ptr);
+            hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+                               (const int*)nullptr, (int*)nullptr, (int*)nullptr, 0);
             scan_i32_excl(indeg.ptr, inoff.ptr, N);
-            const int Ein = read1(inoff.ptr + N - 1) + read1(indeg.ptr + N - 1);
-            DBuf<int>& inedges = memoff;  // reused as in-edge storage
-            inedges.reserve(std::max(Ein, 1));
             HIPCHK(hipMemsetAsync(cursor.ptr, 0, N * sizeof(int), st));
-            hipLaunchKernelGGL(k_ftr_infill, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr,
-                               edges.ptr, nchunk, chunk, inoff.ptr, cursor.ptr, inedges.ptr);
+            hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+                               inoff.ptr, cursor.ptr, inedges.ptr, 1);
             st0.reserve(N);
             st1.reserve(N);
             HIPCHK(hipMemsetAsync(st0.ptr, kUnknown, N, st));
             uint8_t* sa = st0.ptr;
             uint8_t* sb = st1.ptr;
-            for (int round = 0; round <= N + 1; round++) {
-                HIPCHK(hipMemsetAsync(counters.ptr + 4, 0, sizeof(int), st));
-                hipLaunchKernelGGL(k_ftr_round, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, inoff.ptr, indeg.ptr,
-                                   inedges.ptr, sa, sb, counters.ptr + 4);
-                std::swap(sa, sb);
-                if (read1(counters.ptr + 4) == 0) break;
-            }
-            hipLaunchKernelGGL(k_ftr_claim, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, eoff.ptr, ecnt.ptr, edges.ptr,
-                               sa, nchunk, chunk, Fc.ptr);
-            if (nchunk == 8) {
-                for (int lvl = 0; lvl < 3; lvl++) {
-                    const int pairs = 8 >> (lvl + 1);
-                    hipLaunchKernelGGL(k_ftr_merge, dim3(ceil_div(N, 256), pairs), dim3(256), 0, st, N, Fc.ptr, lvl,
-                                       chunk, 8);
+            // Rounds are idempotent once every term is resolved, so several run between
+            // reads of the unresolved count (taken from the last round of a batch).
+            int batch = 2;
+            for (int done = 0; done <= N + 1;) {
+                for (int r = 0; r < batch; r++) {
+                    HIPCHK(hipMemsetAsync(counters.ptr + 4, 0, sizeof(int), st));
+                    hipLaunchKernelGGL(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
+                                       counters.ptr + 4);
+                    std::swap(sa, sb);
                 }
+                done += batch;
+                if (read1(counters.ptr + 4) == 0) break;
+                batch = std::min(batch * 2, 16);
+            }
+            hipLaunchKernelGGL(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);
+            if (N > 10000) {
+                for (int lvl = 0; lvl < 3; lvl++)
+                    hipLaunchKernelGGL(k_ftr_merge, dim3(ceil_div(N, 256), 8 >> (lvl + 1)), b1, 0, st, N, boff, sr,
+                                       Fc.ptr, lvl);
             }
             HIPCHK(hipGetLastError());
         }
-        // reduction groups: roots ascending, members ascending per root
+        // reduction groups
         isroot.reserve(N);
         memkey.reserve(N);
         memval.reserve(N);
@@ -1048,36 +1056,37 @@ struct Engine {
         smval.reserve(N);
         roots.reserve(N);
         jiota.reserve(N);
-        memcnt.reserve(N);
-        hipLaunchKernelGGL(k_ftr_rootflags, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, Fc.ptr, isroot.ptr,
-                           memkey.ptr, memval.ptr);
-        hipLaunchKernelGGL(k_iota, dim3(ceil_div(N, 256)), dim3(256), 0, st, jiota.ptr, N);
+        memcnt.reserve(N + 1);
+        memoff.reserve(N + 1);
+        hipLaunchKernelGGL(k_ftr_rootflags, g1, b1, 0, st, N, boff, sr, Fc.ptr, isroot.ptr, memkey.ptr, memval.ptr);
+        hipLaunchKernelGGL(k_iota, g1, b1, 0, st, jiota.ptr, N);
         cub([&](void* t, size_t& b) {
             return hipcub::DeviceSelect::Flagged(t, b, jiota.ptr, isroot.ptr, roots.ptr, counters.ptr + 5, N, st);
         });
+        int kbits = 1;
+        while (((int64_t)1 << kbits) <= N) kbits++;
+        cub([&](void* t, size_t& b) {
+            return hipcub::DeviceRadixSort::SortPairs(t, b, memkey.ptr, smkey.ptr, memval.ptr, smval.ptr, N, 0, kbits,
+                                                      st);
+        });
+        gbase_dev.reserve(sr + 1);
         const int nroots = read1(counters.ptr + 5);
         const int nm = N - nroots;
+        hipLaunchKernelGGL(k_ftr_gbase, dim3(1), dim3(64), 0, st, nroots, roots.ptr, boff, sr, gbase_dev.ptr);
+        HIPCHK(hipMemsetAsync(memcnt.ptr, 0, (N + 1) * sizeof(int), st));
         if (nm > 0) {
-            cub([&](void* t, size_t& b) {
-                return hipcub::DeviceRadixSort::SortPairs(t, b, memkey.ptr, smkey.ptr, memval.ptr, smval.ptr, N, 0, 32,
-                                                          st);
-            });
+            hipLaunchKernelGGL(k_ftr_memcount, dim3(ceil_div(nm, 256)), b1, 0, st, nm, smkey.ptr, memcnt.ptr);
+            hipLaunchKernelGGL(k_ftr_memslots, dim3(ceil_div(nm, 256)), b1, 0, st, nm, smval.ptr, perm.ptr,
+                               mem_slots.ptr);
         }
-        HIPCHK(hipMemsetAsync(memcnt.ptr, 0, N * sizeof(int), st));
-        if (nm > 0)
-            hipLaunchKernelGGL(k_ftr_memcount, dim3(ceil_div(nm, 256)), dim3(256), 0, st, nm, smkey.ptr, memcnt.ptr);
-        DBuf<int>& moff = cursor;  // member offsets per root (local)
-        moff.reserve(N);
-        scan_i32_excl(memcnt.ptr, moff.ptr, N);
-        if (nm > 0)
-            hipLaunchKernelGGL(k_ftr_memslots, dim3(ceil_div(nm, 256)), dim3(256), 0, st, nm, smval.ptr, perm.ptr,
-                               base, mbase, mem_slots.ptr);
-        hipLaunchKernelGGL(k_ftr_groups, dim3(ceil_div(nroots, 256)), dim3(256), 0, st, nroots, roots.ptr, perm.ptr,
-                           base, gbase, mbase, m, memcnt.ptr, moff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr,
+        scan_i32_excl(memcnt.ptr, memoff.ptr, N);
+        hipLaunchKernelGGL(k_ftr_groups, dim3(ceil_div(std::max(nroots, 1), 256)), b1, 0, st, nroots, roots.ptr,
+                           perm.ptr, boff, sr, memcnt.ptr, memoff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr,
                            grp_m.ptr);
         HIPCHK(hipGetLastError());
-        gbase += nroots;
-        mbase += nm;
+        HIPCHK(hipMemcpyAsync(gbase_of.data(), gbase_dev.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return nroots;
     }
 
     // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).

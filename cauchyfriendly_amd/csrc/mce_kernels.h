@@ -39,36 +39,48 @@ struct StepConst {
 template <int D>
 __device__ inline double dev_one_norm(const double* A) {
     double best = -1;
+#pragma unroll
     for (int c = 0; c < D; c++) {
         double s = 0;
+#pragma unroll
         for (int r = 0; r < D; r++) s += fabs(A[r * D + c]);
         if (s > best) best = s;
     }
     return best;
 }
 
-// In-place partial-pivot LU, cauchy_linalg.hpp:1180-1232.
+// In-place partial-pivot LU, cauchy_linalg.hpp:1180-1232.  Row exchanges are
+// written as predicated swaps over compile-time indices so the matrix stays in
+// registers; the arithmetic and its order are the reference's.
 template <int D>
 __device__ inline bool dev_lu(double* A, int* piv) {
+#pragma unroll
     for (int j = 0; j < D; ++j) {
         double pv = kPluEps;
         int pi = -1;
+#pragma unroll
         for (int i = j; i < D; ++i)
             if (fabs(A[i * D + j]) > fabs(pv)) {
                 pv = A[i * D + j];
                 pi = i;
             }
         if (pi == -1) return false;
-        if (pi != j)
+#pragma unroll
+        for (int r = j + 1; r < D; ++r) {
+            const bool sw = (r == pi);
+#pragma unroll
             for (int k = 0; k < D; k++) {
-                double t = A[j * D + k];
-                A[j * D + k] = A[pi * D + k];
-                A[pi * D + k] = t;
+                const double a = A[j * D + k], b = A[r * D + k];
+                A[j * D + k] = sw ? b : a;
+                A[r * D + k] = sw ? a : b;
             }
+        }
         piv[j] = pi;
+#pragma unroll
         for (int k = j + 1; k < D; ++k) {
             A[k * D + j] /= A[j * D + j];
             const double t = A[k * D + j];
+#pragma unroll
             for (int q = j + 1; q < D; q++) A[k * D + q] -= t * A[j * D + q];
         }
     }
@@ -77,30 +89,49 @@ __device__ inline bool dev_lu(double* A, int* piv) {
 
 template <int D>
 __device__ inline void dev_fwd(const double* LU, double* b) {
+#pragma unroll
     for (int i = 0; i < D; i++) {
         double s = b[i];
+#pragma unroll
         for (int j = 0; j < i; j++) s -= LU[i * D + j] * b[j];
         b[i] = s;
     }
 }
 template <int D>
 __device__ inline void dev_bwd(const double* LU, double* b) {
+#pragma unroll
     for (int i = D - 1; i >= 0; i--) {
         double s = b[i];
+#pragma unroll
         for (int j = D - 1; j > i; j--) s -= LU[i * D + j] * b[j];
         b[i] = s / LU[i * D + i];
     }
 }
+// Inverse of the LAPACK-style pivot sequence (cauchy_linalg.hpp:1258-1277):
+// pt[r] = position row r of the original matrix ends up at.
 template <int D>
 __device__ inline void dev_pivot_inverse(const int* piv, int* pt) {
     int preg[D];
+#pragma unroll
     for (int i = 0; i < D; i++) preg[i] = i;
+#pragma unroll
     for (int i = 0; i < D; i++) {
-        const int t = preg[i];
-        preg[i] = preg[piv[i]];
-        preg[piv[i]] = t;
+        // swap preg[i] and preg[piv[i]] (piv[i] >= i)
+        const int a = preg[i];
+        int b = a;
+#pragma unroll
+        for (int r = i; r < D; r++) b = (r == piv[i]) ? preg[r] : b;
+#pragma unroll
+        for (int r = i; r < D; r++) preg[r] = (r == piv[i]) ? a : preg[r];
+        preg[i] = b;
     }
-    for (int i = 0; i < D; i++) pt[preg[i]] = i;
+#pragma unroll
+    for (int r = 0; r < D; r++) {
+        int v = 0;
+#pragma unroll
+        for (int i = 0; i < D; i++) v = (preg[i] == r) ? i : v;
+        pt[r] = v;
+    }
 }
 
 // One-norm condition number via explicit inverse, cauchy_linalg.hpp:1337-1392.
@@ -108,20 +139,21 @@ template <int D>
 __device__ inline double dev_cond1(double* A, int* piv) {
     const double na = dev_one_norm<D>(A);
     if (!dev_lu<D>(A, piv)) return 1.7976931348623157e308;
-    double W[D * D];
-    for (int i = 0; i < D * D; i++) W[i] = 0;
     int pt[D];
     dev_pivot_inverse<D>(piv, pt);
-    for (int i = 0; i < D; i++) {
-        W[i * D + pt[i]] = 1;
-        dev_fwd<D>(A, W + i * D);
-        dev_bwd<D>(A, W + i * D);
-    }
-    // one-norm of the transpose = largest row sum of W
+    // one-norm of the inverse = largest absolute row sum of W, whose row i solves
+    // A w = e_i (W is the transpose of the inverse before reflect_array)
     double best = -1;
-    for (int c = 0; c < D; c++) {
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        double w[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) w[c] = (c == pt[i]) ? 1.0 : 0.0;
+        dev_fwd<D>(A, w);
+        dev_bwd<D>(A, w);
         double s = 0;
-        for (int r = 0; r < D; r++) s += fabs(W[c * D + r]);
+#pragma unroll
+        for (int r = 0; r < D; r++) s += fabs(w[r]);
         if (s > best) best = s;
     }
     return na * best;
@@ -131,7 +163,13 @@ template <int D>
 __device__ inline void dev_lu_solve(const double* LU, const int* piv, const double* b, double* x) {
     int pt[D];
     dev_pivot_inverse<D>(piv, pt);
-    for (int i = 0; i < D; i++) x[pt[i]] = b[i];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) v = (pt[i] == c) ? b[i] : v;
+        x[c] = v;
+    }
     dev_fwd<D>(LU, x);
     dev_bwd<D>(LU, x);
 }
@@ -319,7 +357,9 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
         const uint8_t* cb = cmb + ci * D;
         double Ac[D * D], bc[D], v[D];
         int piv[D];
+#pragma unroll
         for (int j = 0; j < D; j++) {
+#pragma unroll
             for (int k = 0; k < D; k++) Ac[j * D + k] = A[cb[j] * D + k];
             bc[j] = bp[cb[j]];
         }
@@ -334,10 +374,14 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
             }
             if ((dev_dot<D>(A + ac * D, v) - bp[ac]) < 0) niv |= (1u << ac);
         }
+        uint32_t cbit[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) cbit[k] = 1u << cb[k];
         for (int s = 0; s < (1 << D); s++) {
             uint32_t sv = niv;
+#pragma unroll
             for (int k = 0; k < D; k++)
-                if ((s >> k) & 1) sv |= (1u << cb[k]);
+                if ((s >> k) & 1) sv |= cbit[k];
             atomicOr(&bits[sv >> 5], 1u << (sv & 31));
         }
     }
@@ -817,7 +861,23 @@ __global__ __launch_bounds__(256) void k_mom_chain(const double* __restrict__ gy
         if (tx < kChainComps && c0 + tx < NC) {
             const double* pr = xr + tx * TILE;
             const double* pi = xi + tx * TILE;
-            for (int r = 0; r < n; r++) {
+            int r = 0;
+            for (; r + 8 <= n; r += 8) {
+                double a[8], b[8];
+                bool v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    a[q] = pr[r + q];
+                    b[q] = pi[r + q];
+                    v[q] = val[r + q] != 0;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    acc_re = v[q] ? acc_re + a[q] : acc_re;
+                    acc_im = v[q] ? acc_im + b[q] : acc_im;
+                }
+            }
+            for (; r < n; r++) {
                 const double a = pr[r], b = pi[r];
                 const bool v = val[r] != 0;
                 acc_re = v ? acc_re + a : acc_re;
@@ -868,68 +928,104 @@ __global__ void k_shape_hist(const TermPool OUT, int n, int* __restrict__ hist) 
 }
 
 // ------------------------------------------------------------------ fast term reduction
-// Bucket-local index r -> slot perm[base + r].
-__global__ void k_ftr_keys(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int d, int axis,
-                           uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t s = perm[base + r];
-    keys[r] = f64_radix_key(OUT.b[(int64_t)s * d + axis]);
-    idx[r] = (uint32_t)r;
-}
-
-__global__ void k_ftr_vals(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int d, int axis,
-                           const uint32_t* __restrict__ sidx, double* __restrict__ vals) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    vals[r] = OUT.b[(int64_t)perm[base + sidx[r]] * d + axis];
-}
-
-__device__ inline int upper_bound_f64(const double* v, int n, double x) {
-    int lo = 0, hi = n;
+// All shape buckets are reduced together.  Global position p in [0, N) indexes the
+// gathered order (bucket m occupies [boff[m], boff[m+1]), slot perm[p]); reduction
+// decisions use bucket-local indices, as the reference reduces each bucket on its own
+// (fast_term_reduction_and_create_gtables, cauchy_estimator.hpp:1001-1137).
+__device__ inline int bucket_of(const int* __restrict__ boff, int nb, int p) {
+    int lo = 0, hi = nb - 1;
     while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (v[mid] > x) hi = mid;
-        else lo = mid + 1;
+        const int mid = (lo + hi + 1) >> 1;
+        if (boff[mid] <= p) lo = mid;
+        else hi = mid - 1;
     }
     return lo;
 }
 
+// Chunk of local index l in a bucket of n terms: buckets above 10000 terms are reduced
+// in 8 chunks of n/8 (the last takes the rest), term_reduction.hpp:651-681.
+__device__ inline int ftr_chunk_of(int l, int n) {
+    if (n <= 10000) return 0;
+    const int c = l / (n / 8);
+    return c > 7 ? 7 : c;
+}
+
+__global__ void k_ftr_keys(const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    keys[p] = f64_radix_key(OUT.b[(int64_t)perm[p] * d + axis]);
+    idx[p] = (uint32_t)p;
+}
+
+__global__ void k_ftr_vals(const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
+                           const uint32_t* __restrict__ sidx, double* __restrict__ vals) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    vals[r] = OUT.b[(int64_t)perm[sidx[r]] * d + axis];
+}
+
+// Bucket of each position of the value-sorted order (second, stable sort key).
+__global__ void k_ftr_bkeys(int n, const int* __restrict__ boff, int nb, const uint32_t* __restrict__ sidx,
+                            uint32_t* __restrict__ bkey) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    bkey[r] = (uint32_t)bucket_of(boff, nb, (int)sidx[r]);
+}
+
 // Candidate pairs (i, j > i) of the greedy reduction, term_reduction.hpp:159-276:
 // b_j inside (b_i - ep, b_i + ep] on every axis (the window ftr_binary_search
-// selects, term_reduction.hpp:89-157; a window must hold two points), then the p
-// check and the element-wise check over the first m flattened entries of A.
-// pass 0 counts, pass 1 writes the out-edge list.
+// selects, term_reduction.hpp:89-157, which must hold two points), then the p check
+// and the element-wise check over the first m flattened entries of A.  One thread
+// per sorted position; the window is found by scanning the sorted neighbours.
+// pass 0 counts, pass 1 writes the out-edges (global positions) of term i.
 template <int D>
-__global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ perm, int base, int n, int m,
-                            const StepConst* __restrict__ sc, const double* __restrict__ vals,
-                            const uint32_t* __restrict__ sidx, int* __restrict__ cnt, const int* __restrict__ eoff,
-                            int* __restrict__ edges, int pass) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ perm, int n,
+                            const int* __restrict__ boff, int nb, const StepConst* __restrict__ sc,
+                            const double* __restrict__ vals, const uint32_t* __restrict__ sidx,
+                            int* __restrict__ cnt, const int* __restrict__ eoff, int* __restrict__ edges, int pass) {
+    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp >= n) return;
     const double ep = kReductionEps;
+    const int bk = bucket_of(boff, nb, sp);
+    const int beg = boff[bk], end = boff[bk + 1];
+    const int m = bk;  // bucket index is the shape
     const int MM = OUT.MM;
-    const int64_t si = perm[base + i];
+    const int ig = (int)sidx[sp];
+    const int il = ig - beg;
+    const int64_t si = perm[ig];
     const double* bi = OUT.b + si * D;
     const double* pi = OUT.p + si * MM;
     const double* Ai = OUT.A + si * MM * D;
-    const int a0 = sc->order[0];
-    const int r0 = upper_bound_f64(vals, n, bi[a0] - ep);
-    const int r1 = upper_bound_f64(vals, n, bi[a0] + ep);
+    const double v = vals[sp];
+    const double lo = v - ep, hi = v + ep;
+    // r0 = first sorted position in the bucket with value > lo; r1 = first with value > hi
+    int r0 = sp;
+    if (vals[r0] > lo) {
+        while (r0 > beg && vals[r0 - 1] > lo) r0--;
+    } else {
+        while (r0 < end && vals[r0] <= lo) r0++;
+    }
+    int r1 = sp;
+    if (vals[r1] <= hi) {
+        while (r1 < end && vals[r1] <= hi) r1++;
+    } else {
+        while (r1 > beg && vals[r1 - 1] > hi) r1--;
+    }
     int c = 0;
-    int w = pass ? eoff[i] : 0;
+    int w = pass ? eoff[ig] : 0;
     if (r1 - r0 >= 2) {
         for (int r = r0; r < r1; r++) {
-            const int j = (int)sidx[r];
-            if (j <= i) continue;
-            const int64_t sj = perm[base + j];
+            const int jg = (int)sidx[r];
+            if (jg - beg <= il) continue;
+            const int64_t sj = perm[jg];
             const double* bj = OUT.b + sj * D;
             bool ok = true;
             for (int a = 1; a < D && ok; a++) {
                 const int ax = sc->order[a];
                 const double l2 = bi[ax] - ep, h2 = bi[ax] + ep;
-                const double v = bj[ax];
-                if (!(v > l2 && v <= h2)) ok = false;
+                const double x = bj[ax];
+                if (!(x > l2 && x <= h2)) ok = false;
             }
             if (!ok) continue;
             const double* pj = OUT.p + sj * MM;
@@ -948,42 +1044,31 @@ __global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ per
                 }
             }
             if (!ok) continue;
-            if (pass) edges[w++] = j;
+            if (pass) edges[w++] = jg;
             c++;
         }
     }
-    if (!pass) cnt[i] = c;
+    if (!pass) cnt[ig] = c;
 }
 
-__device__ inline int ftr_chunk(int i, int nchunk, int chunk) {
-    if (nchunk == 1) return 0;
-    const int c = i / chunk;
-    return c >= nchunk ? nchunk - 1 : c;
-}
-
-// In-degree over same-chunk edges.
-__global__ void k_ftr_indeg(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
-                            const int* __restrict__ edges, int nchunk, int chunk, int* __restrict__ indeg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int ci = ftr_chunk(i, nchunk, chunk);
-    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) {
-        const int j = edges[e];
-        if (ftr_chunk(j, nchunk, chunk) == ci) atomicAdd(&indeg[j], 1);
-    }
-}
-
-__global__ void k_ftr_infill(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
-                             const int* __restrict__ edges, int nchunk, int chunk, const int* __restrict__ inoff,
-                             int* __restrict__ cursor, int* __restrict__ inedges) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int ci = ftr_chunk(i, nchunk, chunk);
-    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) {
-        const int j = edges[e];
-        if (ftr_chunk(j, nchunk, chunk) == ci) {
-            const int slot = atomicAdd(&cursor[j], 1);
-            inedges[inoff[j] + slot] = i;
+// In-degree over edges whose ends share a chunk (pass 0), or the in-edge lists (pass 1).
+__global__ void k_ftr_inedges(int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
+                              const int* __restrict__ ecnt, const int* __restrict__ edges, int* __restrict__ indeg,
+                              const int* __restrict__ inoff, int* __restrict__ cursor, int* __restrict__ inedges,
+                              int pass) {
+    const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ig >= n) return;
+    const int bk = bucket_of(boff, nb, ig);
+    const int beg = boff[bk], nbk = boff[bk + 1] - beg;
+    const int ci = ftr_chunk_of(ig - beg, nbk);
+    for (int e = eoff[ig]; e < eoff[ig] + ecnt[ig]; e++) {
+        const int jg = edges[e];
+        if (ftr_chunk_of(jg - beg, nbk) != ci) continue;
+        if (pass == 0) {
+            atomicAdd(&indeg[jg], 1);
+        } else {
+            const int slot = atomicAdd(&cursor[jg], 1);
+            inedges[inoff[jg] + slot] = ig;
         }
     }
 }
@@ -1014,61 +1099,64 @@ __global__ void k_ftr_round(int n, const int* __restrict__ inoff, const int* __r
     st_out[j] = s;
 }
 
-// F_c[j] = min root of chunk c claiming j (identity when unclaimed).
-__global__ void k_ftr_finit(int* __restrict__ F, int64_t total, int n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) F[i] = (int)(i % n);
+// Flag arrays F_c (c < 8) over global positions, holding bucket-local indices:
+// F_c[j] = smallest chunk-c root claiming j, else j itself.  Buckets reduced in one
+// chunk use F_0 only.
+__global__ void k_ftr_finit(int* __restrict__ F, int n, const int* __restrict__ boff, int nb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int bk = bucket_of(boff, nb, p);
+    const int l = p - boff[bk];
+    const int nc = (boff[bk + 1] - boff[bk] > 10000) ? 8 : 1;
+    for (int c = 0; c < nc; c++) F[(int64_t)c * n + p] = l;
 }
 
-__global__ void k_ftr_claim(int n, const int* __restrict__ eoff, const int* __restrict__ ecnt,
-                            const int* __restrict__ edges, const uint8_t* __restrict__ st, int nchunk, int chunk,
-                            int* __restrict__ F) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || st[i] != kRoot) return;
-    int* Fc = F + (int64_t)ftr_chunk(i, nchunk, chunk) * n;
-    for (int e = eoff[i]; e < eoff[i] + ecnt[i]; e++) atomicMin(&Fc[edges[e]], i);
+__global__ void k_ftr_claim(int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
+                            const int* __restrict__ ecnt, const int* __restrict__ edges,
+                            const uint8_t* __restrict__ st, int* __restrict__ F) {
+    const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ig >= n || st[ig] != kRoot) return;
+    const int bk = bucket_of(boff, nb, ig);
+    const int beg = boff[bk], nbk = boff[bk + 1] - beg;
+    const int il = ig - beg;
+    int* Fc = F + (int64_t)ftr_chunk_of(il, nbk) * n;
+    for (int e = eoff[ig]; e < eoff[ig] + ecnt[ig]; e++) atomicMin(&Fc[edges[e]], il);
 }
 
-// One level of the flag-array merge tree, term_reduction.hpp:592-649.  Every
-// value of a flag array is a fixed point of it, so the level is data parallel.
-__global__ void k_ftr_merge(int n, int* __restrict__ F, int level, int chunk, int nchunk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int pairs = nchunk >> (level + 1);
+// One level of the flag-array merge tree of every 8-chunk bucket,
+// term_reduction.hpp:592-649.  Every value of a flag array is a fixed point of it,
+// so a level is data parallel.
+__global__ void k_ftr_merge(int n, const int* __restrict__ boff, int nb, int* __restrict__ F, int level) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
+    if (p >= n) return;
+    const int bk = bucket_of(boff, nb, p);
+    const int beg = boff[bk], nbk = boff[bk + 1] - beg;
+    if (nbk <= 10000) return;
+    const int pairs = 8 >> (level + 1);
     if (k >= pairs) return;
     const int rt = k * (1 << (level + 1));
     const int cm = rt + (1 << level);
-    const int start = cm * chunk;
-    const int idx = start + i;
-    if (idx >= n) return;
-    int* F1 = F + (int64_t)rt * n;
-    const int* F2 = F + (int64_t)cm * n;
-    const int f2 = F2[idx];
-    if (f2 != -1 && f2 != idx && F1[idx] == idx) F1[idx] = (F1[f2] == f2) ? f2 : F1[f2];
+    const int l = p - beg;
+    if (l < cm * (nbk / 8)) return;
+    int* F1 = F + (int64_t)rt * n + beg;
+    const int* F2 = F + (int64_t)cm * n + beg;
+    const int f2 = F2[l];
+    if (f2 != -1 && f2 != l && F1[l] == l) F1[l] = (F1[f2] == f2) ? f2 : F1[f2];
 }
 
-__global__ void k_ftr_rootflags(int n, const int* __restrict__ F, uint8_t* __restrict__ isroot,
-                                uint32_t* __restrict__ memkey, uint32_t* __restrict__ memval) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int f = F[j];
-    isroot[j] = (f == j);
-    memkey[j] = (f == j) ? 0xffffffffu : (uint32_t)f;
-    memval[j] = (uint32_t)j;
-}
-
-// Group table of one bucket: roots (ascending) and their member counts.
-__global__ void k_ftr_groups(int nroots, const uint32_t* __restrict__ roots, const uint32_t* __restrict__ perm,
-                             int base, int gbase, int mbase, int m, const int* __restrict__ memcnt_by_root,
-                             const int* __restrict__ memoff_by_root, int* __restrict__ grp_root,
-                             int* __restrict__ grp_moff, int* __restrict__ grp_mcnt, int* __restrict__ grp_m) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nroots) return;
-    const uint32_t r = roots[g];
-    grp_root[gbase + g] = (int)perm[base + r];
-    grp_moff[gbase + g] = mbase + memoff_by_root[r];
-    grp_mcnt[gbase + g] = memcnt_by_root[r];
-    grp_m[gbase + g] = m;
+// Root flags; members are keyed by their root's global position, roots by n.
+__global__ void k_ftr_rootflags(int n, const int* __restrict__ boff, int nb, const int* __restrict__ F,
+                                uint8_t* __restrict__ isroot, uint32_t* __restrict__ memkey,
+                                uint32_t* __restrict__ memval) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int beg = boff[bucket_of(boff, nb, p)];
+    const int f = F[p];
+    const bool r = (f == p - beg);
+    isroot[p] = r;
+    memkey[p] = r ? (uint32_t)n : (uint32_t)(beg + f);
+    memval[p] = (uint32_t)p;
 }
 
 __global__ void k_ftr_memcount(int nm, const uint32_t* __restrict__ skey, int* __restrict__ cnt) {
@@ -1077,11 +1165,40 @@ __global__ void k_ftr_memcount(int nm, const uint32_t* __restrict__ skey, int* _
     atomicAdd(&cnt[skey[k]], 1);
 }
 
-__global__ void k_ftr_memslots(int nm, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ perm, int base,
-                               int mbase, int* __restrict__ mem_slots) {
+__global__ void k_ftr_memslots(int nm, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ perm,
+                               int* __restrict__ mem_slots) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
-    mem_slots[mbase + k] = (int)perm[base + sval[k]];
+    mem_slots[k] = (int)perm[sval[k]];
+}
+
+// First reduction group of every bucket: gbase[m] = number of roots before boff[m].
+__global__ void k_ftr_gbase(int nroots, const uint32_t* __restrict__ roots, const int* __restrict__ boff, int nb,
+                            int* __restrict__ gbase) {
+    const int m = threadIdx.x;
+    if (m > nb) return;
+    const uint32_t x = (uint32_t)boff[m];
+    int lo = 0, hi = nroots;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (roots[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    gbase[m] = lo;
+}
+
+// Reduction groups in the reference's order: buckets ascending, roots ascending.
+__global__ void k_ftr_groups(int nroots, const uint32_t* __restrict__ roots, const uint32_t* __restrict__ perm,
+                             const int* __restrict__ boff, int nb, const int* __restrict__ memcnt,
+                             const int* __restrict__ memoff, int* __restrict__ grp_root, int* __restrict__ grp_moff,
+                             int* __restrict__ grp_mcnt, int* __restrict__ grp_m) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nroots) return;
+    const uint32_t r = roots[g];
+    grp_root[g] = (int)perm[r];
+    grp_moff[g] = memoff[r];
+    grp_mcnt[g] = memcnt[r];
+    grp_m[g] = bucket_of(boff, nb, (int)r);
 }
 
 // ------------------------------------------------------------------ group B-tables
