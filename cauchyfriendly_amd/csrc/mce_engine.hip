@@ -256,7 +256,7 @@ struct Engine {
     DBuf<uint32_t> barena, b2arena, tmpk;
     DBuf<cplx> scratch, tmpv;
     DBuf<StepConst> dconst;
-    DBuf<double> dbpert, gy, mom_part, mom_out;
+    DBuf<double> dbpert, gy, mom_part, mom_out, mom_x;
     DBuf<uint8_t> dcombos;
     DBuf<int64_t> dcombo_off;
     DBuf<int> dcombo_cnt, dcen_half, din_off;
@@ -717,9 +717,16 @@ struct Engine {
         HIPCHK(hipEventRecord(ev_mu, st));
         HIPCHK(hipStreamWaitEvent(st2, ev_mu, 0));
         if (prof.on) HIPCHK(hipEventRecord(ev_c0, st2));
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_chain<DD>, dim3(T, ceil_div(1 + d + d * d, kChainComps)), dim3(256),
-                                             0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr,
-                                             mom_part.ptr));
+        {
+            int ncp = 0, nyb = 0;
+            MCE_DISPATCH_D(d, (nyb = mom_blocks<DD>(), ncp = nyb * kSumCW / 2));
+            mom_x.reserve((size_t)nyb * nslots * kSumCW);
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_terms<DD>, dim3(ceil_div((int64_t)nslots * ncp, 256)), dim3(256),
+                                                 0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr, T,
+                                                 nslots, mom_x.ptr));
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
+                                                 chain_off.ptr, T, mom_part.ptr));
+        }
         HIPCHK(hipGetLastError());
         if (prof.on) HIPCHK(hipEventRecord(ev_c1, st2));
         cur_T = T;

@@ -787,109 +787,162 @@ __global__ void k_mom_final(const double* __restrict__ part, int nblk, double* _
 // i = t, t+T, ... of every shape bucket (cauchy_estimator.hpp:1575-1604) and adds,
 // per parent, the parent's own term before its children (cache_moments,
 // cauchy_estimator.hpp:307-338 and 1436-1468).  `chain` lists the output slots in
-// exactly that order (slots are parent-first), so chain t is one contiguous range.
-// One block per chain: tiles of 256 slot records are staged in LDS by all threads,
-// then lane c adds component c of every record in order, so the partial sums --
-// and hence fz, x_hat, P_hat and the g-table scale 1/(2 pi Re fz) -- are
-// bit-identical to the reference.
-constexpr int kChainComps = 16;  // moment components per chain block
+// exactly that order (slots are parent-first), so chain t is one contiguous range
+// [chain_off[t], chain_off[t+1]) of chain positions.
+//
+// Two kernels: k_mom_terms writes every summand of every chain position (fully
+// parallel), k_mom_sum adds them in order, one lane per double of a component.
+// Summands are grouped in blocks of kSumCW doubles: X[(yb * npos + pos) * kSumCW + c]
+// holds double yb * kSumCW + c (component / 2, real or imaginary part) of position
+// pos.  Padding and slots without a term hold +0.0: an in-order sum that starts at
+// +0.0 never becomes -0.0, so adding +0.0 leaves it unchanged.
+constexpr int kSumCW = 8;     // doubles per summand block (lanes of the adding wave)
+constexpr int kSumTP = 288;   // positions per LDS tile (tile = 12 doubles per loader thread)
+constexpr int kSumDepth = 3;  // tiles in flight in loader registers
 
 template <int D>
-__global__ __launch_bounds__(256) void k_mom_chain(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
-                                                   const uint32_t* __restrict__ chain,
-                                                   const int* __restrict__ chain_off, double* __restrict__ part) {
+__host__ __device__ constexpr int mom_blocks() {
+    return (2 * (1 + D + D * D) + kSumCW - 1) / kSumCW;
+}
+
+// Products with the reference's operations: g, g*y_j and the negated (g*y_j)*y_k
+// (the reference subtracts the latter; a - b == a + (-b) exactly).
+template <int D>
+__global__ void k_mom_terms(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
+                            const uint32_t* __restrict__ chain, const int* __restrict__ chain_off, int T, int nslots,
+                            double* __restrict__ X) {
     constexpr int NS = 2 + 2 * D;
     constexpr int NC = 1 + D + D * D;
-    constexpr int TILE = 256;
-    constexpr int PER = (TILE * NS + 255) / 256;  // record doubles per thread per tile
-    __shared__ double st[TILE * NS];
-    __shared__ double xr[kChainComps * TILE];
-    __shared__ double xi[kChainComps * TILE];
-    __shared__ uint8_t val[TILE];
+    constexpr int NCP = mom_blocks<D>() * kSumCW / 2;  // padded components per position
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int pos = (int)(i / NCP);
+    const int cc = (int)(i - (int64_t)pos * NCP);
+    if (pos >= nslots) return;
+    const int npos = chain_off[T];
+    if (pos >= npos) return;
+    const uint32_t s = chain[pos];
+    double2 out = make_double2(0.0, 0.0);
+    if (cc < NC && (flags[s] & kValid)) {
+        const double* rec = gy + (int64_t)s * NS;
+        const cplx g = cmake(rec[0], rec[1]);
+        cplx x;
+        if (cc == 0) {
+            x = g;
+        } else if (cc <= D) {
+            x = cmul(g, cmake(rec[2 + 2 * (cc - 1)], rec[3 + 2 * (cc - 1)]));
+        } else {
+            const int j = (cc - 1 - D) / D, kk = (cc - 1 - D) % D;
+            const cplx gyj = cmul(g, cmake(rec[2 + 2 * j], rec[3 + 2 * j]));
+            const cplx q = cmul(gyj, cmake(rec[2 + 2 * kk], rec[3 + 2 * kk]));
+            x = cmake(-q.re, -q.im);
+        }
+        out = make_double2(x.re, x.im);
+    }
+    const int c2 = 2 * cc;
+    const int yb = c2 / kSumCW, c = c2 % kSumCW;
+    *reinterpret_cast<double2*>(X + ((int64_t)yb * npos + pos) * kSumCW + c) = out;
+}
+
+// In-order sums.  Grid (T, mom_blocks), 256 threads: wave 0 adds, lane c < kSumCW
+// owning double blockIdx.y * kSumCW + c; waves 1-3 stream the chain's summand block
+// through a two-tile LDS ring, holding kSumDepth tiles in registers so that the
+// loads of a tile are in flight while the tiles before it are added.  part[t][c2]
+// receives the sum of chain t.
+template <int D>
+__global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, const int* __restrict__ chain_off,
+                                                 int T, double* __restrict__ part) {
+    constexpr int NC2 = 2 * (1 + D + D * D);
+    constexpr int TE = kSumTP * kSumCW;          // doubles per tile
+    constexpr int NL = 192;                      // loader threads
+    constexpr int PER = TE / NL;                 // doubles per loader thread per tile
+    static_assert(PER * NL == TE, "a tile is an exact multiple of the loader threads");
+    __shared__ double ring[2][TE];
     const int t = blockIdx.x;
-    const int c0 = blockIdx.y * kChainComps;
+    const int yb = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int npos = chain_off[T];
     const int beg = chain_off[t], end = chain_off[t + 1];
-    const int tx = threadIdx.x;
-    double acc_re = 0.0, acc_im = 0.0;
-    // register prefetch of one tile
-    double pre[PER];
-    uint8_t pre_v = 0;
-    auto fetch = [&](int base) {
-        const int n = min(TILE, end - base);
+    const int ntiles = (end - beg + kSumTP - 1) / kSumTP;
+    const double* Xb = X + ((int64_t)yb * npos + beg) * kSumCW;  // this chain's block, tile 0 first
+    const int64_t lim = (int64_t)(end - beg) * kSumCW;           // doubles of the chain
+    const bool loader = __builtin_amdgcn_readfirstlane(tid >> 6) != 0;  // wave 0 adds
+    const int l = tid - 64;
+    double rg[kSumDepth][PER];
+
+    // Loads are unconditional (address clamped into the chain) so that no load is
+    // skipped and the wait for the oldest tile stays exact; values past the chain's
+    // end are replaced by +0.0 when the tile is stored to LDS, not at load time,
+    // so the loads stay in flight until then.
+    auto gload = [&](double (&r)[PER], int tile) {
+#pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = tx + q * 256;
-            const int r = e / NS, w = e - r * NS;
-            pre[q] = (r < n) ? gy[(int64_t)chain[base + r] * NS + w] : 0.0;
+            const int64_t o = (int64_t)tile * TE + l + NL * q;
+            r[q] = Xb[o < lim ? o : lim - 1];
         }
-        pre_v = (tx < n) ? (flags[chain[base + tx]] & kValid) : 0;
     };
-    if (beg < end) fetch(beg);
-    for (int base = beg; base < end; base += TILE) {
-        const int n = min(TILE, end - base);
+    auto lstore = [&](const double (&r)[PER], int tile, double* dst) {
+#pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = tx + q * 256;
-            if (e < TILE * NS) st[e] = pre[q];
+            const int64_t o = (int64_t)tile * TE + l + NL * q;
+            dst[l + NL * q] = o < lim ? r[q] : 0.0;
         }
-        val[tx] = pre_v;
-        __syncthreads();
-        if (base + TILE < end) fetch(base + TILE);  // in flight during this tile's work
-        // products with the reference's operations: g, g*y_j and the negated (g*y_j)*y_k
-        // (the reference subtracts it; a - b == a + (-b) exactly)
-        if (tx < n) {
-            const double* rec = st + tx * NS;
-            const cplx g = cmake(rec[0], rec[1]);
-            for (int q = 0; q < kChainComps; q++) {
-                const int cc = c0 + q;
-                if (cc >= NC) break;
-                cplx x;
-                if (cc == 0) {
-                    x = g;
-                } else if (cc <= D) {
-                    x = cmul(g, cmake(rec[2 + 2 * (cc - 1)], rec[3 + 2 * (cc - 1)]));
-                } else {
-                    const int j = (cc - 1 - D) / D, kk = (cc - 1 - D) % D;
-                    const cplx gyj = cmul(g, cmake(rec[2 + 2 * j], rec[3 + 2 * j]));
-                    const cplx p = cmul(gyj, cmake(rec[2 + 2 * kk], rec[3 + 2 * kk]));
-                    x = cmake(-p.re, -p.im);
-                }
-                xr[q * TILE + tx] = x.re;
-                xi[q * TILE + tx] = x.im;
-            }
-        }
-        __syncthreads();
-        // the in-order add chain, one lane per component
-        if (tx < kChainComps && c0 + tx < NC) {
-            const double* pr = xr + tx * TILE;
-            const double* pi = xi + tx * TILE;
-            int r = 0;
-            for (; r + 8 <= n; r += 8) {
-                double a[8], b[8];
-                bool v[8];
+    };
+    // LDS-only barrier: global loads stay in flight across it
+    auto lds_barrier = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+
+    // The two roles run separate loops; each performs exactly nsteps + 1 barriers.
+    // The step count is padded to a multiple of kSumDepth so that every unrolled
+    // step is unconditional (the compiler then keeps the later tiles' loads in
+    // flight); padded tiles hold +0.0 only.
+    const int nsteps = (ntiles + kSumDepth - 1) / kSumDepth * kSumDepth;
+    double acc = 0.0;
+    const int lane = tid & 63;
+    if (loader) {
+        if (nsteps > 0) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    a[q] = pr[r + q];
-                    b[q] = pi[r + q];
-                    v[q] = val[r + q] != 0;
-                }
+            for (int j = 0; j < kSumDepth; j++) gload(rg[j], j);
+            lstore(rg[0], 0, ring[0]);
+            gload(rg[0], kSumDepth);
+        }
+        lds_barrier();
+        for (int k0 = 0; k0 < nsteps; k0 += kSumDepth) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    acc_re = v[q] ? acc_re + a[q] : acc_re;
-                    acc_im = v[q] ? acc_im + b[q] : acc_im;
-                }
-            }
-            for (; r < n; r++) {
-                const double a = pr[r], b = pi[r];
-                const bool v = val[r] != 0;
-                acc_re = v ? acc_re + a : acc_re;
-                acc_im = v ? acc_im + b : acc_im;
+            for (int u = 0; u < kSumDepth; u++) {
+                const int k = k0 + u;
+                // u is a compile-time index once the loop is unrolled
+                lstore(rg[(u + 1) % kSumDepth], k + 1, ring[(k + 1) & 1]);
+                gload(rg[(u + 1) % kSumDepth], k + 1 + kSumDepth);
+                lds_barrier();
             }
         }
-        __syncthreads();
+    } else {
+        lds_barrier();
+        for (int k = 0; k < nsteps; k++) {
+            if (lane < kSumCW) {
+                const double* src = ring[k & 1] + lane;
+                double cur[16], nxt[16];
+#pragma unroll
+                for (int q = 0; q < 16; q++) cur[q] = src[q * kSumCW];
+                for (int r0 = 0; r0 < kSumTP; r0 += 16) {
+                    if (r0 + 16 < kSumTP) {
+#pragma unroll
+                        for (int q = 0; q < 16; q++) nxt[q] = src[(r0 + 16 + q) * kSumCW];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; q++) acc += cur[q];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) cur[q] = nxt[q];
+                }
+            }
+            lds_barrier();
+        }
     }
-    if (tx < kChainComps && c0 + tx < NC) {
-        part[((int64_t)t * NC + c0 + tx) * 2] = acc_re;
-        part[((int64_t)t * NC + c0 + tx) * 2 + 1] = acc_im;
-    }
+    const int c2 = yb * kSumCW + lane;
+    if (!loader && lane < kSumCW && c2 < NC2) part[(int64_t)t * NC2 + c2] = acc;
 }
 
 // Histogram of small integer values privatised in LDS (nbins <= 64).  mode 0: valid
