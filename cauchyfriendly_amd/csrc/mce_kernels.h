@@ -5,8 +5,9 @@
 //   k_btab_simple      B-tables that need no enumeration (copy / identity / empty)
 //   k_btab_enum        time-propagation B-table by vertex enumeration, one block per parent
 //   k_mu               measurement update, one thread per output slot (parent x child)
-//   k_mom_partial/final  deterministic FP64 tree reduction of fz / mean / covariance
-//   k_ftr_*            fast term reduction per shape bucket
+//   k_mom_terms/sum    fz / mean / covariance summands, then in-order sums per
+//                      reference thread (second stream, overlapped with reduction)
+//   k_ftr_*            fast term reduction of all shape buckets in single launches
 //   k_group_btab       B-table of each reduction group's root, one wavefront per group
 //   k_group_eval       g-tables of each group: root, replacement chain, member sums
 //   k_group_sort       key order for groups whose root was replaced
@@ -733,56 +734,6 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
 }
 
 // ------------------------------------------------------------------ moments
-// Per-block partial sums of fz, mean, covariance over 256 slots, summed in slot order.
-template <int D>
-__global__ __launch_bounds__(256) void k_mom_partial(const double* __restrict__ gy, int nslots,
-                                                     const uint8_t* __restrict__ flags, double* __restrict__ part) {
-    constexpr int NS = 2 + 2 * D;
-    constexpr int NC = 1 + D + D * D;
-    __shared__ double st[256 * NS];
-    __shared__ int valid[256];
-    const int base = blockIdx.x * 256;
-    const int s = base + threadIdx.x;
-    valid[threadIdx.x] = (s < nslots) && (flags[s] & kValid);
-    for (int k = 0; k < NS; k++) st[threadIdx.x * NS + k] = (s < nslots) ? gy[(int64_t)s * NS + k] : 0.0;
-    __syncthreads();
-    for (int c = threadIdx.x; c < NC; c += blockDim.x) {
-        cplx acc = cmake(0, 0);
-        for (int r = 0; r < 256; r++) {
-            if (!valid[r]) continue;
-            const double* e = st + r * NS;
-            const cplx g = cmake(e[0], e[1]);
-            if (c == 0) {
-                acc = cadd(acc, g);
-            } else if (c <= D) {
-                const cplx y = cmake(e[2 + 2 * (c - 1)], e[3 + 2 * (c - 1)]);
-                acc = cadd(acc, cmul(g, y));
-            } else {
-                const int jk = c - 1 - D, j = jk / D, k = jk % D;
-                const cplx yj = cmake(e[2 + 2 * j], e[3 + 2 * j]);
-                const cplx yk = cmake(e[2 + 2 * k], e[3 + 2 * k]);
-                acc = csub(acc, cmul(cmul(g, yj), yk));
-            }
-        }
-        part[((int64_t)blockIdx.x * NC + c) * 2] = acc.re;
-        part[((int64_t)blockIdx.x * NC + c) * 2 + 1] = acc.im;
-    }
-}
-
-template <int D>
-__global__ void k_mom_final(const double* __restrict__ part, int nblk, double* __restrict__ out) {
-    constexpr int NC = 1 + D + D * D;
-    for (int c = threadIdx.x; c < NC; c += blockDim.x) {
-        double re = 0, im = 0;
-        for (int b = 0; b < nblk; b++) {
-            re += part[((int64_t)b * NC + c) * 2];
-            im += part[((int64_t)b * NC + c) * 2 + 1];
-        }
-        out[2 * c] = re;
-        out[2 * c + 1] = im;
-    }
-}
-
 // Moment sums in the reference's exact order.  Reference thread t owns parents
 // i = t, t+T, ... of every shape bucket (cauchy_estimator.hpp:1575-1604) and adds,
 // per parent, the parent's own term before its children (cache_moments,
@@ -972,12 +923,6 @@ __global__ void k_hist(int n, int nbins, int mode, const uint8_t* __restrict__ f
 __global__ void k_iota(uint32_t* __restrict__ v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = (uint32_t)i;
-}
-
-__global__ void k_shape_hist(const TermPool OUT, int n, int* __restrict__ hist) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (OUT.flags[i] & kValid) atomicAdd(&hist[OUT.m[i]], 1);
 }
 
 // ------------------------------------------------------------------ fast term reduction
@@ -1715,13 +1660,6 @@ __global__ void k_gtable_first(const TermPool OUT, int n, const uint32_t* __rest
     NEXT.gp_n[r] = cells;
 }
 
-// Per-shape counts of the terms a last update leaves (no reduction follows).
-__global__ void k_skip_counts(const TermPool OUT, int n, int* __restrict__ hist) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    if (OUT.flags[s] & kValid) atomicAdd(&hist[OUT.m[s]], 1);
-}
-
 // Algorithmic bytes of k_group_eval: per group, the B-table keys (4 B/cell) and the
 // output table (key + complex value, 20 B/cell); per evaluated term (root and every
 // member), its record (q, p, A rows, scalars, coalignment maps) and, per cell, the two
@@ -1758,10 +1696,6 @@ __global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, 
 __global__ void k_surv_flags(int n, const int* __restrict__ surv, int* __restrict__ f) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) f[i] = surv[i] >= 0 ? 1 : 0;
-}
-__global__ void k_surv_hist(int n, const int* __restrict__ surv, const int* __restrict__ grp_m, int* __restrict__ h) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && surv[i] >= 0) atomicAdd(&h[grp_m[i]], 1);
 }
 template <int D>
 __global__ void k_det_tp(TermPool P, int n, const double* __restrict__ Tr, const double* __restrict__ bias) {
