@@ -249,6 +249,7 @@ struct Engine {
     int cur_T = 1;
     bool moments_pending = false;
     int max_gp_cells = 1;  // largest parent g-table of the current parents
+    int geval_mode = -1;   // parent lookup mode of the last g-table launch
     PoolStore in, out;
     DBuf<uint32_t> gk[2];
     DBuf<cplx> gv[2];
@@ -893,20 +894,45 @@ struct Engine {
             HIPCHK(hipEventRecord(prof.k0, st));
         }
         {
-            const size_t lds = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
-            if (lds <= 48 * 1024) {
-                HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(k_group_eval<true>, dim3(ng), dim3(64), lds, st, OUT, ng, d, grp_root.ptr,
-                                   grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,
-                                   b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,
-                                   gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+            // parent lookups: rank structure over the 2^(MM-1) half-storage keys when it
+            // fits in LDS (with the parent values too when those fit), else binary search
+            const int wmax = std::max(2, (((1 << (MM - 1)) + 31) >> 5) + 1) & ~1;
+            const size_t rank_b = (size_t)wmax * 8;
+            const size_t vals_b = (size_t)std::max(max_gp_cells, 1) * sizeof(cplx);
+            const size_t keys_b = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
+            const size_t budget = 48 * 1024;
+            int mode;
+            size_t lds;
+            if (MM <= 24 && rank_b + vals_b <= budget) {
+                mode = kLookupRankVals;
+                lds = rank_b + vals_b;
+            } else if (MM <= 24 && rank_b <= budget) {
+                mode = kLookupRank;
+                lds = rank_b;
+            } else if (keys_b <= budget) {
+                mode = kLookupSearchLds;
+                lds = keys_b;
             } else {
-                hipLaunchKernelGGL(k_group_eval<false>, dim3(ng), dim3(64), 0, st, OUT, ng, d, grp_root.ptr,
-                                   grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,
-                                   b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,
-                                   gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);
+                mode = kLookupSearch;
+                lds = 0;
             }
+            geval_mode = mode;
+#define MCE_GEVAL(M_)                                                                                           \
+    do {                                                                                                        \
+        HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<M_>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                   (int)std::max<size_t>(lds, 16)));                                           \
+        hipLaunchKernelGGL(k_group_eval<M_>, dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,       \
+                           grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,         \
+                           b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,   \
+                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);       \
+    } while (0)
+            switch (mode) {
+                case kLookupRankVals: MCE_GEVAL(kLookupRankVals); break;
+                case kLookupRank: MCE_GEVAL(kLookupRank); break;
+                case kLookupSearchLds: MCE_GEVAL(kLookupSearchLds); break;
+                default: MCE_GEVAL(kLookupSearch); break;
+            }
+#undef MCE_GEVAL
         }
         HIPCHK(hipGetLastError());
         if (prof.on) {
@@ -924,6 +950,7 @@ struct Engine {
         }
         const int nsort = read1(counters.ptr + 2);
         const int rare = read1(counters.ptr + 3);
+        if (rare & 4) throw std::runtime_error("parent g-table key outside its half-storage range");
         if (rare) fprintf(stderr, "mce: a parent term joined a reduction group (flag %d); treated as a new child\n", rare);
         if (nsort > 0) {
             tmpk.reserve((size_t)cells_total + 1);

@@ -1321,11 +1321,24 @@ __global__ void k_group_btab(const TermPool OUT, int g0, int ngb, int m, int d, 
 }
 
 // ------------------------------------------------------------------ group g-tables
+// Parent g-table lookup modes of the g-table kernel.
+//   kLookupSearch     binary search over the parent keys in global memory
+//   kLookupSearchLds  binary search over an LDS copy of the parent keys
+//   kLookupRank       rank structure in LDS (bitset of the parent keys + per-word
+//                     prefix counts), values read from global memory
+//   kLookupRankVals   rank structure and an LDS copy of the parent values
+// Every mode finds the same entry; they differ only in where the index comes from.
+enum LookupMode { kLookupSearch = 0, kLookupSearchLds = 1, kLookupRank = 2, kLookupRankVals = 3 };
+
 // Per-term data of one g-table evaluation, staged once per evaluated term: its
-// q, coalignment maps and its parent g-table keys live in LDS, the rest in registers.
+// q, coalignment maps and its parent g-table (keys, or rank structure and values)
+// live in LDS, the rest in registers.
 struct EvalTerm {
     const uint32_t* pkeys;  // parent g-table keys (LDS copy or global)
-    const cplx* pvals;      // parent g-table values (global)
+    const cplx* pvals;      // parent g-table values (LDS copy or global)
+    const uint32_t* rbits;  // rank structure: key bitset
+    const uint32_t* rpre;   // rank structure: keys in the words before each word
+    uint32_t klim;          // 2^(phc-1): every stored key is below it (half storage)
     int pn;                 // parent g-table size
     int m, phc, z, lhp;
     uint32_t hf;
@@ -1333,7 +1346,24 @@ struct EvalTerm {
     double c, dv, gscale;
 };
 
+// Half-storage g numerator lookup in the parent g-table, eval_gs.hpp:94-153.
+template <int MODE>
+__device__ inline cplx lookup_g(const EvalTerm& T, int enc, int top_bit, int rev_mask) {
+    if (MODE >= kLookupRank) {
+        const bool cj = (enc & top_bit) != 0;
+        const uint32_t key = (uint32_t)(cj ? (rev_mask ^ enc) : enc);
+        if (key >= T.klim) return cmake(0, 0);
+        const uint32_t wv = T.rbits[key >> 5];
+        const uint32_t bit = 1u << (key & 31);
+        if (!(wv & bit)) return cmake(0, 0);
+        const cplx v = T.pvals[T.rpre[key >> 5] + __popc(wv & (bit - 1u))];
+        return cj ? cconj(v) : v;
+    }
+    return lookup_num(enc, top_bit, rev_mask, T.pkeys, T.pvals, T.pn);
+}
+
 // g value of one term at one cell (make_gtable loop body, flattening.hpp:129-248).
+template <int MODE>
 __device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8_t* cm, const int8_t* cs,
                                  uint32_t key) {
     double ygi = 0;
@@ -1367,8 +1397,8 @@ __device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8
             l++;
         }
     }
-    const cplx a = lookup_num(lp ^ T.lhp, top_phc, rev_phc, T.pkeys, T.pvals, T.pn);
-    const cplx bm = lookup_num(lmv ^ T.lhp, top_phc, rev_phc, T.pkeys, T.pvals, T.pn);
+    const cplx a = lookup_g<MODE>(T, lp ^ T.lhp, top_phc, rev_phc);
+    const cplx bm = lookup_g<MODE>(T, lmv ^ T.lhp, top_phc, rev_phc);
     const cplx v = csub(cdiv(a, cmake(ygi + T.dv, T.c)), cdiv(bm, cmake(ygi - T.dv, T.c)));
     return cscale(v, T.gscale);
 }
@@ -1380,15 +1410,20 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
     return ps * ps;
 }
 
-// Stages term s for evaluation (one wavefront, blockDim == 64).
-template <bool LDS_KEYS>
+// Stages term s for evaluation (one wavefront, blockDim == 64).  `lds` is the
+// kernel's dynamic LDS: parent keys (kLookupSearchLds), or the rank structure of
+// `wmax` words of bits and `wmax` prefix counts followed by the parent values.
+template <int MODE>
 __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const uint32_t* gk_prev,
-                                      const cplx* gv_prev, double gscale, uint32_t* lds_keys, double* q_s,
-                                      uint8_t* cm_s, int8_t* cs_s) {
+                                      const cplx* gv_prev, double gscale, uint32_t* lds, int wmax, double* q_s,
+                                      uint8_t* cm_s, int8_t* cs_s, int* rare) {
     EvalTerm T;
     const int64_t go = O.gp_off[s];
     T.pn = O.gp_n[s];
     T.pvals = gv_prev + go;
+    T.pkeys = gk_prev + go;
+    T.rbits = nullptr;
+    T.rpre = nullptr;
     T.m = m;
     T.phc = O.phc[s];
     T.z = O.z[s];
@@ -1400,14 +1435,44 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
     T.c = O.c[s];
     T.dv = O.dv[s];
     T.gscale = gscale;
+    T.klim = 1u << (T.phc - 1);
     __syncthreads();
     const int lane = threadIdx.x;
-    if (LDS_KEYS) {
+    if (MODE == kLookupSearchLds) {
         const uint32_t* src = gk_prev + go;
-        for (int i = lane; i < T.pn; i += kWave) lds_keys[i] = src[i];
-        T.pkeys = lds_keys;
-    } else {
-        T.pkeys = gk_prev + go;
+        for (int i = lane; i < T.pn; i += kWave) lds[i] = src[i];
+        T.pkeys = lds;
+    } else if (MODE >= kLookupRank) {
+        // keys are sorted ascending and below klim; W words cover them
+        const uint32_t* src = gk_prev + go;
+        const int W = (int)((T.klim + 31u) >> 5);
+        uint32_t* bits = lds;
+        uint32_t* pre = lds + wmax;
+        for (int w = lane; w < W; w += kWave) bits[w] = 0;
+        __syncthreads();
+        bool bad = false;
+        for (int i = lane; i < T.pn; i += kWave) {
+            const uint32_t key = src[i];
+            if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
+            else bad = true;
+            // pre[x] = index of the first key in word >= x, filled for the words
+            // between the previous key's word and this key's word
+            const int w = min((int)(key >> 5), W - 1);
+            const int wp = i ? min((int)(src[i - 1] >> 5), W - 1) : -1;
+            for (int x = wp + 1; x <= w; x++) pre[x] = i;
+        }
+        const int lastw = T.pn ? min((int)(src[T.pn - 1] >> 5), W - 1) : -1;
+        for (int w = lane; w < W; w += kWave)
+            if (w > lastw) pre[w] = T.pn;
+        if (bad) atomicOr(rare, 4);
+        T.rbits = bits;
+        T.rpre = pre;
+        if (MODE == kLookupRankVals) {
+            cplx* dst = reinterpret_cast<cplx*>(lds + 2 * wmax);
+            const cplx* vs = gv_prev + go;
+            for (int i = lane; i < T.pn; i += kWave) dst[i] = vs[i];
+            T.pvals = dst;
+        }
     }
     const int MM = O.MM;
     if (lane < m) q_s[lane] = O.q[s * MM + lane];
@@ -1426,8 +1491,9 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
 // replacement); then every later non-negligible member's g-table is added into
 // the root's in member order (add_gtables, flattening.hpp:259-315).
 // One wavefront per group; lanes own cells.
-template <bool LDS_KEYS>
-__global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, const int* __restrict__ grp_root,
+template <int MODE>
+__global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, int wmax,
+                                                  const int* __restrict__ grp_root,
                                                   const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
                                                   const int* __restrict__ grp_m, const int* __restrict__ mem_slots,
                                                   const uint32_t* __restrict__ B2, const int64_t* __restrict__ b2_off,
@@ -1437,7 +1503,7 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                                                   cplx* __restrict__ scratch, double gscale, int* __restrict__ surv,
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
                                                   int* __restrict__ sort_count, int* __restrict__ rare) {
-    extern __shared__ uint32_t lds_keys[];
+    extern __shared__ __align__(16) uint32_t lds_dyn[];
     __shared__ double q_s[kMaxM];
     __shared__ uint8_t cm_s[kMaxM];
     __shared__ int8_t cs_s[kMaxM];
@@ -1458,11 +1524,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     int r = grp_root[g];
     uint32_t X = 0;
     {
-        EvalTerm T = stage_term<LDS_KEYS>(OUT, r, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
+        EvalTerm T = stage_term<MODE>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
         const double ps = dev_psum2(OUT, r, m);
         bool small = true;
         for (int i = lane; i < n; i += kWave) {
-            const cplx v = eval_cell(T, q_s, cm_s, cs_s, B[i]);
+            const cplx v = eval_cell<MODE>(T, q_s, cm_s, cs_s, B[i]);
             ov[i] = v;
             if ((ps * hypot(v.re, v.im)) > kApproxEps) small = false;
         }
@@ -1478,11 +1544,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 int sig = orientation_mask(OUT.A + (int64_t)lfr * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
                 if ((uint32_t)sig & top) sig ^= (int)rev;
                 X ^= (uint32_t)sig;
-                EvalTerm Tk = stage_term<LDS_KEYS>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
+                EvalTerm Tk = stage_term<MODE>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
                 for (int i = lane; i < n; i += kWave) {
-                    const cplx v = eval_cell(Tk, q_s, cm_s, cs_s, B[i] ^ X);
+                    const cplx v = eval_cell<MODE>(Tk, q_s, cm_s, cs_s, B[i] ^ X);
                     ov[i] = v;
                     if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
                 }
@@ -1504,11 +1570,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
             const int sig = orientation_mask(Ar, OUT.A + (int64_t)sk * MM * d, m, d);
             const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
-            EvalTerm Tk = stage_term<LDS_KEYS>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_keys, q_s, cm_s, cs_s);
+            EvalTerm Tk = stage_term<MODE>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
             const double psk = dev_psum2(OUT, sk, m);
             bool sm = true;
             for (int i = lane; i < n; i += kWave) {
-                const cplx v = eval_cell(Tk, q_s, cm_s, cs_s, B[i] ^ X ^ sh);
+                const cplx v = eval_cell<MODE>(Tk, q_s, cm_s, cs_s, B[i] ^ X ^ sh);
                 sc[i] = v;
                 if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
             }
