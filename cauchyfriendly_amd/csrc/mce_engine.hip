@@ -917,21 +917,23 @@ struct Engine {
                 lds = 0;
             }
             geval_mode = mode;
-#define MCE_GEVAL(M_)                                                                                           \
-    do {                                                                                                        \
-        HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<M_>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                   (int)std::max<size_t>(lds, 16)));                                           \
-        hipLaunchKernelGGL(k_group_eval<M_>, dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,       \
-                           grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,         \
-                           b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,   \
-                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);       \
+#define MCE_GEVAL(M_, K_)                                                                                        \
+    do {                                                                                                          \
+        HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<M_, K_>,                                             \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 16)));  \
+        hipLaunchKernelGGL((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
+                           grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
+                           b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
+                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);         \
     } while (0)
-            switch (mode) {
-                case kLookupRankVals: MCE_GEVAL(kLookupRankVals); break;
-                case kLookupRank: MCE_GEVAL(kLookupRank); break;
-                case kLookupSearchLds: MCE_GEVAL(kLookupSearchLds); break;
-                default: MCE_GEVAL(kLookupSearch); break;
-            }
+            // rows held in registers: the smallest compile-time bound >= the largest shape
+            if (mode == kLookupRankVals && MM <= 8) MCE_GEVAL(kLookupRankVals, 8);
+            else if (mode == kLookupRankVals && MM <= 16) MCE_GEVAL(kLookupRankVals, 16);
+            else if (mode == kLookupRankVals) MCE_GEVAL(kLookupRankVals, 32);
+            else if (mode == kLookupRank && MM <= 16) MCE_GEVAL(kLookupRank, 16);
+            else if (mode == kLookupRank) MCE_GEVAL(kLookupRank, 32);
+            else if (mode == kLookupSearchLds) MCE_GEVAL(kLookupSearchLds, 32);
+            else MCE_GEVAL(kLookupSearch, 32);
 #undef MCE_GEVAL
         }
         HIPCHK(hipGetLastError());

@@ -1363,13 +1363,15 @@ __device__ inline cplx lookup_g(const EvalTerm& T, int enc, int top_bit, int rev
 }
 
 // g value of one term at one cell (make_gtable loop body, flattening.hpp:129-248).
-template <int MODE>
-__device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8_t* cm, const int8_t* cs,
+// q is held in registers (KM >= m, compile-time bound); ygi is summed in row
+// order as in the reference, with q * (-1) written as -q (the same value).
+template <int MODE, int KM>
+__device__ inline cplx eval_cell(const EvalTerm& T, const double (&qr)[KM], const uint8_t* cm, const int8_t* cs,
                                  uint32_t key) {
     double ygi = 0;
-    for (int k = 0; k < T.m; k++) {
-        const double sb = ((key >> k) & 1u) ? -1 : 1;
-        if (!((T.hf >> k) & 1u)) ygi += q[k] * sb;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        if (k < T.m && !((T.hf >> k) & 1u)) ygi += ((key >> k) & 1u) ? -qr[k] : qr[k];
     }
     const int phc = T.phc;
     const int rev_phc = (phc >= 32) ? -1 : ((1 << phc) - 1);
@@ -1378,6 +1380,18 @@ __device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8
     if (!T.is_new) {
         lp = (int)key & rev_phc;
         lmv = lp;
+    } else if (!T.has_cmap) {
+        // the child's key with bit z of the parent inserted: parent bit k is child
+        // bit k below z and child bit k-1 above it; bit z is 0 in lp and 1 in lmv
+        if (T.z < phc) {
+            const uint32_t low = key & ((1u << T.z) - 1u);
+            const uint32_t high = (key >> T.z) << (T.z + 1);
+            lp = (int)((low | high) & (uint32_t)rev_phc);
+            lmv = lp | (1 << T.z);
+        } else {
+            lp = (int)key & rev_phc;
+            lmv = lp;
+        }
     } else {
         int k = 0, l = 0;
         while (k < phc) {
@@ -1386,9 +1400,7 @@ __device__ inline cplx eval_cell(const EvalTerm& T, const double* q, const uint8
                 k++;
                 if (k == phc) break;
             }
-            int bv;
-            if (T.has_cmap) bv = (((key >> cm[l]) & 1u) ? -1 : 1) * (int)cs[l];
-            else bv = ((key >> l) & 1u) ? -1 : 1;
+            const int bv = (((key >> cm[l]) & 1u) ? -1 : 1) * (int)cs[l];
             if (bv < 0) {
                 lp |= (1 << k);
                 lmv |= (1 << k);
@@ -1413,9 +1425,9 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
 // Stages term s for evaluation (one wavefront, blockDim == 64).  `lds` is the
 // kernel's dynamic LDS: parent keys (kLookupSearchLds), or the rank structure of
 // `wmax` words of bits and `wmax` prefix counts followed by the parent values.
-template <int MODE>
+template <int MODE, int KM>
 __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const uint32_t* gk_prev,
-                                      const cplx* gv_prev, double gscale, uint32_t* lds, int wmax, double* q_s,
+                                      const cplx* gv_prev, double gscale, uint32_t* lds, int wmax, double (&qr)[KM],
                                       uint8_t* cm_s, int8_t* cs_s, int* rare) {
     EvalTerm T;
     const int64_t go = O.gp_off[s];
@@ -1475,7 +1487,8 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         }
     }
     const int MM = O.MM;
-    if (lane < m) q_s[lane] = O.q[s * MM + lane];
+#pragma unroll
+    for (int kk = 0; kk < KM; kk++) qr[kk] = (kk < m) ? O.q[s * MM + kk] : 0.0;
     const int pbc = O.pbc[s];
     if (T.has_cmap && lane < pbc) {
         cm_s[lane] = O.cmap[s * MM + lane];
@@ -1491,7 +1504,7 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
 // replacement); then every later non-negligible member's g-table is added into
 // the root's in member order (add_gtables, flattening.hpp:259-315).
 // One wavefront per group; lanes own cells.
-template <int MODE>
+template <int MODE, int KM>
 __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, int wmax,
                                                   const int* __restrict__ grp_root,
                                                   const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
@@ -1504,7 +1517,7 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
                                                   int* __restrict__ sort_count, int* __restrict__ rare) {
     extern __shared__ __align__(16) uint32_t lds_dyn[];
-    __shared__ double q_s[kMaxM];
+    double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
     __shared__ int8_t cs_s[kMaxM];
     const int g = blockIdx.x;
@@ -1524,11 +1537,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     int r = grp_root[g];
     uint32_t X = 0;
     {
-        EvalTerm T = stage_term<MODE>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
+        EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
         const double ps = dev_psum2(OUT, r, m);
         bool small = true;
         for (int i = lane; i < n; i += kWave) {
-            const cplx v = eval_cell<MODE>(T, q_s, cm_s, cs_s, B[i]);
+            const cplx v = eval_cell<MODE, KM>(T, qr, cm_s, cs_s, B[i]);
             ov[i] = v;
             if ((ps * hypot(v.re, v.im)) > kApproxEps) small = false;
         }
@@ -1544,11 +1557,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 int sig = orientation_mask(OUT.A + (int64_t)lfr * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
                 if ((uint32_t)sig & top) sig ^= (int)rev;
                 X ^= (uint32_t)sig;
-                EvalTerm Tk = stage_term<MODE>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
+                EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
                 for (int i = lane; i < n; i += kWave) {
-                    const cplx v = eval_cell<MODE>(Tk, q_s, cm_s, cs_s, B[i] ^ X);
+                    const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X);
                     ov[i] = v;
                     if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
                 }
@@ -1570,11 +1583,11 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
             const int sig = orientation_mask(Ar, OUT.A + (int64_t)sk * MM * d, m, d);
             const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
-            EvalTerm Tk = stage_term<MODE>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, q_s, cm_s, cs_s, rare);
+            EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
             const double psk = dev_psum2(OUT, sk, m);
             bool sm = true;
             for (int i = lane; i < n; i += kWave) {
-                const cplx v = eval_cell<MODE>(Tk, q_s, cm_s, cs_s, B[i] ^ X ^ sh);
+                const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
                 sc[i] = v;
                 if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
             }
