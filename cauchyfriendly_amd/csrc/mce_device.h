@@ -43,16 +43,18 @@ __device__ __host__ inline cplx cscale(cplx a, double s) { return cmake(a.re * s
 // Complex quotient by Smith's method (R. L. Smith, "Algorithm 116: Complex
 // division", CACM 5(8), 1962).  This is the scheme the reference's compiler
 // runtime uses for `a / b` on GCC complex doubles, so quotients match it bit for bit.
+// Both of Smith's branches are evaluated with selects (no divergence): every
+// quotient, product and sum is the one the taken branch computes.
 __device__ __host__ inline cplx cdiv(cplx a, cplx b) {
     const double c = b.re, d = b.im;
-    if (fabs(c) < fabs(d)) {
-        const double r = c / d;
-        const double den = (c * r) + d;
-        return cmake(((a.re * r) + a.im) / den, ((a.im * r) - a.re) / den);
-    }
-    const double r = d / c;
-    const double den = (d * r) + c;
-    return cmake(((a.im * r) + a.re) / den, (a.im - (a.re * r)) / den);
+    const bool small_c = fabs(c) < fabs(d);
+    const double p = small_c ? c : d, q = small_c ? d : c;
+    const double r = p / q;
+    const double den = (p * r) + q;
+    const double pr = a.re * r, pi = a.im * r;
+    const double xn = small_c ? (pr + a.im) : (pi + a.re);
+    const double yn = small_c ? (pi - a.re) : (a.im - pr);
+    return cmake(xn / den, yn / den);
 }
 
 // Binary search over an ascending key array, gtable.hpp:283-301.

@@ -928,7 +928,10 @@ struct Engine {
     } while (0)
             // rows held in registers: the smallest compile-time bound >= the largest shape
             if (mode == kLookupRankVals && MM <= 8) MCE_GEVAL(kLookupRankVals, 8);
+            else if (mode == kLookupRankVals && MM <= 11) MCE_GEVAL(kLookupRankVals, 11);
+            else if (mode == kLookupRankVals && MM <= 14) MCE_GEVAL(kLookupRankVals, 14);
             else if (mode == kLookupRankVals && MM <= 16) MCE_GEVAL(kLookupRankVals, 16);
+            else if (mode == kLookupRankVals && MM <= 24) MCE_GEVAL(kLookupRankVals, 24);
             else if (mode == kLookupRankVals) MCE_GEVAL(kLookupRankVals, 32);
             else if (mode == kLookupRank && MM <= 16) MCE_GEVAL(kLookupRank, 16);
             else if (mode == kLookupRank) MCE_GEVAL(kLookupRank, 32);

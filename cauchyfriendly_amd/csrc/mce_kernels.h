@@ -1339,6 +1339,7 @@ struct EvalTerm {
     const uint32_t* rbits;  // rank structure: key bitset
     const uint32_t* rpre;   // rank structure: keys in the words before each word
     uint32_t klim;          // 2^(phc-1): every stored key is below it (half storage)
+    uint32_t fm;            // rows that enter ygi: below m and not H-orthogonal
     int pn;                 // parent g-table size
     int m, phc, z, lhp;
     uint32_t hf;
@@ -1350,28 +1351,39 @@ struct EvalTerm {
 template <int MODE>
 __device__ inline cplx lookup_g(const EvalTerm& T, int enc, int top_bit, int rev_mask) {
     if (MODE >= kLookupRank) {
+        // branchless: both LDS reads are issued together, the value read uses
+        // entry 0 when the key is absent and the result is then replaced by zero
         const bool cj = (enc & top_bit) != 0;
-        const uint32_t key = (uint32_t)(cj ? (rev_mask ^ enc) : enc);
-        if (key >= T.klim) return cmake(0, 0);
+        const uint32_t key0 = (uint32_t)(cj ? (rev_mask ^ enc) : enc);
+        const bool inr = key0 < T.klim;
+        const uint32_t key = inr ? key0 : 0u;
         const uint32_t wv = T.rbits[key >> 5];
+        const uint32_t pre = T.rpre[key >> 5];
         const uint32_t bit = 1u << (key & 31);
-        if (!(wv & bit)) return cmake(0, 0);
-        const cplx v = T.pvals[T.rpre[key >> 5] + __popc(wv & (bit - 1u))];
-        return cj ? cconj(v) : v;
+        const bool found = inr && (wv & bit) != 0;
+        const int idx = found ? (int)(pre + __popc(wv & (bit - 1u))) : 0;
+        // the LDS value copy always holds entry 0; a global table may be empty
+        const cplx v = (MODE == kLookupRankVals || found) ? T.pvals[idx] : cmake(0, 0);
+        const double re = found ? v.re : 0.0;
+        const double im = found ? (cj ? -v.im : v.im) : 0.0;
+        return cmake(re, im);
     }
     return lookup_num(enc, top_bit, rev_mask, T.pkeys, T.pvals, T.pn);
 }
 
 // g value of one term at one cell (make_gtable loop body, flattening.hpp:129-248).
-// q is held in registers (KM >= m, compile-time bound); ygi is summed in row
-// order as in the reference, with q * (-1) written as -q (the same value).
+// ygi is summed in row order as in the reference.  qr holds q for the rows that
+// enter the sum and -0.0 for the others (KM >= m, compile-time bound): adding
+// -0.0 leaves any double unchanged, and q * (-1) is q with its sign bit flipped.
 template <int MODE, int KM>
 __device__ inline cplx eval_cell(const EvalTerm& T, const double (&qr)[KM], const uint8_t* cm, const int8_t* cs,
                                  uint32_t key) {
     double ygi = 0;
+    const uint32_t kb = key & T.fm;
 #pragma unroll
     for (int k = 0; k < KM; k++) {
-        if (k < T.m && !((T.hf >> k) & 1u)) ygi += ((key >> k) & 1u) ? -qr[k] : qr[k];
+        const unsigned long long sb = (unsigned long long)((kb >> k) & 1u) << 63;
+        ygi += __longlong_as_double((long long)((unsigned long long)__double_as_longlong(qr[k]) ^ sb));
     }
     const int phc = T.phc;
     const int rev_phc = (phc >= 32) ? -1 : ((1 << phc) - 1);
@@ -1463,15 +1475,28 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         for (int w = lane; w < W; w += kWave) bits[w] = 0;
         __syncthreads();
         bool bad = false;
-        for (int i = lane; i < T.pn; i += kWave) {
-            const uint32_t key = src[i];
-            if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
-            else bad = true;
-            // pre[x] = index of the first key in word >= x, filled for the words
-            // between the previous key's word and this key's word
-            const int w = min((int)(key >> 5), W - 1);
-            const int wp = i ? min((int)(src[i - 1] >> 5), W - 1) : -1;
-            for (int x = wp + 1; x <= w; x++) pre[x] = i;
+        // four keys per lane per round, loads issued before the LDS updates
+        for (int i0 = 0; i0 < T.pn; i0 += 4 * kWave) {
+            uint32_t kc[4], kp[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = min(i0 + lane + u * kWave, T.pn - 1);
+                kc[u] = src[i];
+                kp[u] = src[i > 0 ? i - 1 : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + lane + u * kWave;
+                if (i >= T.pn) continue;
+                const uint32_t key = kc[u];
+                if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
+                else bad = true;
+                // pre[x] = index of the first key in word >= x, filled for the words
+                // between the previous key's word and this key's word
+                const int w = min((int)(key >> 5), W - 1);
+                const int wp = i ? min((int)(kp[u] >> 5), W - 1) : -1;
+                for (int x = wp + 1; x <= w; x++) pre[x] = i;
+            }
         }
         const int lastw = T.pn ? min((int)(src[T.pn - 1] >> 5), W - 1) : -1;
         for (int w = lane; w < W; w += kWave)
@@ -1482,13 +1507,22 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         if (MODE == kLookupRankVals) {
             cplx* dst = reinterpret_cast<cplx*>(lds + 2 * wmax);
             const cplx* vs = gv_prev + go;
-            for (int i = lane; i < T.pn; i += kWave) dst[i] = vs[i];
+            for (int i0 = 0; i0 < T.pn; i0 += 4 * kWave) {
+                cplx vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) vv[u] = vs[min(i0 + lane + u * kWave, T.pn - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (i0 + lane + u * kWave < T.pn) dst[i0 + lane + u * kWave] = vv[u];
+            }
+            if (T.pn == 0 && lane == 0) dst[0] = cmake(0, 0);
             T.pvals = dst;
         }
     }
     const int MM = O.MM;
+    T.fm = (m >= 32 ? 0xffffffffu : ((1u << m) - 1u)) & ~T.hf;
 #pragma unroll
-    for (int kk = 0; kk < KM; kk++) qr[kk] = (kk < m) ? O.q[s * MM + kk] : 0.0;
+    for (int kk = 0; kk < KM; kk++) qr[kk] = ((T.fm >> kk) & 1u) ? O.q[s * MM + kk] : -0.0;
     const int pbc = O.pbc[s];
     if (T.has_cmap && lane < pbc) {
         cm_s[lane] = O.cmap[s * MM + lane];
@@ -1539,13 +1573,17 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     {
         EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
         const double ps = dev_psum2(OUT, r, m);
-        bool small = true;
-        for (int i = lane; i < n; i += kWave) {
-            const cplx v = eval_cell<MODE, KM>(T, qr, cm_s, cs_s, B[i]);
-            ov[i] = v;
-            if ((ps * hypot(v.re, v.im)) > kApproxEps) small = false;
+        // negligible iff every cell is small; the test stops at the first large cell
+        bool negl = true;
+        for (int i0 = 0; i0 < n; i0 += kWave) {
+            const int i = i0 + lane;
+            cplx v = cmake(0, 0);
+            if (i < n) {
+                v = eval_cell<MODE, KM>(T, qr, cm_s, cs_s, B[i]);
+                ov[i] = v;
+            }
+            if (negl) negl = wave_all(i >= n || !((ps * hypot(v.re, v.im)) > kApproxEps));
         }
-        const bool negl = wave_all(small);
         int k = 0;
         if (negl) {
             // replacement chain (flattening.hpp:412-489)
@@ -1560,12 +1598,16 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
-                for (int i = lane; i < n; i += kWave) {
-                    const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X);
-                    ov[i] = v;
-                    if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
+                for (int i0 = 0; i0 < n; i0 += kWave) {
+                    const int i = i0 + lane;
+                    cplx v = cmake(0, 0);
+                    if (i < n) {
+                        v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X);
+                        ov[i] = v;
+                    }
+                    if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
                 }
-                if (!wave_all(sm)) {
+                if (!sm) {
                     r = sk;
                     break;
                 }
@@ -1586,12 +1628,16 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
             const double psk = dev_psum2(OUT, sk, m);
             bool sm = true;
-            for (int i = lane; i < n; i += kWave) {
-                const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
-                sc[i] = v;
-                if ((psk * hypot(v.re, v.im)) > kApproxEps) sm = false;
+            for (int i0 = 0; i0 < n; i0 += kWave) {
+                const int i = i0 + lane;
+                cplx v = cmake(0, 0);
+                if (i < n) {
+                    v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
+                    sc[i] = v;
+                }
+                if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
             }
-            if (wave_all(sm)) continue;
+            if (sm) continue;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             __syncthreads();
             for (int i = lane; i < n; i += kWave) {
