@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -188,6 +189,9 @@ struct Profiler {
     long long geval_launches = 0;
     double geval_bytes = 0;
     double geval_ms = 0;
+    double groups = 0;      // reduction groups evaluated
+    double members = 0;     // members in those groups (terms evaluated = groups + members)
+    double max_group = 0;   // largest group (root + members)
     hipEvent_t k0 = nullptr, k1 = nullptr;
     std::vector<std::pair<int, hipEvent_t>> marks;
     hipEvent_t start_ev = nullptr;
@@ -250,6 +254,11 @@ struct Engine {
     bool moments_pending = false;
     int max_gp_cells = 1;  // largest parent g-table of the current parents
     int geval_mode = -1;   // parent lookup mode of the last g-table launch
+    // MCE_GEVAL_MODE=0..3 selects a lookup mode below the automatic choice (tuning)
+    int geval_mode_override = getenv("MCE_GEVAL_MODE") ? atoi(getenv("MCE_GEVAL_MODE")) : -1;
+    // MCE_BIG_GROUP_MEMBERS: groups with more members add them one wavefront per member
+    int big_group_members = getenv("MCE_BIG_GROUP_MEMBERS") ? std::max(0, atoi(getenv("MCE_BIG_GROUP_MEMBERS")))
+                                                            : kBigGroupMembers;
     PoolStore in, out;
     DBuf<uint32_t> gk[2];
     DBuf<cplx> gv[2];
@@ -280,6 +289,11 @@ struct Engine {
     DBuf<int64_t> ub64, b2off, goff, b2n64;
     DBuf<uint8_t> cub_tmp;
     DBuf<unsigned long long> bytes_acc;
+    DBuf<int> pstat;
+    DBuf<int> defer_k0, big_icnt, big_ioff, big_hcnt, big_hoff;
+    DBuf<int64_t> big_ccnt, big_coff;
+    DBuf<cplx> contrib;
+    DBuf<uint8_t> cneg;
     Profiler prof;
 
     Engine(int num_steps, int d_, int cmcc_, int pncc_, int p_, const double* A0_, const double* p0_,
@@ -893,11 +907,13 @@ struct Engine {
             }
             HIPCHK(hipEventRecord(prof.k0, st));
         }
+        int ge_mode = 0, ge_wmax = 0;
+        size_t ge_lds = 0;
         {
             // parent lookups: rank structure over the 2^(MM-1) half-storage keys when it
             // fits in LDS (with the parent values too when those fit), else binary search
-            const int wmax = std::max(2, (((1 << (MM - 1)) + 31) >> 5) + 1) & ~1;
-            const size_t rank_b = (size_t)wmax * 8;
+            const int wmax = (std::max(4, (((1 << (MM - 1)) + 31) >> 5)) + 3) & ~3;  // multiple of 4 words
+            const size_t rank_b = (size_t)wmax * 12;  // parent bitset, prefix counts, group bitset
             const size_t vals_b = (size_t)std::max(max_gp_cells, 1) * sizeof(cplx);
             const size_t keys_b = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
             const size_t budget = 48 * 1024;
@@ -916,7 +932,16 @@ struct Engine {
                 mode = kLookupSearch;
                 lds = 0;
             }
+            if (geval_mode_override >= 0 && geval_mode_override <= mode) {
+                mode = geval_mode_override;
+                lds = mode == kLookupRank ? rank_b : mode == kLookupSearchLds ? keys_b : mode == kLookupSearch ? 0 : lds;
+            }
             geval_mode = mode;
+            ge_mode = mode;
+            ge_lds = lds;
+            ge_wmax = wmax;
+            const int big_members = mode >= kLookupRank ? big_group_members : INT_MAX;
+            defer_k0.reserve(std::max(ng, 1));
 #define MCE_GEVAL(M_, K_)                                                                                        \
     do {                                                                                                          \
         HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<M_, K_>,                                             \
@@ -924,7 +949,8 @@ struct Engine {
         hipLaunchKernelGGL((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
                            grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
                            b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
-                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3);         \
+                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3,          \
+                           big_members, defer_k0.ptr);                                                            \
     } while (0)
             // rows held in registers: the smallest compile-time bound >= the largest shape
             if (mode == kLookupRankVals && MM <= 8) MCE_GEVAL(kLookupRankVals, 8);
@@ -945,14 +971,27 @@ struct Engine {
             bytes_acc.reserve(1);
             HIPCHK(hipMemsetAsync(bytes_acc.ptr, 0, sizeof(unsigned long long), st));
             hipLaunchKernelGGL(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
-                               grp_mcnt.ptr, b2n.ptr, bytes_acc.ptr);
+                               grp_mcnt.ptr, b2n.ptr, surv.ptr, defer_k0.ptr,
+                               ge_mode >= kLookupRank ? big_group_members : INT_MAX, bytes_acc.ptr);
             const unsigned long long bytes = read1(bytes_acc.ptr);
             float t = 0;
             HIPCHK(hipEventElapsedTime(&t, prof.k0, prof.k1));
             prof.geval_ms += t;
             prof.geval_bytes += (double)bytes;
             prof.geval_launches++;
+            pstat.reserve(2);
+            cub([&](void* t_, size_t& b_) { return hipcub::DeviceReduce::Max(t_, b_, grp_mcnt.ptr, pstat.ptr, ng, st); });
+            cub([&](void* t_, size_t& b_) {
+                return hipcub::DeviceReduce::Sum(t_, b_, grp_mcnt.ptr, pstat.ptr + 1, ng, st);
+            });
+            int ps_[2];
+            HIPCHK(hipMemcpyAsync(ps_, pstat.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            prof.groups += ng;
+            prof.members += ps_[1];
+            prof.max_group = std::max(prof.max_group, (double)(ps_[0] + 1));
         }
+        if (ge_mode >= kLookupRank) big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
         const int nsort = read1(counters.ptr + 2);
         const int rare = read1(counters.ptr + 3);
         if (rare & 4) throw std::runtime_error("parent g-table key outside its half-storage range");
@@ -1001,6 +1040,53 @@ struct Engine {
             Nt += h[m];
         }
         gcur = nx;
+    }
+
+    // Groups with more than big_group_members members: their members' contributions are
+    // evaluated one wavefront per member (k_group_members) and added in member order
+    // (k_group_sum), after k_group_eval has settled each group's root.
+    void big_groups(int ng, int mode, size_t lds, int wmax, double gscale, int nx) {
+        if (ng <= 0) return;
+        big_icnt.reserve(ng);
+        big_ioff.reserve(ng);
+        big_hcnt.reserve(ng);
+        big_hoff.reserve(ng);
+        big_ccnt.reserve(ng);
+        big_coff.reserve(ng);
+        hipLaunchKernelGGL(k_big_prep, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, big_group_members, grp_mcnt.ptr,
+                           b2n.ptr, big_icnt.ptr, big_ccnt.ptr, big_hcnt.ptr);
+        scan_i32_excl(big_icnt.ptr, big_ioff.ptr, ng);
+        scan_i32_excl(big_hcnt.ptr, big_hoff.ptr, ng);
+        scan_i64_excl(big_ccnt.ptr, big_coff.ptr, ng);
+        const int ni = read1(big_ioff.ptr + ng - 1) + read1(big_icnt.ptr + ng - 1);
+        if (ni == 0) return;
+        const int nh = read1(big_hoff.ptr + ng - 1) + read1(big_hcnt.ptr + ng - 1);
+        const int64_t nc = read1(big_coff.ptr + ng - 1) + read1(big_ccnt.ptr + ng - 1);
+        contrib.reserve((size_t)std::max<int64_t>(nc, 1));
+        cneg.reserve(ni);
+        TermPool OUT = out.view();
+#define MCE_GMEM(M_, K_)                                                                                         \
+    do {                                                                                                          \
+        HIPCHK(hipFuncSetAttribute((const void*)k_group_members<M_, K_>,                                          \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 16)));  \
+        hipLaunchKernelGGL((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
+                           ni, big_coff.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr,    \
+                           b2off.ptr, b2n.ptr, gk[gcur].ptr, gv[gcur].ptr, gscale, surv.ptr, gxor.ptr,             \
+                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 3);                                 \
+    } while (0)
+        if (mode == kLookupRankVals && MM <= 8) MCE_GMEM(kLookupRankVals, 8);
+        else if (mode == kLookupRankVals && MM <= 11) MCE_GMEM(kLookupRankVals, 11);
+        else if (mode == kLookupRankVals && MM <= 14) MCE_GMEM(kLookupRankVals, 14);
+        else if (mode == kLookupRankVals && MM <= 16) MCE_GMEM(kLookupRankVals, 16);
+        else if (mode == kLookupRankVals && MM <= 24) MCE_GMEM(kLookupRankVals, 24);
+        else if (mode == kLookupRankVals) MCE_GMEM(kLookupRankVals, 32);
+        else if (mode == kLookupRank && MM <= 16) MCE_GMEM(kLookupRank, 16);
+        else MCE_GMEM(kLookupRank, 32);
+#undef MCE_GMEM
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_group_sum, dim3(nh), dim3(64), 0, st, ng, big_hoff.ptr, nh, big_ioff.ptr, big_coff.ptr,
+                           grp_mcnt.ptr, b2n.ptr, goff.ptr, surv.ptr, defer_k0.ptr, contrib.ptr, cneg.ptr, gv[nx].ptr);
+        HIPCHK(hipGetLastError());
     }
 
     // Fast term reduction of every shape bucket at once.  off[m] is the first gathered
@@ -1501,6 +1587,9 @@ void mce_profile_read(void* handle, double* out) {
     out[kNumPhases + 1] = (double)p.geval_launches;
     out[kNumPhases + 2] = p.geval_bytes;
     out[kNumPhases + 3] = p.geval_ms;
+    out[kNumPhases + 4] = p.groups;
+    out[kNumPhases + 5] = p.members;
+    out[kNumPhases + 6] = p.max_group;
 }
 
 void mce_profile_reset(void* handle) {
@@ -1508,6 +1597,9 @@ void mce_profile_reset(void* handle) {
     for (int i = 0; i < kNumPhases; i++) p.ms[i] = 0;
     p.total = 0;
     p.geval_launches = 0;
+    p.groups = 0;
+    p.members = 0;
+    p.max_group = 0;
     p.geval_bytes = 0;
     p.geval_ms = 0;
 }

@@ -19,6 +19,8 @@
 namespace mce {
 
 constexpr int kWave = 64;
+// Default: groups with more members than this add them with one wavefront per member.
+constexpr int kBigGroupMembers = 16;
 
 enum BtabMode : uint8_t { kModeNone = 0, kModeCopy = 1, kModeIdent = 2, kModeTrivial = 3, kModeEmpty = 4, kModeEnum = 5 };
 
@@ -1436,7 +1438,8 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
 
 // Stages term s for evaluation (one wavefront, blockDim == 64).  `lds` is the
 // kernel's dynamic LDS: parent keys (kLookupSearchLds), or the rank structure of
-// `wmax` words of bits and `wmax` prefix counts followed by the parent values.
+// `wmax` words of bits and `wmax` prefix counts, then `wmax` words of the group's
+// own key bitset (built by the kernel), then the parent values.
 template <int MODE, int KM>
 __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const uint32_t* gk_prev,
                                       const cplx* gv_prev, double gscale, uint32_t* lds, int wmax, double (&qr)[KM],
@@ -1475,19 +1478,26 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         for (int w = lane; w < W; w += kWave) bits[w] = 0;
         __syncthreads();
         bool bad = false;
-        // four keys per lane per round, loads issued before the LDS updates
-        for (int i0 = 0; i0 < T.pn; i0 += 4 * kWave) {
-            uint32_t kc[4], kp[4];
+        // eight keys (and, with LDS values, eight values) per lane per round; all
+        // loads of a round are issued before the LDS updates
+        constexpr int U = 8;
+        cplx* dst = reinterpret_cast<cplx*>(lds + 3 * wmax);
+        const cplx* vs = gv_prev + go;
+        for (int i0 = 0; i0 < T.pn; i0 += U * kWave) {
+            uint32_t kc[U], kp[U];
+            cplx vv[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 const int i = min(i0 + lane + u * kWave, T.pn - 1);
                 kc[u] = src[i];
                 kp[u] = src[i > 0 ? i - 1 : 0];
+                if (MODE == kLookupRankVals) vv[u] = vs[i];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 const int i = i0 + lane + u * kWave;
                 if (i >= T.pn) continue;
+                if (MODE == kLookupRankVals) dst[i] = vv[u];
                 const uint32_t key = kc[u];
                 if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
                 else bad = true;
@@ -1505,16 +1515,6 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         T.rbits = bits;
         T.rpre = pre;
         if (MODE == kLookupRankVals) {
-            cplx* dst = reinterpret_cast<cplx*>(lds + 2 * wmax);
-            const cplx* vs = gv_prev + go;
-            for (int i0 = 0; i0 < T.pn; i0 += 4 * kWave) {
-                cplx vv[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) vv[u] = vs[min(i0 + lane + u * kWave, T.pn - 1)];
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (i0 + lane + u * kWave < T.pn) dst[i0 + lane + u * kWave] = vv[u];
-            }
             if (T.pn == 0 && lane == 0) dst[0] = cmake(0, 0);
             T.pvals = dst;
         }
@@ -1549,7 +1549,8 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                                                   uint32_t* __restrict__ gk_out, cplx* __restrict__ gv_out,
                                                   cplx* __restrict__ scratch, double gscale, int* __restrict__ surv,
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
-                                                  int* __restrict__ sort_count, int* __restrict__ rare) {
+                                                  int* __restrict__ sort_count, int* __restrict__ rare,
+                                                  int big_members, int* __restrict__ defer_k0) {
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
@@ -1570,16 +1571,32 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     const uint32_t rev = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
     int r = grp_root[g];
     uint32_t X = 0;
+    if constexpr (MODE >= kLookupRank) {
+        // bitset of the group's B-table keys (all below 2^(m-1)), for add_gtables
+        uint32_t* bset = lds_dyn + 2 * wmax;
+        const int WB = (int)((top + 31u) >> 5);
+        for (int w = lane; w < WB; w += kWave) bset[w] = 0;
+        __syncthreads();
+        for (int i = lane; i < n; i += kWave) {
+            const uint32_t key = B[i];
+            if (key < top) atomicOr(&bset[key >> 5], 1u << (key & 31));
+            else atomicOr(rare, 4);
+        }
+        // made visible by the barriers of stage_term
+    }
     {
         EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
         const double ps = dev_psum2(OUT, r, m);
         // negligible iff every cell is small; the test stops at the first large cell
         bool negl = true;
+        uint32_t bnext = n > 0 ? B[min(lane, n - 1)] : 0u;  // cell key of the next pass, loaded a pass ahead
         for (int i0 = 0; i0 < n; i0 += kWave) {
             const int i = i0 + lane;
+            const uint32_t bcur = bnext;
+            bnext = B[min(i + kWave, n - 1)];
             cplx v = cmake(0, 0);
             if (i < n) {
-                v = eval_cell<MODE, KM>(T, qr, cm_s, cs_s, B[i]);
+                v = eval_cell<MODE, KM>(T, qr, cm_s, cs_s, bcur);
                 ov[i] = v;
             }
             if (negl) negl = wave_all(i >= n || !((ps * hypot(v.re, v.im)) > kApproxEps));
@@ -1598,11 +1615,14 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
+                uint32_t bnext = n > 0 ? B[min(lane, n - 1)] : 0u;
                 for (int i0 = 0; i0 < n; i0 += kWave) {
                     const int i = i0 + lane;
+                    const uint32_t bcur = bnext;
+                    bnext = B[min(i + kWave, n - 1)];
                     cplx v = cmake(0, 0);
                     if (i < n) {
-                        v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X);
+                        v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, bcur ^ X);
                         ov[i] = v;
                     }
                     if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
@@ -1618,8 +1638,13 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                 return;
             }
         }
-        // later members are added into the root's table
+        // later members are added into the root's table; in a group with more than
+        // big_members members that is left to k_group_members / k_group_sum
         const double* Ar = OUT.A + (int64_t)r * MM * d;
+        if (MODE >= kLookupRank && nmem > big_members) {
+            if (lane == 0) defer_k0[g] = k;
+            k = nmem;
+        }
         for (; k < nmem; k++) {
             const int sk = mem[k];
             if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
@@ -1627,39 +1652,73 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
             EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
             const double psk = dev_psum2(OUT, sk, m);
-            bool sm = true;
-            for (int i0 = 0; i0 < n; i0 += kWave) {
-                const int i = i0 + lane;
-                cplx v = cmake(0, 0);
-                if (i < n) {
-                    v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
-                    sc[i] = v;
+            if constexpr (MODE >= kLookupRank) {
+                // negligibility over the member's own cells B ^ X ^ sh, stopping at the
+                // first large cell
+                bool sm = true;
+                for (int i0 = 0; i0 < n && sm; i0 += kWave) {
+                    const int i = i0 + lane;
+                    cplx v = cmake(0, 0);
+                    if (i < n) v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
+                    sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
                 }
-                if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
+                if (sm) continue;
+                // add_gtables: root cell bi takes the member's value at bi ^ sigma
+                // (reversed and conjugated when that key is in the lower half), which
+                // is evaluated directly; a key outside the member's table adds nothing
+                const uint32_t* bset = lds_dyn + 2 * wmax;
+                for (int i = lane; i < n; i += kWave) {
+                    const uint32_t bi = B[i] ^ X;
+                    uint32_t x = bi ^ (uint32_t)sig;
+                    bool cj = false;
+                    if (x & top) {
+                        cj = true;
+                        x ^= rev;
+                    }
+                    const uint32_t y = x ^ sh ^ X;  // the member's key x as a key of B
+                    if (!((bset[y >> 5] >> (y & 31)) & 1u)) continue;
+                    const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, x);
+                    const cplx cur = ov[i];
+                    ov[i] = cadd(cur, cj ? cconj(v) : v);
+                }
+            } else {
+                bool sm = true;
+                uint32_t bnext = n > 0 ? B[min(lane, n - 1)] : 0u;
+                for (int i0 = 0; i0 < n; i0 += kWave) {
+                    const int i = i0 + lane;
+                    const uint32_t bcur = bnext;
+                    bnext = B[min(i + kWave, n - 1)];
+                    cplx v = cmake(0, 0);
+                    if (i < n) {
+                        v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, bcur ^ X ^ sh);
+                        sc[i] = v;
+                    }
+                    if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
+                }
+                if (sm) continue;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __syncthreads();
+                for (int i = lane; i < n; i += kWave) {
+                    const uint32_t bi = B[i] ^ X;
+                    uint32_t x = bi ^ (uint32_t)sig;
+                    bool cj = false;
+                    if (x & top) {
+                        cj = true;
+                        x ^= rev;
+                    }
+                    cplx v;
+                    if (x == (bi ^ sh)) {
+                        v = sc[i];
+                    } else {
+                        const int ii = bsearch_u32(B, x ^ sh ^ X, n);
+                        if (ii < 0) continue;
+                        v = sc[ii];
+                    }
+                    const cplx cur = ov[i];
+                    ov[i] = cadd(cur, cj ? cconj(v) : v);
+                }
+                __syncthreads();
             }
-            if (sm) continue;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __syncthreads();
-            for (int i = lane; i < n; i += kWave) {
-                const uint32_t bi = B[i] ^ X;
-                uint32_t x = bi ^ (uint32_t)sig;
-                bool cj = false;
-                if (x & top) {
-                    cj = true;
-                    x ^= rev;
-                }
-                cplx v;
-                if (x == (bi ^ sh)) {
-                    v = sc[i];
-                } else {
-                    const int ii = bsearch_u32(B, x ^ sh ^ X, n);
-                    if (ii < 0) continue;
-                    v = sc[ii];
-                }
-                const cplx cur = ov[i];
-                ov[i] = cadd(cur, cj ? cconj(v) : v);
-            }
-            __syncthreads();
         }
     }
     for (int i = lane; i < n; i += kWave) ok[i] = B[i] ^ X;
@@ -1671,6 +1730,137 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
             sort_list[slot] = g;
         }
     }
+}
+
+// Work lists of the groups whose members are added in parallel: per group the
+// member count, the contribution entries (members x cells) and the 64-cell chunks.
+__global__ void k_big_prep(int ng, int big_members, const int* __restrict__ grp_mcnt, const int* __restrict__ b2_n,
+                           int* __restrict__ icnt, int64_t* __restrict__ ccnt, int* __restrict__ hcnt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int nm = grp_mcnt[g];
+    const bool big = nm > big_members;
+    icnt[g] = big ? nm : 0;
+    ccnt[g] = big ? (int64_t)nm * b2_n[g] : 0;
+    hcnt[g] = big ? (b2_n[g] + kWave - 1) / kWave : 0;
+}
+
+// Largest g with off[g] <= item (off ascending, off[0] == 0).
+__device__ inline int owner_of(const int* __restrict__ off, int n, int item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One wavefront per (large group, member k): the member's g-table contribution to
+// every cell of the group's root, in the root's orientation (add_gtables,
+// flattening.hpp:259-315), written to contrib[coff[g] + k * n + i].  A key outside
+// the member's table contributes (-0.0, -0.0), which leaves any sum unchanged; a
+// negligible member is flagged in cneg and contributes nothing.
+template <int MODE, int KM>
+__global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng, int d, int wmax,
+                                                     const int* __restrict__ ioff, int nitems,
+                                                     const int64_t* __restrict__ coff,
+                                                     const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
+                                                     const int* __restrict__ grp_m, const int* __restrict__ mem_slots,
+                                                     const uint32_t* __restrict__ B2, const int64_t* __restrict__ b2_off,
+                                                     const int* __restrict__ b2_n, const uint32_t* __restrict__ gk_prev,
+                                                     const cplx* __restrict__ gv_prev, double gscale,
+                                                     const int* __restrict__ surv, const int* __restrict__ gxor,
+                                                     const int* __restrict__ defer_k0, cplx* __restrict__ contrib,
+                                                     uint8_t* __restrict__ cneg, int* __restrict__ rare) {
+    static_assert(MODE >= kLookupRank, "parallel member sums use the rank lookup");
+    extern __shared__ __align__(16) uint32_t lds_dyn[];
+    double qr[KM];
+    __shared__ uint8_t cm_s[kMaxM];
+    __shared__ int8_t cs_s[kMaxM];
+    const int item = blockIdx.x;
+    if (item >= nitems) return;
+    const int g = owner_of(ioff, ng, item);
+    const int k = item - ioff[g];
+    const int r = surv[g];
+    if (r < 0 || k < defer_k0[g]) return;
+    const int lane = threadIdx.x;
+    const int m = grp_m[g];
+    const int MM = OUT.MM;
+    const uint32_t* B = B2 + b2_off[g];
+    const int n = b2_n[g];
+    const uint32_t X = (uint32_t)gxor[g];
+    const uint32_t top = 1u << (m - 1);
+    const uint32_t rev = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    uint32_t* bset = lds_dyn + 2 * wmax;
+    {
+        const int WB = (int)((top + 31u) >> 5);
+        for (int w = lane; w < WB; w += kWave) bset[w] = 0;
+        __syncthreads();
+        for (int i = lane; i < n; i += kWave) {
+            const uint32_t key = B[i];
+            if (key < top) atomicOr(&bset[key >> 5], 1u << (key & 31));
+        }
+    }
+    const int sk = mem_slots[grp_moff[g] + k];
+    if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
+    const int sig = orientation_mask(OUT.A + (int64_t)r * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
+    const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
+    EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
+    const double psk = dev_psum2(OUT, sk, m);
+    bool sm = true;
+    for (int i0 = 0; i0 < n && sm; i0 += kWave) {
+        const int i = i0 + lane;
+        cplx v = cmake(0, 0);
+        if (i < n) v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
+        sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
+    }
+    if (lane == 0) cneg[item] = sm ? 1 : 0;
+    if (sm) return;
+    cplx* out = contrib + coff[g] + (int64_t)k * n;
+    for (int i = lane; i < n; i += kWave) {
+        const uint32_t bi = B[i] ^ X;
+        uint32_t x = bi ^ (uint32_t)sig;
+        bool cj = false;
+        if (x & top) {
+            cj = true;
+            x ^= rev;
+        }
+        const uint32_t y = x ^ sh ^ X;
+        cplx c = cmake(-0.0, -0.0);
+        if ((bset[y >> 5] >> (y & 31)) & 1u) {
+            const cplx v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, x);
+            c = cj ? cconj(v) : v;
+        }
+        out[i] = c;
+    }
+}
+
+// Adds the member contributions of each large group into its root's table, one
+// lane per cell, members in order (the sum order of add_gtables).
+__global__ __launch_bounds__(64) void k_group_sum(int ng, const int* __restrict__ hoff, int nchunks,
+                                                 const int* __restrict__ ioff, const int64_t* __restrict__ coff,
+                                                 const int* __restrict__ grp_mcnt, const int* __restrict__ b2_n,
+                                                 const int64_t* __restrict__ g_off, const int* __restrict__ surv,
+                                                 const int* __restrict__ defer_k0, const cplx* __restrict__ contrib,
+                                                 const uint8_t* __restrict__ cneg, cplx* __restrict__ gv_out) {
+    const int item = blockIdx.x;
+    if (item >= nchunks) return;
+    const int g = owner_of(hoff, ng, item);
+    if (surv[g] < 0) return;
+    const int n = b2_n[g];
+    const int i = (item - hoff[g]) * kWave + threadIdx.x;
+    if (i >= n) return;
+    const int nm = grp_mcnt[g];
+    const cplx* cb = contrib + coff[g] + i;
+    const uint8_t* ng_ = cneg + ioff[g];
+    cplx acc = gv_out[g_off[g] + i];
+    for (int k = defer_k0[g]; k < nm; k++) {
+        if (ng_[k]) continue;
+        const cplx c = cb[(int64_t)k * n];
+        acc = cadd(acc, c);
+    }
+    gv_out[g_off[g] + i] = acc;
 }
 
 // Restores ascending key order of a group table whose keys were re-oriented by
@@ -1790,13 +1980,18 @@ __global__ void k_gtable_first(const TermPool OUT, int n, const uint32_t* __rest
 // member), its record (q, p, A rows, scalars, coalignment maps) and, per cell, the two
 // parent g-table entries its numerators need (2 x 20 B).
 __global__ void k_geval_bytes(int ng, int d, const int* __restrict__ grp_m, const int* __restrict__ grp_mcnt,
-                              const int* __restrict__ b2_n, unsigned long long* __restrict__ acc) {
+                              const int* __restrict__ b2_n, const int* __restrict__ surv,
+                              const int* __restrict__ defer_k0, int big_members, unsigned long long* __restrict__ acc) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const unsigned long long n = (unsigned long long)b2_n[g];
     const unsigned long long m = (unsigned long long)grp_m[g];
     const unsigned long long rec = m * (16ull + 8ull * d) + 2ull * m + 52ull;
-    const unsigned long long evals = 1ull + (unsigned long long)grp_mcnt[g];
+    // terms evaluated by k_group_eval: the root and the members it adds itself (a
+    // large group's later members are evaluated by k_group_members)
+    const int nm = grp_mcnt[g];
+    const int own = (nm > big_members && surv[g] >= 0) ? defer_k0[g] : nm;
+    const unsigned long long evals = 1ull + (unsigned long long)own;
     atomicAdd(acc, 24ull * n + evals * (40ull * n + rec));
 }
 

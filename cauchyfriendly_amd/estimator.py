@@ -113,7 +113,7 @@ class GpuEstimator:
             L.mce_profile_enable(self.h, int(bool(enable)))
         if reset:
             L.mce_profile_reset(self.h)
-        out = np.zeros(13)
+        out = np.zeros(16)
         L.mce_profile_read(self.h, out.ctypes.data_as(_dp))
         return out
 

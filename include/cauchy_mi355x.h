@@ -153,7 +153,9 @@ void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax
  * out[0..8] per phase [time-prop, B-table enumeration, measurement update, moments, gather,
  * reduction, group B-tables, group g-tables, compaction], out[9] their total, out[10] the
  * launch count of the group g-table kernel, out[11] its algorithmic bytes, out[12] its
- * kernel time.  Recorded only after mce_profile_enable(h, 1); out must hold 13 doubles. */
+ * kernel time, out[13] reduction groups evaluated, out[14] their members, out[15] the
+ * largest group (root + members).  Recorded only after mce_profile_enable(h, 1); out must
+ * hold 16 doubles. */
 void mce_profile_enable(void* handle, int on);
 void mce_profile_read(void* handle, double* out_ms12);
 void mce_profile_reset(void* handle);

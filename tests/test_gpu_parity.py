@@ -96,3 +96,38 @@ def test_gpu_full_window_invariants():
         if u > 0:
             assert got["nt_ftr"] <= got["nt_mu"]
     assert out[-1]["nt_mu"] == 1403563
+
+
+def _run_with_big_group_threshold(s, num_steps, threshold, monkeypatch):
+    monkeypatch.setenv("MCE_BIG_GROUP_MEMBERS", str(threshold))
+    est = GpuEstimator(num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, cmcc=s.cmcc, seed=s.srand_seed)
+    est.profile(enable=True, reset=True)
+    Gt = np.ascontiguousarray(s.Gamma.reshape(-1))
+    B = None if s.B is None else np.ascontiguousarray(s.B.reshape(-1))
+    out = []
+    for k in range(num_steps):
+        for i in range(s.p):
+            out.append(est.step(s.zs[k, i], s.Phi(k).reshape(-1), Gt, s.beta, s.H[i], s.gamma[i], B, s.u(k)))
+    prof = est.profile(enable=False)
+    est.close()
+    return out, prof
+
+
+def test_large_groups_parallel_member_path_is_identical(monkeypatch):
+    """Reduction groups above the member threshold add their members through
+    k_group_members / k_group_sum; the result must equal the one-wavefront-per-group
+    path bit for bit, on a window whose largest group has hundreds of members.  The
+    moments of each update are sums over every g value of the previous update's
+    tables, so equal moments pin the tables too."""
+    s = W.config1()
+    steps = 5
+    ref_out, ref_prof = _run_with_big_group_threshold(s, steps, 1 << 30, monkeypatch)
+    assert ref_prof[15] > 100  # the window does contain large groups
+    for threshold in (16, 0):
+        out, _ = _run_with_big_group_threshold(s, steps, threshold, monkeypatch)
+        for a, b in zip(out, ref_out):
+            assert a["nt_mu"] == b["nt_mu"] and a["nt_ftr"] == b["nt_ftr"]
+            assert np.array_equal(a["shape_ftr"], b["shape_ftr"])
+            assert a["err"] == b["err"]
+            for key in ("fz", "mean", "cov"):
+                assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), (threshold, key)
