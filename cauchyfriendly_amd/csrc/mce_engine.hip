@@ -281,7 +281,10 @@ struct Engine {
     DBuf<int> ecnt, eoff, edges, indeg, inoff, cursor, Fc;
     DBuf<uint8_t> st0, st1, isroot;
     DBuf<uint32_t> memkey, memval, smkey, smval, roots, jiota;
-    DBuf<int> memcnt, memoff, inedges, dboff, gbase_dev;
+    DBuf<int> memcnt, memoff, inedges, dboff, gbase_dev, wlo;
+    DBuf<int64_t> wcnt, woff;
+    DBuf<double> fgb, fgp, fgA;
+    DBuf<uint8_t> pflag;
     DBuf<uint32_t> bkey, bskey;
     std::vector<int> h_boff;
     // groups (all buckets of an update)
@@ -1124,8 +1127,30 @@ struct Engine {
         });
         const uint32_t* sidx = fidx.ptr;
         hipLaunchKernelGGL(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, dconst.ptr,
-                                             fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
+        // candidate windows on the first search axis; the predicate runs one thread per
+        // (term, window slot) pair unless the pairs would not fit, then one per term
+        wlo.reserve(N);
+        wcnt.reserve(N);
+        woff.reserve(N);
+        hipLaunchKernelGGL(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr);
+        scan_i64_excl(wcnt.ptr, woff.ptr, N);
+        const int64_t npairs = read1(woff.ptr + N - 1) + read1(wcnt.ptr + N - 1);
+        const bool by_pairs = npairs < ((int64_t)1 << 31);
+        HIPCHK(hipMemsetAsync(ecnt.ptr, 0, N * sizeof(int), st));
+        if (by_pairs && npairs > 0) {
+            fgb.reserve((size_t)N * d);
+            fgp.reserve((size_t)N * MM);
+            fgA.reserve((size_t)N * MM);
+            pflag.reserve((size_t)npairs);
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_gather<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, sidx,
+                                                 fgb.ptr, fgp.ptr, fgA.ptr));
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs, N,
+                                                 boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr, fgp.ptr,
+                                                 fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr, edges.ptr, 0));
+        } else if (!by_pairs) {
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+                                                 dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
+        }
         HIPCHK(hipGetLastError());
         scan_i32_excl(ecnt.ptr, eoff.ptr, N);
         const int E = read1(eoff.ptr + N - 1) + read1(ecnt.ptr + N - 1);
@@ -1134,8 +1159,17 @@ struct Engine {
         if (E > 0) {
             edges.reserve(E);
             inedges.reserve(E);
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
-                                                 dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
+            cursor.reserve(N);
+            if (by_pairs) {
+                HIPCHK(hipMemsetAsync(cursor.ptr, 0, N * sizeof(int), st));
+                MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
+                                                     N, boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr,
+                                                     fgp.ptr, fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr,
+                                                     edges.ptr, 1));
+            } else {
+                MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+                                                     dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
+            }
             indeg.reserve(N);
             inoff.reserve(N);
             cursor.reserve(N);

@@ -1051,6 +1051,117 @@ __global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ per
     if (!pass) cnt[ig] = c;
 }
 
+// Pair-parallel candidate search.  k_ftr_gather copies what the predicate reads
+// into the value-sorted order: b (d), p (first m) and the first m entries of the
+// flattened A, row stride MM.
+template <int D>
+__global__ void k_ftr_gather(const TermPool OUT, const uint32_t* __restrict__ perm, int n,
+                             const int* __restrict__ boff, int nb, const uint32_t* __restrict__ sidx,
+                             double* __restrict__ gb, double* __restrict__ gp, double* __restrict__ gA) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int m = bucket_of(boff, nb, r);
+    const int MM = OUT.MM;
+    const int64_t s = perm[sidx[r]];
+#pragma unroll
+    for (int a = 0; a < D; a++) gb[(int64_t)r * D + a] = OUT.b[s * D + a];
+    for (int q = 0; q < m; q++) {
+        gp[(int64_t)r * MM + q] = OUT.p[s * MM + q];
+        gA[(int64_t)r * MM + q] = OUT.A[s * MM * D + q];
+    }
+}
+
+// Window of sorted position sp on the first search axis: positions r0 <= r < r1 of
+// its bucket with value in (v - ep, v + ep] (a window must hold two points,
+// ftr_binary_search, term_reduction.hpp:89-157).  wcnt = r1 - r0, or 0.
+__global__ void k_ftr_win(int n, const int* __restrict__ boff, int nb, const double* __restrict__ vals,
+                          int* __restrict__ wlo, int64_t* __restrict__ wcnt) {
+    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp >= n) return;
+    const int bk = bucket_of(boff, nb, sp);
+    const int beg = boff[bk], end = boff[bk + 1];
+    const double v = vals[sp];
+    const double lo = v - kReductionEps, hi = v + kReductionEps;
+    int a = beg, b = end;  // first position with value > lo
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (vals[mid] > lo) b = mid;
+        else a = mid + 1;
+    }
+    const int r0 = a;
+    a = beg;
+    b = end;  // first position with value > hi
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (vals[mid] > hi) b = mid;
+        else a = mid + 1;
+    }
+    const int r1 = a;
+    wlo[sp] = r0;
+    wcnt[sp] = (r1 - r0 >= 2) ? (int64_t)(r1 - r0) : 0;
+}
+
+// One thread per (position, window slot) pair: the rest of the greedy predicate,
+// term_reduction.hpp:159-276 (j > i, every other axis inside (b_i - ep, b_i + ep],
+// |p_j - p_i| <= ep on the first m, and the element-wise check on the first m
+// entries of the flattened A).  pass 0 flags the pair and counts term i's out-edges;
+// pass 1 writes the flagged pairs into term i's list (edge order within a list does
+// not matter to the resolution that follows).
+template <int D>
+__global__ void k_ftr_pairs(int64_t npairs, int n, const int* __restrict__ boff, int nb,
+                            const StepConst* __restrict__ sc, const int64_t* __restrict__ woff,
+                            const int* __restrict__ wlo, const uint32_t* __restrict__ sidx, int MM,
+                            const double* __restrict__ gb, const double* __restrict__ gp,
+                            const double* __restrict__ gA, uint8_t* __restrict__ flag, int* __restrict__ ecnt,
+                            const int* __restrict__ eoff, int* __restrict__ cursor, int* __restrict__ edges,
+                            int pass) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs) return;
+    // owning position: largest sp with woff[sp] <= q
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (woff[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    const int sp = lo;
+    const int r = wlo[sp] + (int)(q - woff[sp]);
+    const int ig = (int)sidx[sp];
+    const int jg = (int)sidx[r];
+    if (pass == 1) {
+        if (flag[q]) {
+            const int slot = atomicAdd(&cursor[ig], 1);
+            edges[eoff[ig] + slot] = jg;
+        }
+        return;
+    }
+    bool ok = jg > ig;
+    if (ok) {
+        const double ep = kReductionEps;
+        const double* bi = gb + (int64_t)sp * D;
+        const double* bj = gb + (int64_t)r * D;
+        for (int a = 1; a < D && ok; a++) {
+            const int ax = sc->order[a];
+            const double l2 = bi[ax] - ep, h2 = bi[ax] + ep;
+            const double x = bj[ax];
+            if (!(x > l2 && x <= h2)) ok = false;
+        }
+        const int m = bucket_of(boff, nb, sp);
+        const double* pi = gp + (int64_t)sp * MM;
+        const double* pj = gp + (int64_t)r * MM;
+        for (int kk = 0; kk < m && ok; kk++)
+            if (fabs(pj[kk] - pi[kk]) > ep) ok = false;
+        const double* Ai = gA + (int64_t)sp * MM;
+        const double* Aj = gA + (int64_t)r * MM;
+        for (int kk = 0; kk < m && ok; kk++) {
+            const double x = Ai[kk], y = Aj[kk];
+            if (!((fabs(x - y) < ep) || (fabs(x + y) < ep))) ok = false;
+        }
+    }
+    flag[q] = ok ? 1 : 0;
+    if (ok) atomicAdd(&ecnt[ig], 1);
+}
+
 // In-degree over edges whose ends share a chunk (pass 0), or the in-edge lists (pass 1).
 __global__ void k_ftr_inedges(int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
                               const int* __restrict__ ecnt, const int* __restrict__ edges, int* __restrict__ indeg,
