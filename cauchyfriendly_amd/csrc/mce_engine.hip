@@ -584,7 +584,8 @@ struct Engine {
             HIPCHK(hipEventElapsedTime(&t, ev_c0, ev_c1));
             prof.ms[kPhMom] += t;
         }
-        if (mu_err) throw std::runtime_error("two H-orthogonal hyperplanes coalign negatively (reference exits here)");
+        if (mu_err & 1) throw std::runtime_error("two H-orthogonal hyperplanes coalign negatively (reference exits here)");
+        if (mu_err) throw std::runtime_error("measurement update: a term's shape exceeds the kernel's row bound");
         std::vector<double> mo(NC * 2, 0.0);
         for (int t = 0; t < T; t++)
             for (int c = 0; c < NC; c++) {
@@ -719,9 +720,22 @@ struct Engine {
         mom_part.reserve((size_t)T * (1 + d + d * d) * 2);
         TermPool OUT = out.view();
         HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
-                                             slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr,
-                                             keys.ptr, chain_key.ptr, counters.ptr + 1));
+        // child rows in registers when the largest shape fits a compile-time bound
+        const int mu_km = MM <= 8 ? 8 : MM <= 14 ? 14 : 0;
+        if (mu_km == 8) {
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
+                                                 np, slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr,
+                                                 gv[gcur].ptr, gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 1));
+        } else if (mu_km == 14 && d <= 5) {
+            MCE_DISPATCH_D(d, if constexpr (DD <= 5) hipLaunchKernelGGL(
+                                  (k_mu_r<DD, 14>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
+                                  slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr, keys.ptr,
+                                  chain_key.ptr, counters.ptr + 1));
+        } else {
+            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
+                                                 slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr,
+                                                 gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 1));
+        }
         HIPCHK(hipGetLastError());
         prof.mark(kPhMu, st);
 

@@ -735,6 +735,263 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
     keys_out[s] = (uint32_t)(((mnew * sc->T) + IN.tid[P]) * 2 + ((t != m) ? 1 : 0));
 }
 
+// Measurement update with every child row in registers (KM >= m, compile-time
+// bound; all row indices are static after unrolling).  Same operations in the same
+// order as k_mu; the child's rows are written once, and only a coalignment that
+// merged rows (a rare case) compacts them in memory afterwards.
+template <int D, int KM>
+__global__ __launch_bounds__(256) void k_mu_r(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
+                                             int nslots, const StepConst* __restrict__ sc,
+                                             const uint32_t* __restrict__ gk_prev, const cplx* __restrict__ gv_prev,
+                                             double* __restrict__ gy, uint32_t* __restrict__ keys_out,
+                                             uint32_t* __restrict__ chain_key, int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slot_base[mid] <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    const int P = lo;
+    const int MM = IN.MM;
+    const int m = IN.m[P];
+    if (m > KM) {
+        atomicOr(err, 2);
+        return;
+    }
+    const int ts = s - slot_base[P];
+    const int t = (ts == 0) ? m : ts - 1;
+    const double* A = IN.A + (int64_t)P * MM * D;
+    const double* p = IN.p + (int64_t)P * MM;
+    const double* b = IN.b + (int64_t)P * D;
+    const double* H = sc->H;
+    uint32_t integ = 0, negs = 0;
+    for (int l = 0; l < m; l++) {
+        const double h = dev_dot<D>(H, A + l * D);
+        if (!(fabs(h) < kMuEps)) {
+            integ |= (1u << l);
+            if (!(h > 0)) negs |= (1u << l);
+        }
+    }
+    double* g_out = gy + (int64_t)s * (2 + 2 * D);
+    if (t < m && !((integ >> t) & 1u)) {
+        OUT.flags[s] = 0;
+        keys_out[s] = 0xffffffffu;
+        chain_key[s] = 15u;
+        for (int kk = 0; kk < 2 + 2 * D; kk++) g_out[kk] = 0.0;
+        return;
+    }
+    chain_key[s] = (uint32_t)IN.tid[P];
+    const double zeta = sc->msmt - dev_dot<D>(H, b);
+    double mut[D];
+    double rho_t;
+    if (t == m) {
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) mut[kk] = 0.0;
+        rho_t = sc->gamma;
+    } else {
+        const double h = dev_dot<D>(H, A + t * D);
+        const double r = 1.0 / h;
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) mut[kk] = A[t * D + kk] * r;
+        rho_t = p[t] * fabs(h);
+    }
+    double cb[D];
+#pragma unroll
+    for (int kk = 0; kk < D; kk++) cb[kk] = b[kk] + zeta * mut[kk];
+    // child rows: row r comes from parent row r below t and row r + 1 from t on;
+    // parent row m stands for the measurement-noise row (-mu_t, gamma)
+    double cA[KM][D], cp[KM];
+    uint32_t hf = 0;
+#pragma unroll
+    for (int r = 0; r < KM; r++) {
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) cA[r][kk] = 0.0;
+        cp[r] = 0.0;
+        if (r < m) {
+            const int src = (t < m && r >= t) ? r + 1 : r;
+            if (src == m) {
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) cA[r][kk] = 0.0 - mut[kk];
+                cp[r] = sc->gamma;
+            } else {
+                const double* Ar = A + src * D;
+                const double h = dev_dot<D>(H, Ar);
+                if (fabs(h) < kMuEps) {
+#pragma unroll
+                    for (int kk = 0; kk < D; kk++) cA[r][kk] = Ar[kk];
+                    cp[r] = p[src];
+                    hf |= (1u << r);
+                } else {
+                    const double rr = 1.0 / h;
+                    cp[r] = p[src] * fabs(h);
+#pragma unroll
+                    for (int kk = 0; kk < D; kk++) cA[r][kk] = (Ar[kk] * rr) - mut[kk];
+                }
+            }
+        }
+    }
+    const int phc = IN.phc[P];
+    OUT.c[s] = zeta;
+    OUT.dv[s] = rho_t;
+    OUT.phc[s] = phc;
+    OUT.pbc[s] = m;
+    OUT.z[s] = t;
+    OUT.gp_off[s] = IN.gp_off[P];
+    OUT.gp_n[s] = IN.gp_n[P];
+    const uint32_t enc = negs;
+    uint32_t lhp = enc;
+    if (phc < m) lhp &= (1u << phc) - 1u;
+    uint32_t sg = enc;
+    const uint32_t rev_m = (m >= 32) ? 0xffffffffu : ((1u << m) - 1u);
+    if (sg & (1u << (m - 1))) sg ^= rev_m;
+    OUT.lhp[s] = (int)lhp;
+    OUT.bt_off[s] = IN.bt_off[P];
+    OUT.bt_n[s] = IN.bt_n[P];
+    OUT.bt_sgn[s] = sc->skip ? 0 : (int)sg;
+    OUT.tid[s] = IN.tid[P];
+    double* ob = OUT.b + (int64_t)s * D;
+#pragma unroll
+    for (int kk = 0; kk < D; kk++) ob[kk] = cb[kk];
+    // moments: g and y_ei of this term at the root point (eval_g_yei,
+    // cauchy_term.hpp:312-401; cache_moments, cauchy_estimator.hpp:307-338)
+    {
+        double ty[D];
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) ty[kk] = 0.0;
+        double ygi = 0.0;
+        uint32_t neg = 0;
+#pragma unroll
+        for (int l = 0; l < KM; l++) {
+            if (l < m) {
+                double dr = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) dr += cA[l][kk] * sc->root[kk];
+                const double sgn = dr > 0 ? 1 : -1;
+                if (sgn < 0) neg |= (1u << l);
+                const double scl = cp[l] * sgn;
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) ty[kk] += scl * cA[l][kk];
+                if (!((hf >> l) & 1u)) ygi += cp[l] * sgn;
+            }
+        }
+        cplx gpl, gmn;
+        if (sc->first) {
+            gpl = cmake(1, 0);
+            gmn = cmake(1, 0);
+        } else {
+            const int top = 1 << (phc - 1), rev = (phc >= 32) ? -1 : ((1 << phc) - 1);
+            int lp = 0, lmv = 0, kq = 0;
+            for (int l = 0; l < m; l++) {
+                if (kq < phc) {
+                    if (kq == t) {
+                        lmv |= (1 << kq);
+                        kq++;
+                        if (kq == phc) continue;
+                    }
+                    if ((neg >> l) & 1u) {
+                        lp |= (1 << kq);
+                        lmv |= (1 << kq);
+                    }
+                    kq++;
+                }
+            }
+            const int64_t go = IN.gp_off[P];
+            gpl = lookup_num(lp ^ (int)lhp, top, rev, gk_prev + go, gv_prev + go, IN.gp_n[P]);
+            gmn = lookup_num(lmv ^ (int)lhp, top, rev, gk_prev + go, gv_prev + go, IN.gp_n[P]);
+        }
+        const cplx g = cscale(csub(cdiv(gpl, cmake(ygi + rho_t, zeta)), cdiv(gmn, cmake(ygi - rho_t, zeta))),
+                              kInvTwoPi);
+        g_out[0] = g.re;
+        g_out[1] = g.im;
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) {
+            g_out[2 + 2 * kk] = -ty[kk];
+            g_out[3 + 2 * kk] = cb[kk];
+        }
+    }
+    int mnew = m;
+    uint8_t fl = kValid | ((t != m) ? kIsNew : 0);
+    double* oA = OUT.A + (int64_t)s * MM * D;
+    double* op = OUT.p + (int64_t)s * MM;
+    double* oq = OUT.q + (int64_t)s * MM;
+    bool in_memory = false;  // rows already written and coaligned in memory
+    if (!sc->skip && !sc->first) {
+        // L1 norms of the rows (normalize_hps, cauchy_term.hpp:458-474)
+        double n1[KM];
+#pragma unroll
+        for (int r = 0; r < KM; r++) {
+            n1[r] = 1.0;
+            if (r < m) {
+                double a = 0;
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) a += fabs(cA[r][kk]);
+                n1[r] = a;
+            }
+        }
+        bool cand = false;
+        if (t != m) {
+            // Two normalised rows coalign (mu_coalign, cauchy_term.hpp:533-745) only
+            // if |x_j - x_k| < eps or |x_j + x_k| < eps on every coordinate, so only
+            // if ||x_j0| - |x_k0|| < eps on the first (in floating point the latter
+            // equals one of the former exactly).  Without such a pair nothing merges.
+            double x0[KM];
+#pragma unroll
+            for (int r = 0; r < KM; r++) x0[r] = (r < m) ? fabs(cA[r][0] / n1[r]) : 0.0;
+#pragma unroll
+            for (int j = 0; j < KM - 1; j++)
+#pragma unroll
+                for (int kx = j + 1; kx < KM; kx++)
+                    if (kx < m && fabs(x0[j] - x0[kx]) < kCoalignEps) cand = true;
+        }
+        if (cand) {
+            // possible merge: the in-memory coalignment on the rows as built
+#pragma unroll
+            for (int r = 0; r < KM; r++)
+                if (r < m) {
+#pragma unroll
+                    for (int kk = 0; kk < D; kk++) oA[r * D + kk] = cA[r][kk];
+                    op[r] = cp[r];
+                }
+            uint8_t* cm = OUT.cmap + (int64_t)s * MM;
+            int8_t* cs = OUT.csmap + (int64_t)s * MM;
+            mnew = dev_mu_coalign<D>(oA, op, oq, cm, cs, &hf, m, err);
+            if (mnew < m) fl |= kHasCmap;
+            in_memory = true;
+        } else {
+            // q <- p, rows normalised; a new child's maps are the identity
+            uint8_t* cm = OUT.cmap + (int64_t)s * MM;
+            int8_t* cs = OUT.csmap + (int64_t)s * MM;
+#pragma unroll
+            for (int r = 0; r < KM; r++)
+                if (r < m) {
+                    oq[r] = cp[r];
+                    cp[r] *= n1[r];
+#pragma unroll
+                    for (int kk = 0; kk < D; kk++) cA[r][kk] /= n1[r];
+                    if (t != m) {
+                        cm[r] = (uint8_t)r;
+                        cs[r] = 1;
+                    }
+                }
+        }
+    }
+    if (!in_memory) {
+#pragma unroll
+        for (int r = 0; r < KM; r++)
+            if (r < m) {
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) oA[r * D + kk] = cA[r][kk];
+                op[r] = cp[r];
+            }
+    }
+    OUT.hflag[s] = hf;
+    OUT.flags[s] = fl;
+    OUT.m[s] = mnew;
+    keys_out[s] = (uint32_t)(((mnew * sc->T) + IN.tid[P]) * 2 + ((t != m) ? 1 : 0));
+}
+
 // ------------------------------------------------------------------ moments
 // Moment sums in the reference's exact order.  Reference thread t owns parents
 // i = t, t+T, ... of every shape bucket (cauchy_estimator.hpp:1575-1604) and adds,
