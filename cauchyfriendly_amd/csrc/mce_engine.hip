@@ -694,8 +694,11 @@ struct Engine {
             HIPCHK(hipGetLastError());
             const int nenum = read1(counters.ptr);
             if (nenum > 0) {
+                // 2^MM-bit candidate set plus the 2^(MM-1)-bit parent key set
                 const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
-                const size_t lds = (size_t)words * 4;
+                const size_t lds = (size_t)words * 4 + (size_t)std::max(1, words / 2) * 4;
+                if (lds > 160 * 1024)
+                    throw std::runtime_error("hyperplane count too large for the cell enumerator's LDS sign-vector set");
                 MCE_DISPATCH_D(d, {
                     HIPCHK(hipFuncSetAttribute((const void*)k_btab_enum<DD>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

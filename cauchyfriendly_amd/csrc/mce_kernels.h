@@ -138,10 +138,33 @@ __device__ inline void dev_pivot_inverse(const int* piv, int* pt) {
 }
 
 // One-norm condition number via explicit inverse, cauchy_linalg.hpp:1337-1392.
+// The callers only compare it with kCondEps.  A cheap bound decides first: every
+// entry of A^-1 is a cofactor over det A, and by Hadamard's inequality a cofactor is
+// at most rn^(D-1), rn the largest row 1-norm, so cond1 <= ||A||_1 * D * rn^(D-1) /
+// |det A|, with |det A| the product of the LU pivots.  When that bound is below
+// kCondScreen (three orders of magnitude under kCondEps, far beyond the rounding of
+// the pivots) the bound is returned; otherwise the reference's value is computed.
+constexpr double kCondScreen = 1e9;
 template <int D>
 __device__ inline double dev_cond1(double* A, int* piv) {
     const double na = dev_one_norm<D>(A);
+    double rn = 0;
+#pragma unroll
+    for (int r = 0; r < D; r++) {
+        double s = 0;
+#pragma unroll
+        for (int c = 0; c < D; c++) s += fabs(A[r * D + c]);
+        rn = fmax(rn, s);
+    }
     if (!dev_lu<D>(A, piv)) return 1.7976931348623157e308;
+    {
+        double det = 1.0, bnd = na * (double)D;
+#pragma unroll
+        for (int i = 0; i < D; i++) det *= fabs(A[i * D + i]);
+#pragma unroll
+        for (int i = 0; i < D - 1; i++) bnd *= rn;
+        if (bnd <= kCondScreen * det) return bnd / det;
+    }
     int pt[D];
     dev_pivot_inverse<D>(piv, pt);
     // one-norm of the inverse = largest absolute row sum of W, whose row i solves
@@ -350,8 +373,23 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
     const int m = P.m[i];
     const int phc = P.phc[i];
     const int W = m >= 5 ? (1 << (m - 5)) : 1;
+    // bitset of the parent's g-table keys (all below 2^(phc-1)), after the 2^MM-bit
+    // set of candidate sign vectors
+    uint32_t* pbits = bits + (MM >= 5 ? (1 << (MM - 5)) : 1);
+    const uint32_t ptop = 1u << (phc - 1);
+    const int PW = (int)((ptop + 31u) >> 5);
     for (int w = threadIdx.x; w < W; w += blockDim.x) bits[w] = 0;
+    for (int w = threadIdx.x; w < PW; w += blockDim.x) pbits[w] = 0;
     __syncthreads();
+    {
+        const uint32_t* gk = gkeys + P.gp_off[i];
+        const int gn = P.gp_n[i];
+        for (int q = threadIdx.x; q < gn; q += blockDim.x) {
+            const uint32_t key = gk[q];
+            if (key < ptop) atomicOr(&pbits[key >> 5], 1u << (key & 31));
+            else atomicOr(err, 4);
+        }
+    }
     const double* A = P.A + (int64_t)i * MM * D;
     const double* bp = bpert + (int64_t)P.tid[i] * MM;
     const uint8_t* cmb = combos + combo_off[m];
@@ -390,8 +428,6 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
     }
     __syncthreads();
     // keep sign vectors whose parent part is a cell of the parent arrangement
-    const uint32_t* gk = gkeys + P.gp_off[i];
-    const int gn = P.gp_n[i];
     const uint32_t top_phc = 1u << (phc - 1), rev_phc = (phc >= 32) ? 0xffffffffu : ((1u << phc) - 1u);
     const uint32_t valid_mask = (m >= 5) ? 0xffffffffu : ((1u << (1 << m)) - 1u);
     for (int w = threadIdx.x; w < W; w += blockDim.x) {
@@ -401,7 +437,7 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
             x &= x - 1;
             uint32_t psv = ((uint32_t)(w * 32 + bt)) & rev_phc;
             if (psv & top_phc) psv ^= rev_phc;
-            if (bsearch_u32(gk, psv, gn) >= 0) keep |= (1u << bt);
+            if ((pbits[psv >> 5] >> (psv & 31)) & 1u) keep |= (1u << bt);
         }
         bits[w] = keep;
     }
