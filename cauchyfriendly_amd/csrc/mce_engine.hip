@@ -293,7 +293,10 @@ struct Engine {
     DBuf<uint8_t> cub_tmp;
     DBuf<unsigned long long> bytes_acc;
     DBuf<int> pstat;
-    DBuf<int> defer_k0, big_icnt, big_ioff, big_hcnt, big_hoff;
+    static constexpr size_t kStageBytes = 4096;
+    uint8_t* h_stage = nullptr;  // pinned staging for device-to-host scalar reads
+    DBuf<int64_t> d_scal;        // device scalars (scan totals)
+    DBuf<int> defer_k0, big_icnt, big_ioff, big_hcnt, big_hoff, hist_mx;
     DBuf<int64_t> big_ccnt, big_coff;
     DBuf<cplx> contrib;
     DBuf<uint8_t> cneg;
@@ -326,6 +329,8 @@ struct Engine {
         last_fz = cmake(0, 0);
         HIPCHK(hipStreamCreate(&st));
         HIPCHK(hipStreamCreate(&st2));
+        HIPCHK(hipHostMalloc((void**)&h_stage, kStageBytes, hipHostMallocDefault));
+        d_scal.reserve(16);
         HIPCHK(hipEventCreateWithFlags(&ev_mu, hipEventDisableTiming));
         HIPCHK(hipEventCreate(&ev_c0));
         HIPCHK(hipEventCreate(&ev_c1));
@@ -370,6 +375,7 @@ struct Engine {
     }
 
     ~Engine() {
+        if (h_stage) (void)hipHostFree(h_stage);
         if (st) (void)hipStreamDestroy(st);
         if (st2) (void)hipStreamDestroy(st2);
         if (ev_mu) (void)hipEventDestroy(ev_mu);
@@ -410,12 +416,31 @@ struct Engine {
         HIPCHK(f((void*)cub_tmp.ptr, bytes));
     }
 
+    // Device scalars are read through pinned host memory, one copy and one stream
+    // synchronisation per read (several values at once where possible).
     template <class T>
     T read1(const T* dptr) {
-        T v;
-        HIPCHK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(h_stage, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        T v;
+        memcpy(&v, h_stage, sizeof(T));
         return v;
+    }
+    template <class T>
+    void readn(const T* dptr, T* out_, int n) {
+        HIPCHK(hipMemcpyAsync(h_stage, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        memcpy(out_, h_stage, n * sizeof(T));
+    }
+    // off[n-1] + cnt[n-1] (the total of an exclusive scan) into device scalar `slot`.
+    template <class A, class B>
+    void enqueue_total(const A* off, const B* cnt, int n, int slot) {
+        hipLaunchKernelGGL((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
+    }
+    template <class A, class B>
+    int64_t read_total(const A* off, const B* cnt, int n) {
+        enqueue_total(off, cnt, n, 0);
+        return read1(d_scal.ptr);
     }
 
     // Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.
@@ -684,9 +709,7 @@ struct Engine {
             // B arena offsets (64-bit)
             widen(btcap.ptr, btcap64.ptr, np);
             scan_i64_excl(btcap64.ptr, bt_base.ptr, np);
-            int64_t last_off = read1(bt_base.ptr + np - 1);
-            int last_cap = read1(btcap.ptr + np - 1);
-            barena.reserve((size_t)(last_off + last_cap) + 1);
+            barena.reserve((size_t)read_total(bt_base.ptr, btcap.ptr, np) + 1);
             enum_list.reserve(np);
             HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
             hipLaunchKernelGGL(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
@@ -889,7 +912,7 @@ struct Engine {
                            (int*)b2n.ptr);
         widen(b2n.ptr, ub64.ptr, ng);
         scan_i64_excl(ub64.ptr, b2off.ptr, ng);
-        const int64_t ub_total = read1(b2off.ptr + ng - 1) + read1(ub64.ptr + ng - 1);
+        const int64_t ub_total = read_total(b2off.ptr, ub64.ptr, ng);
         b2arena.reserve((size_t)ub_total + 1);
         for (int m = 1; m < sr; m++) {
             const int ngb = gbase_of[m + 1] - gbase_of[m];
@@ -907,7 +930,7 @@ struct Engine {
         }
         widen(b2n.ptr, b2n64.ptr, ng);
         scan_i64_excl(b2n64.ptr, goff.ptr, ng);
-        const int64_t cells_total = read1(goff.ptr + ng - 1) + read1(b2n64.ptr + ng - 1);
+        const int64_t cells_total = read_total(goff.ptr, b2n64.ptr, ng);
         prof.mark(kPhGBtab, st);
 
         // ---- group g-tables need the update's g scale 1/(2 pi Re fz)
@@ -1012,8 +1035,10 @@ struct Engine {
             prof.max_group = std::max(prof.max_group, (double)(ps_[0] + 1));
         }
         if (ge_mode >= kLookupRank) big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
-        const int nsort = read1(counters.ptr + 2);
-        const int rare = read1(counters.ptr + 3);
+        int nr_[2];
+        readn(counters.ptr + 2, nr_, 2);
+        const int nsort = nr_[0];
+        const int rare = nr_[1];
         if (rare & 4) throw std::runtime_error("parent g-table key outside its half-storage range");
         if (rare) fprintf(stderr, "mce: a parent term joined a reduction group (flag %d); treated as a new child\n", rare);
         if (nsort > 0) {
@@ -1034,8 +1059,7 @@ struct Engine {
         spos.reserve(ng);
         hipLaunchKernelGGL(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
         scan_i32_excl(sflag.ptr, spos.ptr, ng);
-        const int nsurv = read1(spos.ptr + ng - 1) + read1(sflag.ptr + ng - 1);
-        in.reserve(std::max(nsurv, 1));
+        in.reserve(std::max(ng, 1));  // survivors <= groups; their count comes with the histogram
         TermPool NEXT = in.view();
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
                                              surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
@@ -1043,14 +1067,16 @@ struct Engine {
         HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
         hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
                            (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist.ptr);
-        std::vector<int> h(sr + 1);
-        HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        std::vector<int> h(sr + 2);
         {
-            DBuf<int>& mx = gbase_dev;  // scalar scratch
-            mx.reserve(1);
-            cub([&](void* t_, size_t& b_) { return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, mx.ptr, ng, st); });
-            max_gp_cells = read1(mx.ptr);
+            // histogram and the largest g-table (the next update's parent tables) in one read
+            hist_mx.reserve(sr + 2);
+            HIPCHK(hipMemcpyAsync(hist_mx.ptr, hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
+            cub([&](void* t_, size_t& b_) {
+                return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, hist_mx.ptr + sr + 1, ng, st);
+            });
+            readn(hist_mx.ptr, h.data(), sr + 2);
+            max_gp_cells = h[sr + 1];
         }
         prof.mark(kPhCompact, st);
         in_count.assign(sr, 0);
@@ -1078,10 +1104,15 @@ struct Engine {
         scan_i32_excl(big_icnt.ptr, big_ioff.ptr, ng);
         scan_i32_excl(big_hcnt.ptr, big_hoff.ptr, ng);
         scan_i64_excl(big_ccnt.ptr, big_coff.ptr, ng);
-        const int ni = read1(big_ioff.ptr + ng - 1) + read1(big_icnt.ptr + ng - 1);
+        enqueue_total(big_ioff.ptr, big_icnt.ptr, ng, 0);
+        enqueue_total(big_hoff.ptr, big_hcnt.ptr, ng, 1);
+        enqueue_total(big_coff.ptr, big_ccnt.ptr, ng, 2);
+        int64_t tt[3];
+        readn(d_scal.ptr, tt, 3);
+        const int ni = (int)tt[0];
         if (ni == 0) return;
-        const int nh = read1(big_hoff.ptr + ng - 1) + read1(big_hcnt.ptr + ng - 1);
-        const int64_t nc = read1(big_coff.ptr + ng - 1) + read1(big_ccnt.ptr + ng - 1);
+        const int nh = (int)tt[1];
+        const int64_t nc = tt[2];
         contrib.reserve((size_t)std::max<int64_t>(nc, 1));
         cneg.reserve(ni);
         TermPool OUT = out.view();
@@ -1151,7 +1182,7 @@ struct Engine {
         woff.reserve(N);
         hipLaunchKernelGGL(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr);
         scan_i64_excl(wcnt.ptr, woff.ptr, N);
-        const int64_t npairs = read1(woff.ptr + N - 1) + read1(wcnt.ptr + N - 1);
+        const int64_t npairs = read_total(woff.ptr, wcnt.ptr, N);
         const bool by_pairs = npairs < ((int64_t)1 << 31);
         HIPCHK(hipMemsetAsync(ecnt.ptr, 0, N * sizeof(int), st));
         if (by_pairs && npairs > 0) {
@@ -1170,7 +1201,7 @@ struct Engine {
         }
         HIPCHK(hipGetLastError());
         scan_i32_excl(ecnt.ptr, eoff.ptr, N);
-        const int E = read1(eoff.ptr + N - 1) + read1(ecnt.ptr + N - 1);
+        const int E = (int)read_total(eoff.ptr, ecnt.ptr, N);
         Fc.reserve((size_t)8 * N);
         hipLaunchKernelGGL(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
         if (E > 0) {
@@ -1246,23 +1277,18 @@ struct Engine {
                                                       st);
         });
         gbase_dev.reserve(sr + 1);
-        const int nroots = read1(counters.ptr + 5);
-        const int nm = N - nroots;
-        hipLaunchKernelGGL(k_ftr_gbase, dim3(1), dim3(64), 0, st, nroots, roots.ptr, boff, sr, gbase_dev.ptr);
+        const int* d_nroots = counters.ptr + 5;
+        hipLaunchKernelGGL(k_ftr_gbase, dim3(1), dim3(64), 0, st, d_nroots, roots.ptr, boff, sr, gbase_dev.ptr);
         HIPCHK(hipMemsetAsync(memcnt.ptr, 0, (N + 1) * sizeof(int), st));
-        if (nm > 0) {
-            hipLaunchKernelGGL(k_ftr_memcount, dim3(ceil_div(nm, 256)), b1, 0, st, nm, smkey.ptr, memcnt.ptr);
-            hipLaunchKernelGGL(k_ftr_memslots, dim3(ceil_div(nm, 256)), b1, 0, st, nm, smval.ptr, perm.ptr,
-                               mem_slots.ptr);
-        }
+        hipLaunchKernelGGL(k_ftr_memcount, g1, b1, 0, st, N, d_nroots, smkey.ptr, memcnt.ptr);
+        hipLaunchKernelGGL(k_ftr_memslots, g1, b1, 0, st, N, d_nroots, smval.ptr, perm.ptr, mem_slots.ptr);
         scan_i32_excl(memcnt.ptr, memoff.ptr, N);
-        hipLaunchKernelGGL(k_ftr_groups, dim3(ceil_div(std::max(nroots, 1), 256)), b1, 0, st, nroots, roots.ptr,
-                           perm.ptr, boff, sr, memcnt.ptr, memoff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr,
-                           grp_m.ptr);
+        hipLaunchKernelGGL(k_ftr_groups, g1, b1, 0, st, d_nroots, roots.ptr, perm.ptr, boff, sr, memcnt.ptr,
+                           memoff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(gbase_of.data(), gbase_dev.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        return nroots;
+        // group offsets per shape; the last one is the root count
+        readn(gbase_dev.ptr, gbase_of.data(), sr + 1);
+        return gbase_of[sr];
     }
 
     // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).

@@ -1563,24 +1563,27 @@ __global__ void k_ftr_rootflags(int n, const int* __restrict__ boff, int nb, con
     memval[p] = (uint32_t)p;
 }
 
-__global__ void k_ftr_memcount(int nm, const uint32_t* __restrict__ skey, int* __restrict__ cnt) {
+// The root count is read on the device (grids are sized by the term count n).
+__global__ void k_ftr_memcount(int n, const int* __restrict__ nroots, const uint32_t* __restrict__ skey,
+                               int* __restrict__ cnt) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= n - *nroots) return;
     atomicAdd(&cnt[skey[k]], 1);
 }
 
-__global__ void k_ftr_memslots(int nm, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ perm,
-                               int* __restrict__ mem_slots) {
+__global__ void k_ftr_memslots(int n, const int* __restrict__ nroots, const uint32_t* __restrict__ sval,
+                               const uint32_t* __restrict__ perm, int* __restrict__ mem_slots) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= n - *nroots) return;
     mem_slots[k] = (int)perm[sval[k]];
 }
 
 // First reduction group of every bucket: gbase[m] = number of roots before boff[m].
-__global__ void k_ftr_gbase(int nroots, const uint32_t* __restrict__ roots, const int* __restrict__ boff, int nb,
-                            int* __restrict__ gbase) {
+__global__ void k_ftr_gbase(const int* __restrict__ d_nroots, const uint32_t* __restrict__ roots,
+                            const int* __restrict__ boff, int nb, int* __restrict__ gbase) {
     const int m = threadIdx.x;
     if (m > nb) return;
+    const int nroots = *d_nroots;
     const uint32_t x = (uint32_t)boff[m];
     int lo = 0, hi = nroots;
     while (lo < hi) {
@@ -1592,12 +1595,13 @@ __global__ void k_ftr_gbase(int nroots, const uint32_t* __restrict__ roots, cons
 }
 
 // Reduction groups in the reference's order: buckets ascending, roots ascending.
-__global__ void k_ftr_groups(int nroots, const uint32_t* __restrict__ roots, const uint32_t* __restrict__ perm,
-                             const int* __restrict__ boff, int nb, const int* __restrict__ memcnt,
-                             const int* __restrict__ memoff, int* __restrict__ grp_root, int* __restrict__ grp_moff,
-                             int* __restrict__ grp_mcnt, int* __restrict__ grp_m) {
+__global__ void k_ftr_groups(const int* __restrict__ nroots, const uint32_t* __restrict__ roots,
+                             const uint32_t* __restrict__ perm, const int* __restrict__ boff, int nb,
+                             const int* __restrict__ memcnt, const int* __restrict__ memoff,
+                             int* __restrict__ grp_root, int* __restrict__ grp_moff, int* __restrict__ grp_mcnt,
+                             int* __restrict__ grp_m) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nroots) return;
+    if (g >= *nroots) return;
     const uint32_t r = roots[g];
     grp_root[g] = (int)perm[r];
     grp_moff[g] = memoff[r];
@@ -2413,6 +2417,12 @@ __global__ void k_chain_off(const uint32_t* __restrict__ skeys, int n, int T, in
 }
 
 // ------------------------------------------------------------------ small helpers
+// Total of an exclusive scan: off[n-1] + cnt[n-1].
+template <class A, class B>
+__global__ void k_scan_total(const A* __restrict__ off, const B* __restrict__ cnt, int n, int64_t* __restrict__ out) {
+    out[0] = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
+}
+
 __global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (int64_t)src[i];
