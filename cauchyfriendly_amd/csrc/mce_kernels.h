@@ -771,12 +771,40 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
     keys_out[s] = (uint32_t)(((mnew * sc->T) + IN.tid[P]) * 2 + ((t != m) ? 1 : 0));
 }
 
-// Measurement update with every child row in registers (KM >= m, compile-time
-// bound; all row indices are static after unrolling).  Same operations in the same
-// order as k_mu; the child's rows are written once, and only a coalignment that
-// merged rows (a rare case) compacts them in memory afterwards.
+// Child row r of slot (P, t) as k_mu builds it: parent row r below t and row r + 1
+// from t on, parent row m standing for the measurement-noise row (-mu_t, gamma).
+// Returns the row's p value; sets *horth when the parent row is H-orthogonal.
+template <int D>
+__device__ inline double dev_child_row(const double* A, const double* p, const double* H, const double* mut,
+                                       double gamma, int m, int t, int r, double* row, bool* horth) {
+    const int src = (t < m && r >= t) ? r + 1 : r;
+    *horth = false;
+    if (src == m) {
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) row[kk] = 0.0 - mut[kk];
+        return gamma;
+    }
+    const double* Ar = A + src * D;
+    const double h = dev_dot<D>(H, Ar);
+    if (fabs(h) < kMuEps) {
+#pragma unroll
+        for (int kk = 0; kk < D; kk++) row[kk] = Ar[kk];
+        *horth = true;
+        return p[src];
+    }
+    const double rr = 1.0 / h;
+#pragma unroll
+    for (int kk = 0; kk < D; kk++) row[kk] = (Ar[kk] * rr) - mut[kk];
+    return p[src] * fabs(h);
+}
+
+// Measurement update streamed row by row (KM >= m, compile-time bound).  The same
+// operations in the same order as k_mu: each child row is built, enters the moment
+// sums, is L1-normalised and written once; only the normalised first coordinates
+// are kept, for the coalignment filter.  When the filter finds a possible merge (a
+// rare case) the rows are rebuilt in memory and coaligned there, as k_mu does.
 template <int D, int KM>
-__global__ __launch_bounds__(256) void k_mu_r(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
+__global__ __launch_bounds__(256, 4) void k_mu_r(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
                                              int nslots, const StepConst* __restrict__ sc,
                                              const uint32_t* __restrict__ gk_prev, const cplx* __restrict__ gv_prev,
                                              double* __restrict__ gy, uint32_t* __restrict__ keys_out,
@@ -836,38 +864,6 @@ __global__ __launch_bounds__(256) void k_mu_r(TermPool IN, int np, const int* __
     double cb[D];
 #pragma unroll
     for (int kk = 0; kk < D; kk++) cb[kk] = b[kk] + zeta * mut[kk];
-    // child rows: row r comes from parent row r below t and row r + 1 from t on;
-    // parent row m stands for the measurement-noise row (-mu_t, gamma)
-    double cA[KM][D], cp[KM];
-    uint32_t hf = 0;
-#pragma unroll
-    for (int r = 0; r < KM; r++) {
-#pragma unroll
-        for (int kk = 0; kk < D; kk++) cA[r][kk] = 0.0;
-        cp[r] = 0.0;
-        if (r < m) {
-            const int src = (t < m && r >= t) ? r + 1 : r;
-            if (src == m) {
-#pragma unroll
-                for (int kk = 0; kk < D; kk++) cA[r][kk] = 0.0 - mut[kk];
-                cp[r] = sc->gamma;
-            } else {
-                const double* Ar = A + src * D;
-                const double h = dev_dot<D>(H, Ar);
-                if (fabs(h) < kMuEps) {
-#pragma unroll
-                    for (int kk = 0; kk < D; kk++) cA[r][kk] = Ar[kk];
-                    cp[r] = p[src];
-                    hf |= (1u << r);
-                } else {
-                    const double rr = 1.0 / h;
-                    cp[r] = p[src] * fabs(h);
-#pragma unroll
-                    for (int kk = 0; kk < D; kk++) cA[r][kk] = (Ar[kk] * rr) - mut[kk];
-                }
-            }
-        }
-    }
     const int phc = IN.phc[P];
     OUT.c[s] = zeta;
     OUT.dv[s] = rho_t;
@@ -890,28 +886,51 @@ __global__ __launch_bounds__(256) void k_mu_r(TermPool IN, int np, const int* __
     double* ob = OUT.b + (int64_t)s * D;
 #pragma unroll
     for (int kk = 0; kk < D; kk++) ob[kk] = cb[kk];
-    // moments: g and y_ei of this term at the root point (eval_g_yei,
-    // cauchy_term.hpp:312-401; cache_moments, cauchy_estimator.hpp:307-338)
-    {
-        double ty[D];
+    double* oA = OUT.A + (int64_t)s * MM * D;
+    double* op = OUT.p + (int64_t)s * MM;
+    double* oq = OUT.q + (int64_t)s * MM;
+    const bool normalise = !sc->skip && !sc->first;
+    // rows: moment contributions (eval_g_yei, cauchy_term.hpp:312-401), then
+    // q <- p and the L1 normalisation (normalize_hps, cauchy_term.hpp:458-474)
+    double ty[D];
 #pragma unroll
-        for (int kk = 0; kk < D; kk++) ty[kk] = 0.0;
-        double ygi = 0.0;
-        uint32_t neg = 0;
+    for (int kk = 0; kk < D; kk++) ty[kk] = 0.0;
+    double ygi = 0.0;
+    uint32_t neg = 0, hf = 0;
+    double x0[KM];
 #pragma unroll
-        for (int l = 0; l < KM; l++) {
-            if (l < m) {
-                double dr = 0.0;
+    for (int r = 0; r < KM; r++) {
+        x0[r] = 0.0;
+        if (r < m) {
+            double row[D];
+            bool ho;
+            double pr = dev_child_row<D>(A, p, H, mut, sc->gamma, m, t, r, row, &ho);
+            if (ho) hf |= (1u << r);
+            double dr = 0.0;
 #pragma unroll
-                for (int kk = 0; kk < D; kk++) dr += cA[l][kk] * sc->root[kk];
-                const double sgn = dr > 0 ? 1 : -1;
-                if (sgn < 0) neg |= (1u << l);
-                const double scl = cp[l] * sgn;
+            for (int kk = 0; kk < D; kk++) dr += row[kk] * sc->root[kk];
+            const double sgn = dr > 0 ? 1 : -1;
+            if (sgn < 0) neg |= (1u << r);
+            const double scl = pr * sgn;
 #pragma unroll
-                for (int kk = 0; kk < D; kk++) ty[kk] += scl * cA[l][kk];
-                if (!((hf >> l) & 1u)) ygi += cp[l] * sgn;
+            for (int kk = 0; kk < D; kk++) ty[kk] += scl * row[kk];
+            if (!ho) ygi += pr * sgn;
+            if (normalise) {
+                double n1 = 0;
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) n1 += fabs(row[kk]);
+                oq[r] = pr;
+                pr *= n1;
+#pragma unroll
+                for (int kk = 0; kk < D; kk++) row[kk] /= n1;
+                x0[r] = fabs(row[0]);
             }
+#pragma unroll
+            for (int kk = 0; kk < D; kk++) oA[r * D + kk] = row[kk];
+            op[r] = pr;
         }
+    }
+    {
         cplx gpl, gmn;
         if (sc->first) {
             gpl = cmake(1, 0);
@@ -949,78 +968,38 @@ __global__ __launch_bounds__(256) void k_mu_r(TermPool IN, int np, const int* __
     }
     int mnew = m;
     uint8_t fl = kValid | ((t != m) ? kIsNew : 0);
-    double* oA = OUT.A + (int64_t)s * MM * D;
-    double* op = OUT.p + (int64_t)s * MM;
-    double* oq = OUT.q + (int64_t)s * MM;
-    bool in_memory = false;  // rows already written and coaligned in memory
-    if (!sc->skip && !sc->first) {
-        // L1 norms of the rows (normalize_hps, cauchy_term.hpp:458-474)
-        double n1[KM];
-#pragma unroll
-        for (int r = 0; r < KM; r++) {
-            n1[r] = 1.0;
-            if (r < m) {
-                double a = 0;
-#pragma unroll
-                for (int kk = 0; kk < D; kk++) a += fabs(cA[r][kk]);
-                n1[r] = a;
-            }
-        }
+    if (normalise && t != m) {
+        // Two normalised rows coalign (mu_coalign, cauchy_term.hpp:533-745) only if
+        // |x_j - x_k| < eps or |x_j + x_k| < eps on every coordinate, so only if
+        // ||x_j0| - |x_k0|| < eps on the first (in floating point the latter equals
+        // one of the former exactly).  Without such a pair nothing merges.
         bool cand = false;
-        if (t != m) {
-            // Two normalised rows coalign (mu_coalign, cauchy_term.hpp:533-745) only
-            // if |x_j - x_k| < eps or |x_j + x_k| < eps on every coordinate, so only
-            // if ||x_j0| - |x_k0|| < eps on the first (in floating point the latter
-            // equals one of the former exactly).  Without such a pair nothing merges.
-            double x0[KM];
 #pragma unroll
-            for (int r = 0; r < KM; r++) x0[r] = (r < m) ? fabs(cA[r][0] / n1[r]) : 0.0;
+        for (int j = 0; j < KM - 1; j++)
 #pragma unroll
-            for (int j = 0; j < KM - 1; j++)
-#pragma unroll
-                for (int kx = j + 1; kx < KM; kx++)
-                    if (kx < m && fabs(x0[j] - x0[kx]) < kCoalignEps) cand = true;
-        }
+            for (int kx = j + 1; kx < KM; kx++)
+                if (kx < m && fabs(x0[j] - x0[kx]) < kCoalignEps) cand = true;
+        uint8_t* cm = OUT.cmap + (int64_t)s * MM;
+        int8_t* cs = OUT.csmap + (int64_t)s * MM;
         if (cand) {
-            // possible merge: the in-memory coalignment on the rows as built
+            // possible merge: rebuild the rows as built before normalisation and run
+            // the in-memory coalignment
+            for (int r = 0; r < m; r++) {
+                double row[D];
+                bool ho;
+                op[r] = dev_child_row<D>(A, p, H, mut, sc->gamma, m, t, r, row, &ho);
 #pragma unroll
-            for (int r = 0; r < KM; r++)
-                if (r < m) {
-#pragma unroll
-                    for (int kk = 0; kk < D; kk++) oA[r * D + kk] = cA[r][kk];
-                    op[r] = cp[r];
-                }
-            uint8_t* cm = OUT.cmap + (int64_t)s * MM;
-            int8_t* cs = OUT.csmap + (int64_t)s * MM;
+                for (int kk = 0; kk < D; kk++) oA[r * D + kk] = row[kk];
+            }
             mnew = dev_mu_coalign<D>(oA, op, oq, cm, cs, &hf, m, err);
             if (mnew < m) fl |= kHasCmap;
-            in_memory = true;
         } else {
-            // q <- p, rows normalised; a new child's maps are the identity
-            uint8_t* cm = OUT.cmap + (int64_t)s * MM;
-            int8_t* cs = OUT.csmap + (int64_t)s * MM;
-#pragma unroll
-            for (int r = 0; r < KM; r++)
-                if (r < m) {
-                    oq[r] = cp[r];
-                    cp[r] *= n1[r];
-#pragma unroll
-                    for (int kk = 0; kk < D; kk++) cA[r][kk] /= n1[r];
-                    if (t != m) {
-                        cm[r] = (uint8_t)r;
-                        cs[r] = 1;
-                    }
-                }
-        }
-    }
-    if (!in_memory) {
-#pragma unroll
-        for (int r = 0; r < KM; r++)
-            if (r < m) {
-#pragma unroll
-                for (int kk = 0; kk < D; kk++) oA[r * D + kk] = cA[r][kk];
-                op[r] = cp[r];
+            // a new child's maps are the identity
+            for (int r = 0; r < m; r++) {
+                cm[r] = (uint8_t)r;
+                cs[r] = 1;
             }
+        }
     }
     OUT.hflag[s] = hf;
     OUT.flags[s] = fl;
