@@ -254,7 +254,7 @@ struct Engine {
     bool moments_pending = false;
     int max_gp_cells = 1;  // largest parent g-table of the current parents
     int geval_mode = -1;   // parent lookup mode of the last g-table launch
-    // MCE_GEVAL_MODE=0..3 selects a lookup mode below the automatic choice (tuning)
+    // MCE_GEVAL_MODE=0..3 selects the g-table lookup mode (tuning; unsupported choices are ignored)
     int geval_mode_override = getenv("MCE_GEVAL_MODE") ? atoi(getenv("MCE_GEVAL_MODE")) : -1;
     // MCE_BIG_GROUP_MEMBERS: groups with more members add them one wavefront per member
     int big_group_members = getenv("MCE_BIG_GROUP_MEMBERS") ? std::max(0, atoi(getenv("MCE_BIG_GROUP_MEMBERS")))
@@ -962,10 +962,10 @@ struct Engine {
             const size_t budget = 48 * 1024;
             int mode;
             size_t lds;
-            if (MM <= 24 && rank_b + vals_b <= budget) {
-                mode = kLookupRankVals;
-                lds = rank_b + vals_b;
-            } else if (MM <= 24 && rank_b <= budget) {
+            // the rank structure with values read from global memory is the default:
+            // the parent values are L2-resident and the smaller LDS footprint keeps more
+            // wavefronts resident than staging the values (measured on MI355X)
+            if (MM <= 24 && rank_b <= budget) {
                 mode = kLookupRank;
                 lds = rank_b;
             } else if (keys_b <= budget) {
@@ -975,9 +975,14 @@ struct Engine {
                 mode = kLookupSearch;
                 lds = 0;
             }
-            if (geval_mode_override >= 0 && geval_mode_override <= mode) {
+            if (geval_mode_override >= 0 && geval_mode_override <= kLookupRankVals &&
+                (geval_mode_override < kLookupRank || mode == kLookupRank) &&
+                (geval_mode_override != kLookupRankVals || rank_b + vals_b <= budget)) {
                 mode = geval_mode_override;
-                lds = mode == kLookupRank ? rank_b : mode == kLookupSearchLds ? keys_b : mode == kLookupSearch ? 0 : lds;
+                lds = mode == kLookupRankVals ? rank_b + vals_b
+                      : mode == kLookupRank   ? rank_b
+                      : mode == kLookupSearchLds ? keys_b
+                                                 : 0;
             }
             geval_mode = mode;
             ge_mode = mode;

@@ -131,3 +131,16 @@ def test_large_groups_parallel_member_path_is_identical(monkeypatch):
             assert a["err"] == b["err"]
             for key in ("fz", "mean", "cov"):
                 assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), (threshold, key)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3])
+@pytest.mark.parametrize("name", ["lti3_w8", "ltv5_p2", "horthog3"])
+def test_gtable_lookup_modes_match_reference(name, mode, monkeypatch):
+    """The g-table kernel's fallback parent lookups (binary search over keys in global
+    memory or LDS, rank structure with LDS-staged values) give the reference's results."""
+    monkeypatch.setenv("MCE_GEVAL_MODE", str(mode))
+    rec = load_golden(name)
+    s = W.System.from_dict(rec["system"])
+    _, out, _ = run_gpu(s, rec["num_steps"])
+    for u, (got, exp) in enumerate(zip(out, rec["updates"])):
+        compare_update(got, exp, TOL, where=f"{name} mode {mode} update {u}")
