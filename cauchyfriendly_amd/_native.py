@@ -45,6 +45,36 @@ _ipp = ct.POINTER(_ip)
 DYN_CALLBACK = ct.CFUNCTYPE(None, ct.c_void_p)
 
 
+
+def declare_pycauchy(L, sym=lambda name: name):
+    """Declares the argument and result types of the pycauchy_* C ABI on the CDLL L;
+    sym maps a pycauchy_* name to the symbol L exports for it."""
+    init_args = [ct.c_int] + [_dp, ct.c_int] * 9
+    getattr(L, sym("pycauchy_initialize_lti")).restype = ct.c_void_p
+    getattr(L, sym("pycauchy_initialize_lti")).argtypes = init_args + [ct.c_double, ct.c_int, ct.c_bool]
+    getattr(L, sym("pycauchy_initialize_ltv")).restype = ct.c_void_p
+    getattr(L, sym("pycauchy_initialize_ltv")).argtypes = init_args + [ct.c_void_p, ct.c_double, ct.c_int, ct.c_bool]
+    getattr(L, sym("pycauchy_single_step_ltiv")).restype = None
+    getattr(L, sym("pycauchy_single_step_ltiv")).argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int] + \
+        [_dpp, _ip] * 12 + [_ipp, _ip]
+    getattr(L, sym("pycauchy_single_step_reset")).argtypes = [ct.c_void_p] + [_dp, ct.c_int] * 4
+    getattr(L, sym("pycauchy_single_step_set_master_step")).argtypes = [ct.c_void_p, ct.c_int]
+    getattr(L, sym("pycauchy_single_step_set_window_number")).argtypes = [ct.c_void_p, ct.c_int]
+    getattr(L, sym("pycauchy_single_step_get_number_of_terms")).restype = ct.c_int
+    getattr(L, sym("pycauchy_single_step_get_number_of_terms")).argtypes = [ct.c_void_p]
+    getattr(L, sym("pycauchy_single_step_deterministic_transform")).argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int]
+    getattr(L, sym("pycauchy_single_step_get_duc")).restype = ct.c_void_p
+    getattr(L, sym("pycauchy_single_step_get_duc")).argtypes = [ct.c_void_p]
+    getattr(L, sym("pycauchy_single_step_shutdown")).argtypes = [ct.c_void_p]
+    getattr(L, sym("pycauchy_set_tr_search_idxs_ordering")).restype = ct.c_int
+    getattr(L, sym("pycauchy_set_tr_search_idxs_ordering")).argtypes = [_ip, ct.c_int]
+    getattr(L, sym("pycauchy_speyers_window_init")).restype = None
+    getattr(L, sym("pycauchy_speyers_window_init")).argtypes = [_dp, ct.c_int] * 3 + [ct.c_double, ct.c_double] + [_dpp, _ip] * 3
+    getattr(L, sym("pycauchy_get_reinitialization_statistics")).restype = None
+    getattr(L, sym("pycauchy_get_reinitialization_statistics")).argtypes = [ct.c_void_p, ct.c_double] + [_dp, ct.c_int] * 3 + \
+        [ct.c_double] + [_dpp, _ip] * 3
+
+
 def lib():
     """The loaded engine; raises if the HIP library is not built."""
     global _lib
@@ -53,25 +83,7 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"HIP engine {LIB_PATH} is not built; run __graft_entry__.build()")
     L = ct.CDLL(LIB_PATH)
-    init_args = [ct.c_int] + [_dp, ct.c_int] * 9
-    L.pycauchy_initialize_lti.restype = ct.c_void_p
-    L.pycauchy_initialize_lti.argtypes = init_args + [ct.c_double, ct.c_int, ct.c_bool]
-    L.pycauchy_initialize_ltv.restype = ct.c_void_p
-    L.pycauchy_initialize_ltv.argtypes = init_args + [ct.c_void_p, ct.c_double, ct.c_int, ct.c_bool]
-    L.pycauchy_single_step_ltiv.restype = None
-    L.pycauchy_single_step_ltiv.argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int] + \
-        [_dpp, _ip] * 12 + [_ipp, _ip]
-    L.pycauchy_single_step_reset.argtypes = [ct.c_void_p] + [_dp, ct.c_int] * 4
-    L.pycauchy_single_step_set_master_step.argtypes = [ct.c_void_p, ct.c_int]
-    L.pycauchy_single_step_set_window_number.argtypes = [ct.c_void_p, ct.c_int]
-    L.pycauchy_single_step_get_number_of_terms.restype = ct.c_int
-    L.pycauchy_single_step_get_number_of_terms.argtypes = [ct.c_void_p]
-    L.pycauchy_single_step_deterministic_transform.argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int]
-    L.pycauchy_single_step_get_duc.restype = ct.c_void_p
-    L.pycauchy_single_step_get_duc.argtypes = [ct.c_void_p]
-    L.pycauchy_single_step_shutdown.argtypes = [ct.c_void_p]
-    L.pycauchy_set_tr_search_idxs_ordering.restype = ct.c_int
-    L.pycauchy_set_tr_search_idxs_ordering.argtypes = [_ip, ct.c_int]
+    declare_pycauchy(L)
     L.mce_estimator_create.restype = ct.c_void_p
     L.mce_estimator_create.argtypes = [_dp, _dp, _dp] + [ct.c_int] * 5 + [ct.c_bool]
     L.mce_estimator_step.restype = ct.c_int

@@ -19,6 +19,7 @@ constexpr double kMuEps = 1e-10;
 constexpr double kApproxEps = 1e-15;
 constexpr double kPluEps = 1e-15;
 constexpr double kCondEps = 1e12;
+constexpr double kCovEigenvalueTol = -1e-5;  // COV_EIGENVALUE_TOLERANCE, cauchy_constants.hpp:102
 constexpr double kReductionEps = 1e-8;
 constexpr double kInvTwoPi = 1.0 / (2.0 * 3.14159265358979323846);
 

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/cauchy_mi355x.h"
+#include "mce_host_linalg.h"
 #include "mce_kernels.h"
 
 using namespace mce;
@@ -145,37 +146,6 @@ int cells_central(int m, int d) {
 }
 
 double rand_u() { return ((double)rand() + 1.0) / ((double)RAND_MAX + 1.0); }  // random_variables.hpp:22-26
-
-// Symmetric eigenvalues by cyclic Jacobi; they only feed the sign test of
-// covariance_checker (cauchy_util.hpp:1882-2004).
-void sym_eigvals(const double* S, double* ev, int n) {
-    std::vector<double> a(S, S + n * n);
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0;
-        for (int i = 0; i < n; i++)
-            for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
-        if (off < 1e-300) break;
-        for (int pp = 0; pp < n; pp++)
-            for (int qq = pp + 1; qq < n; qq++) {
-                double apq = a[pp * n + qq];
-                if (fabs(apq) < 1e-300) continue;
-                double theta = (a[qq * n + qq] - a[pp * n + pp]) / (2 * apq);
-                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                double c = 1 / sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < n; k++) {
-                    double akp = a[k * n + pp], akq = a[k * n + qq];
-                    a[k * n + pp] = c * akp - s * akq;
-                    a[k * n + qq] = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; k++) {
-                    double apk = a[pp * n + k], aqk = a[qq * n + k];
-                    a[pp * n + k] = c * apk - s * aqk;
-                    a[qq * n + k] = s * apk + c * aqk;
-                }
-            }
-    }
-    for (int i = 0; i < n; i++) ev[i] = a[i * n + i];
-}
 
 int g_search_order[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
 
@@ -532,9 +502,12 @@ struct Engine {
                 cr[i] = cov[i].re;
                 ci[i] = cov[i].im;
             }
-            sym_eigvals(cr.data(), ev.data(), d);
+            {
+                std::vector<double> evec((size_t)d * d);
+                host::sym_eig(cr.data(), ev.data(), evec.data(), d);
+            }
             for (int i = 0; i < d; i++)
-                if (ev[i] < -1e-5) cf |= 1;
+                if (ev[i] < kCovEigenvalueTol) cf |= 1;
             for (int i = 0; i < d - 1; i++) {
                 double sii = sqrt(cr[i * d + i]);
                 for (int j = i + 1; j < d; j++) {
@@ -1568,6 +1541,38 @@ int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering) {
     }
     for (int i = 0; i < size_ordering; i++) g_search_order[i] = ordering[i];
     return 0;
+}
+
+void pycauchy_speyers_window_init(double* xhat, int size_xhat, double* Phat, int size_Phat, double* H, int size_H,
+                                  double gamma, double z, double** out_A0, int* size_out_A0, double** out_p0,
+                                  int* size_out_p0, double** out_b0, int* size_out_b0) {
+    const int n = size_xhat;
+    if (size_Phat != n * n || size_H != n) {
+        fprintf(stderr, "[cauchy_mi355x] pycauchy_speyers_window_init: size mismatch\n");
+        exit(1);
+    }
+    *out_A0 = mallocd<double>((size_t)n * n);
+    *size_out_A0 = n * n;
+    *out_p0 = mallocd<double>(n);
+    *size_out_p0 = n;
+    *out_b0 = mallocd<double>(n);
+    *size_out_b0 = n;
+    std::vector<double> var(Phat, Phat + (size_t)n * n);
+    host::speyers_window_init(n, xhat, var.data(), H, gamma, z, *out_A0, *out_p0, *out_b0, kCovEigenvalueTol);
+}
+
+void pycauchy_get_reinitialization_statistics(void* handle, double z, double* xhat, int size_xhat, double* Phat,
+                                               int size_Phat, double* H, int size_H, double gamma, double** out_A0,
+                                               int* size_out_A0, double** out_p0, int* size_out_p0, double** out_b0,
+                                               int* size_out_b0) {
+    Handle* h = (Handle*)handle;
+    const int n = h->duc.n;
+    if (size_xhat != n || size_Phat != n * n || size_H != n) {
+        fprintf(stderr, "[cauchy_mi355x] pycauchy_get_reinitialization_statistics: size mismatch\n");
+        exit(1);
+    }
+    pycauchy_speyers_window_init(xhat, size_xhat, Phat, size_Phat, H, size_H, gamma, z, out_A0, size_out_A0, out_p0,
+                                 size_out_p0, out_b0, size_out_b0);
 }
 
 void* mce_estimator_create(double* A0, double* p0, double* b0, int steps, int d, int cmcc, int pncc, int p,

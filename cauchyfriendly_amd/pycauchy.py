@@ -4,11 +4,17 @@ Same function names, argument order and return tuples as the SWIG bindings of
 scripts/swig/cauchy/pycauchy.i (numpy.i typemaps: an input `double* x, int
 size_x` pair is one numpy array; `double** out, int* size_out` pairs are
 returned as numpy arrays that own malloc'd memory), so the reference's
-`cauchy_estimator.py` (PyCauchyEstimator) runs unchanged on the HIP engine.
+`cauchy_estimator.py` (PyCauchyEstimator, PySlidingWindowManager) runs
+unchanged on the HIP engine.
+
+The module-level functions are bound to the MI355X library.  `bind(cdll, sym)`
+returns the same functions bound to any other library that exports this C ABI
+(sym maps a pycauchy_* name to the library's symbol).
 """
 from __future__ import annotations
 
 import ctypes as ct
+import types
 
 import numpy as np
 
@@ -36,87 +42,131 @@ def _take_i(ptr, n):
     return out
 
 
-def _check(handle, what):
-    if not handle:
-        msg = _native.lib().mce_last_error().decode()
-        raise RuntimeError(f"{what} failed: {msg}")
-    return handle
-
-
-def pycauchy_initialize_lti(num_steps, A0, p0, b0, Phi, Gamma, B, beta, H, gamma, dt, init_step, debug_print):
-    L = _native.lib()
-    args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
+def _three_outputs(call):
+    """Calls a function whose last six arguments are three (double**, int*) outputs."""
+    outs = [_dp() for _ in range(3)]
+    sizes = [ct.c_int(0) for _ in range(3)]
     flat = []
-    for a in args:
-        flat += [a[1], a[2]]
-    h = L.pycauchy_initialize_lti(int(num_steps), *flat, float(dt), int(init_step), bool(debug_print))
-    return _check(h, "pycauchy_initialize_lti")
+    for o, n in zip(outs, sizes):
+        flat += [ct.byref(o), ct.byref(n)]
+    call(*flat)
+    return tuple(_take_d(o, n.value) for o, n in zip(outs, sizes))
 
 
-def pycauchy_initialize_ltv(num_steps, A0, p0, b0, Phi, Gamma, B, beta, H, gamma, f_duc_ptr, dt, init_step,
-                            debug_print):
-    L = _native.lib()
-    args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
-    flat = []
-    for a in args:
-        flat += [a[1], a[2]]
-    if not isinstance(f_duc_ptr, int):
-        f_duc_ptr = ct.cast(f_duc_ptr, ct.c_void_p).value
-    h = L.pycauchy_initialize_ltv(int(num_steps), *flat, ct.c_void_p(f_duc_ptr), float(dt), int(init_step),
-                                  bool(debug_print))
-    return _check(h, "pycauchy_initialize_ltv")
+class _Bindings:
+    """The SWIG module's functions over one library."""
+
+    def __init__(self, get_lib, sym=lambda name: name, last_error=None):
+        self._get_lib = get_lib
+        self._sym = sym
+        self._last_error = last_error
+
+    def _f(self, name):
+        return getattr(self._get_lib(), self._sym(name))
+
+    def _check(self, handle, what):
+        if not handle:
+            msg = self._last_error() if self._last_error else ""
+            raise RuntimeError(f"{what} failed: {msg}")
+        return handle
+
+    def pycauchy_initialize_lti(self, num_steps, A0, p0, b0, Phi, Gamma, B, beta, H, gamma, dt, init_step,
+                                debug_print):
+        args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
+        flat = []
+        for a in args:
+            flat += [a[1], a[2]]
+        h = self._f("pycauchy_initialize_lti")(int(num_steps), *flat, float(dt), int(init_step), bool(debug_print))
+        return self._check(h, "pycauchy_initialize_lti")
+
+    def pycauchy_initialize_ltv(self, num_steps, A0, p0, b0, Phi, Gamma, B, beta, H, gamma, f_duc_ptr, dt,
+                                init_step, debug_print):
+        args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
+        flat = []
+        for a in args:
+            flat += [a[1], a[2]]
+        if not isinstance(f_duc_ptr, int):
+            f_duc_ptr = ct.cast(f_duc_ptr, ct.c_void_p).value
+        h = self._f("pycauchy_initialize_ltv")(int(num_steps), *flat, ct.c_void_p(f_duc_ptr), float(dt),
+                                               int(init_step), bool(debug_print))
+        return self._check(h, "pycauchy_initialize_ltv")
+
+    def pycauchy_single_step_ltiv(self, handle, msmts, controls):
+        """Returns (Phi, Gamma, B, H, beta, gamma, fz, xhat, Phat, cerr_fz, cerr_xhat, cerr_Phat, err_code)."""
+        m_ = _arr(msmts)
+        c_ = _arr(controls)
+        outs_d = [(_dp(), ct.c_int()) for _ in range(12)]
+        out_i = (_ip(), ct.c_int())
+        args = [ct.c_void_p(handle), m_[1], m_[2], c_[1] if c_[2] > 0 else None, c_[2]]
+        for p, n in outs_d:
+            args += [ct.byref(p), ct.byref(n)]
+        args += [ct.byref(out_i[0]), ct.byref(out_i[1])]
+        self._f("pycauchy_single_step_ltiv")(*args)
+        res = [_take_d(p, n.value) for p, n in outs_d]
+        res.append(_take_i(out_i[0], out_i[1].value))
+        return tuple(res)
+
+    def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
+        a = [_arr(v) for v in (A0, p0, b0, xbar)]
+        self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],
+                                              a[2][2], a[3][1], a[3][2])
+
+    def pycauchy_single_step_set_master_step(self, handle, step):
+        self._f("pycauchy_single_step_set_master_step")(ct.c_void_p(handle), int(step))
+
+    def pycauchy_single_step_set_window_number(self, handle, win_num):
+        self._f("pycauchy_single_step_set_window_number")(ct.c_void_p(handle), int(win_num))
+
+    def pycauchy_single_step_get_number_of_terms(self, handle):
+        return int(self._f("pycauchy_single_step_get_number_of_terms")(ct.c_void_p(handle)))
+
+    def pycauchy_single_step_deterministic_transform(self, handle, Trans, bias):
+        t = _arr(Trans)
+        b = _arr(bias)
+        self._f("pycauchy_single_step_deterministic_transform")(ct.c_void_p(handle), t[1], t[2], b[1], b[2])
+
+    def pycauchy_single_step_get_duc(self, handle):
+        return self._f("pycauchy_single_step_get_duc")(ct.c_void_p(handle))
+
+    def pycauchy_single_step_shutdown(self, handle):
+        self._f("pycauchy_single_step_shutdown")(ct.c_void_p(handle))
+
+    def pycauchy_set_tr_search_idxs_ordering(self, ordering):
+        o = np.ascontiguousarray(np.asarray(ordering, dtype=np.int32).reshape(-1))
+        return int(self._f("pycauchy_set_tr_search_idxs_ordering")(o.ctypes.data_as(_ip), int(o.size)))
+
+    def pycauchy_speyers_window_init(self, xhat, Phat, H, gamma, z):
+        """(A0, p0, b0) of a window restarted about (xhat, Phat) after measurement z
+        (pycauchy.hpp:973-993)."""
+        x, P, h = _arr(xhat), _arr(Phat), _arr(H)
+        f = self._f("pycauchy_speyers_window_init")
+        return _three_outputs(lambda *o: f(x[1], x[2], P[1], P[2], h[1], h[2], float(gamma), float(z), *o))
+
+    def pycauchy_get_reinitialization_statistics(self, handle, z, xhat, Phat, H, gamma):
+        """(A0, p0, b0) that restart an estimator about (xhat, Phat) after measurement z
+        (pycauchy.hpp:948-971)."""
+        x, P, h = _arr(xhat), _arr(Phat), _arr(H)
+        f = self._f("pycauchy_get_reinitialization_statistics")
+        return _three_outputs(lambda *o: f(ct.c_void_p(handle), float(z), x[1], x[2], P[1], P[2], h[1], h[2],
+                                           float(gamma), *o))
 
 
-def pycauchy_single_step_ltiv(handle, msmts, controls):
-    """Returns (Phi, Gamma, B, H, beta, gamma, fz, xhat, Phat, cerr_fz, cerr_xhat, cerr_Phat, err_code)."""
-    L = _native.lib()
-    m_ = _arr(msmts)
-    c_ = _arr(controls)
-    outs_d = [(_dp(), ct.c_int()) for _ in range(12)]
-    out_i = (_ip(), ct.c_int())
-    args = [ct.c_void_p(handle), m_[1], m_[2], c_[1] if c_[2] > 0 else None, c_[2]]
-    for p, n in outs_d:
-        args += [ct.byref(p), ct.byref(n)]
-    args += [ct.byref(out_i[0]), ct.byref(out_i[1])]
-    L.pycauchy_single_step_ltiv(*args)
-    res = [_take_d(p, n.value) for p, n in outs_d]
-    res.append(_take_i(out_i[0], out_i[1].value))
-    return tuple(res)
+_NAMES = [n for n in dir(_Bindings) if n.startswith("pycauchy_")]
 
 
-def pycauchy_single_step_reset(handle, A0, p0, b0, xbar):
-    L = _native.lib()
-    a = [_arr(v) for v in (A0, p0, b0, xbar)]
-    L.pycauchy_single_step_reset(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1], a[2][2],
-                                 a[3][1], a[3][2])
+def bind(cdll, sym=lambda name: name, name="pycauchy"):
+    """A module with the SWIG module's functions over another library exporting this
+    C ABI (argument and result types are declared on it)."""
+    _native.declare_pycauchy(cdll, sym)
+    b = _Bindings(lambda: cdll, sym)
+    mod = types.ModuleType(name)
+    mod.__doc__ = __doc__
+    for n in _NAMES:
+        setattr(mod, n, getattr(b, n))
+    return mod
 
 
-def pycauchy_single_step_set_master_step(handle, step):
-    _native.lib().pycauchy_single_step_set_master_step(ct.c_void_p(handle), int(step))
-
-
-def pycauchy_single_step_set_window_number(handle, win_num):
-    _native.lib().pycauchy_single_step_set_window_number(ct.c_void_p(handle), int(win_num))
-
-
-def pycauchy_single_step_get_number_of_terms(handle):
-    return int(_native.lib().pycauchy_single_step_get_number_of_terms(ct.c_void_p(handle)))
-
-
-def pycauchy_single_step_deterministic_transform(handle, Trans, bias):
-    t = _arr(Trans)
-    b = _arr(bias)
-    _native.lib().pycauchy_single_step_deterministic_transform(ct.c_void_p(handle), t[1], t[2], b[1], b[2])
-
-
-def pycauchy_single_step_get_duc(handle):
-    return _native.lib().pycauchy_single_step_get_duc(ct.c_void_p(handle))
-
-
-def pycauchy_single_step_shutdown(handle):
-    _native.lib().pycauchy_single_step_shutdown(ct.c_void_p(handle))
-
-
-def pycauchy_set_tr_search_idxs_ordering(ordering):
-    o = np.ascontiguousarray(np.asarray(ordering, dtype=np.int32).reshape(-1))
-    return int(_native.lib().pycauchy_set_tr_search_idxs_ordering(o.ctypes.data_as(_ip), int(o.size)))
+_default = _Bindings(_native.lib, last_error=lambda: _native.lib().mce_last_error().decode())
+for _n in _NAMES:
+    globals()[_n] = getattr(_default, _n)
+del _n

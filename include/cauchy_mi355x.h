@@ -101,6 +101,21 @@ void pycauchy_single_step_shutdown(void* handle);
  * Returns 1 on error (more than 12 entries), 0 otherwise. */
 int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering);
 
+/* Replaces pycauchy_speyers_window_init, scripts/swig/cauchy/pycauchy.hpp:973-993: the
+ * initial terms (A0 n x n rows, p0, b0; malloc'd) of a window restarted about the estimate
+ * (xhat, Phat) after scalar measurement z = H x + v, v ~ Cauchy(0, gamma)
+ * (speyers_window_init, include/cauchy_util.hpp:23-113).  Phat is not modified. */
+void pycauchy_speyers_window_init(double* xhat, int size_xhat, double* Phat, int size_Phat, double* H, int size_H,
+                                  double gamma, double z, double** out_A0, int* size_out_A0, double** out_p0,
+                                  int* size_out_p0, double** out_b0, int* size_out_b0);
+
+/* Replaces pycauchy_get_reinitialization_statistics, scripts/swig/cauchy/pycauchy.hpp:948-971
+ * (the same computation for the estimator's state dimension). */
+void pycauchy_get_reinitialization_statistics(void* handle, double z, double* xhat, int size_xhat, double* Phat,
+                                               int size_Phat, double* H, int size_H, double gamma, double** out_A0,
+                                               int* size_out_A0, double** out_p0, int* size_out_p0, double** out_b0,
+                                               int* size_out_b0);
+
 /* ---- the C++ CauchyEstimator interface (include/cauchy_estimator.hpp:47-1432) --------- */
 
 /* Replaces the CauchyEstimator constructor, include/cauchy_estimator.hpp:87-185: steps time

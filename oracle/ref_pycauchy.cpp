@@ -1,0 +1,63 @@
+// Reference build of the SWIG-level estimator functions
+// (scripts/swig/cauchy/pycauchy.hpp, compiled from the reference tree where it
+// lies), re-exported with C linkage under refpy_ names.  TEST INFRASTRUCTURE
+// ONLY: tests/golden/make_window_golden.py binds these to run the reference's
+// own Python classes on the reference backend and record fixtures.
+#include "pycauchy.hpp"
+
+extern "C" {
+
+void* refpy_initialize_lti(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0, int size_b0,
+                           double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B, int size_B,
+                           double* beta, int size_beta, double* H, int size_H, double* gamma, int size_gamma,
+                           double dt, int init_step, bool debug_print) {
+    return pycauchy_initialize_lti(num_steps, A0, size_A0, p0, size_p0, b0, size_b0, Phi, size_Phi, Gamma, size_Gamma,
+                                   B, size_B, beta, size_beta, H, size_H, gamma, size_gamma, dt, init_step,
+                                   debug_print);
+}
+
+void* refpy_initialize_ltv(int num_steps, double* A0, int size_A0, double* p0, int size_p0, double* b0, int size_b0,
+                           double* Phi, int size_Phi, double* Gamma, int size_Gamma, double* B, int size_B,
+                           double* beta, int size_beta, double* H, int size_H, double* gamma, int size_gamma,
+                           void (*cb)(CauchyDynamicsUpdateContainer*), double dt, int init_step, bool debug_print) {
+    return pycauchy_initialize_ltv(num_steps, A0, size_A0, p0, size_p0, b0, size_b0, Phi, size_Phi, Gamma, size_Gamma,
+                                   B, size_B, beta, size_beta, H, size_H, gamma, size_gamma, cb, dt, init_step,
+                                   debug_print);
+}
+
+void refpy_single_step_ltiv(void* h, double* msmts, int size_msmts, double* controls, int size_controls,
+                            double** o1, int* s1, double** o2, int* s2, double** o3, int* s3, double** o4, int* s4,
+                            double** o5, int* s5, double** o6, int* s6, double** o7, int* s7, double** o8, int* s8,
+                            double** o9, int* s9, double** o10, int* s10, double** o11, int* s11, double** o12,
+                            int* s12, int** o13, int* s13) {
+    pycauchy_single_step_ltiv(h, msmts, size_msmts, controls, size_controls, o1, s1, o2, s2, o3, s3, o4, s4, o5, s5,
+                              o6, s6, o7, s7, o8, s8, o9, s9, o10, s10, o11, s11, o12, s12, o13, s13);
+}
+
+void refpy_single_step_reset(void* h, double* A0, int sA0, double* p0, int sp0, double* b0, int sb0, double* xbar,
+                             int sxbar) {
+    pycauchy_single_step_reset(h, A0, sA0, p0, sp0, b0, sb0, xbar, sxbar);
+}
+
+void refpy_single_step_set_master_step(void* h, int step) { pycauchy_single_step_set_master_step(h, step); }
+void refpy_single_step_set_window_number(void* h, int w) { pycauchy_single_step_set_window_number(h, w); }
+int refpy_single_step_get_number_of_terms(void* h) { return pycauchy_single_step_get_number_of_terms(h); }
+void refpy_single_step_deterministic_transform(void* h, double* T, int sT, double* b, int sb) {
+    pycauchy_single_step_deterministic_transform(h, T, sT, b, sb);
+}
+void* refpy_single_step_get_duc(void* h) { return pycauchy_single_step_get_duc(h); }
+void refpy_single_step_shutdown(void* h) { pycauchy_single_step_shutdown(h); }
+int refpy_set_tr_search_idxs_ordering(int* ordering, int n) { return pycauchy_set_tr_search_idxs_ordering(ordering, n); }
+
+void refpy_speyers_window_init(double* xhat, int sx, double* Phat, int sP, double* H, int sH, double gamma, double z,
+                               double** oA, int* sA, double** op, int* sp, double** ob, int* sb) {
+    pycauchy_speyers_window_init(xhat, sx, Phat, sP, H, sH, gamma, z, oA, sA, op, sp, ob, sb);
+}
+
+void refpy_get_reinitialization_statistics(void* h, double z, double* xhat, int sx, double* Phat, int sP, double* H,
+                                           int sH, double gamma, double** oA, int* sA, double** op, int* sp,
+                                           double** ob, int* sb) {
+    pycauchy_get_reinitialization_statistics(h, z, xhat, sx, Phat, sP, H, sH, gamma, oA, sA, op, sp, ob, sb);
+}
+
+}  // extern "C"

@@ -17,8 +17,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP engine)")
 
 
+# fixtures of other record types (window bank, window re-initialisation)
+NON_ESTIMATOR_FIXTURES = ("speyer_", "window_")
+
+
 def golden_names():
-    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.json")))
+    """Names of the per-update estimator fixtures."""
+    names = (os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.json")))
+    return sorted(n for n in names if not n.startswith(NON_ESTIMATOR_FIXTURES))
 
 
 def load_golden(name):
