@@ -95,6 +95,8 @@ def lib():
     L.mce_last_update.restype = ct.c_int
     L.mce_last_update.argtypes = [ct.c_void_p, _dp, _ip]
     L.mce_shape_range.restype = ct.c_int
+    L.mce_terms_generated.restype = ct.c_longlong
+    L.mce_terms_generated.argtypes = [ct.c_void_p]
     L.mce_shape_range.argtypes = [ct.c_void_p]
     L.mce_term.restype = ct.c_int
     L.mce_term.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, _ip, _dp, _dp, _dp, ct.POINTER(ct.c_uint), _dp]

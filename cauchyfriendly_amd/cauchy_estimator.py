@@ -59,7 +59,9 @@ def _flat(a):
 class PyCauchyEstimator:
     MODES = ("lti", "ltv")
 
-    def __init__(self, mode, num_steps, debug_print=False):
+    def __init__(self, mode, num_steps, debug_print=False, backend=None):
+        """backend: a module with the pycauchy functions (default: this library's)."""
+        self._pc = backend if backend is not None else pycauchy
         mode = str(mode).lower()
         if mode not in self.MODES:
             raise ValueError(f"mode {mode!r} is not supported by this engine (modes: {self.MODES})")
@@ -96,7 +98,7 @@ class PyCauchyEstimator:
         if self.mode != "lti":
             raise RuntimeError(f"initialize_lti called on an estimator in mode {self.mode}")
         self._store_inputs(A0, p0, b0, Phi, B, Gamma, beta, H, gamma)
-        self.py_handle = pycauchy.pycauchy_initialize_lti(self.num_steps, self._A0, self._p0, self._b0, self._Phi,
+        self.py_handle = self._pc.pycauchy_initialize_lti(self.num_steps, self._A0, self._p0, self._b0, self._Phi,
                                                           self._Gamma, self._B, self._beta, self._H, self._gamma,
                                                           float(dt), int(init_step), self.debug_print)
         self.is_initialized = True
@@ -108,7 +110,7 @@ class PyCauchyEstimator:
         cb_type = ct.CFUNCTYPE(None, ct.POINTER(C_CauchyDynamicsUpdateContainer))
         self._f_duc = cb_type(dynamics_update_callback)  # kept alive as long as the estimator
         self.f_duc_ptr1 = ct.cast(self._f_duc, ct.c_void_p).value
-        self.py_handle = pycauchy.pycauchy_initialize_ltv(self.num_steps, self._A0, self._p0, self._b0, self._Phi,
+        self.py_handle = self._pc.pycauchy_initialize_ltv(self.num_steps, self._A0, self._p0, self._b0, self._Phi,
                                                           self._Gamma, self._B, self._beta, self._H, self._gamma,
                                                           self.f_duc_ptr1, float(dt), int(init_step),
                                                           self.debug_print)
@@ -130,7 +132,7 @@ class PyCauchyEstimator:
         return z, u
 
     def _call_step(self, z, u, full_info):
-        out = pycauchy.pycauchy_single_step_ltiv(self.py_handle, z, u)
+        out = self._pc.pycauchy_single_step_ltiv(self.py_handle, z, u)
         (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._cerr_fz, self._cerr_x, self._cerr_P,
          self._err_code) = out
         if self.mode == "ltv":  # the dynamics of this step, as the callback left them
@@ -169,14 +171,19 @@ class PyCauchyEstimator:
         b = np.asarray(bias, dtype=np.float64).reshape(-1)
         if T.size != self.n * self.n or b.size != self.n:
             raise ValueError("Trans must be n x n and bias n")
-        pycauchy.pycauchy_single_step_deterministic_transform(self.py_handle, T, b)
+        self._pc.pycauchy_single_step_deterministic_transform(self.py_handle, T, b)
+
+    def terms_generated(self):
+        """Terms generated by this estimator so far (this library's backend only)."""
+        from . import _native
+        return int(_native.lib().mce_terms_generated(self.py_handle)) if self.py_handle else 0
 
     def get_num_CF_terms(self):
         if not self.is_initialized:
             return 0
         if self.step_count == 0:
             return 1
-        return pycauchy.pycauchy_single_step_get_number_of_terms(self.py_handle)
+        return self._pc.pycauchy_single_step_get_number_of_terms(self.py_handle)
 
     def get_last_mean_cov(self):
         return self.moment_info["x"][-1].copy(), self.moment_info["P"][-1].copy()
@@ -195,7 +202,7 @@ class PyCauchyEstimator:
             raise RuntimeError("the estimator has not processed a measurement yet")
         i = self._msmt_index(msmt_idx)
         n = self.n
-        A0, p0, b0 = pycauchy.pycauchy_get_reinitialization_statistics(
+        A0, p0, b0 = self._pc.pycauchy_get_reinitialization_statistics(
             self.py_handle, self._msmts[i], self._x[i * n:(i + 1) * n].copy(),
             self._P[i * n * n:(i + 1) * n * n].copy(), self._H[i * n:(i + 1) * n].copy(), self._gamma[i])
         return A0.reshape(n, n), p0, b0
@@ -212,7 +219,7 @@ class PyCauchyEstimator:
         self._p0 = _flat(p0)
         self._b0 = _flat(b0)
         self._xbar = _flat(xbar)
-        pycauchy.pycauchy_single_step_reset(self.py_handle, self._A0, self._p0, self._b0, self._xbar)
+        self._pc.pycauchy_single_step_reset(self.py_handle, self._A0, self._p0, self._b0, self._xbar)
 
     def reset_about_estimator(self, other_estimator, msmt_idx=-1):
         """Restarts this window about other_estimator's estimate after its msmt_idx-th
@@ -230,23 +237,23 @@ class PyCauchyEstimator:
         A0, p0, b0 = other_estimator.get_reinitialization_statistics(i)
         self.reset(A0, p0, b0)
         xs, Ps = self._call_step(z, _EMPTY, False)
-        pycauchy.pycauchy_single_step_set_master_step(self.py_handle, self.p)
+        self._pc.pycauchy_single_step_set_master_step(self.py_handle, self.p)
         return xs, Ps
 
     def reset_with_last_measurement(self, z_scalar, A0, p0, b0, xbar=None):
         z = np.asarray(z_scalar, dtype=np.float64).reshape(-1).copy()
         self.reset(A0, p0, b0, xbar)
         xs, Ps = self._call_step(z, _EMPTY, False)
-        pycauchy.pycauchy_single_step_set_master_step(self.py_handle, self.p)
+        self._pc.pycauchy_single_step_set_master_step(self.py_handle, self.p)
         return xs, Ps
 
     def set_window_number(self, win_num):
-        pycauchy.pycauchy_single_step_set_window_number(self.py_handle, int(win_num))
+        self._pc.pycauchy_single_step_set_window_number(self.py_handle, int(win_num))
 
     def shutdown(self):
         if not self.is_initialized:
             return
-        pycauchy.pycauchy_single_step_shutdown(self.py_handle)
+        self._pc.pycauchy_single_step_shutdown(self.py_handle)
         self.py_handle = None
         self.is_initialized = False
 
@@ -265,7 +272,7 @@ class PySlidingWindowManager:
     COV_UNSTABLE = 2
     COV_DNE = 8
 
-    def __init__(self, mode, num_windows, swm_debug_print=True, win_debug_print=False):
+    def __init__(self, mode, num_windows, swm_debug_print=True, win_debug_print=False, backend=None):
         self.num_windows = int(num_windows)
         if self.num_windows >= 20:
             raise ValueError("at most 19 windows")
@@ -277,7 +284,7 @@ class PySlidingWindowManager:
         self.step_idx = 0
         self.win_idxs = np.arange(self.num_windows)
         self.win_counts = np.zeros(self.num_windows, dtype=np.int64)
-        self.cauchyEsts = [PyCauchyEstimator(self.mode, self.num_windows, bool(win_debug_print))
+        self.cauchyEsts = [PyCauchyEstimator(self.mode, self.num_windows, bool(win_debug_print), backend)
                            for _ in range(self.num_windows)]
         self.moment_info = {k: [] for k in _MOMENT_KEYS + ("win_idx",)}
         self.avg_moment_info = {k: [] for k in _MOMENT_KEYS + ("win_idx",)}

@@ -224,6 +224,7 @@ struct Engine {
     bool moments_pending = false;
     int max_gp_cells = 1;  // largest parent g-table of the current parents
     int geval_mode = -1;   // parent lookup mode of the last g-table launch
+    long long terms_generated = 0;  // terms after MU/MUC (after MU on a window's last update), summed over updates
     // MCE_GEVAL_MODE=0..3 selects the g-table lookup mode (tuning; unsupported choices are ignored)
     int geval_mode_override = getenv("MCE_GEVAL_MODE") ? atoi(getenv("MCE_GEVAL_MODE")) : -1;
     // MCE_BIG_GROUP_MEMBERS: groups with more members add them one wavefront per member
@@ -780,6 +781,7 @@ struct Engine {
                 Nt += h[m];
             }
             diag_cnt[1] = Nt;
+            terms_generated += Nt;
         } else {
             // ---- gather into shape buckets in the reference's term order
             iota.reserve(nslots);
@@ -803,6 +805,7 @@ struct Engine {
                 nt_mu += cnt[m];
             }
             diag_cnt[1] = nt_mu;
+            terms_generated += nt_mu;
             if (first) {
                 first_gtables(nt_mu);
             } else {
@@ -1641,6 +1644,8 @@ int mce_last_update(void* handle, double* out_mom, int* out_cnt) {
 }
 
 int mce_shape_range(void* handle) { return ((Handle*)handle)->eng->sr; }
+
+long long mce_terms_generated(void* handle) { return ((Handle*)handle)->eng->terms_generated; }
 
 int mce_term(void* handle, int shape, int idx, int* info, double* A, double* p, double* b, unsigned int* keys,
              double* gvals) {

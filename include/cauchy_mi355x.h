@@ -153,6 +153,10 @@ int mce_last_update(void* handle, double* out_mom, int* out_cnt);
 /* Shape range (largest hyperplane count + 1) of the estimator. */
 int mce_shape_range(void* handle);
 
+/* Terms generated since the estimator was created: the sum over its updates of the terms
+ * after MU/MUC (after MU alone on a window's last update), the numerator of terms/s. */
+long long mce_terms_generated(void* handle);
+
 /* Copies term `idx` of shape bucket `shape` (after an update that ran term reduction):
  * info = [m, g-table size, m]; A (m x d), p (m), b (d), keys / g (re,im interleaved) of its g-table.
  * Returns 0, or -1 when there is no such term. */
