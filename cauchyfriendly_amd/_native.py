@@ -57,6 +57,12 @@ def declare_pycauchy(L, sym=lambda name: name):
     getattr(L, sym("pycauchy_single_step_ltiv")).restype = None
     getattr(L, sym("pycauchy_single_step_ltiv")).argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int] + \
         [_dpp, _ip] * 12 + [_ipp, _ip]
+    getattr(L, sym("pycauchy_initialize_nonlin")).restype = ct.c_void_p
+    getattr(L, sym("pycauchy_initialize_nonlin")).argtypes = [ct.c_int] + [_dp, ct.c_int] * 6 + \
+        [ct.c_void_p] * 3 + [ct.c_int, ct.c_double, ct.c_int, ct.c_bool]
+    getattr(L, sym("pycauchy_single_step_nonlin")).restype = None
+    getattr(L, sym("pycauchy_single_step_nonlin")).argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int, ct.c_bool] + \
+        [_dpp, _ip] * 14 + [_ipp, _ip]
     getattr(L, sym("pycauchy_single_step_reset")).argtypes = [ct.c_void_p] + [_dp, ct.c_int] * 4
     getattr(L, sym("pycauchy_single_step_set_master_step")).argtypes = [ct.c_void_p, ct.c_int]
     getattr(L, sym("pycauchy_single_step_set_window_number")).argtypes = [ct.c_void_p, ct.c_int]
@@ -92,6 +98,8 @@ def lib():
     L.mce_estimator_moments.argtypes = [ct.c_void_p, _dp, _dp, _dp]
     L.mce_estimator_reset.argtypes = [ct.c_void_p]
     L.mce_estimator_destroy.argtypes = [ct.c_void_p]
+    L.mce_estimator_finalize_extended_moments.restype = ct.c_int
+    L.mce_estimator_finalize_extended_moments.argtypes = [ct.c_void_p, _dp]
     L.mce_last_update.restype = ct.c_int
     L.mce_last_update.argtypes = [ct.c_void_p, _dp, _ip]
     L.mce_shape_range.restype = ct.c_int

@@ -1,5 +1,5 @@
 """Python estimator classes over the MI355X engine, mirroring the reference's
-scripts/swig/cauchy/cauchy_estimator.py interface for the linear modes:
+scripts/swig/cauchy/cauchy_estimator.py interface (modes lti, ltv and nonlin):
 
 * PyCauchyEstimator (cauchy_estimator.py:535-950): one estimator window, stepped
   with the p scalar measurements of a time step; keeps `moment_info` and the
@@ -9,8 +9,8 @@ scripts/swig/cauchy/cauchy_estimator.py interface for the linear modes:
 * PySlidingWindowManager (cauchy_estimator.py:1092-1335): a bank of windows,
   each restarted about the best full window when the oldest one completes.
 * speyers_window_init, set_tr_search_idxs_ordering (cauchy_estimator.py:18-37).
-
-The nonlinear (extended) mode is not part of this engine.
+* Py_CauchyDynamicsUpdateContainer (cauchy_estimator.py:60-136): numpy access to the
+  C container that the ltv / nonlin callbacks receive.
 """
 from __future__ import annotations
 
@@ -48,6 +48,92 @@ class C_CauchyDynamicsUpdateContainer(ct.Structure):
                 ("is_xbar_set_for_ece", ct.c_bool), ("other_stuff", ct.c_void_p)]
 
 
+class Py_CauchyDynamicsUpdateContainer:
+    """The callbacks' view of a C_CauchyDynamicsUpdateContainer: cget_* return numpy
+    copies, cset_* write through to the C arrays (same method names as the reference)."""
+
+    def __init__(self, cduc):
+        self.cduc = cduc
+        c = cduc.contents
+        self.n, self.pncc, self.cmcc, self.p = c.n, c.pncc, c.cmcc, c.p
+
+    def _get(self, field, size, shape=None):
+        ptr = getattr(self.cduc.contents, field)
+        a = np.ctypeslib.as_array(ptr, shape=(size,)).copy() if size > 0 else np.zeros(0)
+        return a if shape is None else a.reshape(shape)
+
+    def _set(self, field, value, size):
+        v = np.asarray(value, dtype=np.float64).reshape(-1)
+        if v.size != size:
+            raise ValueError(f"{field} must have {size} entries, got {v.size}")
+        if size > 0:
+            np.ctypeslib.as_array(getattr(self.cduc.contents, field), shape=(size,))[:] = v
+
+    def cget_step(self):
+        return self.cduc.contents.step
+
+    def cget_dt(self):
+        return self.cduc.contents.dt
+
+    def cset_dt(self, dt):
+        self.cduc.contents.dt = float(dt)
+
+    def cget_x(self):
+        return self._get("x", self.n)
+
+    def cset_x(self, x):
+        self._set("x", x, self.n)
+
+    def cget_u(self):
+        return self._get("u", self.cmcc)
+
+    def cget_Phi(self):
+        return self._get("Phi", self.n * self.n, (self.n, self.n))
+
+    def cset_Phi(self, Phi):
+        self._set("Phi", Phi, self.n * self.n)
+
+    def cget_Gamma(self):
+        return self._get("Gamma", self.n * self.pncc, (self.n, self.pncc))
+
+    def cset_Gamma(self, Gamma):
+        self._set("Gamma", Gamma, self.n * self.pncc)
+
+    def cget_B(self):
+        return self._get("B", self.n * self.cmcc, (self.n, self.cmcc))
+
+    def cset_B(self, B):
+        self._set("B", B, self.n * self.cmcc)
+
+    def cget_beta(self):
+        return self._get("beta", self.pncc)
+
+    def cset_beta(self, beta):
+        self._set("beta", beta, self.pncc)
+
+    def cget_H(self):
+        return self._get("H", self.p * self.n, (self.p, self.n))
+
+    def cset_H(self, H):
+        self._set("H", H, self.p * self.n)
+
+    def cget_gamma(self):
+        return self._get("gamma", self.p)
+
+    def cset_gamma(self, gamma):
+        self._set("gamma", gamma, self.p)
+
+    def cset_is_xbar_set_for_ece(self):
+        self.cduc.contents.is_xbar_set_for_ece = True
+
+    def cset_zbar(self, c_zbar, zbar):
+        z = np.asarray(zbar, dtype=np.float64).reshape(-1)
+        if z.size != self.p:
+            raise ValueError(f"zbar must have {self.p} entries, got {z.size}")
+        for i in range(self.p):
+            c_zbar[i] = float(z[i])
+
+
 _EMPTY = np.array([], dtype=np.float64)
 _MOMENT_KEYS = ("fz", "x", "P", "cerr_x", "cerr_P", "cerr_fz", "err_code")
 
@@ -57,7 +143,7 @@ def _flat(a):
 
 
 class PyCauchyEstimator:
-    MODES = ("lti", "ltv")
+    MODES = ("lti", "ltv", "nonlin")
 
     def __init__(self, mode, num_steps, debug_print=False, backend=None):
         """backend: a module with the pycauchy functions (default: this library's)."""
@@ -116,6 +202,37 @@ class PyCauchyEstimator:
                                                           self.debug_print)
         self.is_initialized = True
 
+    def initialize_nonlin(self, x0, A0, p0, b0, beta, gamma, dynamics_update_callback, nonlinear_msmt_model,
+                          extended_msmt_update_callback, cmcc, dt=0, step=0):
+        """Extended estimator: x0 is the initial x_bar; the estimator runs on the deviation
+        from x_bar, with Phi, Gamma (dynamics_update_callback) and H (extended_msmt_update_callback)
+        the Jacobians at x_bar, and the residual z - h(x_bar) (nonlinear_msmt_model)."""
+        if self.mode != "nonlin":
+            raise RuntimeError(f"initialize_nonlin called on an estimator in mode {self.mode}")
+        n = np.asarray(p0).size
+        if np.asarray(A0).size != n * n or np.asarray(b0).size != n or np.asarray(x0).size != n:
+            raise ValueError("x0 (n), A0 (n x n), p0 (n) and b0 (n) sizes disagree")
+        self.n = n
+        self.pncc = 0 if beta is None else np.asarray(beta).size
+        self.p = np.asarray(gamma).size
+        self.cmcc = int(cmcc)
+        t1 = ct.CFUNCTYPE(None, ct.POINTER(C_CauchyDynamicsUpdateContainer))
+        t2 = ct.CFUNCTYPE(None, ct.POINTER(C_CauchyDynamicsUpdateContainer), ct.POINTER(ct.c_double))
+        # the ctypes callbacks live as long as the estimator
+        self._f_duc1 = t1(dynamics_update_callback)
+        self._f_duc2 = t2(nonlinear_msmt_model)
+        self._f_duc3 = t1(extended_msmt_update_callback)
+        self.f_duc_ptr1 = ct.cast(self._f_duc1, ct.c_void_p).value
+        self.f_duc_ptr2 = ct.cast(self._f_duc2, ct.c_void_p).value
+        self.f_duc_ptr3 = ct.cast(self._f_duc3, ct.c_void_p).value
+        self._x0, self._A0, self._p0, self._b0 = _flat(x0), _flat(A0), _flat(p0), _flat(b0)
+        self._beta, self._gamma = _flat(beta), _flat(gamma)
+        self.py_handle = self._pc.pycauchy_initialize_nonlin(self.num_steps, self._x0, self._A0, self._p0, self._b0,
+                                                             self._beta, self._gamma, self.f_duc_ptr1,
+                                                             self.f_duc_ptr2, self.f_duc_ptr3, self.cmcc, float(dt),
+                                                             int(step), self.debug_print)
+        self.is_initialized = True
+
     # -- stepping -------------------------------------------------------------
     def _checked_inputs(self, msmts, controls):
         z = np.asarray(msmts, dtype=np.float64).reshape(-1).copy()
@@ -132,10 +249,16 @@ class PyCauchyEstimator:
         return z, u
 
     def _call_step(self, z, u, full_info):
-        out = self._pc.pycauchy_single_step_ltiv(self.py_handle, z, u)
-        (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._cerr_fz, self._cerr_x, self._cerr_P,
-         self._err_code) = out
-        if self.mode == "ltv":  # the dynamics of this step, as the callback left them
+        if self.mode == "nonlin":
+            # the first step of a window only updates; every later one propagates first
+            out = self._pc.pycauchy_single_step_nonlin(self.py_handle, z, u, self.step_count != 0)
+            (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._xbar, self._zbar, self._cerr_fz,
+             self._cerr_x, self._cerr_P, self._err_code) = out
+        else:
+            out = self._pc.pycauchy_single_step_ltiv(self.py_handle, z, u)
+            (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._cerr_fz, self._cerr_x, self._cerr_P,
+             self._err_code) = out
+        if self.mode != "lti":  # the dynamics of this step, as the callbacks left them
             self._Phi, self._Gamma, self._B, self._H, self._beta, self._gamma = Phi, Gamma, B, H, beta, gamma
         n = self.n
         k = z.size
@@ -197,15 +320,23 @@ class PyCauchyEstimator:
 
     def get_reinitialization_statistics(self, msmt_idx=-1):
         """(A0, p0, b0) restarting a window about this estimator's estimate after its
-        msmt_idx-th measurement of the last step."""
+        msmt_idx-th measurement of the last step; in nonlin mode the restart is about the
+        deviation from x_bar (residual z - zbar), and x_bar is returned as a fourth value."""
         if not self.is_initialized or self.step_count == 0:
             raise RuntimeError("the estimator has not processed a measurement yet")
         i = self._msmt_index(msmt_idx)
         n = self.n
-        A0, p0, b0 = self._pc.pycauchy_get_reinitialization_statistics(
-            self.py_handle, self._msmts[i], self._x[i * n:(i + 1) * n].copy(),
-            self._P[i * n * n:(i + 1) * n * n].copy(), self._H[i * n:(i + 1) * n].copy(), self._gamma[i])
-        return A0.reshape(n, n), p0, b0
+        z = self._msmts[i]
+        x = self._x[i * n:(i + 1) * n].copy()
+        P = self._P[i * n * n:(i + 1) * n * n].copy()
+        H = self._H[i * n:(i + 1) * n].copy()
+        if self.mode != "nonlin":
+            A0, p0, b0 = self._pc.pycauchy_get_reinitialization_statistics(self.py_handle, z, x, P, H, self._gamma[i])
+            return A0.reshape(n, n), p0, b0
+        xbar = self._xbar[i * n:(i + 1) * n].copy()
+        A0, p0, b0 = self._pc.pycauchy_get_reinitialization_statistics(self.py_handle, z - self._zbar[i], x - xbar, P,
+                                                                       H, self._gamma[i])
+        return A0.reshape(n, n), p0, b0, xbar
 
     def reset(self, A0=None, p0=None, b0=None, xbar=None):
         if not self.is_initialized:
@@ -234,8 +365,7 @@ class PyCauchyEstimator:
         i = self._msmt_index(msmt_idx)
         z = other_estimator._msmts[i:].copy()
         self._msmts = z.copy()
-        A0, p0, b0 = other_estimator.get_reinitialization_statistics(i)
-        self.reset(A0, p0, b0)
+        self.reset(*other_estimator.get_reinitialization_statistics(i))
         xs, Ps = self._call_step(z, _EMPTY, False)
         self._pc.pycauchy_single_step_set_master_step(self.py_handle, self.p)
         return xs, Ps
@@ -311,6 +441,18 @@ class PySlidingWindowManager:
         for i, est in enumerate(self.cauchyEsts):
             est.initialize_ltv(A0, p0, b0, Phi, B, Gamma, beta, H, gamma, dynamics_update_callback, int(step) + i,
                                float(dt))
+            est.set_window_number(i + 1)
+        self.is_initialized = True
+        self._dims()
+
+    def initialize_nonlin(self, x0, A0, p0, b0, beta, gamma, dynamics_update_callback, nonlinear_msmt_model,
+                          extended_msmt_update_callback, cmcc, dt=0, step=0, reinit_func=None):
+        if np.asarray(p0).size == 1:
+            raise ValueError("the window bank is for systems of dimension > 1")
+        self.reinit_func = reinit_func
+        for i, est in enumerate(self.cauchyEsts):
+            est.initialize_nonlin(x0, A0, p0, b0, beta, gamma, dynamics_update_callback, nonlinear_msmt_model,
+                                  extended_msmt_update_callback, cmcc, float(dt), int(step) + i)
             est.set_window_number(i + 1)
         self.is_initialized = True
         self._dims()

@@ -1275,6 +1275,11 @@ struct Engine {
     // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).
     void deterministic_transform(const double* Trans, const double* bias);
 
+    // Extended-mode finish of a measurement update (cauchy_estimator.hpp:1358-1394): the
+    // terms' b-vectors move by -Re(mean) (skipped on a window's last update), the mean
+    // becomes x_bar + mean, and x_bar takes its real part.
+    void finalize_extended_moments(double* x_bar);
+
     // Host copy of parent term `idx` of bucket `shape`.
     int term(int shape, int idx, int* info, double* A, double* p_, double* b, unsigned int* keys_, double* g) {
         if (shape < 0 || shape >= sr || idx < 0 || idx >= in_count[shape]) return -1;
@@ -1320,6 +1325,19 @@ void Engine::deterministic_transform(const double* Trans, const double* bias) {
     HIPCHK(hipStreamSynchronize(st));
 }
 
+void Engine::finalize_extended_moments(double* x_bar) {
+    if (!skip) {
+        VecArg dx;
+        for (int i = 0; i < d; i++) dx.v[i] = mean[i].re;
+        TermPool IN = in.view();
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_sub_b<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dx));
+        HIPCHK(hipGetLastError());
+    }
+    for (int i = 0; i < d; i++) mean[i].re += x_bar[i];
+    for (int i = 0; i < d; i++) x_bar[i] = mean[i].re;
+    if (!skip) HIPCHK(hipStreamSynchronize(st));
+}
+
 // ======================================================================== C ABI
 namespace {
 
@@ -1327,7 +1345,10 @@ struct Handle {
     CauchyDynamicsUpdateContainer duc;
     Engine* eng = nullptr;
     void (*cb)(CauchyDynamicsUpdateContainer*) = nullptr;
-    std::vector<double> Phi, Gamma, B, H, beta, gamma, x;
+    // extended (nonlinear) mode: h(x) into zbar, then the measurement Jacobian
+    void (*msmt_model)(CauchyDynamicsUpdateContainer*, double*) = nullptr;
+    void (*ext_update)(CauchyDynamicsUpdateContainer*) = nullptr;
+    std::vector<double> Phi, Gamma, B, H, beta, gamma, x, zbar;
 };
 
 void set_err(const char* where, const std::exception& e) {
@@ -1496,6 +1517,156 @@ void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, doub
     }
 }
 
+void* pycauchy_initialize_nonlin(int num_steps, double* x0, int size_x0, double* A0, int size_A0, double* p0,
+                                 int size_p0, double* b0, int size_b0, double* beta, int size_beta, double* gamma,
+                                 int size_gamma, void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                                 void (*f_nonlinear_msmt_model)(CauchyDynamicsUpdateContainer*, double*),
+                                 void (*f_extended_msmt_update_callback)(CauchyDynamicsUpdateContainer*), int cmcc,
+                                 double dt, int init_step, bool debug_print) {
+    (void)size_A0;
+    (void)size_p0;
+    (void)debug_print;
+    try {
+        const int n = size_b0;
+        const int pncc = size_beta;
+        const int p = size_gamma;
+        if (size_x0 != n) throw std::runtime_error("x0 must hold n = size_b0 entries");
+        if (!f_dyn_update_callback || !f_nonlinear_msmt_model || !f_extended_msmt_update_callback)
+            throw std::runtime_error("all three callbacks are required");
+        Handle* h = new Handle;
+        h->x.assign(x0, x0 + n);
+        h->Phi.assign(std::max(n * n, 1), 0.0);
+        h->Gamma.assign(std::max(n * pncc, 1), 0.0);
+        h->B.assign(std::max(n * cmcc, 1), 0.0);
+        h->H.assign(std::max(p * n, 1), 0.0);
+        h->beta.assign(std::max(pncc, 1), 0.0);
+        std::copy(beta, beta + size_beta, h->beta.begin());
+        h->gamma.assign(std::max(p, 1), 0.0);
+        std::copy(gamma, gamma + size_gamma, h->gamma.begin());
+        h->zbar.assign(std::max(p, 1), 0.0);
+        CauchyDynamicsUpdateContainer& duc = h->duc;
+        memset(&duc, 0, sizeof(duc));
+        duc.x = h->x.data();
+        duc.dt = dt;
+        duc.step = init_step;
+        duc.n = n;
+        duc.cmcc = cmcc;
+        duc.pncc = pncc;
+        duc.p = p;
+        duc.Phi = h->Phi.data();
+        duc.Gamma = h->Gamma.data();
+        duc.B = h->B.data();
+        duc.H = h->H.data();
+        duc.beta = h->beta.data();
+        duc.gamma = h->gamma.data();
+        h->cb = f_dyn_update_callback;
+        h->msmt_model = f_nonlinear_msmt_model;
+        h->ext_update = f_extended_msmt_update_callback;
+        // the estimator propagates the state deviation, which has no control term
+        h->eng = new Engine(num_steps, n, 0, pncc, p, A0, p0, b0);
+        return h;
+    } catch (const std::exception& e) {
+        set_err("pycauchy_initialize_nonlin", e);
+        return nullptr;
+    }
+}
+
+void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, double* controls, int size_controls,
+                                 bool with_propagate, double** out_Phi, int* size_out_Phi, double** out_Gamma,
+                                 int* size_out_Gamma, double** out_B, int* size_out_B, double** out_H,
+                                 int* size_out_H, double** out_beta, int* size_out_beta, double** out_gamma,
+                                 int* size_out_gamma, double** out_fz, int* size_out_fz, double** out_xhat,
+                                 int* size_out_xhat, double** out_Phat, int* size_out_Phat, double** out_xbar,
+                                 int* size_out_xbar, double** out_zbar, int* size_out_zbar, double** out_cerr_fz,
+                                 int* size_out_cerr_fz, double** out_cerr_xhat, int* size_out_cerr_xhat,
+                                 double** out_cerr_Phat, int* size_out_cerr_Phat, int** out_err_code,
+                                 int* size_out_err_code) {
+    (void)size_controls;
+    Handle* h = (Handle*)handle;
+    Engine* e = h->eng;
+    CauchyDynamicsUpdateContainer& duc = h->duc;
+    duc.u = controls;
+    const int p = duc.p, n = duc.n, cmcc = duc.cmcc, pncc = duc.pncc;
+    const int nm = size_msmts;
+    // outputs (pycauchy.hpp:580-604); the dynamics are this step's, or the current
+    // ones when the step does not propagate
+    *size_out_Phi = n * n;
+    *out_Phi = mallocd<double>(n * n);
+    *size_out_Gamma = n * pncc;
+    *out_Gamma = mallocd<double>(n * pncc);
+    *size_out_B = n * cmcc;
+    *out_B = mallocd<double>(n * cmcc);
+    *size_out_H = nm * n;
+    *out_H = mallocd<double>(nm * n);
+    *size_out_beta = pncc;
+    *out_beta = mallocd<double>(pncc);
+    *size_out_gamma = nm;
+    *out_gamma = mallocd<double>(nm);
+    *size_out_fz = nm;
+    *out_fz = mallocd<double>(nm);
+    *size_out_xhat = nm * n;
+    *out_xhat = mallocd<double>(nm * n);
+    *size_out_Phat = nm * n * n;
+    *out_Phat = mallocd<double>(nm * n * n);
+    *size_out_xbar = nm * n;
+    *out_xbar = mallocd<double>(nm * n);
+    *size_out_zbar = nm;
+    *out_zbar = mallocd<double>(nm);
+    *size_out_cerr_fz = nm;
+    *out_cerr_fz = mallocd<double>(nm);
+    *size_out_cerr_xhat = nm;
+    *out_cerr_xhat = mallocd<double>(nm);
+    *size_out_cerr_Phat = nm;
+    *out_cerr_Phat = mallocd<double>(nm);
+    *size_out_err_code = nm;
+    *out_err_code = mallocd<int>(nm);
+    if (with_propagate) {
+        // x_bar(k+1) = f(x_bar(k)) and the Jacobians, from the caller (pycauchy.hpp:606-619)
+        duc.is_xbar_set_for_ece = false;
+        h->cb(&duc);
+        if (!duc.is_xbar_set_for_ece) {
+            g_last_error = "pycauchy_single_step_nonlin: the dynamics callback did not set is_xbar_set_for_ece";
+            fprintf(stderr, "[cauchy_mi355x] %s\n", g_last_error.c_str());
+        }
+    }
+    memcpy(*out_Phi, duc.Phi, n * n * sizeof(double));
+    memcpy(*out_B, duc.B, n * cmcc * sizeof(double));
+    memcpy(*out_Gamma, duc.Gamma, n * pncc * sizeof(double));
+    memcpy(*out_beta, duc.beta, pncc * sizeof(double));
+    const int start = p - nm;
+    double* zbar = h->zbar.data();
+    for (int i = 0; i < nm; i++) {
+        const int idx = start + i;
+        // zbar = h(x), then H = dh/dx at x (pycauchy.hpp:628-629); x is x_bar before the
+        // first update and the updated estimate after it
+        h->msmt_model(&duc, zbar);
+        h->ext_update(&duc);
+        memcpy(*out_xbar + i * n, duc.x, n * sizeof(double));
+        (*out_zbar)[i] = zbar[idx];
+        const double dz = msmts[i] - zbar[idx];
+        memcpy(*out_H + i * n, duc.H + idx * n, n * sizeof(double));
+        (*out_gamma)[i] = duc.gamma[idx];
+        try {
+            e->update(dz, duc.Phi, duc.Gamma, duc.beta, duc.H + idx * n, duc.gamma[idx], nullptr, nullptr);
+            e->finalize_extended_moments(duc.x);
+        } catch (const std::exception& ex) {
+            set_err("pycauchy_single_step_nonlin", ex);
+        }
+        (*out_fz)[i] = e->fz.re;
+        for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
+        for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
+        (*out_cerr_fz)[i] = e->fz.im;
+        double mx = fabs(e->mean[0].im);
+        for (int j = 1; j < n; j++) mx = std::max(mx, fabs(e->mean[j].im));
+        (*out_cerr_xhat)[i] = mx;
+        double mp = fabs(e->cov[0].im);
+        for (int j = 1; j < n * n; j++) mp = std::max(mp, fabs(e->cov[j].im));
+        (*out_cerr_Phat)[i] = mp;
+        (*out_err_code)[i] = e->errs;
+    }
+    duc.step += 1;
+}
+
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar) {
     Handle* h = (Handle*)handle;
@@ -1634,6 +1805,16 @@ void mce_estimator_reset(void* handle) {
 }
 
 void mce_estimator_destroy(void* handle) { pycauchy_single_step_shutdown(handle); }
+
+int mce_estimator_finalize_extended_moments(void* handle, double* x_bar) {
+    try {
+        ((Handle*)handle)->eng->finalize_extended_moments(x_bar);
+        return 0;
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_finalize_extended_moments", ex);
+        return -1;
+    }
+}
 
 int mce_last_update(void* handle, double* out_mom, int* out_cnt) {
     Engine* e = ((Handle*)handle)->eng;

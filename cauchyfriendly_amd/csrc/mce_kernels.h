@@ -2436,5 +2436,19 @@ __global__ void k_det_tp(TermPool P, int n, const double* __restrict__ Tr, const
         for (int r = 0; r < D; r++) b[r] += bias[r];
 }
 
+// A d-vector passed by value as a kernel argument.
+struct VecArg {
+    double v[kMaxD];
+};
+
+// b <- b - dx for every term (cauchy_estimator.hpp:1371-1380, sub_vecs).
+template <int D>
+__global__ void k_sub_b(TermPool P, int n, VecArg dx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* b = P.b + (int64_t)i * D;
+#pragma unroll
+    for (int r = 0; r < D; r++) b[r] -= dx.v[r];
+}
 
 }  // namespace mce

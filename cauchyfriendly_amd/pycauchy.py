@@ -106,6 +106,33 @@ class _Bindings:
         res.append(_take_i(out_i[0], out_i[1].value))
         return tuple(res)
 
+    def pycauchy_initialize_nonlin(self, num_steps, x0, A0, p0, b0, beta, gamma, f_duc_ptr1, f_duc_ptr2, f_duc_ptr3,
+                                   cmcc, dt, init_step, debug_print):
+        args = [_arr(v) for v in (x0, A0, p0, b0, beta, gamma)]
+        flat = []
+        for a in args:
+            flat += [a[1], a[2]]
+        cbs = [f if isinstance(f, int) else ct.cast(f, ct.c_void_p).value for f in (f_duc_ptr1, f_duc_ptr2, f_duc_ptr3)]
+        h = self._f("pycauchy_initialize_nonlin")(int(num_steps), *flat, *[ct.c_void_p(f) for f in cbs], int(cmcc),
+                                                  float(dt), int(init_step), bool(debug_print))
+        return self._check(h, "pycauchy_initialize_nonlin")
+
+    def pycauchy_single_step_nonlin(self, handle, msmts, controls, with_propagate):
+        """Returns (Phi, Gamma, B, H, beta, gamma, fz, xhat, Phat, xbar, zbar, cerr_fz, cerr_xhat, cerr_Phat,
+        err_code)."""
+        m_ = _arr(msmts)
+        c_ = _arr(controls)
+        outs_d = [(_dp(), ct.c_int()) for _ in range(14)]
+        out_i = (_ip(), ct.c_int())
+        args = [ct.c_void_p(handle), m_[1], m_[2], c_[1] if c_[2] > 0 else None, c_[2], bool(with_propagate)]
+        for p, n in outs_d:
+            args += [ct.byref(p), ct.byref(n)]
+        args += [ct.byref(out_i[0]), ct.byref(out_i[1])]
+        self._f("pycauchy_single_step_nonlin")(*args)
+        res = [_take_d(p, n.value) for p, n in outs_d]
+        res.append(_take_i(out_i[0], out_i[1].value))
+        return tuple(res)
+
     def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
         a = [_arr(v) for v in (A0, p0, b0, xbar)]
         self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],

@@ -236,3 +236,128 @@ def reference_demo_systems() -> List[System]:
          2.7335350536863903, -0.3754139600950176, 0.6986657326616188,
          0.52558307773279223, 0.82802377147093642, 0.98211422248186553], srand_seed=1))
     return out
+
+
+# ------------------------------------------------------------ nonlinear (extended) mode
+@dataclasses.dataclass
+class Pendulum:
+    """A damped pendulum, state (angle, rate), for the extended estimator.
+
+    Propagation is one classical Runge-Kutta 4 step of
+    d(angle)/dt = rate, d(rate)/dt = -(g/L) sin(angle) - damp * rate; Phi and Gamma are
+    the second-order Taylor discretisations of its Jacobian (process noise enters the
+    rate).  Measurements are the bob's horizontal position L sin(angle) and, when p = 2,
+    its height -L cos(angle).  The setup follows the reference's pendulum example
+    (scripts/swig/cauchy/test_pycauchy_nonlin.py), with a nonlinear measurement here."""
+
+    name: str
+    p: int
+    x0: np.ndarray              # true initial state
+    xbar0: np.ndarray           # the estimator's initial x_bar
+    A0: np.ndarray
+    p0: np.ndarray
+    b0: np.ndarray
+    beta: np.ndarray            # 1 process-noise scale
+    gamma: np.ndarray           # p measurement-noise scales
+    zs: np.ndarray              # (steps, p) measurements
+    L: float = 0.3
+    g: float = 9.81
+    damp: float = 0.3
+    dt: float = 0.05
+    srand_seed: int = 1
+
+    n = 2
+    pncc = 1
+
+    @property
+    def num_steps(self) -> int:
+        return int(self.zs.shape[0])
+
+    def ode(self, x):
+        return np.array([x[1], -self.g / self.L * np.sin(x[0]) - self.damp * x[1]])
+
+    def propagate(self, x):
+        h = self.dt
+        k1 = self.ode(x)
+        k2 = self.ode(x + 0.5 * h * k1)
+        k3 = self.ode(x + 0.5 * h * k2)
+        k4 = self.ode(x + h * k3)
+        return x + h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+
+    def jacobian(self, x):
+        return np.array([[0.0, 1.0], [-self.g / self.L * np.cos(x[0]), -self.damp]])
+
+    def discretise(self, x):
+        J = self.jacobian(x)
+        h = self.dt
+        Phi = np.eye(2) + J * h + J @ J * (h * h / 2.0)
+        Gc = np.array([0.0, 1.0])
+        Gamma = (np.eye(2) * h + J * (h * h / 2.0) + J @ J * (h ** 3 / 6.0)) @ Gc
+        return Phi, Gamma.reshape(2, 1)
+
+    def h(self, x):
+        z = [self.L * np.sin(x[0]), -self.L * np.cos(x[0])]
+        return np.array(z[:self.p])
+
+    def H(self, x):
+        rows = [[self.L * np.cos(x[0]), 0.0], [self.L * np.sin(x[0]), 0.0]]
+        return np.array(rows[:self.p])
+
+    # the three callbacks of the extended estimator, over the container view
+    def callbacks(self, view_type):
+        def dynamics(c_duc):
+            v = view_type(c_duc)
+            x = v.cget_x()
+            Phi, Gamma = self.discretise(x)
+            v.cset_x(self.propagate(x))
+            v.cset_is_xbar_set_for_ece()
+            v.cset_Phi(Phi)
+            v.cset_Gamma(Gamma)
+
+        def msmt_model(c_duc, c_zbar):
+            v = view_type(c_duc)
+            v.cset_zbar(c_zbar, self.h(v.cget_x()))
+
+        def msmt_jacobian(c_duc):
+            v = view_type(c_duc)
+            v.cset_H(self.H(v.cget_x()))
+
+        return dynamics, msmt_model, msmt_jacobian
+
+    def to_dict(self) -> dict:
+        out = {}
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            out[f.name] = v.tolist() if isinstance(v, np.ndarray) else v
+        return out
+
+    @staticmethod
+    def from_dict(o: dict) -> "Pendulum":
+        kw = {}
+        for f in dataclasses.fields(Pendulum):
+            v = o[f.name]
+            kw[f.name] = np.asarray(v, dtype=np.float64) if isinstance(v, list) else v
+        return Pendulum(**kw)
+
+
+def pendulum_system(p: int = 1, steps: int = 8, seed_offset: int = 40, name: str = "pendulum") -> Pendulum:
+    """Seeded pendulum run: true trajectory with Cauchy process and measurement noise,
+    and the estimator's initial conditions."""
+    rng = np.random.default_rng(BASE_SEED + seed_offset)
+    beta = np.array([0.08])
+    gamma = np.full(p, 0.05)
+    # A0 couples angle and rate, so that the rate's deviation is not left at exactly zero
+    # by the first angle-only measurements (the moment checks divide by its size)
+    A0 = np.array([[1.0, 0.4], [-0.3, 1.0]])
+    proto = Pendulum(name=name, p=p, x0=np.array([np.pi / 3, 0.0]), xbar0=np.zeros(2), A0=A0,
+                     p0=np.array([0.1, 0.1]), b0=np.zeros(2), beta=beta, gamma=gamma, zs=np.zeros((0, p)))
+    x = proto.x0.copy()
+    zs = []
+    for k in range(steps):
+        if k > 0:
+            _, G = proto.discretise(x)
+            x = proto.propagate(x) + G.reshape(-1) * beta[0] * rng.standard_cauchy()
+        zs.append(proto.h(x) + gamma * rng.standard_cauchy(p))
+    proto.xbar0 = proto.x0 + proto.p0 * rng.standard_cauchy(2) * 0.5
+    proto.zs = np.array(zs)
+    return proto

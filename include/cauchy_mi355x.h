@@ -73,6 +73,37 @@ void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, doub
                                double** out_cerr_xhat, int* size_out_cerr_xhat, double** out_cerr_Phat,
                                int* size_out_cerr_Phat, int** out_err_code, int* size_out_err_code);
 
+/* Replaces pycauchy_initialize_nonlin, scripts/swig/cauchy/pycauchy.hpp:365-396: the
+ * extended (linearised) estimator.  n = size_b0, pncc = size_beta, p = size_gamma; x0 is the
+ * initial x_bar.  The estimator itself runs on the state deviation, without a control term.
+ * The three callbacks run on the calling thread: f_dyn_update_callback propagates duc->x and
+ * sets Phi, Gamma, B, beta and is_xbar_set_for_ece; f_nonlinear_msmt_model writes h(duc->x) into
+ * its p-vector argument; f_extended_msmt_update_callback sets H (and gamma) at duc->x.
+ * Returns NULL on failure. */
+void* pycauchy_initialize_nonlin(int num_steps, double* x0, int size_x0, double* A0, int size_A0, double* p0,
+                                 int size_p0, double* b0, int size_b0, double* beta, int size_beta, double* gamma,
+                                 int size_gamma, void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                                 void (*f_nonlinear_msmt_model)(CauchyDynamicsUpdateContainer*, double*),
+                                 void (*f_extended_msmt_update_callback)(CauchyDynamicsUpdateContainer*), int cmcc,
+                                 double dt, int init_step, bool debug_print);
+
+/* Replaces pycauchy_single_step_nonlin, scripts/swig/cauchy/pycauchy.hpp:545-652.  With
+ * with_propagate, the dynamics callback runs first.  Then, for rows p-size_msmts .. p-1: zbar
+ * and H from the callbacks, one update with the residual z - zbar, and the extended finish
+ * (terms shifted by the estimated deviation, x_bar <- Re(x_bar + deviation)).  All 15 outputs
+ * are malloc'd; the dynamics outputs hold the current dynamics when the step does not
+ * propagate (the reference leaves them unset). */
+void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, double* controls, int size_controls,
+                                 bool with_propagate, double** out_Phi, int* size_out_Phi, double** out_Gamma,
+                                 int* size_out_Gamma, double** out_B, int* size_out_B, double** out_H,
+                                 int* size_out_H, double** out_beta, int* size_out_beta, double** out_gamma,
+                                 int* size_out_gamma, double** out_fz, int* size_out_fz, double** out_xhat,
+                                 int* size_out_xhat, double** out_Phat, int* size_out_Phat, double** out_xbar,
+                                 int* size_out_xbar, double** out_zbar, int* size_out_zbar, double** out_cerr_fz,
+                                 int* size_out_cerr_fz, double** out_cerr_xhat, int* size_out_cerr_xhat,
+                                 double** out_cerr_Phat, int* size_out_cerr_Phat, int** out_err_code,
+                                 int* size_out_err_code);
+
 /* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar);
@@ -139,6 +170,11 @@ void mce_estimator_reset(void* handle);
 
 /* Releases an estimator made by mce_estimator_create (same as pycauchy_single_step_shutdown). */
 void mce_estimator_destroy(void* handle);
+
+/* Replaces CauchyEstimator::finalize_extended_moments, include/cauchy_estimator.hpp:1358-1394:
+ * b -= Re(mean) for every term (not after a window's last update), mean += x_bar,
+ * x_bar <- Re(mean).  Returns 0, or -1 on a device error. */
+int mce_estimator_finalize_extended_moments(void* handle, double* x_bar);
 
 /* ---- diagnostics (no reference counterpart) ---------------------------------------- */
 

@@ -58,6 +58,7 @@ def lib(kind: str = "ref"):
         L.ref_step.restype = ct.c_int
         L.ref_step.argtypes = [ct.c_void_p, ct.c_double, _dp, _dp, _dp, _dp, ct.c_double, _dp, _dp, _dp, _ip]
         L.ref_public_moments.argtypes = [ct.c_void_p, _dp, _dp, _dp]
+        L.ref_finalize_extended_moments.argtypes = [ct.c_void_p, _dp]
         L.ref_term.restype = ct.c_int
         L.ref_term.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, _ip, _dp, _dp, _dp,
                                ct.POINTER(ct.c_uint), _dp]
@@ -140,6 +141,12 @@ class RefEstimator:
             "shape_mu": cnt[4:4 + sr].copy(),
             "shape_ftr": cnt[4 + sr:4 + 2 * sr].copy(),
         }
+
+    def finalize_extended(self, x_bar):
+        """Extended-mode finish of the last update; returns the new x_bar."""
+        xb = np.ascontiguousarray(np.asarray(x_bar, dtype=np.float64).reshape(-1).copy())
+        lib(self.kind).ref_finalize_extended_moments(self.h, xb.ctypes.data_as(_dp))
+        return xb
 
     def public_moments(self):
         d = self.d

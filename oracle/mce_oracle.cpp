@@ -1483,6 +1483,23 @@ int oracle_step(void* h, double msmt, const double* Phi, const double* Gamma, co
     return E.errs;
 }
 
+// Extended-mode finish of an update, cauchy_estimator.hpp:1358-1394: unless this was the
+// window's last update, every term's b moves by -Re(mean) (sub_vecs); then mean += x_bar
+// and x_bar <- Re(mean).
+void oracle_finalize_extended_moments(void* h, double* x_bar) {
+    Est& E = *(Est*)h;
+    const int d = E.d;
+    if (!E.skip_post_mu) {
+        std::vector<double> dx(d);
+        for (int i = 0; i < d; i++) dx[i] = cre(E.mean[i]);
+        for (auto& bk : E.bucket)
+            for (auto& t : bk)
+                for (int j = 0; j < d; j++) t.b[j] -= dx[j];
+    }
+    for (int i = 0; i < d; i++) E.mean[i] = mk(cre(E.mean[i]) + x_bar[i], cim(E.mean[i]));
+    for (int i = 0; i < d; i++) x_bar[i] = cre(E.mean[i]);
+}
+
 void oracle_public_moments(void* h, double* fz, double* xhat, double* Phat) {
     Est& E = *(Est*)h;
     fz[0] = cre(E.fz);

@@ -29,6 +29,8 @@ int oracle_step(void* h, double msmt, const double* Phi, const double* Gamma, co
                 const double* H, double gamma, const double* B, const double* u,
                 double* out_mom, int* out_cnt);
 void oracle_public_moments(void* h, double* fz, double* xhat, double* Phat);
+/* Extended-mode finish of the last update; x_bar (d) in/out. */
+void oracle_finalize_extended_moments(void* h, double* x_bar);
 int oracle_term(void* h, int shape, int idx, int* info, double* A, double* p, double* b,
                 unsigned int* keys, double* gvals);
 int oracle_set_tr_search_idxs_ordering(const int* ordering, int n);

@@ -152,6 +152,13 @@ void ref_public_moments(void* hv, double* fz, double* xhat, double* Phat) {
     for (int i = 0; i < d * d; i++) Phat[i] = creal(e->conditional_variance[i]);
 }
 
+// Extended-mode finish of an update (CauchyEstimator::finalize_extended_moments,
+// include/cauchy_estimator.hpp:1358-1394), x_bar in/out.
+void ref_finalize_extended_moments(void* hv, double* x_bar) {
+    RefHandle* h = (RefHandle*)hv;
+    h->est->finalize_extended_moments(x_bar);
+}
+
 // Term introspection after a step that ran FTR: every term is a parent whose
 // g-table (sorted by key) is gtable_p (cauchy_term.hpp:748-755).
 //   info: [m, cells_gtable_p, phc]

@@ -34,6 +34,27 @@ void refpy_single_step_ltiv(void* h, double* msmts, int size_msmts, double* cont
                               o6, s6, o7, s7, o8, s8, o9, s9, o10, s10, o11, s11, o12, s12, o13, s13);
 }
 
+void* refpy_initialize_nonlin(int num_steps, double* x0, int size_x0, double* A0, int size_A0, double* p0,
+                              int size_p0, double* b0, int size_b0, double* beta, int size_beta, double* gamma,
+                              int size_gamma, void (*f_dyn)(CauchyDynamicsUpdateContainer*),
+                              void (*f_msmt)(CauchyDynamicsUpdateContainer*, double*),
+                              void (*f_ext)(CauchyDynamicsUpdateContainer*), int cmcc, double dt, int init_step,
+                              bool debug_print) {
+    return pycauchy_initialize_nonlin(num_steps, x0, size_x0, A0, size_A0, p0, size_p0, b0, size_b0, beta, size_beta,
+                                      gamma, size_gamma, f_dyn, f_msmt, f_ext, cmcc, dt, init_step, debug_print);
+}
+
+void refpy_single_step_nonlin(void* h, double* msmts, int size_msmts, double* controls, int size_controls,
+                              bool with_propagate, double** o1, int* s1, double** o2, int* s2, double** o3, int* s3,
+                              double** o4, int* s4, double** o5, int* s5, double** o6, int* s6, double** o7, int* s7,
+                              double** o8, int* s8, double** o9, int* s9, double** o10, int* s10, double** o11,
+                              int* s11, double** o12, int* s12, double** o13, int* s13, double** o14, int* s14,
+                              int** o15, int* s15) {
+    pycauchy_single_step_nonlin(h, msmts, size_msmts, controls, size_controls, with_propagate, o1, s1, o2, s2, o3, s3,
+                                o4, s4, o5, s5, o6, s6, o7, s7, o8, s8, o9, s9, o10, s10, o11, s11, o12, s12, o13,
+                                s13, o14, s14, o15, s15);
+}
+
 void refpy_single_step_reset(void* h, double* A0, int sA0, double* p0, int sp0, double* b0, int sb0, double* xbar,
                              int sxbar) {
     pycauchy_single_step_reset(h, A0, sA0, p0, sp0, b0, sb0, xbar, sxbar);
