@@ -63,6 +63,15 @@ def declare_pycauchy(L, sym=lambda name: name):
     getattr(L, sym("pycauchy_single_step_nonlin")).restype = None
     getattr(L, sym("pycauchy_single_step_nonlin")).argtypes = [ct.c_void_p, _dp, ct.c_int, _dp, ct.c_int, ct.c_bool] + \
         [_dpp, _ip] * 14 + [_ipp, _ip]
+    if hasattr(L, sym("pycauchy_batch_single_step_ltiv")):  # this library's batched stepping
+        getattr(L, sym("pycauchy_batch_single_step_ltiv")).restype = ct.c_int
+        getattr(L, sym("pycauchy_batch_single_step_ltiv")).argtypes = [ct.POINTER(ct.c_void_p), ct.c_int, _dpp, _ip,
+                                                                       _dpp, _ip, ct.POINTER(ct.c_void_p), _ip,
+                                                                       ct.c_int]
+        getattr(L, sym("pycauchy_batch_single_step_nonlin")).restype = ct.c_int
+        getattr(L, sym("pycauchy_batch_single_step_nonlin")).argtypes = [ct.POINTER(ct.c_void_p), ct.c_int, _dpp,
+                                                                         _ip, _dpp, _ip, _ip,
+                                                                         ct.POINTER(ct.c_void_p), _ip, ct.c_int]
     getattr(L, sym("pycauchy_single_step_reset")).argtypes = [ct.c_void_p] + [_dp, ct.c_int] * 4
     getattr(L, sym("pycauchy_single_step_set_master_step")).argtypes = [ct.c_void_p, ct.c_int]
     getattr(L, sym("pycauchy_single_step_set_window_number")).argtypes = [ct.c_void_p, ct.c_int]

@@ -252,10 +252,16 @@ class PyCauchyEstimator:
         if self.mode == "nonlin":
             # the first step of a window only updates; every later one propagates first
             out = self._pc.pycauchy_single_step_nonlin(self.py_handle, z, u, self.step_count != 0)
+        else:
+            out = self._pc.pycauchy_single_step_ltiv(self.py_handle, z, u)
+        return self._absorb_step(out, z, full_info)
+
+    def _absorb_step(self, out, z, full_info):
+        """Keeps the outputs of one step (a single or batched pycauchy step) as step() does."""
+        if self.mode == "nonlin":
             (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._xbar, self._zbar, self._cerr_fz,
              self._cerr_x, self._cerr_P, self._err_code) = out
         else:
-            out = self._pc.pycauchy_single_step_ltiv(self.py_handle, z, u)
             (Phi, Gamma, B, H, beta, gamma, self._fz, self._x, self._P, self._cerr_fz, self._cerr_x, self._cerr_P,
              self._err_code) = out
         if self.mode != "lti":  # the dynamics of this step, as the callbacks left them
@@ -394,6 +400,41 @@ class PyCauchyEstimator:
             pass
 
 
+def step_estimators(estimators, msmts, controls=None, full_info=False, num_threads=0):
+    """Steps every estimator once, as est.step(msmts[i], controls[i]) would, but concurrently:
+    the estimators run on their own HIP streams from up to num_threads host threads (0: one
+    per hardware thread).  All must be initialized estimators of one mode on this library.
+    Returns the list of est.step return values."""
+    ests = list(estimators)
+    if not ests:
+        return []
+    mode = ests[0].mode
+    pc = ests[0]._pc
+    if any(e.mode != mode or e._pc is not pc for e in ests):
+        raise ValueError("the estimators must share one mode and one backend")
+    if not hasattr(pc, "pycauchy_batch_single_step_ltiv"):
+        raise RuntimeError("the backend has no batched stepping")
+    if controls is None:
+        controls = [None] * len(ests)
+    zs, us = [], []
+    for e, z, u in zip(ests, msmts, controls):
+        if not e.is_initialized:
+            raise RuntimeError("estimator is not initialized")
+        if e.step_count == e.num_steps:
+            raise RuntimeError(f"an estimator has already processed its {e.num_steps} steps; reset it first")
+        z_, u_ = e._checked_inputs(z, u)
+        e._msmts = z_.copy()
+        e._controls = u_.copy()
+        zs.append(z_)
+        us.append(u_)
+    handles = [e.py_handle for e in ests]
+    if mode == "nonlin":
+        outs = pc.pycauchy_batch_single_step_nonlin(handles, zs, us, [e.step_count != 0 for e in ests], num_threads)
+    else:
+        outs = pc.pycauchy_batch_single_step_ltiv(handles, zs, us, num_threads)
+    return [e._absorb_step(o, z, bool(full_info)) for e, o, z in zip(ests, outs, zs)]
+
+
 class PySlidingWindowManager:
     """A bank of num_windows estimators, each spanning num_windows time steps: window
     i starts i steps after window 0; when the oldest window completes it is reset,
@@ -402,7 +443,12 @@ class PySlidingWindowManager:
     COV_UNSTABLE = 2
     COV_DNE = 8
 
-    def __init__(self, mode, num_windows, swm_debug_print=True, win_debug_print=False, backend=None):
+    def __init__(self, mode, num_windows, swm_debug_print=True, win_debug_print=False, backend=None,
+                 concurrent=False):
+        """concurrent: step the active windows together (step_estimators) instead of one
+        after another; the results are the same, and LTV / nonlin callbacks then run on
+        worker threads."""
+        self.concurrent = bool(concurrent)
         self.num_windows = int(num_windows)
         if self.num_windows >= 20:
             raise ValueError("at most 19 windows")
@@ -538,10 +584,15 @@ class PySlidingWindowManager:
         else:
             max_idx = int(np.argmax(self.win_counts))
             min_idx = int(np.argmin(self.win_counts))
-            for i, count in zip(self.win_idxs, self.win_counts):
-                if count > 0:
+            active = [i for i, count in zip(self.win_idxs, self.win_counts) if count > 0]
+            if self.concurrent:
+                step_estimators([self.cauchyEsts[i] for i in active], [msmts] * len(active),
+                                [controls] * len(active))
+            else:
+                for i in active:
                     self.cauchyEsts[i].step(msmts, controls)
-                    self.win_counts[i] += 1
+            for i in active:
+                self.win_counts[i] += 1
             self.min_idx = min_idx
         best, usable = self._best_window_est()
         wx, wP = self._weighted_average_win_est(usable)

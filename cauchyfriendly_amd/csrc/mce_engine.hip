@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <climits>
@@ -16,6 +17,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cauchy_mi355x.h"
@@ -26,7 +28,8 @@ using namespace mce;
 
 namespace {
 
-std::string g_last_error;
+// per host thread, so that estimators stepped concurrently keep their own message
+thread_local std::string g_last_error;
 
 #define HIPCHK(x)                                                                                   \
     do {                                                                                            \
@@ -38,17 +41,39 @@ std::string g_last_error;
 
 int ceil_div(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
-// Grow-only device buffer.
+// Allocations outgrown while an engine works on this host thread.  They are released by the
+// engine once its streams are idle (hipFree synchronises the whole device, which would stall
+// every other estimator stepped concurrently); outside an engine they are freed at once.
+struct Graveyard {
+    std::vector<void*> ptrs;
+    size_t bytes = 0;
+};
+thread_local Graveyard* t_graveyard = nullptr;
+
+struct GraveyardScope {
+    Graveyard* saved;
+    explicit GraveyardScope(Graveyard* g) : saved(t_graveyard) { t_graveyard = g; }
+    ~GraveyardScope() { t_graveyard = saved; }
+};
+
+// Grow-only device buffer; growing discards the contents.
 template <class T>
 struct DBuf {
     T* ptr = nullptr;
     size_t cap = 0;
     void reserve(size_t n) {
         if (n <= cap) return;
-        // kernels still in flight may read the old allocation
-        HIPCHK(hipDeviceSynchronize());
-        if (ptr) HIPCHK(hipFree(ptr));
-        size_t nn = std::max(n, cap + cap / 2);
+        if (ptr) {
+            if (t_graveyard) {
+                // kernels still in flight may read the old allocation: it stays valid
+                t_graveyard->ptrs.push_back(ptr);
+                t_graveyard->bytes += cap * sizeof(T);
+            } else {
+                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipFree(ptr));
+            }
+        }
+        size_t nn = std::max(n, 2 * cap);
         nn = std::max<size_t>(nn, 64);
         HIPCHK(hipMalloc(&ptr, nn * sizeof(T)));
         cap = nn;
@@ -272,6 +297,18 @@ struct Engine {
     DBuf<cplx> contrib;
     DBuf<uint8_t> cneg;
     Profiler prof;
+    Graveyard graveyard;
+    static constexpr size_t kGraveyardBytes = size_t(4) << 30;  // released early beyond this
+
+    // Frees the outgrown allocations once this engine's streams are idle.
+    void bury() {
+        if (graveyard.ptrs.empty()) return;
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(st2));
+        for (void* q : graveyard.ptrs) HIPCHK(hipFree(q));
+        graveyard.ptrs.clear();
+        graveyard.bytes = 0;
+    }
 
     Engine(int num_steps, int d_, int cmcc_, int pncc_, int p_, const double* A0_, const double* p0_,
            const double* b0_)
@@ -298,8 +335,10 @@ struct Engine {
         last_cov = cov;
         fz = cmake(0, 0);
         last_fz = cmake(0, 0);
-        HIPCHK(hipStreamCreate(&st));
-        HIPCHK(hipStreamCreate(&st2));
+        // non-blocking: estimators stepped from several host threads do not serialise on the
+        // null stream; every copy of the engine is ordered on its own streams
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHK(hipHostMalloc((void**)&h_stage, kStageBytes, hipHostMallocDefault));
         d_scal.reserve(16);
         HIPCHK(hipEventCreateWithFlags(&ev_mu, hipEventDisableTiming));
@@ -329,13 +368,13 @@ struct Engine {
         }
         coff[sr] = (int64_t)cmb.size();
         dcombos.reserve(std::max<size_t>(cmb.size(), 1));
-        if (!cmb.empty()) HIPCHK(hipMemcpy(dcombos.ptr, cmb.data(), cmb.size(), hipMemcpyHostToDevice));
+        if (!cmb.empty()) copy_sync(dcombos.ptr, cmb.data(), cmb.size(), hipMemcpyHostToDevice);
         dcombo_off.reserve(sr + 1);
-        HIPCHK(hipMemcpy(dcombo_off.ptr, coff.data(), (sr + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        copy_sync(dcombo_off.ptr, coff.data(), (sr + 1) * sizeof(int64_t), hipMemcpyHostToDevice);
         dcombo_cnt.reserve(sr);
-        HIPCHK(hipMemcpy(dcombo_cnt.ptr, ccnt.data(), sr * sizeof(int), hipMemcpyHostToDevice));
+        copy_sync(dcombo_cnt.ptr, ccnt.data(), sr * sizeof(int), hipMemcpyHostToDevice);
         dcen_half.reserve(sr + 1);
-        HIPCHK(hipMemcpy(dcen_half.ptr, cen.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice));
+        copy_sync(dcen_half.ptr, cen.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice);
         dbpert.reserve(8 * MM);
         upload_draws();
         dconst.reserve(1);
@@ -346,6 +385,9 @@ struct Engine {
     }
 
     ~Engine() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (st2) (void)hipStreamSynchronize(st2);
+        for (void* q : graveyard.ptrs) (void)hipFree(q);
         if (h_stage) (void)hipHostFree(h_stage);
         if (st) (void)hipStreamDestroy(st);
         if (st2) (void)hipStreamDestroy(st2);
@@ -355,23 +397,25 @@ struct Engine {
     }
 
     void upload_draws() {
-        HIPCHK(hipMemcpy(dbpert.ptr, bpert.data(), 8 * MM * sizeof(double), hipMemcpyHostToDevice));
+        copy_sync(dbpert.ptr, bpert.data(), 8 * MM * sizeof(double), hipMemcpyHostToDevice);
     }
 
     // First term (setup_first_term, cauchy_term.hpp:772-785).
     void reset_terms() {
+        bury();
+        GraveyardScope gs(&graveyard);
         in.reserve(1);
         std::vector<double> A(MM * d, 0.0), pp(MM, 0.0);
         memcpy(A.data(), A0.data(), d * d * sizeof(double));
         memcpy(pp.data(), p0.data(), d * sizeof(double));
-        HIPCHK(hipMemcpy(in.A.ptr, A.data(), MM * d * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(in.p.ptr, pp.data(), MM * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(in.b.ptr, b0.data(), d * sizeof(double), hipMemcpyHostToDevice));
+        copy_sync(in.A.ptr, A.data(), MM * d * sizeof(double), hipMemcpyHostToDevice);
+        copy_sync(in.p.ptr, pp.data(), MM * sizeof(double), hipMemcpyHostToDevice);
+        copy_sync(in.b.ptr, b0.data(), d * sizeof(double), hipMemcpyHostToDevice);
         int mm = d, zero = 0;
         int64_t z64 = 0;
-        HIPCHK(hipMemcpy(in.m.ptr, &mm, sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(in.gp_n.ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(in.gp_off.ptr, &z64, sizeof(int64_t), hipMemcpyHostToDevice));
+        copy_sync(in.m.ptr, &mm, sizeof(int), hipMemcpyHostToDevice);
+        copy_sync(in.gp_n.ptr, &zero, sizeof(int), hipMemcpyHostToDevice);
+        copy_sync(in.gp_off.ptr, &z64, sizeof(int64_t), hipMemcpyHostToDevice);
         in_count.assign(sr, 0);
         in_count[d] = 1;
         Nt = 1;
@@ -402,6 +446,12 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(h_stage, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         memcpy(out_, h_stage, n * sizeof(T));
+    }
+    // Blocking copy ordered on the engine's stream.
+    void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+        if (bytes == 0) return;
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+        HIPCHK(hipStreamSynchronize(st));
     }
     // off[n-1] + cnt[n-1] (the total of an exclusive scan) into device scalar `slot`.
     template <class A, class B>
@@ -619,6 +669,17 @@ struct Engine {
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
+        int r;
+        {
+            GraveyardScope gs(&graveyard);
+            r = update_terms(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u);
+        }
+        if (graveyard.bytes > kGraveyardBytes) bury();
+        return r;
+    }
+
+    int update_terms(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
+                     double gamma, const double* Bm, const double* u) {
         if (errs & (1 << 9)) {
             fprintf(stderr, "[Window %d:] ERROR_FZ_NEGATIVE triggered. Cannot continue stepping until this estimator has been reset!\n", win_num);
             return errs;
@@ -1020,8 +1081,9 @@ struct Engine {
         readn(counters.ptr + 2, nr_, 2);
         const int nsort = nr_[0];
         const int rare = nr_[1];
+        // bits 1 and 2 note parent (old) terms among a group's members, which stage_term
+        // evaluates through their own tables; bit 4 is a broken invariant
         if (rare & 4) throw std::runtime_error("parent g-table key outside its half-storage range");
-        if (rare) fprintf(stderr, "mce: a parent term joined a reduction group (flag %d); treated as a new child\n", rare);
         if (nsort > 0) {
             tmpk.reserve((size_t)cells_total + 1);
             tmpv.reserve((size_t)cells_total + 1);
@@ -1289,18 +1351,18 @@ struct Engine {
         int mm;
         int gn;
         int64_t go;
-        HIPCHK(hipMemcpy(&mm, in.m.ptr + o, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&gn, in.gp_n.ptr + o, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&go, in.gp_off.ptr + o, sizeof(int64_t), hipMemcpyDeviceToHost));
+        copy_sync(&mm, in.m.ptr + o, sizeof(int), hipMemcpyDeviceToHost);
+        copy_sync(&gn, in.gp_n.ptr + o, sizeof(int), hipMemcpyDeviceToHost);
+        copy_sync(&go, in.gp_off.ptr + o, sizeof(int64_t), hipMemcpyDeviceToHost);
         info[0] = mm;
         info[1] = gn;
         info[2] = mm;
-        if (A) HIPCHK(hipMemcpy(A, in.A.ptr + (int64_t)o * MM * d, mm * d * sizeof(double), hipMemcpyDeviceToHost));
-        if (p_) HIPCHK(hipMemcpy(p_, in.p.ptr + (int64_t)o * MM, mm * sizeof(double), hipMemcpyDeviceToHost));
-        if (b) HIPCHK(hipMemcpy(b, in.b.ptr + (int64_t)o * d, d * sizeof(double), hipMemcpyDeviceToHost));
+        if (A) copy_sync(A, in.A.ptr + (int64_t)o * MM * d, mm * d * sizeof(double), hipMemcpyDeviceToHost);
+        if (p_) copy_sync(p_, in.p.ptr + (int64_t)o * MM, mm * sizeof(double), hipMemcpyDeviceToHost);
+        if (b) copy_sync(b, in.b.ptr + (int64_t)o * d, d * sizeof(double), hipMemcpyDeviceToHost);
         if (keys_ && g && gn > 0) {
-            HIPCHK(hipMemcpy(keys_, gk[gcur].ptr + go, gn * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(g, gv[gcur].ptr + go, gn * sizeof(cplx), hipMemcpyDeviceToHost));
+            copy_sync(keys_, gk[gcur].ptr + go, gn * sizeof(uint32_t), hipMemcpyDeviceToHost);
+            copy_sync(g, gv[gcur].ptr + go, gn * sizeof(cplx), hipMemcpyDeviceToHost);
         }
         return 0;
     }
@@ -1316,9 +1378,9 @@ void Engine::widen(const int* src, int64_t* dst, int n) {
 void Engine::deterministic_transform(const double* Trans, const double* bias) {
     DBuf<double> dT;
     dT.reserve(d * d + d);
-    HIPCHK(hipMemcpy(dT.ptr, Trans, d * d * sizeof(double), hipMemcpyHostToDevice));
+    copy_sync(dT.ptr, Trans, d * d * sizeof(double), hipMemcpyHostToDevice);
     const bool shift = !skip && bias != nullptr;
-    if (shift) HIPCHK(hipMemcpy(dT.ptr + d * d, bias, d * sizeof(double), hipMemcpyHostToDevice));
+    if (shift) copy_sync(dT.ptr + d * d, bias, d * sizeof(double), hipMemcpyHostToDevice);
     TermPool IN = in.view();
     MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_det_tp<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dT.ptr,
                                          shift ? dT.ptr + d * d : nullptr));
@@ -1354,6 +1416,46 @@ struct Handle {
 void set_err(const char* where, const std::exception& e) {
     g_last_error = std::string(where) + ": " + e.what();
     fprintf(stderr, "[cauchy_mi355x] %s\n", g_last_error.c_str());
+}
+
+// Runs step(i) for i in [0, count) on up to num_threads host threads (0: one per hardware
+// thread), estimators with the most terms first.  Each estimator is stepped by one thread
+// at a time, on its own streams.  Returns the number of estimators whose step reported an
+// error; the calling thread's last error is the first of them.
+template <class F>
+int run_concurrently(void** handles, int count, int num_threads, int (*terms_of)(void*), F step) {
+    if (count <= 0) return 0;
+    std::vector<int> order(count);
+    for (int i = 0; i < count; i++) order[i] = i;
+    std::vector<int> nt(count);
+    for (int i = 0; i < count; i++) nt[i] = terms_of(handles[i]);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nt[a] > nt[b]; });
+    int nthreads = num_threads > 0 ? num_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nthreads = std::min(nthreads, count);
+    std::atomic<int> next{0};
+    std::vector<std::string> errors(count);
+    auto work = [&]() {
+        int k;
+        while ((k = next.fetch_add(1)) < count) {
+            const int i = order[k];
+            g_last_error.clear();
+            step(i);
+            errors[i] = g_last_error;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    int failed = 0;
+    std::string first;
+    for (int i = 0; i < count; i++)
+        if (!errors[i].empty()) {
+            if (!failed) first = "estimator " + std::to_string(i) + ": " + errors[i];
+            failed++;
+        }
+    g_last_error = first;
+    return failed;
 }
 
 template <class T>
@@ -1665,6 +1767,39 @@ void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, do
         (*out_err_code)[i] = e->errs;
     }
     duc.step += 1;
+}
+
+int pycauchy_batch_single_step_ltiv(void** handles, int count, double** msmts, const int* size_msmts,
+                                    double** controls, const int* size_controls, void** outs, int* out_sizes,
+                                    int num_threads) {
+    return run_concurrently(handles, count, num_threads, pycauchy_single_step_get_number_of_terms, [&](int i) {
+        void** o = outs + 13 * (int64_t)i;
+        int* n = out_sizes + 13 * (int64_t)i;
+        double* u = controls ? controls[i] : nullptr;
+        const int nu = size_controls ? size_controls[i] : 0;
+        pycauchy_single_step_ltiv(handles[i], msmts[i], size_msmts[i], u, nu, (double**)&o[0], &n[0],
+                                  (double**)&o[1], &n[1], (double**)&o[2], &n[2], (double**)&o[3], &n[3],
+                                  (double**)&o[4], &n[4], (double**)&o[5], &n[5], (double**)&o[6], &n[6],
+                                  (double**)&o[7], &n[7], (double**)&o[8], &n[8], (double**)&o[9], &n[9],
+                                  (double**)&o[10], &n[10], (double**)&o[11], &n[11], (int**)&o[12], &n[12]);
+    });
+}
+
+int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts, const int* size_msmts,
+                                     double** controls, const int* size_controls, const int* with_propagate,
+                                     void** outs, int* out_sizes, int num_threads) {
+    return run_concurrently(handles, count, num_threads, pycauchy_single_step_get_number_of_terms, [&](int i) {
+        void** o = outs + 15 * (int64_t)i;
+        int* n = out_sizes + 15 * (int64_t)i;
+        double* u = controls ? controls[i] : nullptr;
+        const int nu = size_controls ? size_controls[i] : 0;
+        pycauchy_single_step_nonlin(handles[i], msmts[i], size_msmts[i], u, nu, with_propagate[i] != 0,
+                                    (double**)&o[0], &n[0], (double**)&o[1], &n[1], (double**)&o[2], &n[2],
+                                    (double**)&o[3], &n[3], (double**)&o[4], &n[4], (double**)&o[5], &n[5],
+                                    (double**)&o[6], &n[6], (double**)&o[7], &n[7], (double**)&o[8], &n[8],
+                                    (double**)&o[9], &n[9], (double**)&o[10], &n[10], (double**)&o[11], &n[11],
+                                    (double**)&o[12], &n[12], (double**)&o[13], &n[13], (int**)&o[14], &n[14]);
+    });
 }
 
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,

@@ -133,6 +133,42 @@ class _Bindings:
         res.append(_take_i(out_i[0], out_i[1].value))
         return tuple(res)
 
+    def _batch(self, fname, nout, handles, msmts, controls, extra, num_threads):
+        count = len(handles)
+        hs = (ct.c_void_p * count)(*[ct.c_void_p(h) for h in handles])
+        m_ = [_arr(m) for m in msmts]
+        c_ = [_arr(c) for c in controls]
+        mp = (_dp * count)(*[a[1] for a in m_])
+        mn = (ct.c_int * count)(*[a[2] for a in m_])
+        cp = (_dp * count)(*[a[1] if a[2] > 0 else _dp() for a in c_])
+        cn = (ct.c_int * count)(*[a[2] for a in c_])
+        outs = (ct.c_void_p * (nout * count))()
+        sizes = (ct.c_int * (nout * count))()
+        failed = self._f(fname)(hs, count, mp, mn, cp, cn, *extra, outs, sizes, int(num_threads))
+        res = []
+        for i in range(count):
+            r = []
+            for j in range(nout):
+                ptr, n = outs[nout * i + j], sizes[nout * i + j]
+                if j == nout - 1:
+                    r.append(_take_i(ct.cast(ptr, _ip), n))
+                else:
+                    r.append(_take_d(ct.cast(ptr, _dp), n))
+            res.append(tuple(r))
+        if failed:
+            msg = self._last_error() if self._last_error else ""
+            raise RuntimeError(f"{fname}: {failed} estimator(s) failed: {msg}")
+        return res
+
+    def pycauchy_batch_single_step_ltiv(self, handles, msmts, controls, num_threads=0):
+        """pycauchy_single_step_ltiv of every handle, concurrently; a list of its 13-tuples."""
+        return self._batch("pycauchy_batch_single_step_ltiv", 13, handles, msmts, controls, (), num_threads)
+
+    def pycauchy_batch_single_step_nonlin(self, handles, msmts, controls, with_propagate, num_threads=0):
+        """pycauchy_single_step_nonlin of every handle, concurrently; a list of its 15-tuples."""
+        wp = (ct.c_int * len(handles))(*[1 if w else 0 for w in with_propagate])
+        return self._batch("pycauchy_batch_single_step_nonlin", 15, handles, msmts, controls, (wp,), num_threads)
+
     def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
         a = [_arr(v) for v in (A0, p0, b0, xbar)]
         self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],

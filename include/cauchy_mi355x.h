@@ -104,6 +104,25 @@ void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, do
                                  double** out_cerr_Phat, int* size_out_cerr_Phat, int** out_err_code,
                                  int* size_out_err_code);
 
+/* Batched stepping (no counterpart in the reference, whose windows and estimators step one
+ * at a time or in forked processes).  Steps count estimators concurrently from up to
+ * num_threads host threads (0: one per hardware thread); estimator i runs exactly
+ * pycauchy_single_step_ltiv(handles[i], msmts[i], size_msmts[i], controls[i], size_controls[i],
+ * ...) on its own HIP streams, so each result is that of the single step.  Its 13 outputs are
+ * outs[13*i + j] / out_sizes[13*i + j] in the single step's order (j = 12, the error codes, is
+ * an int*), malloc'd as there.  controls / size_controls may be NULL.  LTV callbacks run on
+ * the worker threads.  Returns the number of estimators that reported an error (message in
+ * mce_last_error()). */
+int pycauchy_batch_single_step_ltiv(void** handles, int count, double** msmts, const int* size_msmts,
+                                    double** controls, const int* size_controls, void** outs, int* out_sizes,
+                                    int num_threads);
+
+/* Batched pycauchy_single_step_nonlin, as above with 15 outputs per estimator (j = 14, the
+ * error codes, is an int*) and with_propagate[i] != 0 for the step's propagation. */
+int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts, const int* size_msmts,
+                                     double** controls, const int* size_controls, const int* with_propagate,
+                                     void** outs, int* out_sizes, int num_threads);
+
 /* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar);

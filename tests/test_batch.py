@@ -1,0 +1,91 @@
+"""Batched stepping: many independent estimators stepped concurrently
+(pycauchy_batch_single_step_ltiv / step_estimators).
+
+The batch is the ragged recipe of cauchyfriendly_amd/workloads.batch_systems: state
+dimension 2..5, windows of 4..8 steps, 30% of the measurements 50x outliers.  Each
+estimator stepped in the batch must give what it gives stepped alone (same kernels,
+its own streams), and both must agree with the CPU restatement of the reference
+(counts and error bits exact, moments within 1e-9)."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from cauchyfriendly_amd import workloads as W
+from oracle import driver as dr
+
+TOL = 1e-9
+COUNT = 16
+
+
+def _systems():
+    return W.batch_systems(count=COUNT)
+
+
+def _make(s):
+    from cauchyfriendly_amd.cauchy_estimator import PyCauchyEstimator
+    ct.CDLL("libc.so.6").srand(ct.c_uint(s.srand_seed))
+    e = PyCauchyEstimator("lti", s.num_steps, False)
+    e.initialize_lti(s.A0, s.p0, s.b0, s.Phi(0), None, s.Gamma, s.beta, s.H, s.gamma)
+    return e
+
+
+def _record(e):
+    return (e._fz.copy(), e._x.copy(), e._P.copy(), e._err_code.copy(), e.get_num_CF_terms())
+
+
+def test_batch_recipe_is_ragged():
+    ss = _systems()
+    assert len({s.n for s in ss}) >= 3 and len({s.num_steps for s in ss}) >= 3
+
+
+@pytest.mark.gpu
+def test_batch_step_matches_single_steps_and_oracle():
+    from cauchyfriendly_amd.cauchy_estimator import step_estimators
+    ss = _systems()
+    single = []
+    for s in ss:
+        e = _make(s)
+        recs = []
+        for k in range(s.num_steps):
+            e.step(s.zs[k], None)
+            recs.append(_record(e))
+        e.shutdown()
+        single.append(recs)
+    ests = [_make(s) for s in ss]
+    batched = [[] for _ in ss]
+    for k in range(max(s.num_steps for s in ss)):
+        live = [i for i, s in enumerate(ss) if k < s.num_steps]
+        step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], num_threads=4)
+        for i in live:
+            batched[i].append(_record(ests[i]))
+    for e in ests:
+        e.shutdown()
+    for i, s in enumerate(ss):
+        _, oracle_recs = dr.run_system(s, kind="oracle")
+        for k in range(s.num_steps):
+            where = f"{s.name} step {k}"
+            fz, x, P, err, nt = batched[i][k]
+            fz1, x1, P1, err1, nt1 = single[i][k]
+            assert nt == nt1 and list(err) == list(err1), f"{where}: batched counts/errors differ from single"
+            for a, b, what in ((fz, fz1, "fz"), (x, x1, "xhat"), (P, P1, "Phat")):
+                assert rel_err(a, b) <= 1e-12, f"{where}: batched {what} differs from single"
+            o = oracle_recs[k]
+            assert nt == o["nt_ftr"], f"{where}: terms {nt} != oracle {o['nt_ftr']}"
+            assert int(err[-1]) == o["err"], f"{where}: error bits"
+            assert rel_err(x, o["mean"].real) <= TOL, f"{where}: xhat vs oracle"
+            assert rel_err(P, o["cov"].real.reshape(-1)) <= TOL, f"{where}: Phat vs oracle"
+            assert rel_err(np.array([fz[-1]]), np.array([o["fz_public"].real])) <= TOL, f"{where}: fz vs oracle"
+
+
+@pytest.mark.gpu
+def test_batch_rejects_mixed_modes_and_finished_estimators():
+    from cauchyfriendly_amd.cauchy_estimator import step_estimators
+    s = _systems()[0]
+    e = _make(s)
+    for k in range(s.num_steps):
+        step_estimators([e], [s.zs[k]])
+    with pytest.raises(RuntimeError):
+        step_estimators([e], [s.zs[0]])
+    e.shutdown()

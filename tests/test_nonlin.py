@@ -149,13 +149,14 @@ def test_nonlin_matches_reference(name):
 
 
 @pytest.mark.gpu
-def test_nonlin_window_bank_matches_reference():
+@pytest.mark.parametrize("concurrent", [False, True], ids=["sequential", "concurrent"])
+def test_nonlin_window_bank_matches_reference(concurrent):
     import ctypes as ct
     from cauchyfriendly_amd.cauchy_estimator import PySlidingWindowManager, Py_CauchyDynamicsUpdateContainer
     rec = _load("window_nonlin_pend")
     s = W.Pendulum.from_dict(rec["system"])
     ct.CDLL("libc.so.6").srand(ct.c_uint(s.srand_seed))
-    swm = PySlidingWindowManager("nonlin", rec["num_windows"], swm_debug_print=False)
+    swm = PySlidingWindowManager("nonlin", rec["num_windows"], swm_debug_print=False, concurrent=concurrent)
     dyn, msmt, ext = s.callbacks(Py_CauchyDynamicsUpdateContainer)
     swm.initialize_nonlin(s.xbar0, s.A0, s.p0, s.b0, s.beta, s.gamma, dyn, msmt, ext, 0, s.dt, 0)
     for k, e in enumerate(rec["outputs"]):

@@ -68,12 +68,12 @@ def test_speyer_init_matches_reference():
         assert np.array_equal(b0, np.array(c["b0"])), f"n={n}: b0 differs"
 
 
-def _run_bank(rec):
+def _run_bank(rec, concurrent=False):
     from cauchyfriendly_amd.cauchy_estimator import PySlidingWindowManager
     s = W.System.from_dict(rec["system"])
     n = s.n
     ct.CDLL("libc.so.6").srand(ct.c_uint(rec["srand"]))
-    swm = PySlidingWindowManager(rec["mode"], rec["num_windows"], swm_debug_print=False)
+    swm = PySlidingWindowManager(rec["mode"], rec["num_windows"], swm_debug_print=False, concurrent=concurrent)
     Gamma = s.Gamma if s.pncc > 1 else s.Gamma.reshape(-1)
     H = s.H if s.p > 1 else s.H.reshape(-1)
     beta = np.asarray(s.beta, dtype=np.float64).reshape(-1)
@@ -99,10 +99,11 @@ def _run_bank(rec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("concurrent", [False, True], ids=["sequential", "concurrent"])
 @pytest.mark.parametrize("name", ["window_lti3", "window_ltv4_p2"])
-def test_window_bank_matches_reference(name):
+def test_window_bank_matches_reference(name, concurrent):
     rec = _load(name)
-    got = _run_bank(rec)
+    got = _run_bank(rec, concurrent)
     for k, (g, e) in enumerate(zip(got, rec["outputs"])):
         xhat, Phat, wx, wP, best, fz, err, counts = g
         where = f"{name} step {k}"
