@@ -314,6 +314,24 @@ class PyCauchyEstimator:
             return 1
         return self._pc.pycauchy_single_step_get_number_of_terms(self.py_handle)
 
+    def get_marginal_1D_pointwise_cpdf(self, marg_idx, gridx_low, gridx_high, gridx_resolution, log_dir=None):
+        """Marginal density of state marg_idx on the grid gridx_low, gridx_low + gridx_resolution, ...
+        (cauchy_estimator.py:1003-1031); returns (X, Y), or (None, None) before the first step."""
+        if not self.is_initialized or self.step_count < 1:
+            return None, None
+        if not (gridx_high > gridx_low and gridx_resolution > 0 and 0 <= int(marg_idx) < self.n):
+            raise ValueError("the grid needs high > low, resolution > 0 and a state index below n")
+        pts, num = self._pc.pycauchy_get_marginal_1D_pointwise_cpdf(self.py_handle, int(marg_idx), float(gridx_low),
+                                                                    float(gridx_high), float(gridx_resolution),
+                                                                    log_dir)
+        pts = pts.reshape(num, 2)
+        return pts[:, 0], pts[:, 1]
+
+    def get_1D_pointwise_cpdf(self, gridx_low, gridx_high, gridx_resolution, log_dir=None):
+        if self.n != 1:
+            raise ValueError("get_1D_pointwise_cpdf is for a 1-state system; use get_marginal_1D_pointwise_cpdf")
+        return self.get_marginal_1D_pointwise_cpdf(0, gridx_low, gridx_high, gridx_resolution, log_dir)
+
     def get_last_mean_cov(self):
         return self.moment_info["x"][-1].copy(), self.moment_info["P"][-1].copy()
 

@@ -21,6 +21,7 @@ constexpr double kPluEps = 1e-15;
 constexpr double kCondEps = 1e12;
 constexpr double kCovEigenvalueTol = -1e-5;  // COV_EIGENVALUE_TOLERANCE, cauchy_constants.hpp:102
 constexpr double kReductionEps = 1e-8;
+constexpr double kPi = 3.14159265358979323846;
 constexpr double kInvTwoPi = 1.0 / (2.0 * 3.14159265358979323846);
 
 struct cplx {

@@ -1337,6 +1337,16 @@ struct Engine {
     // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).
     void deterministic_transform(const double* Trans, const double* bias);
 
+    // 1D marginal density of state `marg` on the grid lo, lo + res, ... (clamped to hi), as
+    // CauchyCPDFGridDispatcher1D::evaluate_point_grid does (cpdf_ndim.hpp:2061-2130): the
+    // first point by the direct per-term formula, every later point by the cached form.
+    // Fills xy with (x, f(x)) pairs; returns 1 (no points) after a window's last update,
+    // whose g-tables are not built.
+    int cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy);
+    std::vector<double> bar_nu;  // drawn at the first density evaluation (cpdf_ndim.hpp:384-387)
+    DBuf<double> cpdf_direct, cpdf_x, cpdf_y, cpdf_part, cpdf_sum;
+    DBuf<CpdfTerm1D> cpdf_cache;
+
     // Extended-mode finish of a measurement update (cauchy_estimator.hpp:1358-1394): the
     // terms' b-vectors move by -Re(mean) (skipped on a window's last update), the mean
     // becomes x_bar + mean, and x_bar takes its real part.
@@ -1385,6 +1395,62 @@ void Engine::deterministic_transform(const double* Trans, const double* bias) {
     MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_det_tp<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dT.ptr,
                                          shift ? dT.ptr + d * d : nullptr));
     HIPCHK(hipStreamSynchronize(st));
+}
+
+int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
+    if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
+    if (marg < 0 || marg >= d) throw std::runtime_error("marginal index outside the state");
+    if (!(hi > lo) || !(res > 0)) throw std::runtime_error("the grid needs high > low and resolution > 0");
+    if (master_step == n_mu) {
+        fprintf(stderr, "[cauchy_mi355x] the density cannot be evaluated after the window's last update: its g-tables "
+                        "were not built\n");
+        return 1;
+    }
+    if (bar_nu.empty()) {
+        bar_nu.resize(d);
+        for (int i = 0; i < d; i++) bar_nu[i] = 2 * (((double)rand() + 1.0) / ((double)RAND_MAX + 2.0));
+    }
+    const int npts = (int)((hi - lo + res - 1e-15) / res) + 1;
+    std::vector<double> xs(npts), ys(npts, 0.0);
+    for (int i = 0; i < npts; i++) xs[i] = std::min(lo + i * res, hi);
+    const double norm = fz.re;
+    if (Nt > 0) {
+        GraveyardScope gs(&graveyard);
+        const int nchunks = ceil_div(Nt, kCpdfChunk);
+        cpdf_direct.reserve(Nt);
+        cpdf_cache.reserve(Nt);
+        cpdf_x.reserve(npts);
+        cpdf_y.reserve(npts);
+        cpdf_part.reserve((size_t)nchunks * npts);
+        cpdf_sum.reserve(1);
+        HIPCHK(hipMemcpyAsync(cpdf_x.ptr, xs.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
+        VecArg bn;
+        for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
+        TermPool IN = in.view();
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_cpdf1d_terms<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt,
+                                             marg, bn, gk[gcur].ptr, gv[gcur].ptr, xs[0], cpdf_direct.ptr,
+                                             cpdf_cache.ptr));
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
+        if (npts > 1) {
+            hipLaunchKernelGGL(k_cpdf1d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
+                               cpdf_cache.ptr, Nt, cpdf_x.ptr + 1, npts - 1, cpdf_part.ptr);
+            hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
+                               nchunks, npts - 1, norm, cpdf_y.ptr + 1);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(ys.data() + 1, cpdf_y.ptr + 1, (npts - 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+        }
+        double direct_sum = 0.0;
+        HIPCHK(hipMemcpyAsync(&direct_sum, cpdf_sum.ptr, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        ys[0] = direct_sum * kInvTwoPi / norm;
+    }
+    xy.resize(2 * (size_t)npts);
+    for (int i = 0; i < npts; i++) {
+        xy[2 * i] = xs[i];
+        xy[2 * i + 1] = ys[i];
+    }
+    return 0;
 }
 
 void Engine::finalize_extended_moments(double* x_bar) {
@@ -1800,6 +1866,31 @@ int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts,
                                     (double**)&o[9], &n[9], (double**)&o[10], &n[10], (double**)&o[11], &n[11],
                                     (double**)&o[12], &n[12], (double**)&o[13], &n[13], (int**)&o[14], &n[14]);
     });
+}
+
+void pycauchy_get_marginal_1D_pointwise_cpdf(void* handle, int marg_idx1, double gridx_low, double gridx_high,
+                                             double gridx_resolution, char* log_dir, double** out_cpdf_data,
+                                             int* size_out_cpdf_data, int* out_num_gridx) {
+    (void)log_dir;
+    std::vector<double> xy;
+    int rc = 1;
+    try {
+        rc = ((Handle*)handle)->eng->cpdf_1d(marg_idx1, gridx_low, gridx_high, gridx_resolution, xy);
+    } catch (const std::exception& ex) {
+        set_err("pycauchy_get_marginal_1D_pointwise_cpdf", ex);
+    }
+    if (rc != 0) xy.clear();
+    *out_num_gridx = (int)(xy.size() / 2);
+    *size_out_cpdf_data = (int)xy.size();
+    *out_cpdf_data = mallocd<double>(xy.size());
+    if (!xy.empty()) memcpy(*out_cpdf_data, xy.data(), xy.size() * sizeof(double));
+}
+
+void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,
+                                    char* log_dir, double** out_cpdf_data, int* size_out_cpdf_data,
+                                    int* out_num_gridx) {
+    pycauchy_get_marginal_1D_pointwise_cpdf(handle, 0, gridx_low, gridx_high, gridx_resolution, log_dir, out_cpdf_data,
+                                            size_out_cpdf_data, out_num_gridx);
 }
 
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,

@@ -2441,6 +2441,114 @@ struct VecArg {
     double v[kMaxD];
 };
 
+// ------------------------------------------------------------ 1D marginal CPDF
+// Per-term closed form of the 1D marginal density (cpdf_ndim.hpp:1233-1368): the
+// arrangement is cut along state `marg`; w = sum_j p_j |A_j,marg|, the sign vectors of
+// the two half-lines select the numerators, and the term contributes
+// Re(g_lhs / (w + i(b_marg - x)) - g_rhs / (-w + i(b_marg - x))) at x.  The second form,
+// (a x + bb) / (w^2 + (x - s)^2) with s = b_marg, a = Im(g_rhs) / pi and
+// bb = (Re(g_rhs) w - Im(g_rhs) s) / pi, is the one every later grid point uses.
+struct CpdfTerm1D {
+    double a, bb, w2, s;
+};
+
+template <int D>
+__global__ void k_cpdf1d_terms(TermPool P, int n, int marg, VecArg bar_nu, const uint32_t* __restrict__ gkeys,
+                               const cplx* __restrict__ gvals, double x0, double* __restrict__ direct,
+                               CpdfTerm1D* __restrict__ cached) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = P.m[i];
+    const double* A = P.A + (int64_t)i * P.MM * D;
+    const double* p = P.p + (int64_t)i * P.MM;
+    const double bm = P.b[(int64_t)i * D + marg];
+    double w = 0.0;
+    int lhs = 0, rhs = 0;
+    for (int j = 0; j < m; j++) {
+        const double acj = A[j * D + marg];
+        const double fa = fabs(acj);
+        w += p[j] * fa;
+        bool pos;
+        if (fa > 1e-15) {
+            pos = acj > 0;
+        } else {
+            // a row orthogonal to the state: its side is taken from the drawn direction
+            double ip = 0.0;
+            for (int k = 0; k < D; k++) ip += A[j * D + k] * bar_nu.v[k];
+            pos = ip > 0;
+        }
+        if (pos) lhs |= 1 << j;
+        else rhs |= 1 << j;
+    }
+    const int top = 1 << (m - 1), rev = (1 << m) - 1;
+    const int64_t go = P.gp_off[i];
+    const int gn = P.gp_n[i];
+    const cplx gl = lookup_num(lhs, top, rev, gkeys + go, gvals + go, gn);
+    const cplx gr = lookup_num(rhs, top, rev, gkeys + go, gvals + go, gn);
+    const double bc = bm - x0;
+    const cplx v = csub(cdiv(gl, cmake(w, bc)), cdiv(gr, cmake(-w, bc)));
+    direct[i] = v.re;
+    CpdfTerm1D c;
+    c.s = bm;
+    c.a = gr.im / kPi;
+    c.bb = (gr.re * w - gr.im * bm) / kPi;
+    c.w2 = w * w;
+    cached[i] = c;
+}
+
+// Deterministic sum of n values by one block (fixed strided partials, then a fixed tree).
+__global__ void k_sum_fixed(const double* __restrict__ v, int n, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < n; i += blockDim.x) s += v[i];
+    sh[t] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (t < h) sh[t] += sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[0] = sh[0];
+}
+
+constexpr int kCpdfChunk = 1024;   // terms staged in LDS per block
+constexpr int kCpdfPoints = 256;   // grid points per block (one per thread)
+
+// Partial sums over one chunk of terms for a block of grid points:
+// part[chunk * npts + k] = sum over the chunk of (a x_k + bb) / (w2 + (x_k - s)^2).
+__global__ void __launch_bounds__(kCpdfPoints) k_cpdf1d_grid(const CpdfTerm1D* __restrict__ cached, int n,
+                                                             const double* __restrict__ xs, int npts,
+                                                             double* __restrict__ part) {
+    __shared__ CpdfTerm1D sh[kCpdfChunk];
+    const int chunk = blockIdx.y;
+    const int t0 = chunk * kCpdfChunk;
+    const int nt = min(kCpdfChunk, n - t0);
+    for (int j = threadIdx.x; j < nt; j += blockDim.x) sh[j] = cached[t0 + j];
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    const double x = xs[k];
+    double acc = 0.0;
+    for (int j = 0; j < nt; j++) {
+        const CpdfTerm1D c = sh[j];
+        double xm = x - c.s;
+        xm *= xm;
+        acc += (c.a * x + c.bb) / (c.w2 + xm);
+    }
+    part[(int64_t)chunk * npts + k] = acc;
+}
+
+// y_k = (sum over chunks of part) / norm, the chunks added in order.
+__global__ void k_cpdf1d_finish(const double* __restrict__ part, int nchunks, int npts, double inv_norm,
+                                double* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s += part[(int64_t)c * npts + k];
+    y[k] = s * inv_norm;
+}
+
+
 // b <- b - dx for every term (cauchy_estimator.hpp:1371-1380, sub_vecs).
 template <int D>
 __global__ void k_sub_b(TermPool P, int n, VecArg dx) {

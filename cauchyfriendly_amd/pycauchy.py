@@ -169,6 +169,20 @@ class _Bindings:
         wp = (ct.c_int * len(handles))(*[1 if w else 0 for w in with_propagate])
         return self._batch("pycauchy_batch_single_step_nonlin", 15, handles, msmts, controls, (wp,), num_threads)
 
+    def pycauchy_get_marginal_1D_pointwise_cpdf(self, handle, marg_idx, gridx_low, gridx_high, gridx_resolution,
+                                                log_dir=None):
+        """(cpdf_points as x, f(x) pairs, num_gridx) of the marginal density of one state."""
+        out, size, num = _dp(), ct.c_int(0), ct.c_int(0)
+        ld = None if log_dir is None else str(log_dir).encode()
+        self._f("pycauchy_get_marginal_1D_pointwise_cpdf")(ct.c_void_p(handle), int(marg_idx), float(gridx_low),
+                                                           float(gridx_high), float(gridx_resolution), ld,
+                                                           ct.byref(out), ct.byref(size), ct.byref(num))
+        return _take_d(out, size.value), int(num.value)
+
+    def pycauchy_get_1D_pointwise_cpdf(self, handle, gridx_low, gridx_high, gridx_resolution, log_dir=None):
+        return self.pycauchy_get_marginal_1D_pointwise_cpdf(handle, 0, gridx_low, gridx_high, gridx_resolution,
+                                                            log_dir)
+
     def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
         a = [_arr(v) for v in (A0, p0, b0, xbar)]
         self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],

@@ -123,6 +123,21 @@ int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts,
                                      double** controls, const int* size_controls, const int* with_propagate,
                                      void** outs, int* out_sizes, int num_threads);
 
+/* Replaces pycauchy_get_marginal_1D_pointwise_cpdf, scripts/swig/cauchy/pycauchy.hpp:890-932:
+ * the marginal density of state marg_idx1 on the grid gridx_low + i * gridx_resolution
+ * (clamped to gridx_high), as (x, f(x)) pairs, malloc'd; out_num_gridx points.  No points
+ * after a window's last update (its g-tables are not built) or on error.  The first call on a
+ * handle draws d values from glibc rand(), as the reference's density object does.  log_dir
+ * is accepted and not written. */
+void pycauchy_get_marginal_1D_pointwise_cpdf(void* handle, int marg_idx1, double gridx_low, double gridx_high,
+                                             double gridx_resolution, char* log_dir, double** out_cpdf_data,
+                                             int* size_out_cpdf_data, int* out_num_gridx);
+
+/* Replaces pycauchy_get_1D_pointwise_cpdf, scripts/swig/cauchy/pycauchy.hpp:934-945 (state 0). */
+void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,
+                                    char* log_dir, double** out_cpdf_data, int* size_out_cpdf_data,
+                                    int* out_num_gridx);
+
 /* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar);

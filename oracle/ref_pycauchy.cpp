@@ -55,6 +55,16 @@ void refpy_single_step_nonlin(void* h, double* msmts, int size_msmts, double* co
                                 s13, o14, s14, o15, s15);
 }
 
+void refpy_get_marginal_1D_pointwise_cpdf(void* h, int marg_idx1, double lo, double hi, double res, char* log_dir,
+                                          double** out, int* size, int* num) {
+    pycauchy_get_marginal_1D_pointwise_cpdf(h, marg_idx1, lo, hi, res, log_dir, out, size, num);
+}
+
+void refpy_get_1D_pointwise_cpdf(void* h, double lo, double hi, double res, char* log_dir, double** out, int* size,
+                                 int* num) {
+    pycauchy_get_1D_pointwise_cpdf(h, lo, hi, res, log_dir, out, size, num);
+}
+
 void refpy_single_step_reset(void* h, double* A0, int sA0, double* p0, int sp0, double* b0, int sb0, double* xbar,
                              int sxbar) {
     pycauchy_single_step_reset(h, A0, sA0, p0, sp0, b0, sb0, xbar, sxbar);

@@ -17,8 +17,8 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP engine)")
 
 
-# fixtures of other record types (window bank, window re-initialisation, extended mode)
-NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_")
+# fixtures of other record types (window bank, window re-initialisation, extended mode, densities)
+NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_")
 
 
 def golden_names():

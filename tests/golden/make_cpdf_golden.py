@@ -1,0 +1,89 @@
+"""Writes the 1D marginal density fixtures under tests/golden/ from the reference.
+
+Run in the container that holds the reference tree (after `make -C oracle ref`):
+
+    python tests/golden/make_cpdf_golden.py
+
+The reference evaluates the marginal density through
+pycauchy_get_marginal_1D_pointwise_cpdf (scripts/swig/cauchy/pycauchy.hpp:890-945,
+include/cpdf_ndim.hpp:1233-1368 and 2017-2130), reached from its Python class
+(PyCauchyEstimator.get_marginal_1D_pointwise_cpdf, cauchy_estimator.py:1003-1043).  As
+for the window-bank fixtures, the SWIG module is the reference's own C++ functions
+bound through ctypes (oracle/_ref/libpycauchy_ref.so) and the reference's Python file
+is loaded from the reference tree in this container only.  What is written is data.
+"""
+import ctypes as ct
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from make_window_golden import REF_LIB, lst, reference_module  # noqa: E402
+from cauchyfriendly_amd import workloads as W  # noqa: E402
+
+
+def run(ref, sysobj, mode, steps, grids):
+    """Steps a reference estimator, then evaluates the marginal densities in `grids`
+    (marg_idx, low, high, resolution; low/high relative to the estimate)."""
+    ct.CDLL("libc.so.6").srand(ct.c_uint(sysobj.srand_seed))
+    est = ref.PyCauchyEstimator(mode, sysobj.num_steps, False)
+    Gamma = sysobj.Gamma if sysobj.pncc > 1 else sysobj.Gamma.reshape(-1)
+    H = sysobj.H if sysobj.p > 1 else sysobj.H.reshape(-1)
+    beta = np.asarray(sysobj.beta, dtype=np.float64).reshape(-1)
+    gamma = np.asarray(sysobj.gamma, dtype=np.float64).reshape(-1)
+    n = sysobj.n
+    if mode == "lti":
+        est.initialize_lti(sysobj.A0.copy(), sysobj.p0.copy(), sysobj.b0.copy(), sysobj.Phi(0).copy(), None,
+                           Gamma.copy(), beta.copy(), H.copy(), gamma.copy())
+    else:
+        def dyn(c_duc):
+            duc = c_duc.contents
+            k = int(duc.step)
+            Phi = sysobj.Phi(k).reshape(-1)
+            for i in range(n * n):
+                duc.Phi[i] = float(Phi[i])
+            duc.step = k + 1
+        est.initialize_ltv(sysobj.A0.copy(), sysobj.p0.copy(), sysobj.b0.copy(), sysobj.Phi(0).copy(), None,
+                           Gamma.copy(), beta.copy(), H.copy(), gamma.copy(), dyn)
+    for k in range(steps):
+        est.step(np.asarray(sysobj.zs[k], dtype=np.float64).reshape(-1), None)
+    xhat = est.moment_info["x"][-1]
+    out = []
+    for marg, lo, hi, res in grids:
+        lo_, hi_ = float(xhat[marg] + lo), float(xhat[marg] + hi)
+        if n == 1:
+            X, Y = est.get_1D_pointwise_cpdf(lo_, hi_, res)
+        else:
+            X, Y = est.get_marginal_1D_pointwise_cpdf(marg, lo_, hi_, res)
+        out.append({"marg": marg, "low": lo_, "high": hi_, "res": res, "X": lst(X), "Y": lst(Y)})
+    rec = {"system": sysobj.to_dict(), "mode": mode, "steps": steps, "xhat": lst(xhat),
+           "Nt": int(est.get_num_CF_terms()), "grids": out}
+    est.shutdown()
+    return rec
+
+
+def main():
+    if not os.path.exists(REF_LIB):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    ref = reference_module()
+    cases = [
+        ("cpdf_lti3", W.lti_system(d=3, steps=8, seed_offset=0, name="lti3_w8"), "lti", 5,
+         [(0, -2.0, 2.0, 0.05), (1, -2.0, 2.0, 0.05), (2, -1.0, 1.5, 0.1), (0, -30.0, 30.0, 0.5)]),
+        ("cpdf_ltv5_p2", W.ltv_system(d=5, steps=4, seed_offset=1, name="ltv5_p2_w4"), "ltv", 3,
+         [(0, -1.0, 1.0, 0.04), (4, -1.0, 1.0, 0.04)]),
+        ("cpdf_scalar1", W.lti_system(d=1, steps=6, seed_offset=5, name="scalar1"), "lti", 4,
+         [(0, -1.5, 1.5, 0.03)]),
+    ]
+    for fname, sysobj, mode, steps, grids in cases:
+        rec = run(ref, sysobj, mode, steps, grids)
+        with open(os.path.join(HERE, fname + ".json"), "w") as fh:
+            json.dump(rec, fh, separators=(",", ":"))
+        print(f"{fname}: {rec['Nt']} terms, grids of {[len(g['X']) for g in rec['grids']]} points")
+
+
+if __name__ == "__main__":
+    main()
