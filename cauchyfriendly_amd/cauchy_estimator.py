@@ -327,6 +327,30 @@ class PyCauchyEstimator:
         pts = pts.reshape(num, 2)
         return pts[:, 0], pts[:, 1]
 
+    def get_marginal_2D_pointwise_cpdf(self, marg_idx1, marg_idx2, gridx_low, gridx_high, gridx_resolution,
+                                       gridy_low, gridy_high, gridy_resolution, log_dir=None, reset_cache=True):
+        """Joint marginal density of states marg_idx1 < marg_idx2 on a grid
+        (cauchy_estimator.py:954-989); returns meshgrid-shaped (X, Y, Z), or (None, None, None)
+        before the first step."""
+        if not self.is_initialized or self.step_count < 1:
+            return None, None, None
+        if not (gridx_high > gridx_low and gridy_high > gridy_low and gridx_resolution > 0 and gridy_resolution > 0
+                and 0 <= int(marg_idx1) < int(marg_idx2) < self.n):
+            raise ValueError("the grid needs high > low, resolutions > 0 and state indices i1 < i2 < n")
+        pts, nx, ny = self._pc.pycauchy_get_marginal_2D_pointwise_cpdf(
+            self.py_handle, int(marg_idx1), int(marg_idx2), float(gridx_low), float(gridx_high),
+            float(gridx_resolution), float(gridy_low), float(gridy_high), float(gridy_resolution), log_dir,
+            bool(reset_cache))
+        pts = pts.reshape(ny, nx, 3)
+        return pts[:, :, 0], pts[:, :, 1], pts[:, :, 2]
+
+    def get_2D_pointwise_cpdf(self, gridx_low, gridx_high, gridx_resolution, gridy_low, gridy_high, gridy_resolution,
+                              log_dir=None):
+        if self.n != 2:
+            raise ValueError("get_2D_pointwise_cpdf is for a 2-state system; use get_marginal_2D_pointwise_cpdf")
+        return self.get_marginal_2D_pointwise_cpdf(0, 1, gridx_low, gridx_high, gridx_resolution, gridy_low,
+                                                   gridy_high, gridy_resolution, log_dir)
+
     def get_1D_pointwise_cpdf(self, gridx_low, gridx_high, gridx_resolution, log_dir=None):
         if self.n != 1:
             raise ValueError("get_1D_pointwise_cpdf is for a 1-state system; use get_marginal_1D_pointwise_cpdf")

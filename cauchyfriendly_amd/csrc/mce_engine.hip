@@ -1343,9 +1343,17 @@ struct Engine {
     // Fills xy with (x, f(x)) pairs; returns 1 (no points) after a window's last update,
     // whose g-tables are not built.
     int cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy);
+    // 2D marginal density of states (i1, i2) on the grid of CauchyCPDFGridDispatcher2D
+    // (cpdf_ndim.hpp:1816-1900): x runs fastest; fills xyz with (x, y, f) triples.
+    int cpdf_2d(int i1, int i2, double lox, double hix, double resx, double loy, double hiy, double resy,
+                std::vector<double>& xyz, int& nx, int& ny);
+    void draw_bar_nu();
     std::vector<double> bar_nu;  // drawn at the first density evaluation (cpdf_ndim.hpp:384-387)
     DBuf<double> cpdf_direct, cpdf_x, cpdf_y, cpdf_part, cpdf_sum;
     DBuf<CpdfTerm1D> cpdf_cache;
+    DBuf<int> c2_mc, c2_err;
+    DBuf<double> c2_b2, c2_sn, c2_cs, c2_g1r, c2_g2r, c2_px, c2_py;
+    DBuf<cplx> c2_g;
 
     // Extended-mode finish of a measurement update (cauchy_estimator.hpp:1358-1394): the
     // terms' b-vectors move by -Re(mean) (skipped on a window's last update), the mean
@@ -1406,10 +1414,7 @@ int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<doub
                         "were not built\n");
         return 1;
     }
-    if (bar_nu.empty()) {
-        bar_nu.resize(d);
-        for (int i = 0; i < d; i++) bar_nu[i] = 2 * (((double)rand() + 1.0) / ((double)RAND_MAX + 2.0));
-    }
+    draw_bar_nu();
     const int npts = (int)((hi - lo + res - 1e-15) / res) + 1;
     std::vector<double> xs(npts), ys(npts, 0.0);
     for (int i = 0; i < npts; i++) xs[i] = std::min(lo + i * res, hi);
@@ -1449,6 +1454,104 @@ int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<doub
     for (int i = 0; i < npts; i++) {
         xy[2 * i] = xs[i];
         xy[2 * i + 1] = ys[i];
+    }
+    return 0;
+}
+
+void Engine::draw_bar_nu() {
+    if (!bar_nu.empty()) return;
+    bar_nu.resize(d);
+    for (int i = 0; i < d; i++) bar_nu[i] = 2 * (((double)rand() + 1.0) / ((double)RAND_MAX + 2.0));
+}
+
+int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double loy, double hiy, double resy,
+                    std::vector<double>& xyz, int& nx, int& ny) {
+    nx = ny = 0;
+    if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
+    if (!(0 <= i1 && i1 < i2 && i2 < d)) throw std::runtime_error("marginal indices need 0 <= i1 < i2 < n");
+    if (!(hix > lox) || !(resx > 0) || !(hiy > loy) || !(resy > 0))
+        throw std::runtime_error("the grid needs high > low and resolution > 0 on both axes");
+    if (master_step == n_mu) {
+        fprintf(stderr, "[cauchy_mi355x] the density cannot be evaluated after the window's last update: its g-tables "
+                        "were not built\n");
+        return 1;
+    }
+    draw_bar_nu();
+    nx = (int)((hix - lox + resx - 1e-15) / resx) + 1;
+    ny = (int)((hiy - loy + resy - 1e-15) / resy) + 1;
+    const int npts = nx * ny;
+    std::vector<double> xs(npts), ys(npts), zs(npts, 0.0);
+    for (int i = 0; i < ny; i++) {
+        const double y = std::min(loy + i * resy, hiy);
+        for (int j = 0; j < nx; j++) {
+            xs[i * nx + j] = std::min(lox + j * resx, hix);
+            ys[i * nx + j] = y;
+        }
+    }
+    const double norm = fz.re;
+    const double R = kInvTwoPi;
+    if (Nt > 0) {
+        GraveyardScope gs(&graveyard);
+        const int nchunks = ceil_div(Nt, kCpdfChunk);
+        const size_t cells = (size_t)Nt * MM;
+        c2_mc.reserve(Nt);
+        c2_b2.reserve(2 * (size_t)Nt);
+        c2_sn.reserve(cells + Nt);
+        c2_cs.reserve(cells + Nt);
+        c2_g1r.reserve(cells);
+        c2_g2r.reserve(cells);
+        c2_g.reserve(cells);
+        c2_err.reserve(1);
+        c2_px.reserve(npts);
+        c2_py.reserve(npts);
+        cpdf_direct.reserve(Nt);
+        cpdf_y.reserve(npts);
+        cpdf_part.reserve((size_t)nchunks * npts);
+        cpdf_sum.reserve(1);
+        HIPCHK(hipMemcpyAsync(c2_px.ptr, xs.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c2_py.ptr, ys.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(c2_err.ptr, 0, sizeof(int), st));
+        Cpdf2D C;
+        C.MM = MM;
+        C.mc = c2_mc.ptr;
+        C.b2 = c2_b2.ptr;
+        C.sn = c2_sn.ptr;
+        C.cs = c2_cs.ptr;
+        C.g1r = c2_g1r.ptr;
+        C.g2r = c2_g2r.ptr;
+        C.gc = c2_g.ptr;
+        VecArg bn;
+        for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
+        TermPool IN = in.view();
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_cpdf2d_terms<DD>, dim3(ceil_div(Nt, 128)), dim3(128), 0, st, IN, Nt,
+                                             i1, i2, bn, gk[gcur].ptr, gv[gcur].ptr, xs[0], ys[0], C,
+                                             cpdf_direct.ptr, c2_err.ptr));
+        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
+        if (npts > 1) {
+            hipLaunchKernelGGL(k_cpdf2d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
+                               C, Nt, c2_px.ptr + 1, c2_py.ptr + 1, npts - 1, cpdf_part.ptr, c2_err.ptr);
+            hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
+                               nchunks, npts - 1, 1.0, cpdf_y.ptr + 1);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(zs.data() + 1, cpdf_y.ptr + 1, (npts - 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+        }
+        double direct_sum = 0.0;
+        int singular = 0;
+        HIPCHK(hipMemcpyAsync(&direct_sum, cpdf_sum.ptr, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&singular, c2_err.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (singular)
+            throw std::runtime_error("a grid point meets a term's singularity (gamma1 = gamma2 = 0); move the grid");
+        zs[0] = direct_sum;
+        // fx = 2 * sum * (1 / 2pi)^2 / Re fz, cpdf_ndim.hpp:1451
+        for (int k = 0; k < npts; k++) zs[k] = 2 * zs[k] * R * R / norm;
+    }
+    xyz.resize(3 * (size_t)npts);
+    for (int k = 0; k < npts; k++) {
+        xyz[3 * k] = xs[k];
+        xyz[3 * k + 1] = ys[k];
+        xyz[3 * k + 2] = zs[k];
     }
     return 0;
 }
@@ -1891,6 +1994,41 @@ void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx
                                     int* out_num_gridx) {
     pycauchy_get_marginal_1D_pointwise_cpdf(handle, 0, gridx_low, gridx_high, gridx_resolution, log_dir, out_cpdf_data,
                                             size_out_cpdf_data, out_num_gridx);
+}
+
+void pycauchy_get_marginal_2D_pointwise_cpdf(void* handle, int marg_idx1, int marg_idx2, double gridx_low,
+                                             double gridx_high, double gridx_resolution, double gridy_low,
+                                             double gridy_high, double gridy_resolution, char* log_dir,
+                                             bool reset_cache, double** out_cpdf_data, int* size_out_cpdf_data,
+                                             int* out_num_gridx, int* out_num_gridy) {
+    (void)log_dir;
+    (void)reset_cache;
+    std::vector<double> xyz;
+    int rc = 1, nx = 0, ny = 0;
+    try {
+        rc = ((Handle*)handle)->eng->cpdf_2d(marg_idx1, marg_idx2, gridx_low, gridx_high, gridx_resolution, gridy_low,
+                                            gridy_high, gridy_resolution, xyz, nx, ny);
+    } catch (const std::exception& ex) {
+        set_err("pycauchy_get_marginal_2D_pointwise_cpdf", ex);
+    }
+    if (rc != 0) {
+        xyz.clear();
+        nx = ny = 0;
+    }
+    *out_num_gridx = nx;
+    *out_num_gridy = ny;
+    *size_out_cpdf_data = (int)xyz.size();
+    *out_cpdf_data = mallocd<double>(xyz.size());
+    if (!xyz.empty()) memcpy(*out_cpdf_data, xyz.data(), xyz.size() * sizeof(double));
+}
+
+void pycauchy_get_2D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,
+                                    double gridy_low, double gridy_high, double gridy_resolution, char* log_dir,
+                                    bool reset_cache, double** out_cpdf_data, int* size_out_cpdf_data,
+                                    int* out_num_gridx, int* out_num_gridy) {
+    pycauchy_get_marginal_2D_pointwise_cpdf(handle, 0, 1, gridx_low, gridx_high, gridx_resolution, gridy_low,
+                                            gridy_high, gridy_resolution, log_dir, reset_cache, out_cpdf_data,
+                                            size_out_cpdf_data, out_num_gridx, out_num_gridy);
 }
 
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,

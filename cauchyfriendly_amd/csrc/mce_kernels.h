@@ -2539,13 +2539,13 @@ __global__ void __launch_bounds__(kCpdfPoints) k_cpdf1d_grid(const CpdfTerm1D* _
 }
 
 // y_k = (sum over chunks of part) / norm, the chunks added in order.
-__global__ void k_cpdf1d_finish(const double* __restrict__ part, int nchunks, int npts, double inv_norm,
+__global__ void k_cpdf1d_finish(const double* __restrict__ part, int nchunks, int npts, double norm,
                                 double* __restrict__ y) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npts) return;
     double s = 0.0;
     for (int c = 0; c < nchunks; c++) s += part[(int64_t)c * npts + k];
-    y[k] = s * inv_norm;
+    y[k] = s / norm;
 }
 
 
@@ -2558,5 +2558,230 @@ __global__ void k_sub_b(TermPool P, int n, VecArg dx) {
 #pragma unroll
     for (int r = 0; r < D; r++) b[r] -= dx.v[r];
 }
+
+// ------------------------------------------------------------ 2D marginal CPDF
+// Per-term cell data of the 2D marginal density (cpdf_ndim.hpp:1400-1700): the
+// arrangement cut to states (i1, i2), zero rows removed and +/- equal rows merged
+// (marg2d_remove_zeros_and_coalign), the cell walls' angles sorted over a full turn,
+// and for each of the m' cells of a half turn its g numerator and the real parts of
+// gamma1, gamma2.  Arrays have a stride of MM cells per term (MM + 1 angles).
+struct Cpdf2D {
+    int MM;
+    int* mc;           // cells per term
+    double* b2;        // b of the two states
+    double* sn;        // sin of the wall angles
+    double* cs;        // cos of the wall angles
+    double* g1r;       // Re gamma1 per cell
+    double* g2r;       // Re gamma2 per cell
+    cplx* gc;          // g numerator per cell
+};
+
+constexpr double kIntegralGammaEps = 1e-8;  // INTEGRAL_GAMMA_EPS, cauchy_constants.hpp:76
+constexpr int kZeroHpMarker = 32;           // ZERO_HP_MARKER_VALUE, cpdf_ndim.hpp:371
+
+// Integral of one cell at (x1, x2) (marg2d_eval_term_for_cpdf, cpdf_ndim.hpp:1603-1636);
+// `singular` is set where the reference stops with a singularity error.
+__device__ inline double cpdf2d_cell(double s1, double c1, double s2, double c2, double g1re, double g2re,
+                                     double g1im, double g2im, cplx g, bool check_g1, bool& singular) {
+    cplx ga1 = cmake(g1re, g1im), ga2 = cmake(g2re, g2im);
+    if (check_g1 && fabs(g1re) < kIntegralGammaEps) {
+        if (fabs(g2re) < kIntegralGammaEps && fabs(g2im) < kIntegralGammaEps) {
+            singular = true;
+            return 0.0;
+        }
+        const cplx lo = cdiv(csub(cscale(ga1, s1), cscale(ga2, c1)), cadd(cscale(ga1, c1), cscale(ga2, s1)));
+        const cplx hi = cdiv(csub(cscale(ga1, s2), cscale(ga2, c2)), cadd(cscale(ga1, c2), cscale(ga2, s2)));
+        const cplx sq = cadd(cmul(ga1, ga1), cmul(ga2, ga2));
+        return cmul(csub(hi, lo), cdiv(g, sq)).re;
+    }
+    ga2 = cmul(ga2, ga1);
+    ga1 = cmul(ga1, ga1);
+    const cplx lo = cdiv(cmake(s1, 0.0), cadd(cscale(ga1, c1), cscale(ga2, s1)));
+    const cplx hi = cdiv(cmake(s2, 0.0), cadd(cscale(ga1, c2), cscale(ga2, s2)));
+    return cmul(csub(hi, lo), g).re;
+}
+
+template <int D>
+__global__ void k_cpdf2d_terms(TermPool P, int n, int i1, int i2, VecArg bar_nu, const uint32_t* __restrict__ gkeys,
+                               const cplx* __restrict__ gvals, double x0, double y0, Cpdf2D C,
+                               double* __restrict__ direct, int* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = P.m[i];
+    const int MM = C.MM;
+    const double* A = P.A + (int64_t)i * P.MM * D;
+    const double* p = P.p + (int64_t)i * P.MM;
+    double wa[2 * kMaxM], wp[kMaxM], wa2[2 * kMaxM], wp2[kMaxM], th[2 * kMaxM];
+    int cmap[kMaxM], csm[kMaxM], fidx[kMaxM];
+    bool F[kMaxM];
+    // columns of the two states; zero rows marked, the others L1-normalised
+    for (int r = 0; r < m; r++) {
+        double a0 = A[r * D + i1], a1 = A[r * D + i2];
+        const double f0 = fabs(a0), f1 = fabs(a1);
+        wp[r] = p[r];
+        if (f0 < kMuEps && f1 < kMuEps) {
+            cmap[r] = csm[r] = fidx[r] = kZeroHpMarker;
+            F[r] = false;
+        } else {
+            const double sa = f0 + f1;
+            wa[2 * r] = a0 / sa;
+            wa[2 * r + 1] = a1 / sa;
+            wp[r] *= sa;
+            F[r] = true;
+        }
+    }
+    // +/- equal rows merged into the first of them
+    int u = 0;
+    for (int r = 0; r < m; r++) {
+        if (!F[r]) continue;
+        cmap[r] = u;
+        csm[r] = 1;
+        fidx[r] = r;
+        wp2[u] = wp[r];
+        for (int j = r + 1; j < m; j++) {
+            if (!F[j]) continue;
+            const bool pos = (fabs(wa[2 * r] - wa[2 * j]) < kCoalignEps) && (fabs(wa[2 * r + 1] - wa[2 * j + 1]) < kCoalignEps);
+            if (pos) {
+                cmap[j] = u;
+                csm[j] = 1;
+                F[j] = false;
+                fidx[j] = r;
+                wp2[u] += wp[j];
+                continue;
+            }
+            const bool neg = (fabs(wa[2 * r] + wa[2 * j]) < kCoalignEps) && (fabs(wa[2 * r + 1] + wa[2 * j + 1]) < kCoalignEps);
+            if (neg) {
+                cmap[j] = u;
+                csm[j] = -1;
+                F[j] = false;
+                fidx[j] = r;
+                wp2[u] += wp[j];
+            }
+        }
+        u++;
+    }
+    {
+        int k = 0;
+        for (int r = 0; r < m; r++)
+            if (fidx[r] == r) {
+                wa2[2 * k] = wa[2 * r];
+                wa2[2 * k + 1] = wa[2 * r + 1];
+                k++;
+            }
+    }
+    // cell wall angles over a full turn, ascending (marg2d_get_cell_wall_angles)
+    for (int r = 0; r < u; r++) {
+        const double a0 = wa2[2 * r], a1 = wa2[2 * r + 1];
+        double px, py;
+        if (fabs(a0) < fabs(a1)) {
+            px = 1.0;
+            py = -a0 / a1;
+        } else {
+            px = -a1 / a0;
+            py = 1.0;
+        }
+        double t1 = atan2(py, px);
+        if (t1 < 0) t1 += kPi;
+        th[2 * r] = t1;
+        th[2 * r + 1] = t1 + kPi;
+    }
+    for (int a = 1; a < 2 * u; a++) {
+        const double v = th[a];
+        int b = a - 1;
+        while (b >= 0 && th[b] > v) {
+            th[b + 1] = th[b];
+            b--;
+        }
+        th[b + 1] = v;
+    }
+    const double bx = P.b[(int64_t)i * D + i1], by = P.b[(int64_t)i * D + i2];
+    const double g1im = bx - x0, g2im = by - y0;
+    const bool check_g1 = fabs(g1im) < kIntegralGammaEps;
+    const int top = 1 << (m - 1), rev = (1 << m) - 1;
+    const int64_t go = P.gp_off[i];
+    const int gn = P.gp_n[i];
+    const int64_t co = (int64_t)i * MM;
+    double total = 0.0;
+    bool singular = false;
+    double s1 = 0.0, c1 = 0.0;
+    if (u > 0) {
+        s1 = sin(th[0]);
+        c1 = cos(th[0]);
+    }
+    for (int c = 0; c < u; c++) {
+        // sign vector of the cell from its middle direction (marg2d_get_SVs)
+        const double t = (th[c + 1] + th[c]) / 2.0;
+        const double ptx = cos(t), pty = sin(t);
+        uint32_t neg = 0;  // bit j: row j of the cut arrangement is negative in this cell
+        for (int j = 0; j < u; j++)
+            if (!(wa2[2 * j] * ptx + wa2[2 * j + 1] * pty > 0)) neg |= 1u << j;
+        int enc = 0;
+        for (int j = 0; j < m; j++) {
+            if (cmap[j] == kZeroHpMarker) {
+                double ip = 0.0;
+                for (int k = 0; k < D; k++) ip += A[j * D + k] * bar_nu.v[k];
+                if (ip < 0) enc |= 1 << j;
+            } else {
+                const int sv = ((neg >> cmap[j]) & 1u) ? -1 : 1;
+                if (sv * csm[j] < 0) enc |= 1 << j;
+            }
+        }
+        const cplx g = lookup_num(enc, top, rev, gkeys + go, gvals + go, gn);
+        double g1re = 0.0, g2re = 0.0;
+        for (int j = 0; j < u; j++) {
+            const double sv = ((neg >> j) & 1u) ? -1.0 : 1.0;
+            g1re -= (wa2[2 * j] * wp2[j]) * sv;
+            g2re -= (wa2[2 * j + 1] * wp2[j]) * sv;
+        }
+        const double s2 = sin(th[c + 1]), c2 = cos(th[c + 1]);
+        total += cpdf2d_cell(s1, c1, s2, c2, g1re, g2re, g1im, g2im, g, check_g1, singular);
+        C.sn[co + i + c] = s1;
+        C.cs[co + i + c] = c1;
+        C.g1r[co + c] = g1re;
+        C.g2r[co + c] = g2re;
+        C.gc[co + c] = g;
+        s1 = s2;
+        c1 = c2;
+    }
+    C.sn[co + i + u] = s1;
+    C.cs[co + i + u] = c1;
+    C.mc[i] = u;
+    C.b2[2 * i] = bx;
+    C.b2[2 * i + 1] = by;
+    direct[i] = total;
+    if (singular) atomicOr(err, 1);
+}
+
+// Partial sums over one chunk of terms for a block of grid points (the cached cell
+// integrals, marg2d_cached_eval_term_for_cpdf, cpdf_ndim.hpp:1654-1735).
+__global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, const double* __restrict__ px,
+                                                             const double* __restrict__ py, int npts,
+                                                             double* __restrict__ part, int* __restrict__ err) {
+    const int chunk = blockIdx.y;
+    const int t0 = chunk * kCpdfChunk;
+    const int t1 = min(n, t0 + kCpdfChunk);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    const double x = px[k], y = py[k];
+    const int MM = C.MM;
+    double acc = 0.0;
+    bool singular = false;
+    for (int t = t0; t < t1; t++) {
+        const int u = C.mc[t];
+        const double g1im = C.b2[2 * t] - x, g2im = C.b2[2 * t + 1] - y;
+        const bool check_g1 = fabs(g1im) < kIntegralGammaEps;
+        const int64_t co = (int64_t)t * MM;
+        const double* sn = C.sn + co + t;
+        const double* cs = C.cs + co + t;
+        double term = 0.0;
+        for (int c = 0; c < u; c++)
+            term += cpdf2d_cell(sn[c], cs[c], sn[c + 1], cs[c + 1], C.g1r[co + c], C.g2r[co + c], g1im, g2im,
+                                C.gc[co + c], check_g1, singular);
+        acc += term;
+    }
+    part[(int64_t)chunk * npts + k] = acc;
+    if (singular) atomicOr(err, 1);
+}
+
 
 }  // namespace mce

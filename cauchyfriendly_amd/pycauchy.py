@@ -183,6 +183,24 @@ class _Bindings:
         return self.pycauchy_get_marginal_1D_pointwise_cpdf(handle, 0, gridx_low, gridx_high, gridx_resolution,
                                                             log_dir)
 
+    def pycauchy_get_marginal_2D_pointwise_cpdf(self, handle, marg_idx1, marg_idx2, gridx_low, gridx_high,
+                                                gridx_resolution, gridy_low, gridy_high, gridy_resolution,
+                                                log_dir=None, reset_cache=True):
+        """(cpdf_points as x, y, f triples, num_gridx, num_gridy) of the joint marginal density."""
+        out, size, nx, ny = _dp(), ct.c_int(0), ct.c_int(0), ct.c_int(0)
+        ld = None if log_dir is None else str(log_dir).encode()
+        self._f("pycauchy_get_marginal_2D_pointwise_cpdf")(
+            ct.c_void_p(handle), int(marg_idx1), int(marg_idx2), float(gridx_low), float(gridx_high),
+            float(gridx_resolution), float(gridy_low), float(gridy_high), float(gridy_resolution), ld,
+            bool(reset_cache), ct.byref(out), ct.byref(size), ct.byref(nx), ct.byref(ny))
+        return _take_d(out, size.value), int(nx.value), int(ny.value)
+
+    def pycauchy_get_2D_pointwise_cpdf(self, handle, gridx_low, gridx_high, gridx_resolution, gridy_low, gridy_high,
+                                       gridy_resolution, log_dir=None, reset_cache=True):
+        return self.pycauchy_get_marginal_2D_pointwise_cpdf(handle, 0, 1, gridx_low, gridx_high, gridx_resolution,
+                                                            gridy_low, gridy_high, gridy_resolution, log_dir,
+                                                            reset_cache)
+
     def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
         a = [_arr(v) for v in (A0, p0, b0, xbar)]
         self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],

@@ -138,6 +138,24 @@ void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx
                                     char* log_dir, double** out_cpdf_data, int* size_out_cpdf_data,
                                     int* out_num_gridx);
 
+/* Replaces pycauchy_get_marginal_2D_pointwise_cpdf, scripts/swig/cauchy/pycauchy.hpp:822-872:
+ * the joint marginal density of states marg_idx1 < marg_idx2 on the grid (x from gridx_low
+ * by gridx_resolution, y likewise, each clamped to its high end; x runs fastest), as (x, y, f)
+ * triples, malloc'd; out_num_gridx x out_num_gridy points.  No points after a window's last
+ * update, or on error (mce_last_error()).  Every call builds its per-term cell data afresh, so
+ * reset_cache changes nothing; log_dir is accepted and not written. */
+void pycauchy_get_marginal_2D_pointwise_cpdf(void* handle, int marg_idx1, int marg_idx2, double gridx_low,
+                                             double gridx_high, double gridx_resolution, double gridy_low,
+                                             double gridy_high, double gridy_resolution, char* log_dir,
+                                             bool reset_cache, double** out_cpdf_data, int* size_out_cpdf_data,
+                                             int* out_num_gridx, int* out_num_gridy);
+
+/* Replaces pycauchy_get_2D_pointwise_cpdf, scripts/swig/cauchy/pycauchy.hpp:874-888 (states 0, 1). */
+void pycauchy_get_2D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,
+                                    double gridy_low, double gridy_high, double gridy_resolution, char* log_dir,
+                                    bool reset_cache, double** out_cpdf_data, int* size_out_cpdf_data,
+                                    int* out_num_gridx, int* out_num_gridy);
+
 /* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar);

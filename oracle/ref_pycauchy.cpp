@@ -65,6 +65,13 @@ void refpy_get_1D_pointwise_cpdf(void* h, double lo, double hi, double res, char
     pycauchy_get_1D_pointwise_cpdf(h, lo, hi, res, log_dir, out, size, num);
 }
 
+void refpy_get_marginal_2D_pointwise_cpdf(void* h, int i1, int i2, double lox, double hix, double resx, double loy,
+                                          double hiy, double resy, char* log_dir, bool reset_cache, double** out,
+                                          int* size, int* nx, int* ny) {
+    pycauchy_get_marginal_2D_pointwise_cpdf(h, i1, i2, lox, hix, resx, loy, hiy, resy, log_dir, reset_cache, out, size,
+                                            nx, ny);
+}
+
 void refpy_single_step_reset(void* h, double* A0, int sA0, double* p0, int sp0, double* b0, int sb0, double* xbar,
                              int sxbar) {
     pycauchy_single_step_reset(h, A0, sA0, p0, sp0, b0, sb0, xbar, sxbar);

@@ -1,13 +1,13 @@
-"""Writes the 1D marginal density fixtures under tests/golden/ from the reference.
+"""Writes the marginal density fixtures (1D and 2D) under tests/golden/ from the reference.
 
 Run in the container that holds the reference tree (after `make -C oracle ref`):
 
     python tests/golden/make_cpdf_golden.py
 
-The reference evaluates the marginal density through
-pycauchy_get_marginal_1D_pointwise_cpdf (scripts/swig/cauchy/pycauchy.hpp:890-945,
-include/cpdf_ndim.hpp:1233-1368 and 2017-2130), reached from its Python class
-(PyCauchyEstimator.get_marginal_1D_pointwise_cpdf, cauchy_estimator.py:1003-1043).  As
+The reference evaluates the marginal densities through
+pycauchy_get_marginal_1D_pointwise_cpdf and pycauchy_get_marginal_2D_pointwise_cpdf
+(scripts/swig/cauchy/pycauchy.hpp:822-945, include/cpdf_ndim.hpp:1233-1735 and
+1774-2130), reached from its Python class (PyCauchyEstimator, cauchy_estimator.py:954-1043).  As
 for the window-bank fixtures, the SWIG module is the reference's own C++ functions
 bound through ctypes (oracle/_ref/libpycauchy_ref.so) and the reference's Python file
 is loaded from the reference tree in this container only.  What is written is data.
@@ -26,9 +26,10 @@ from make_window_golden import REF_LIB, lst, reference_module  # noqa: E402
 from cauchyfriendly_amd import workloads as W  # noqa: E402
 
 
-def run(ref, sysobj, mode, steps, grids):
+def run(ref, sysobj, mode, steps, grids, grids2d=()):
     """Steps a reference estimator, then evaluates the marginal densities in `grids`
-    (marg_idx, low, high, resolution; low/high relative to the estimate)."""
+    (marg_idx, low, high, resolution; low/high relative to the estimate) and the joint
+    marginal densities in `grids2d` (i1, i2, then low, high, resolution per axis)."""
     ct.CDLL("libc.so.6").srand(ct.c_uint(sysobj.srand_seed))
     est = ref.PyCauchyEstimator(mode, sysobj.num_steps, False)
     Gamma = sysobj.Gamma if sysobj.pncc > 1 else sysobj.Gamma.reshape(-1)
@@ -60,8 +61,17 @@ def run(ref, sysobj, mode, steps, grids):
         else:
             X, Y = est.get_marginal_1D_pointwise_cpdf(marg, lo_, hi_, res)
         out.append({"marg": marg, "low": lo_, "high": hi_, "res": res, "X": lst(X), "Y": lst(Y)})
+    out2 = []
+    for i1, i2, lox, hix, rx, loy, hiy, ry in grids2d:
+        lims = (float(xhat[i1] + lox), float(xhat[i1] + hix), float(xhat[i2] + loy), float(xhat[i2] + hiy))
+        if n == 2:
+            X, Y, Z = est.get_2D_pointwise_cpdf(lims[0], lims[1], rx, lims[2], lims[3], ry)
+        else:
+            X, Y, Z = est.get_marginal_2D_pointwise_cpdf(i1, i2, lims[0], lims[1], rx, lims[2], lims[3], ry)
+        out2.append({"i1": i1, "i2": i2, "xlow": lims[0], "xhigh": lims[1], "xres": rx, "ylow": lims[2],
+                     "yhigh": lims[3], "yres": ry, "shape": list(Z.shape), "X": lst(X), "Y": lst(Y), "Z": lst(Z)})
     rec = {"system": sysobj.to_dict(), "mode": mode, "steps": steps, "xhat": lst(xhat),
-           "Nt": int(est.get_num_CF_terms()), "grids": out}
+           "Nt": int(est.get_num_CF_terms()), "grids": out, "grids2d": out2}
     est.shutdown()
     return rec
 
@@ -72,17 +82,22 @@ def main():
     ref = reference_module()
     cases = [
         ("cpdf_lti3", W.lti_system(d=3, steps=8, seed_offset=0, name="lti3_w8"), "lti", 5,
-         [(0, -2.0, 2.0, 0.05), (1, -2.0, 2.0, 0.05), (2, -1.0, 1.5, 0.1), (0, -30.0, 30.0, 0.5)]),
+         [(0, -2.0, 2.0, 0.05), (1, -2.0, 2.0, 0.05), (2, -1.0, 1.5, 0.1), (0, -30.0, 30.0, 0.5)],
+         [(0, 1, -1.5, 1.5, 0.15, -1.5, 1.5, 0.15), (1, 2, -1.0, 1.2, 0.2, -0.8, 1.0, 0.15)]),
         ("cpdf_ltv5_p2", W.ltv_system(d=5, steps=4, seed_offset=1, name="ltv5_p2_w4"), "ltv", 3,
-         [(0, -1.0, 1.0, 0.04), (4, -1.0, 1.0, 0.04)]),
+         [(0, -1.0, 1.0, 0.04), (4, -1.0, 1.0, 0.04)],
+         [(0, 4, -1.0, 1.0, 0.2, -1.0, 1.0, 0.25)]),
         ("cpdf_scalar1", W.lti_system(d=1, steps=6, seed_offset=5, name="scalar1"), "lti", 4,
-         [(0, -1.5, 1.5, 0.03)]),
+         [(0, -1.5, 1.5, 0.03)], []),
+        ("cpdf_lti2", W.lti_system(d=2, steps=7, seed_offset=6, name="lti2_w7"), "lti", 5,
+         [(1, -1.0, 1.0, 0.05)], [(0, 1, -1.2, 1.2, 0.1, -1.2, 1.2, 0.1)]),
     ]
-    for fname, sysobj, mode, steps, grids in cases:
-        rec = run(ref, sysobj, mode, steps, grids)
+    for fname, sysobj, mode, steps, grids, grids2d in cases:
+        rec = run(ref, sysobj, mode, steps, grids, grids2d)
         with open(os.path.join(HERE, fname + ".json"), "w") as fh:
             json.dump(rec, fh, separators=(",", ":"))
-        print(f"{fname}: {rec['Nt']} terms, grids of {[len(g['X']) for g in rec['grids']]} points")
+        print(f"{fname}: {rec['Nt']} terms, grids of {[len(g['X']) for g in rec['grids']]} points, "
+              f"2D grids {[g['shape'] for g in rec['grids2d']]}")
 
 
 if __name__ == "__main__":
