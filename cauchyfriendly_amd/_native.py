@@ -78,6 +78,9 @@ def declare_pycauchy(L, sym=lambda name: name):
     getattr(L, sym("pycauchy_get_marginal_2D_pointwise_cpdf")).restype = None
     getattr(L, sym("pycauchy_get_marginal_2D_pointwise_cpdf")).argtypes = [ct.c_void_p, ct.c_int, ct.c_int] + \
         [ct.c_double] * 6 + [ct.c_char_p, ct.c_bool, _dpp, _ip, _ip, _ip]
+    getattr(L, sym("pycauchy_single_step_eval_2d_rsys_cpdf")).restype = None
+    getattr(L, sym("pycauchy_single_step_eval_2d_rsys_cpdf")).argtypes = [_dp, ct.c_int, ct.c_void_p, ct.c_void_p] + \
+        [ct.c_double] * 7 + [_dpp, _ip] * 7 + [_ip, _ip]
     getattr(L, sym("pycauchy_single_step_reset")).argtypes = [ct.c_void_p] + [_dp, ct.c_int] * 4
     getattr(L, sym("pycauchy_single_step_set_master_step")).argtypes = [ct.c_void_p, ct.c_int]
     getattr(L, sym("pycauchy_single_step_set_window_number")).argtypes = [ct.c_void_p, ct.c_int]

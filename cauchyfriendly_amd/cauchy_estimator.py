@@ -442,6 +442,19 @@ class PyCauchyEstimator:
             pass
 
 
+def eval_2d_relative_system_cpdf(Trel, secondary, primary, rsys_approx_eps, xlow, xhigh, delta_x, ylow, yhigh,
+                                 delta_y):
+    """Moments and density of the relative system secondary - primary transformed by Trel
+    (2 x n), as pycauchy_single_step_eval_2d_rsys_cpdf returns them; the grid limits are
+    relative to the relative mean (the reference calls this from
+    scripts/swig/leo/gmat_mce.py:950-975)."""
+    if secondary._pc is not primary._pc:
+        raise ValueError("the two estimators must share one backend")
+    return primary._pc.pycauchy_single_step_eval_2d_rsys_cpdf(
+        np.asarray(Trel, dtype=np.float64).reshape(-1), secondary.py_handle, primary.py_handle,
+        float(rsys_approx_eps), float(xlow), float(xhigh), float(delta_x), float(ylow), float(yhigh), float(delta_y))
+
+
 def step_estimators(estimators, msmts, controls=None, full_info=False, num_threads=0):
     """Steps every estimator once, as est.step(msmts[i], controls[i]) would, but concurrently:
     the estimators run on their own HIP streams from up to num_threads host threads (0: one

@@ -1351,6 +1351,10 @@ struct Engine {
     std::vector<double> bar_nu;  // drawn at the first density evaluation (cpdf_ndim.hpp:384-387)
     DBuf<double> cpdf_direct, cpdf_x, cpdf_y, cpdf_part, cpdf_sum;
     DBuf<CpdfTerm1D> cpdf_cache;
+    // relative-system scratch (held by the primary estimator of a pair)
+    DBuf<double> rs_mom, rs_sums;
+    DBuf<uint8_t> rs_keep;
+    DBuf<int> rs_idx, rs_cnt;
     DBuf<int> c2_mc, c2_err;
     DBuf<double> c2_b2, c2_sn, c2_cs, c2_g1r, c2_g2r, c2_px, c2_py;
     DBuf<cplx> c2_g;
@@ -1529,7 +1533,8 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
         hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
         if (npts > 1) {
             hipLaunchKernelGGL(k_cpdf2d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
-                               C, Nt, c2_px.ptr + 1, c2_py.ptr + 1, npts - 1, cpdf_part.ptr, c2_err.ptr);
+                               C, Nt, (const int*)nullptr, c2_px.ptr + 1, c2_py.ptr + 1, npts - 1, cpdf_part.ptr,
+                               c2_err.ptr);
             hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
                                nchunks, npts - 1, 1.0, cpdf_y.ptr + 1);
             HIPCHK(hipGetLastError());
@@ -1554,6 +1559,144 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
         xyz[3 * k + 2] = zs[k];
     }
     return 0;
+}
+
+// The relative system sec - prim, transformed by Trel (2 x d): its moments and its density
+// on a grid (pycauchy_single_step_eval_2d_rsys_cpdf, pycauchy.hpp:673-750, with
+// get_marg2d_relative_and_transformed_cpdf and grid_eval_marg2d_relative_and_transformed_cpdf,
+// cauchy_prediction.hpp:653-972, 1102-1200).  The grid limits are relative to the relative
+// mean.  Points are ordered with y fastest (point i * ny + j is (x_i, y_j)), as the reference's
+// grid is.  Returns the number of relative terms kept by the approximation.
+static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, double xlow, double xhigh, double dx,
+                       double ylow, double yhigh, double dy, cplx& nf, cplx mean[2], cplx cov[4],
+                       std::vector<double>& xyz, int& nx, int& ny) {
+    const int d = P.d;
+    if (S.d != d) throw std::runtime_error("the two estimators have different state dimensions");
+    if (P.master_step == 0 || S.master_step == 0) throw std::runtime_error("both estimators need a measurement update");
+    if (P.master_step == P.n_mu || S.master_step == S.n_mu)
+        throw std::runtime_error("an estimator is past its window's last update, whose g-tables are not built");
+    if (!(xhigh > xlow) || !(yhigh > ylow) || !(dx > 0) || !(dy > 0))
+        throw std::runtime_error("the grid needs high > low and resolution > 0 on both axes");
+    // the direction used on rows that vanish under Trel (cauchy_prediction.hpp:703-713)
+    RsysArgs R;
+    memset(&R, 0, sizeof(R));
+    for (int i = 0; i < 2 * d; i++) R.T[i] = Trel[i];
+    for (int i = 0; i < d; i++) R.nu[i] = 2 * (((double)rand() + 1.0) / ((double)RAND_MAX + 2.0)) - 1;
+    for (int k = 0; k < 2; k++) {
+        double acc = 0.0;
+        for (int j = 0; j < d; j++) acc += Trel[k * d + j] * R.nu[j];
+        R.nu2[k] = acc;
+    }
+    R.norm_fact = kInvTwoPi * kInvTwoPi / P.fz.re / S.fz.re;
+    R.eps = eps;
+    const int np = P.Nt, ns = S.Nt;
+    const int64_t npairs = (int64_t)np * ns;
+    const int CM = P.MM + S.MM;
+    if (CM > kMaxM) throw std::runtime_error("the relative arrangement would exceed 31 hyperplanes");
+    const double bytes = (double)npairs * (CM * 40.0 + 8.0 * (CM + 1) * 2 + 14 * 8 + 32);
+    if (bytes > 64.0 * (1ull << 30))
+        throw std::runtime_error("the relative system's pairs need more than 64 GiB; use estimators with fewer terms");
+    GraveyardScope gs(&P.graveyard);
+    hipStream_t st = P.st;
+    HIPCHK(hipStreamSynchronize(S.st));
+    P.c2_mc.reserve(npairs);
+    P.c2_b2.reserve(2 * npairs);
+    P.c2_sn.reserve(npairs * (CM + 1));
+    P.c2_cs.reserve(npairs * (CM + 1));
+    P.c2_g1r.reserve(npairs * CM);
+    P.c2_g2r.reserve(npairs * CM);
+    P.c2_g.reserve(npairs * CM);
+    P.rs_mom.reserve(npairs * 14);
+    P.rs_keep.reserve(npairs);
+    P.rs_idx.reserve(npairs);
+    P.rs_cnt.reserve(1);
+    P.rs_sums.reserve(14);
+    P.c2_err.reserve(1);
+    Cpdf2D C;
+    C.MM = CM;
+    C.mc = P.c2_mc.ptr;
+    C.b2 = P.c2_b2.ptr;
+    C.sn = P.c2_sn.ptr;
+    C.cs = P.c2_cs.ptr;
+    C.g1r = P.c2_g1r.ptr;
+    C.g2r = P.c2_g2r.ptr;
+    C.gc = P.c2_g.ptr;
+    TermPool PI = P.in.view(), SI = S.in.view();
+    if (npairs > 0) {
+        const unsigned blocks = (unsigned)((npairs + 127) / 128);
+        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_rsys_pairs<DD>, dim3(blocks), dim3(128), 0, st, PI, np,
+                                             P.gk[P.gcur].ptr, P.gv[P.gcur].ptr, SI, ns, S.gk[S.gcur].ptr,
+                                             S.gv[S.gcur].ptr, R, C, P.rs_mom.ptr, P.rs_keep.ptr));
+        hipLaunchKernelGGL(k_colsum_fixed, dim3(14), dim3(1024), 0, st, P.rs_mom.ptr, npairs, 14, P.rs_sums.ptr);
+        P.cub([&](void* t, size_t& b) {
+            return hipcub::DeviceSelect::Flagged(t, b, hipcub::CountingInputIterator<int>(0), P.rs_keep.ptr,
+                                                 P.rs_idx.ptr, P.rs_cnt.ptr, (int)npairs, st);
+        });
+        HIPCHK(hipGetLastError());
+    }
+    double sums[14] = {0};
+    int nkeep = 0;
+    if (npairs > 0) {
+        HIPCHK(hipMemcpyAsync(sums, P.rs_sums.ptr, sizeof(sums), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&nkeep, P.rs_cnt.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    // finalize_rel_sys_moments (cauchy_prediction.hpp:288-304)
+    nf = cmake(sums[0], sums[1]);
+    const cplx ifz = cmake(0.0, nf.re);
+    for (int i = 0; i < 2; i++) mean[i] = cdiv(cmake(sums[2 + 2 * i], sums[3 + 2 * i]), ifz);
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const cplx c = cmake(-sums[6 + 2 * (2 * i + j)], -sums[7 + 2 * (2 * i + j)]);
+            cov[2 * i + j] = csub(cdiv(c, nf), cmul(mean[i], mean[j]));
+        }
+    // the grid about the relative mean (grid_eval_marg2d_relative_and_transformed_cpdf)
+    const double x0 = xlow + mean[0].re, x1 = xhigh + mean[0].re;
+    const double y0 = ylow + mean[1].re, y1 = yhigh + mean[1].re;
+    nx = (int)ceil((x1 - x0) / dx + 1);
+    ny = (int)ceil((y1 - y0) / dy + 1);
+    std::vector<double> xs(nx), ys(ny);
+    for (int i = 0; i < nx; i++) xs[i] = x0 + dx * i;
+    xs[nx - 1] = x1;
+    for (int j = 0; j < ny; j++) ys[j] = y0 + dy * j;
+    ys[ny - 1] = y1;
+    const int npts = nx * ny;
+    std::vector<double> px(npts), py(npts), z(npts, 0.0);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            px[i * ny + j] = xs[i];
+            py[i * ny + j] = ys[j];
+        }
+    if (nkeep > 0) {
+        const int nchunks = ceil_div(nkeep, kCpdfChunk);
+        P.c2_px.reserve(npts);
+        P.c2_py.reserve(npts);
+        P.cpdf_part.reserve((size_t)nchunks * npts);
+        P.cpdf_y.reserve(npts);
+        HIPCHK(hipMemcpyAsync(P.c2_px.ptr, px.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(P.c2_py.ptr, py.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(P.c2_err.ptr, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_cpdf2d_grid, dim3(ceil_div(npts, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st, C,
+                           nkeep, P.rs_idx.ptr, P.c2_px.ptr, P.c2_py.ptr, npts, P.cpdf_part.ptr, P.c2_err.ptr);
+        hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts, 256)), dim3(256), 0, st, P.cpdf_part.ptr, nchunks, npts,
+                           1.0, P.cpdf_y.ptr);
+        HIPCHK(hipGetLastError());
+        int singular = 0;
+        HIPCHK(hipMemcpyAsync(z.data(), P.cpdf_y.ptr, npts * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&singular, P.c2_err.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (singular)
+            throw std::runtime_error("a grid point meets a relative term's singularity (gamma1 = gamma2 = 0)");
+        // 2 * sum * (1 / 2pi)^2 / (Re fz_p Re fz_s), cauchy_prediction.hpp:1072
+        for (int k = 0; k < npts; k++) z[k] = 2 * z[k] * kInvTwoPi * kInvTwoPi / (P.fz.re * S.fz.re);
+    }
+    xyz.resize(3 * (size_t)npts);
+    for (int k = 0; k < npts; k++) {
+        xyz[3 * k] = px[k];
+        xyz[3 * k + 1] = py[k];
+        xyz[3 * k + 2] = z[k];
+    }
+    return nkeep;
 }
 
 void Engine::finalize_extended_moments(double* x_bar) {
@@ -2029,6 +2172,56 @@ void pycauchy_get_2D_pointwise_cpdf(void* handle, double gridx_low, double gridx
     pycauchy_get_marginal_2D_pointwise_cpdf(handle, 0, 1, gridx_low, gridx_high, gridx_resolution, gridy_low,
                                             gridy_high, gridy_resolution, log_dir, reset_cache, out_cpdf_data,
                                             size_out_cpdf_data, out_num_gridx, out_num_gridy);
+}
+
+void pycauchy_single_step_eval_2d_rsys_cpdf(double* Trel, int size_Trel, void* _s_pcdh, void* _p_pcdh,
+                                            double RSYS_APPROX_EPS, double xlow, double xhigh, double delta_x,
+                                            double ylow, double yhigh, double delta_y, double** out_rsys_fz,
+                                            int* size_out_rsys_fz, double** out_rsys_xhat, int* size_out_rsys_xhat,
+                                            double** out_rsys_Phat, int* size_out_rsys_Phat,
+                                            double** out_rsys_cerr_fz, int* size_out_rsys_cerr_fz,
+                                            double** out_rsys_cerr_xhat, int* size_out_rsys_cerr_xhat,
+                                            double** out_rsys_cerr_Phat, int* size_out_rsys_cerr_Phat,
+                                            double** out_cpdf_data, int* size_out_cpdf_data, int* out_num_gridx,
+                                            int* out_num_gridy) {
+    *size_out_rsys_fz = *size_out_rsys_cerr_fz = 1;
+    *size_out_rsys_xhat = *size_out_rsys_cerr_xhat = 2;
+    *size_out_rsys_Phat = *size_out_rsys_cerr_Phat = 4;
+    *out_rsys_fz = mallocd<double>(1);
+    *out_rsys_cerr_fz = mallocd<double>(1);
+    *out_rsys_xhat = mallocd<double>(2);
+    *out_rsys_cerr_xhat = mallocd<double>(2);
+    *out_rsys_Phat = mallocd<double>(4);
+    *out_rsys_cerr_Phat = mallocd<double>(4);
+    cplx nf = cmake(0, 0), mean[2] = {cmake(0, 0), cmake(0, 0)}, cov[4];
+    for (auto& c : cov) c = cmake(0, 0);
+    std::vector<double> xyz;
+    int nx = 0, ny = 0;
+    try {
+        Engine* s = ((Handle*)_s_pcdh)->eng;
+        Engine* p = ((Handle*)_p_pcdh)->eng;
+        if (size_Trel != 2 * p->d) throw std::runtime_error("Trel must be 2 x n");
+        rsys_2d(*p, *s, Trel, RSYS_APPROX_EPS, xlow, xhigh, delta_x, ylow, yhigh, delta_y, nf, mean, cov, xyz, nx, ny);
+    } catch (const std::exception& ex) {
+        set_err("pycauchy_single_step_eval_2d_rsys_cpdf", ex);
+        xyz.clear();
+        nx = ny = 0;
+    }
+    (*out_rsys_fz)[0] = nf.re;
+    (*out_rsys_cerr_fz)[0] = nf.im;
+    for (int i = 0; i < 2; i++) {
+        (*out_rsys_xhat)[i] = mean[i].re;
+        (*out_rsys_cerr_xhat)[i] = mean[i].im;
+    }
+    for (int i = 0; i < 4; i++) {
+        (*out_rsys_Phat)[i] = cov[i].re;
+        (*out_rsys_cerr_Phat)[i] = cov[i].im;
+    }
+    *size_out_cpdf_data = (int)xyz.size();
+    *out_cpdf_data = mallocd<double>(xyz.size());
+    if (!xyz.empty()) memcpy(*out_cpdf_data, xyz.data(), xyz.size() * sizeof(double));
+    *out_num_gridx = nx;
+    *out_num_gridy = ny;
 }
 
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,

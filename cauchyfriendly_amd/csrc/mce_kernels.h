@@ -2754,7 +2754,9 @@ __global__ void k_cpdf2d_terms(TermPool P, int n, int i1, int i2, VecArg bar_nu,
 
 // Partial sums over one chunk of terms for a block of grid points (the cached cell
 // integrals, marg2d_cached_eval_term_for_cpdf, cpdf_ndim.hpp:1654-1735).
-__global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, const double* __restrict__ px,
+// `idx`, when given, lists the terms to add (the kept ones of a relative system).
+__global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, const int* __restrict__ idx,
+                                                             const double* __restrict__ px,
                                                              const double* __restrict__ py, int npts,
                                                              double* __restrict__ part, int* __restrict__ err) {
     const int chunk = blockIdx.y;
@@ -2766,7 +2768,8 @@ __global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, co
     const int MM = C.MM;
     double acc = 0.0;
     bool singular = false;
-    for (int t = t0; t < t1; t++) {
+    for (int tt = t0; tt < t1; tt++) {
+        const int t = idx ? idx[tt] : tt;
         const int u = C.mc[t];
         const double g1im = C.b2[2 * t] - x, g2im = C.b2[2 * t + 1] - y;
         const bool check_g1 = fabs(g1im) < kIntegralGammaEps;
@@ -2783,5 +2786,262 @@ __global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, co
     if (singular) atomicOr(err, 1);
 }
 
+
+// ------------------------------------------------------------ relative system (two estimators)
+// The relative system rel = secondary - primary, transformed to 2D by Trel (2 x d),
+// as get_marg2d_relative_and_transformed_cpdf builds it (cauchy_prediction.hpp:497-650):
+// one relative term per (primary term, secondary term) pair, A = [-A_p; A_s] Trel^T,
+// p = [p_p; p_s], b = Trel (b_s - b_p).  Per pair: its moment contribution
+// (eval_rel_sys_moments_for_term, 207-286), its cell data for the density
+// (cache_rel_marg2d_term_for_cpdf, 306-443) and whether the term is approximated out
+// (approx_out_rel_marg2d_term_for_cpdf, 445-471).
+struct RsysArgs {
+    double T[2 * kMaxD];   // Trel, row-major 2 x d
+    double nu2[2];         // Trel bar_nu
+    double nu[kMaxD];      // bar_nu (full state)
+    double norm_fact;      // (1 / 2pi)^2 / (Re fz_p Re fz_s)
+    double eps;            // term approximation threshold
+};
+
+__device__ inline double sgn_d(double x) { return (double)((x > 0) - (x < 0)); }
+
+template <int D>
+__global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pkeys, const cplx* __restrict__ pvals,
+                             TermPool S, int ns, const uint32_t* __restrict__ skeys, const cplx* __restrict__ svals,
+                             RsysArgs R, Cpdf2D C, double* __restrict__ mom, uint8_t* __restrict__ keep) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)np * ns) return;
+    const int tp = (int)(q / ns), ts = (int)(q % ns);
+    const int mp = P.m[tp], ms = S.m[ts];
+    const int m = mp + ms;
+    const double* Ap = P.A + (int64_t)tp * P.MM * D;
+    const double* As = S.A + (int64_t)ts * S.MM * D;
+    double a2[2 * kMaxM], p2[kMaxM], wa2[2 * kMaxM], wp2[kMaxM], th[2 * kMaxM];
+    int cmap[kMaxM], csm[kMaxM], fidx[kMaxM];
+    bool F[kMaxM];
+    // A_2d = A_rel Trel^T (matmatmul with the second operand transposed), p_2d = p_rel
+    for (int i = 0; i < m; i++) {
+        const bool pr = i < mp;
+        const double* ar = pr ? Ap + i * D : As + (i - mp) * D;
+        const double sg = pr ? -1.0 : 1.0;
+        for (int k = 0; k < 2; k++) {
+            double acc = 0.0;
+            for (int j = 0; j < D; j++) acc += (ar[j] * sg) * R.T[k * D + j];
+            a2[2 * i + k] = acc;
+        }
+        p2[i] = pr ? P.p[(int64_t)tp * P.MM + i] : S.p[(int64_t)ts * S.MM + (i - mp)];
+    }
+    double b2[2];
+    {
+        double br[kMaxD];
+        for (int j = 0; j < D; j++) br[j] = S.b[(int64_t)ts * D + j] - P.b[(int64_t)tp * D + j];
+        for (int k = 0; k < 2; k++) {
+            double acc = 0.0;
+            for (int j = 0; j < D; j++) acc += R.T[k * D + j] * br[j];
+            b2[k] = acc;
+        }
+    }
+    // zero rows marked, the others L1-normalised in place; +/- equal rows merged
+    for (int r = 0; r < m; r++) {
+        const double f0 = fabs(a2[2 * r]), f1 = fabs(a2[2 * r + 1]);
+        if (f0 < kMuEps && f1 < kMuEps) {
+            cmap[r] = csm[r] = fidx[r] = kZeroHpMarker;
+            F[r] = false;
+        } else {
+            const double sa = f0 + f1;
+            a2[2 * r] /= sa;
+            a2[2 * r + 1] /= sa;
+            p2[r] *= sa;
+            F[r] = true;
+        }
+    }
+    int u = 0;
+    for (int r = 0; r < m; r++) {
+        if (!F[r]) continue;
+        cmap[r] = u;
+        csm[r] = 1;
+        fidx[r] = r;
+        wp2[u] = p2[r];
+        for (int j = r + 1; j < m; j++) {
+            if (!F[j]) continue;
+            const bool pos = (fabs(a2[2 * r] - a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] - a2[2 * j + 1]) < kCoalignEps);
+            if (pos) {
+                cmap[j] = u;
+                csm[j] = 1;
+                F[j] = false;
+                fidx[j] = r;
+                wp2[u] += p2[j];
+                continue;
+            }
+            const bool neg = (fabs(a2[2 * r] + a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] + a2[2 * j + 1]) < kCoalignEps);
+            if (neg) {
+                cmap[j] = u;
+                csm[j] = -1;
+                F[j] = false;
+                fidx[j] = r;
+                wp2[u] += p2[j];
+            }
+        }
+        u++;
+    }
+    {
+        int k = 0;
+        for (int r = 0; r < m; r++)
+            if (fidx[r] == r) {
+                wa2[2 * k] = a2[2 * r];
+                wa2[2 * k + 1] = a2[2 * r + 1];
+                k++;
+            }
+    }
+    const uint32_t* pk = pkeys + P.gp_off[tp];
+    const cplx* pv = pvals + P.gp_off[tp];
+    const int pn = P.gp_n[tp];
+    const uint32_t* sk = skeys + S.gp_off[ts];
+    const cplx* sv_ = svals + S.gp_off[ts];
+    const int sn_ = S.gp_n[ts];
+    // moment contribution: the cell of bar_nu (Trel bar_nu in 2D, bar_nu on zero rows)
+    {
+        double ty0 = 0.0, ty1 = 0.0;
+        int encp = 0, encs = 0;
+        for (int i = 0; i < m; i++) {
+            double ds;
+            if (cmap[i] != kZeroHpMarker) {
+                ds = a2[2 * i] * R.nu2[0] + a2[2 * i + 1] * R.nu2[1];
+            } else {
+                const bool pr = i < mp;
+                const double* ar = pr ? Ap + i * D : As + (i - mp) * D;
+                const double sg = pr ? -1.0 : 1.0;
+                ds = 0.0;
+                for (int j = 0; j < D; j++) ds += (ar[j] * sg) * R.nu[j];
+            }
+            const double sgi = sgn_d(ds);
+            ty0 += a2[2 * i] * (p2[i] * sgi);
+            ty1 += a2[2 * i + 1] * (p2[i] * sgi);
+            if (sgi < 0) {
+                if (i < mp) encp |= 1 << i;
+                else encs |= 1 << (i - mp);
+            }
+        }
+        const cplx gp = lookup_num(encp, 1 << (mp - 1), (1 << mp) - 1, pk, pv, pn);
+        const cplx gs = lookup_num(encs, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
+        const cplx g = cmul(gp, gs);
+        const cplx y0 = cmake(-ty0, b2[0]), y1 = cmake(-ty1, b2[1]);
+        double* o = mom + q * 14;
+        const cplx gy0 = cmul(g, y0), gy1 = cmul(g, y1);
+        const cplx c00 = cmul(gy0, y0), c01 = cmul(gy0, y1), c10 = cmul(gy1, y0), c11 = cmul(gy1, y1);
+        o[0] = g.re;
+        o[1] = g.im;
+        o[2] = gy0.re;
+        o[3] = gy0.im;
+        o[4] = gy1.re;
+        o[5] = gy1.im;
+        o[6] = c00.re;
+        o[7] = c00.im;
+        o[8] = c01.re;
+        o[9] = c01.im;
+        o[10] = c10.re;
+        o[11] = c10.im;
+        o[12] = c11.re;
+        o[13] = c11.im;
+    }
+    // cell data over a half turn
+    for (int r = 0; r < u; r++) {
+        const double x0 = wa2[2 * r], x1 = wa2[2 * r + 1];
+        double px, py;
+        if (fabs(x0) < fabs(x1)) {
+            px = 1.0;
+            py = -x0 / x1;
+        } else {
+            px = -x1 / x0;
+            py = 1.0;
+        }
+        double t1 = atan2(py, px);
+        if (t1 < 0) t1 += kPi;
+        th[2 * r] = t1;
+        th[2 * r + 1] = t1 + kPi;
+    }
+    for (int a = 1; a < 2 * u; a++) {
+        const double v = th[a];
+        int b = a - 1;
+        while (b >= 0 && th[b] > v) {
+            th[b + 1] = th[b];
+            b--;
+        }
+        th[b + 1] = v;
+    }
+    const int MM = C.MM;
+    const int64_t co = q * MM;
+    const int mpmask = (1 << mp) - 1;
+    double total = 0.0;
+    bool keep_term = false;
+    for (int c = 0; c < u; c++) {
+        const double t = (th[c + 1] + th[c]) / 2.0;
+        const double ptx = cos(t), pty = sin(t);
+        uint32_t negb = 0;
+        for (int j = 0; j < u; j++)
+            if (!(wa2[2 * j] * ptx + wa2[2 * j + 1] * pty > 0)) negb |= 1u << j;
+        int enc = 0;
+        for (int j = 0; j < m; j++) {
+            if (cmap[j] == kZeroHpMarker) {
+                const bool pr = j < mp;
+                const double* ar = pr ? Ap + j * D : As + (j - mp) * D;
+                const double sg = pr ? -1.0 : 1.0;
+                double ip = 0.0;
+                for (int k = 0; k < D; k++) ip += (ar[k] * sg) * R.nu[k];
+                if (ip < 0) enc |= 1 << j;
+            } else {
+                const int svv = ((negb >> cmap[j]) & 1u) ? -1 : 1;
+                if (svv * csm[j] < 0) enc |= 1 << j;
+            }
+        }
+        const cplx gp = lookup_num(enc & mpmask, 1 << (mp - 1), mpmask, pk, pv, pn);
+        const cplx gs = lookup_num(enc >> mp, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
+        const cplx g = cmul(gp, gs);
+        double g1re = 0.0, g2re = 0.0;
+        for (int j = 0; j < u; j++) {
+            const double svd = ((negb >> j) & 1u) ? -1.0 : 1.0;
+            g1re -= (wa2[2 * j] * wp2[j]) * svd;
+            g2re -= (wa2[2 * j + 1] * wp2[j]) * svd;
+        }
+        const double s1 = sin(th[c]), c1 = cos(th[c]);
+        const double s2 = sin(th[c + 1]), c2 = cos(th[c + 1]);
+        C.sn[co + q + c] = s1;
+        C.cs[co + q + c] = c1;
+        C.g1r[co + c] = g1re;
+        C.g2r[co + c] = g2re;
+        C.gc[co + c] = g;
+        // the approximation's bound on the cell's contribution
+        const double den1 = g1re * (g1re * c1 + g2re * s1);
+        const double den2 = g1re * (g1re * c2 + g2re * s2);
+        if (fabs(den1) < 1e-14 || fabs(den2) < 1e-14) keep_term = true;
+        const double gr = hypot(g.re, g.im);
+        total += fabs((gr * s2) / den2) + fabs((gr * s1) / den1);
+    }
+    if (u > 0) {
+        C.sn[co + q + u] = sin(th[u]);
+        C.cs[co + q + u] = cos(th[u]);
+    }
+    C.mc[q] = u;
+    C.b2[2 * q] = b2[0];
+    C.b2[2 * q + 1] = b2[1];
+    total *= (2 * R.norm_fact);
+    keep[q] = (keep_term || !(fabs(total) < R.eps)) ? 1 : 0;
+}
+
+// Column sums of a row-major n x cols array, one block per column, in a fixed order.
+__global__ void k_colsum_fixed(const double* __restrict__ v, int64_t n, int cols, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int64_t i = t; i < n; i += blockDim.x) s += v[i * cols + c];
+    sh[t] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (t < h) sh[t] += sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[c] = sh[0];
+}
 
 }  // namespace mce

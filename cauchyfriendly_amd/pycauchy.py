@@ -201,6 +201,22 @@ class _Bindings:
                                                             gridy_low, gridy_high, gridy_resolution, log_dir,
                                                             reset_cache)
 
+    def pycauchy_single_step_eval_2d_rsys_cpdf(self, Trel, s_handle, p_handle, rsys_approx_eps, xlow, xhigh,
+                                               delta_x, ylow, yhigh, delta_y):
+        """(rsys_fz, rsys_xhat, rsys_Phat, rsys_cerr_fz, rsys_cerr_xhat, rsys_cerr_Phat, cpdf_points,
+        num_gridx, num_gridy) of the relative system secondary - primary transformed by Trel."""
+        t = _arr(Trel)
+        outs = [(_dp(), ct.c_int()) for _ in range(7)]
+        nx, ny = ct.c_int(0), ct.c_int(0)
+        flat = []
+        for p, n in outs:
+            flat += [ct.byref(p), ct.byref(n)]
+        self._f("pycauchy_single_step_eval_2d_rsys_cpdf")(
+            t[1], t[2], ct.c_void_p(s_handle), ct.c_void_p(p_handle), float(rsys_approx_eps), float(xlow), float(xhigh),
+            float(delta_x), float(ylow), float(yhigh), float(delta_y), *flat, ct.byref(nx), ct.byref(ny))
+        res = [_take_d(p, n.value) for p, n in outs]
+        return tuple(res) + (int(nx.value), int(ny.value))
+
     def pycauchy_single_step_reset(self, handle, A0, p0, b0, xbar):
         a = [_arr(v) for v in (A0, p0, b0, xbar)]
         self._f("pycauchy_single_step_reset")(ct.c_void_p(handle), a[0][1], a[0][2], a[1][1], a[1][2], a[2][1],

@@ -156,6 +156,28 @@ void pycauchy_get_2D_pointwise_cpdf(void* handle, double gridx_low, double gridx
                                     bool reset_cache, double** out_cpdf_data, int* size_out_cpdf_data,
                                     int* out_num_gridx, int* out_num_gridy);
 
+/* Replaces pycauchy_single_step_eval_2d_rsys_cpdf, scripts/swig/cauchy/pycauchy.hpp:673-750: the
+ * relative system (secondary - primary) of two estimators of one state dimension n,
+ * transformed by Trel (2 x n, row-major).  Every pair of their terms forms one relative
+ * term; terms whose bounded contribution is below RSYS_APPROX_EPS are left out of the
+ * density (cauchy_prediction.hpp:445-471), not out of the moments.  Outputs (malloc'd): the
+ * relative normalisation factor, mean (2) and covariance (4) with their imaginary parts,
+ * and the density on the grid [xlow, xhigh] x [ylow, yhigh] about the relative mean as
+ * (x, y, f) triples with y fastest, out_num_gridx x out_num_gridy points.  Both estimators
+ * must have stepped and not be past their window's last update; on error the density is
+ * empty and mce_last_error() says why.  Draws n values from glibc rand() per call, as the
+ * reference does. */
+void pycauchy_single_step_eval_2d_rsys_cpdf(double* Trel, int size_Trel, void* _s_pcdh, void* _p_pcdh,
+                                            double RSYS_APPROX_EPS, double xlow, double xhigh, double delta_x,
+                                            double ylow, double yhigh, double delta_y, double** out_rsys_fz,
+                                            int* size_out_rsys_fz, double** out_rsys_xhat, int* size_out_rsys_xhat,
+                                            double** out_rsys_Phat, int* size_out_rsys_Phat,
+                                            double** out_rsys_cerr_fz, int* size_out_rsys_cerr_fz,
+                                            double** out_rsys_cerr_xhat, int* size_out_rsys_cerr_xhat,
+                                            double** out_rsys_cerr_Phat, int* size_out_rsys_cerr_Phat,
+                                            double** out_cpdf_data, int* size_out_cpdf_data, int* out_num_gridx,
+                                            int* out_num_gridy);
+
 /* Replaces pycauchy_single_step_reset, scripts/swig/cauchy/pycauchy.hpp:798-819. */
 void pycauchy_single_step_reset(void* handle, double* A0, int size_A0, double* p0, int size_p0, double* b0,
                                 int size_b0, double* xbar, int size_xbar);

@@ -72,6 +72,15 @@ void refpy_get_marginal_2D_pointwise_cpdf(void* h, int i1, int i2, double lox, d
                                             nx, ny);
 }
 
+void refpy_single_step_eval_2d_rsys_cpdf(double* Trel, int sT, void* sh, void* ph, double eps, double xlo,
+                                         double xhi, double dx, double ylo, double yhi, double dy, double** o1,
+                                         int* s1, double** o2, int* s2, double** o3, int* s3, double** o4, int* s4,
+                                         double** o5, int* s5, double** o6, int* s6, double** o7, int* s7, int* nx,
+                                         int* ny) {
+    pycauchy_single_step_eval_2d_rsys_cpdf(Trel, sT, sh, ph, eps, xlo, xhi, dx, ylo, yhi, dy, o1, s1, o2, s2, o3, s3,
+                                           o4, s4, o5, s5, o6, s6, o7, s7, nx, ny);
+}
+
 void refpy_single_step_reset(void* h, double* A0, int sA0, double* p0, int sp0, double* b0, int sb0, double* xbar,
                              int sxbar) {
     pycauchy_single_step_reset(h, A0, sA0, p0, sp0, b0, sb0, xbar, sxbar);

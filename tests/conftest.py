@@ -17,8 +17,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP engine)")
 
 
-# fixtures of other record types (window bank, window re-initialisation, extended mode, densities)
-NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_")
+# fixtures of other record types (window bank, window re-initialisation, extended mode, densities,
+# relative systems)
+NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_", "rsys_")
 
 
 def golden_names():
