@@ -1352,7 +1352,7 @@ struct Engine {
     DBuf<double> cpdf_direct, cpdf_x, cpdf_y, cpdf_part, cpdf_sum;
     DBuf<CpdfTerm1D> cpdf_cache;
     // relative-system scratch (held by the primary estimator of a pair)
-    DBuf<double> rs_mom, rs_sums;
+    DBuf<double> rs_mom, rs_sums, rs_part;
     DBuf<uint8_t> rs_keep;
     DBuf<int> rs_idx, rs_cnt;
     DBuf<int> c2_mc, c2_err;
@@ -1611,6 +1611,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     P.rs_idx.reserve(npairs);
     P.rs_cnt.reserve(1);
     P.rs_sums.reserve(14);
+    P.rs_part.reserve(14 * kColSlices);
     P.c2_err.reserve(1);
     Cpdf2D C;
     C.MM = CM;
@@ -1627,7 +1628,9 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_rsys_pairs<DD>, dim3(blocks), dim3(128), 0, st, PI, np,
                                              P.gk[P.gcur].ptr, P.gv[P.gcur].ptr, SI, ns, S.gk[S.gcur].ptr,
                                              S.gv[S.gcur].ptr, R, C, P.rs_mom.ptr, P.rs_keep.ptr));
-        hipLaunchKernelGGL(k_colsum_fixed, dim3(14), dim3(1024), 0, st, P.rs_mom.ptr, npairs, 14, P.rs_sums.ptr);
+        hipLaunchKernelGGL(k_colsum_part, dim3(kColSlices, 14), dim3(256), 0, st, P.rs_mom.ptr, npairs,
+                           P.rs_part.ptr);
+        hipLaunchKernelGGL(k_colsum_finish, dim3(1), dim3(64), 0, st, P.rs_part.ptr, 14, P.rs_sums.ptr);
         P.cub([&](void* t, size_t& b) {
             return hipcub::DeviceSelect::Flagged(t, b, hipcub::CountingInputIterator<int>(0), P.rs_keep.ptr,
                                                  P.rs_idx.ptr, P.rs_cnt.ptr, (int)npairs, st);

@@ -2594,11 +2594,13 @@ __device__ inline double cpdf2d_cell(double s1, double c1, double s2, double c2,
         const cplx sq = cadd(cmul(ga1, ga1), cmul(ga2, ga2));
         return cmul(csub(hi, lo), cdiv(g, sq)).re;
     }
-    ga2 = cmul(ga2, ga1);
-    ga1 = cmul(ga1, ga1);
-    const cplx lo = cdiv(cmake(s1, 0.0), cadd(cscale(ga1, c1), cscale(ga2, s1)));
-    const cplx hi = cdiv(cmake(s2, 0.0), cadd(cscale(ga1, c2), cscale(ga2, s2)));
-    return cmul(csub(hi, lo), g).re;
+    // The reference's form, sin t2 / (g1 (g1 cos t2 + g2 sin t2)) - (the same at t1), equals
+    // sin(t2 - t1) / (D1 D2) with Dk = g1 cos tk + g2 sin tk exactly: one complex quotient
+    // instead of two, and no cancellation between the two limits.
+    const cplx d1 = cadd(cscale(ga1, c1), cscale(ga2, s1));
+    const cplx d2 = cadd(cscale(ga1, c2), cscale(ga2, s2));
+    const double sd = s2 * c1 - s1 * c2;
+    return cdiv(cscale(g, sd), cmul(d1, d2)).re;
 }
 
 template <int D>
@@ -2926,23 +2928,14 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         const cplx gs = lookup_num(encs, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
         const cplx g = cmul(gp, gs);
         const cplx y0 = cmake(-ty0, b2[0]), y1 = cmake(-ty1, b2[1]);
-        double* o = mom + q * 14;
+        // component c of pair q at mom[c * npairs + q] (column-major: contiguous column sums)
+        const int64_t npq = (int64_t)np * ns;
         const cplx gy0 = cmul(g, y0), gy1 = cmul(g, y1);
         const cplx c00 = cmul(gy0, y0), c01 = cmul(gy0, y1), c10 = cmul(gy1, y0), c11 = cmul(gy1, y1);
-        o[0] = g.re;
-        o[1] = g.im;
-        o[2] = gy0.re;
-        o[3] = gy0.im;
-        o[4] = gy1.re;
-        o[5] = gy1.im;
-        o[6] = c00.re;
-        o[7] = c00.im;
-        o[8] = c01.re;
-        o[9] = c01.im;
-        o[10] = c10.re;
-        o[11] = c10.im;
-        o[12] = c11.re;
-        o[13] = c11.im;
+        const double v[14] = {g.re,   g.im,   gy0.re, gy0.im, gy1.re, gy1.im, c00.re,
+                              c00.im, c01.re, c01.im, c10.re, c10.im, c11.re, c11.im};
+#pragma unroll
+        for (int c = 0; c < 14; c++) mom[c * npq + q] = v[c];
     }
     // cell data over a half turn
     for (int r = 0; r < u; r++) {
@@ -3028,20 +3021,32 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
     keep[q] = (keep_term || !(fabs(total) < R.eps)) ? 1 : 0;
 }
 
-// Column sums of a row-major n x cols array, one block per column, in a fixed order.
-__global__ void k_colsum_fixed(const double* __restrict__ v, int64_t n, int cols, double* __restrict__ out) {
-    __shared__ double sh[1024];
-    const int c = blockIdx.x;
-    const int t = threadIdx.x;
+// Partial sums of the columns of a column-major n x cols array: block (b, c) adds the
+// fixed slice b of column c; k_colsum_finish adds the slices in order.
+constexpr int kColSlices = 256;
+__global__ void k_colsum_part(const double* __restrict__ v, int64_t n, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int b = blockIdx.x, c = blockIdx.y, t = threadIdx.x;
+    const int64_t per = (n + kColSlices - 1) / kColSlices;
+    const int64_t lo = b * per, hi = min(n, lo + per);
+    const double* col = v + c * n;
     double s = 0.0;
-    for (int64_t i = t; i < n; i += blockDim.x) s += v[i * cols + c];
+    for (int64_t i = lo + t; i < hi; i += blockDim.x) s += col[i];
     sh[t] = s;
     __syncthreads();
     for (int h = blockDim.x / 2; h > 0; h >>= 1) {
         if (t < h) sh[t] += sh[t + h];
         __syncthreads();
     }
-    if (t == 0) out[c] = sh[0];
+    if (t == 0) part[c * kColSlices + b] = sh[0];
+}
+
+__global__ void k_colsum_finish(const double* __restrict__ part, int cols, double* __restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= cols) return;
+    double s = 0.0;
+    for (int b = 0; b < kColSlices; b++) s += part[c * kColSlices + b];
+    out[c] = s;
 }
 
 }  // namespace mce
