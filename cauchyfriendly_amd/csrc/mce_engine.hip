@@ -1418,6 +1418,7 @@ int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<doub
                         "were not built\n");
         return 1;
     }
+    if ((hi - lo) / res > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     draw_bar_nu();
     const int npts = (int)((hi - lo + res - 1e-15) / res) + 1;
     std::vector<double> xs(npts), ys(npts, 0.0);
@@ -1480,6 +1481,7 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
                         "were not built\n");
         return 1;
     }
+    if (((hix - lox) / resx + 1) * ((hiy - loy) / resy + 1) > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     draw_bar_nu();
     nx = (int)((hix - lox + resx - 1e-15) / resx) + 1;
     ny = (int)((hiy - loy + resy - 1e-15) / resy) + 1;
@@ -1656,6 +1658,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     // the grid about the relative mean (grid_eval_marg2d_relative_and_transformed_cpdf)
     const double x0 = xlow + mean[0].re, x1 = xhigh + mean[0].re;
     const double y0 = ylow + mean[1].re, y1 = yhigh + mean[1].re;
+    if (((x1 - x0) / dx + 2) * ((y1 - y0) / dy + 2) > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     nx = (int)ceil((x1 - x0) / dx + 1);
     ny = (int)ceil((y1 - y0) / dy + 1);
     std::vector<double> xs(nx), ys(ny);
