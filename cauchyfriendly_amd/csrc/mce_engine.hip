@@ -11,6 +11,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cstdio>
 #include <climits>
 #include <cstdlib>
@@ -224,6 +227,25 @@ struct Profiler {
     }
 };
 
+// ---------------------------------------------------------------- per-thread streams
+// An engine runs on the streams and pinned staging buffer of the host thread that calls
+// it (calls on one engine never overlap, and every call leaves its streams idle), so a
+// process holds two streams per calling thread instead of two per estimator: thousands of
+// live estimators do not multiply the streams that every device-wide synchronisation
+// (hipFree, hipDeviceSynchronize) has to visit.  They live as long as the process.
+constexpr size_t kStageBytes = 4096;
+struct ThreadStreams {
+    hipStream_t st = nullptr, st2 = nullptr;
+    uint8_t* stage = nullptr;
+    void init() {
+        if (st) return;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc((void**)&stage, kStageBytes, hipHostMallocDefault));
+    }
+};
+thread_local ThreadStreams t_streams;
+
 // ---------------------------------------------------------------- engine
 struct Engine {
     int d, cmcc, pncc, p, n_mu, sr, MM;
@@ -289,8 +311,15 @@ struct Engine {
     DBuf<uint8_t> cub_tmp;
     DBuf<unsigned long long> bytes_acc;
     DBuf<int> pstat;
-    static constexpr size_t kStageBytes = 4096;
-    uint8_t* h_stage = nullptr;  // pinned staging for device-to-host scalar reads
+    uint8_t* h_stage = nullptr;  // pinned staging for device-to-host scalar reads (the thread's)
+
+    // Takes the calling thread's streams and staging buffer; every entry point starts here.
+    void bind() {
+        t_streams.init();
+        st = t_streams.st;
+        st2 = t_streams.st2;
+        h_stage = t_streams.stage;
+    }
     DBuf<int64_t> d_scal;        // device scalars (scan totals)
     DBuf<int> defer_k0, big_icnt, big_ioff, big_hcnt, big_hoff, hist_mx;
     DBuf<int64_t> big_ccnt, big_coff;
@@ -303,6 +332,7 @@ struct Engine {
     // Frees the outgrown allocations once this engine's streams are idle.
     void bury() {
         if (graveyard.ptrs.empty()) return;
+        bind();
         HIPCHK(hipStreamSynchronize(st));
         HIPCHK(hipStreamSynchronize(st2));
         for (void* q : graveyard.ptrs) HIPCHK(hipFree(q));
@@ -314,6 +344,7 @@ struct Engine {
            const double* b0_)
         : d(d_), cmcc(cmcc_), pncc(pncc_), p(p_) {
         if (d < 1 || d > kMaxD) throw std::runtime_error("state dimension outside 1..8 is not supported");
+        bind();
         n_mu = p * num_steps;
         const int maxhp = d > 1 ? (num_steps - 1) * pncc + d : d + pncc;
         sr = maxhp + 1;
@@ -335,11 +366,6 @@ struct Engine {
         last_cov = cov;
         fz = cmake(0, 0);
         last_fz = cmake(0, 0);
-        // non-blocking: estimators stepped from several host threads do not serialise on the
-        // null stream; every copy of the engine is ordered on its own streams
-        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
-        HIPCHK(hipHostMalloc((void**)&h_stage, kStageBytes, hipHostMallocDefault));
         d_scal.reserve(16);
         HIPCHK(hipEventCreateWithFlags(&ev_mu, hipEventDisableTiming));
         HIPCHK(hipEventCreate(&ev_c0));
@@ -385,23 +411,23 @@ struct Engine {
     }
 
     ~Engine() {
-        if (st) (void)hipStreamSynchronize(st);
-        if (st2) (void)hipStreamSynchronize(st2);
+        bind();
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(st2);
         for (void* q : graveyard.ptrs) (void)hipFree(q);
-        if (h_stage) (void)hipHostFree(h_stage);
-        if (st) (void)hipStreamDestroy(st);
-        if (st2) (void)hipStreamDestroy(st2);
         if (ev_mu) (void)hipEventDestroy(ev_mu);
         if (ev_c0) (void)hipEventDestroy(ev_c0);
         if (ev_c1) (void)hipEventDestroy(ev_c1);
     }
 
     void upload_draws() {
+        bind();
         copy_sync(dbpert.ptr, bpert.data(), 8 * MM * sizeof(double), hipMemcpyHostToDevice);
     }
 
     // First term (setup_first_term, cauchy_term.hpp:772-785).
     void reset_terms() {
+        bind();
         bury();
         GraveyardScope gs(&graveyard);
         in.reserve(1);
@@ -669,6 +695,7 @@ struct Engine {
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
+        bind();
         int r;
         {
             GraveyardScope gs(&graveyard);
@@ -1366,6 +1393,7 @@ struct Engine {
 
     // Host copy of parent term `idx` of bucket `shape`.
     int term(int shape, int idx, int* info, double* A, double* p_, double* b, unsigned int* keys_, double* g) {
+        bind();
         if (shape < 0 || shape >= sr || idx < 0 || idx >= in_count[shape]) return -1;
         int o = 0;
         for (int m = 0; m < shape; m++) o += in_count[m];
@@ -1398,6 +1426,7 @@ void Engine::widen(const int* src, int64_t* dst, int n) {
 }
 
 void Engine::deterministic_transform(const double* Trans, const double* bias) {
+    bind();
     DBuf<double> dT;
     dT.reserve(d * d + d);
     copy_sync(dT.ptr, Trans, d * d * sizeof(double), hipMemcpyHostToDevice);
@@ -1410,6 +1439,7 @@ void Engine::deterministic_transform(const double* Trans, const double* bias) {
 }
 
 int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
+    bind();
     if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
     if (marg < 0 || marg >= d) throw std::runtime_error("marginal index outside the state");
     if (!(hi > lo) || !(res > 0)) throw std::runtime_error("the grid needs high > low and resolution > 0");
@@ -1471,6 +1501,7 @@ void Engine::draw_bar_nu() {
 
 int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double loy, double hiy, double resy,
                     std::vector<double>& xyz, int& nx, int& ny) {
+    bind();
     nx = ny = 0;
     if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
     if (!(0 <= i1 && i1 < i2 && i2 < d)) throw std::runtime_error("marginal indices need 0 <= i1 < i2 < n");
@@ -1572,6 +1603,8 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
 static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, double xlow, double xhigh, double dx,
                        double ylow, double yhigh, double dy, cplx& nf, cplx mean[2], cplx cov[4],
                        std::vector<double>& xyz, int& nx, int& ny) {
+    P.bind();
+    S.bind();
     const int d = P.d;
     if (S.d != d) throw std::runtime_error("the two estimators have different state dimensions");
     if (P.master_step == 0 || S.master_step == 0) throw std::runtime_error("both estimators need a measurement update");
@@ -1706,6 +1739,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
 }
 
 void Engine::finalize_extended_moments(double* x_bar) {
+    bind();
     if (!skip) {
         VecArg dx;
         for (int i = 0; i < d; i++) dx.v[i] = mean[i].re;
@@ -1736,6 +1770,62 @@ void set_err(const char* where, const std::exception& e) {
     fprintf(stderr, "[cauchy_mi355x] %s\n", g_last_error.c_str());
 }
 
+// Helper threads for the batch calls.  They live as long as the process (each keeps its
+// streams), so a batch call starts no thread; one batch call runs at a time.
+class WorkerPool {
+   public:
+    // Runs work() on the calling thread and on `helpers` pool threads; returns when all end.
+    void run(int helpers, const std::function<void()>& work) {
+        std::lock_guard<std::mutex> call(call_mu_);
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            while ((int)threads_.size() < helpers) {
+                const int id = (int)threads_.size();
+                threads_.emplace_back([this, id] { loop(id); });
+                threads_.back().detach();
+            }
+            job_ = &work;
+            active_ = helpers;
+            pending_ = helpers;
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+   private:
+    void loop(int id) {
+        unsigned long long seen = 0;
+        for (;;) {
+            const std::function<void()>* job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (id >= active_) continue;
+                job = job_;
+            }
+            (*job)();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> threads_;
+    const std::function<void()>* job_ = nullptr;
+    int active_ = 0, pending_ = 0;
+    unsigned long long gen_ = 0;
+};
+
+WorkerPool& worker_pool() {
+    static WorkerPool* pool = new WorkerPool();  // never destroyed: its threads outlive main
+    return *pool;
+}
+
 // Runs step(i) for i in [0, count) on up to num_threads host threads (0: one per hardware
 // thread), estimators with the most terms first.  Each estimator is stepped by one thread
 // at a time, on its own streams.  Returns the number of estimators whose step reported an
@@ -1761,10 +1851,12 @@ int run_concurrently(void** handles, int count, int num_threads, int (*terms_of)
             errors[i] = g_last_error;
         }
     };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; t++) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
+    if (nthreads > 1) {
+        const std::function<void()> job = work;
+        worker_pool().run(nthreads - 1, job);
+    } else {
+        work();
+    }
     int failed = 0;
     std::string first;
     for (int i = 0; i < count; i++)

@@ -361,3 +361,16 @@ def pendulum_system(p: int = 1, steps: int = 8, seed_offset: int = 40, name: str
     proto.xbar0 = proto.x0 + proto.p0 * rng.standard_cauchy(2) * 0.5
     proto.zs = np.array(zs)
     return proto
+
+
+def lti_sequences(count: int = 64, steps: int = 6, seed_offset: int = 3) -> List[System]:
+    """One 3-state LTI system (the 8-step recipe's dynamics and scales) with `count` seeded
+    measurement sequences of `steps` steps: the likelihood-gradient workload."""
+    base = lti_system(d=3, steps=steps, seed_offset=0, name="lti3")
+    out = []
+    for j in range(count):
+        rng = np.random.default_rng(BASE_SEED + seed_offset + 1000 * (j + 1))
+        xs, zs = _simulate(rng, base.Phis, base.Gamma, base.H, base.beta, base.gamma, base.p0, base.b0, steps)
+        out.append(System(f"lti3_seq{j}", 3, 1, 1, base.A0, base.p0, base.b0, base.Phis, base.Gamma, base.beta,
+                          base.H, base.gamma, zs, xs, BASE_SEED + seed_offset + 1000 * (j + 1)))
+    return out

@@ -1,0 +1,89 @@
+"""Window log-likelihood and its central-difference gradient (cauchyfriendly_amd.likelihood).
+
+L = sum_k log fz_k over the updates of a 6-step window of the 3-state LTI system, for
+seeded measurement sequences; the gradient is over (beta, gamma, Phi, H).  CPU tests pin
+the oracle side: fz of the first update has the closed form s / (pi (s^2 + r^2)) with
+s = sum_l p0_l |A0_l . H| + gamma and r = z - H b0, so d log fz / d gamma = 1/s - 2 s / (s^2 + r^2),
+which the oracle's central difference must reproduce.  GPU tests compare this library's
+likelihoods with the oracle's (1e-9) and its gradient with the oracle's central
+differences at the same step (1e-6), component by component where the discrete
+structure allows it."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from cauchyfriendly_amd import workloads as W
+from oracle import driver as dr
+
+H_STEP = 1e-6
+
+
+def _oracle_L(s, theta, steps=None):
+    from cauchyfriendly_amd.likelihood import _unpack
+    beta, gamma, Phi, H = _unpack(theta, s.n)
+    steps = s.num_steps if steps is None else steps
+    est = dr.RefEstimator(steps, s.n, 1, 1, s.A0, s.p0, s.b0, seed=s.srand_seed, kind="oracle")
+    L = 0.0
+    structure = []
+    for k in range(steps):
+        r = est.step(s.zs[k, 0], Phi.reshape(-1), s.Gamma.reshape(-1), beta, H.reshape(-1), gamma[0])
+        L += np.log(r["fz"].real)
+        structure.append((r["nt_mu"], r["nt_ftr"]))
+    est.close()
+    return L, structure
+
+
+def _oracle_grad(s, params, steps=None):
+    from cauchyfriendly_amd.likelihood import theta_of
+    th = theta_of(s)
+    g = []
+    for p in params:
+        tp, tm = th.copy(), th.copy()
+        tp[p] += H_STEP
+        tm[p] -= H_STEP
+        g.append((_oracle_L(s, tp, steps)[0] - _oracle_L(s, tm, steps)[0]) / (2 * H_STEP))
+    return np.array(g)
+
+
+def test_first_update_likelihood_gradient_closed_form():
+    s = W.lti_sequences(count=1)[0]
+    from cauchyfriendly_amd.likelihood import theta_of
+    th = theta_of(s)
+    H = th[11:14]
+    sc = float(np.sum(s.p0 * np.abs(s.A0 @ H)) + th[1])
+    r = float(s.zs[0, 0] - H @ s.b0)
+    L1, _ = _oracle_L(s, th, steps=1)
+    assert abs(L1 - np.log(sc / (np.pi * (sc * sc + r * r)))) < 1e-12
+    g = _oracle_grad(s, [1], steps=1)[0]
+    assert abs(g - (1 / sc - 2 * sc / (sc * sc + r * r))) < 1e-6 * abs(g)
+
+
+def test_sequences_are_seeded_and_distinct():
+    a, b = W.lti_sequences(count=2), W.lti_sequences(count=2)
+    assert np.array_equal(a[0].zs, b[0].zs) and not np.array_equal(a[0].zs, a[1].zs)
+
+
+@pytest.mark.gpu
+def test_likelihoods_match_oracle():
+    from cauchyfriendly_amd.likelihood import log_likelihoods, theta_of
+    ss = W.lti_sequences(count=6)
+    L, structure = log_likelihoods(ss, [theta_of(s) for s in ss], num_threads=4)
+    for s, l, st in zip(ss, L, structure):
+        lo, so = _oracle_L(s, theta_of(s))
+        assert st == so, f"{s.name}: structure"
+        assert abs(l - lo) <= 1e-9 * abs(lo), f"{s.name}: L {l} != {lo}"
+
+
+@pytest.mark.gpu
+def test_gradient_matches_oracle_central_differences():
+    from cauchyfriendly_amd.likelihood import loglik_gradient
+    ss = W.lti_sequences(count=3)
+    params = [0, 1, 2, 6, 11, 13]   # beta, gamma, two Phi entries, two H entries
+    L0, grad, valid = loglik_gradient(ss, h=H_STEP, params=params, num_threads=4)
+    assert valid.mean() > 0.5
+    for i, s in enumerate(ss):
+        go = _oracle_grad(s, params)
+        for j in range(len(params)):
+            if valid[i, j]:
+                assert abs(grad[i, j] - go[j]) <= 1e-6 * max(1.0, abs(go[j])), f"{s.name} param {params[j]}"
