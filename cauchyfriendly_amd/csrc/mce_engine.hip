@@ -344,7 +344,6 @@ struct Engine {
            const double* b0_)
         : d(d_), cmcc(cmcc_), pncc(pncc_), p(p_) {
         if (d < 1 || d > kMaxD) throw std::runtime_error("state dimension outside 1..8 is not supported");
-        bind();
         n_mu = p * num_steps;
         const int maxhp = d > 1 ? (num_steps - 1) * pncc + d : d + pncc;
         sr = maxhp + 1;
@@ -360,6 +359,9 @@ struct Engine {
         bpert.assign(8 * MM, 0.0);
         for (int h = 0; h < 8; h++)
             for (int i = 0; i < MM; i++) bpert[h * MM + i] = 2 * rand_u() - 1;
+        // the first HIP call of a process may itself draw from rand(): streams only after
+        // the draws above
+        bind();
         mean.assign(d, cmake(0, 0));
         cov.assign(d * d, cmake(0, 0));
         last_mean = mean;
@@ -1071,6 +1073,8 @@ struct Engine {
             else if (mode == kLookupRankVals && MM <= 16) MCE_GEVAL(kLookupRankVals, 16);
             else if (mode == kLookupRankVals && MM <= 24) MCE_GEVAL(kLookupRankVals, 24);
             else if (mode == kLookupRankVals) MCE_GEVAL(kLookupRankVals, 32);
+            else if (mode == kLookupRank && MM <= 8) MCE_GEVAL(kLookupRank, 8);
+            else if (mode == kLookupRank && MM <= 14) MCE_GEVAL(kLookupRank, 14);
             else if (mode == kLookupRank && MM <= 16) MCE_GEVAL(kLookupRank, 16);
             else if (mode == kLookupRank) MCE_GEVAL(kLookupRank, 32);
             else if (mode == kLookupSearchLds) MCE_GEVAL(kLookupSearchLds, 32);
@@ -1201,6 +1205,8 @@ struct Engine {
         else if (mode == kLookupRankVals && MM <= 16) MCE_GMEM(kLookupRankVals, 16);
         else if (mode == kLookupRankVals && MM <= 24) MCE_GMEM(kLookupRankVals, 24);
         else if (mode == kLookupRankVals) MCE_GMEM(kLookupRankVals, 32);
+        else if (mode == kLookupRank && MM <= 8) MCE_GMEM(kLookupRank, 8);
+        else if (mode == kLookupRank && MM <= 14) MCE_GMEM(kLookupRank, 14);
         else if (mode == kLookupRank && MM <= 16) MCE_GMEM(kLookupRank, 16);
         else MCE_GMEM(kLookupRank, 32);
 #undef MCE_GMEM
@@ -1439,7 +1445,6 @@ void Engine::deterministic_transform(const double* Trans, const double* bias) {
 }
 
 int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
-    bind();
     if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
     if (marg < 0 || marg >= d) throw std::runtime_error("marginal index outside the state");
     if (!(hi > lo) || !(res > 0)) throw std::runtime_error("the grid needs high > low and resolution > 0");
@@ -1450,6 +1455,7 @@ int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<doub
     }
     if ((hi - lo) / res > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     draw_bar_nu();
+    bind();  // after the draws: a thread's first stream creation must not come between them
     const int npts = (int)((hi - lo + res - 1e-15) / res) + 1;
     std::vector<double> xs(npts), ys(npts, 0.0);
     for (int i = 0; i < npts; i++) xs[i] = std::min(lo + i * res, hi);
@@ -1501,7 +1507,6 @@ void Engine::draw_bar_nu() {
 
 int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double loy, double hiy, double resy,
                     std::vector<double>& xyz, int& nx, int& ny) {
-    bind();
     nx = ny = 0;
     if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
     if (!(0 <= i1 && i1 < i2 && i2 < d)) throw std::runtime_error("marginal indices need 0 <= i1 < i2 < n");
@@ -1514,6 +1519,7 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
     }
     if (((hix - lox) / resx + 1) * ((hiy - loy) / resy + 1) > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     draw_bar_nu();
+    bind();
     nx = (int)((hix - lox + resx - 1e-15) / resx) + 1;
     ny = (int)((hiy - loy + resy - 1e-15) / resy) + 1;
     const int npts = nx * ny;
@@ -1603,8 +1609,6 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
 static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, double xlow, double xhigh, double dx,
                        double ylow, double yhigh, double dy, cplx& nf, cplx mean[2], cplx cov[4],
                        std::vector<double>& xyz, int& nx, int& ny) {
-    P.bind();
-    S.bind();
     const int d = P.d;
     if (S.d != d) throw std::runtime_error("the two estimators have different state dimensions");
     if (P.master_step == 0 || S.master_step == 0) throw std::runtime_error("both estimators need a measurement update");
@@ -1622,6 +1626,8 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         for (int j = 0; j < d; j++) acc += Trel[k * d + j] * R.nu[j];
         R.nu2[k] = acc;
     }
+    P.bind();  // after the draws
+    S.bind();
     R.norm_fact = kInvTwoPi * kInvTwoPi / P.fz.re / S.fz.re;
     R.eps = eps;
     const int np = P.Nt, ns = S.Nt;

@@ -1,6 +1,7 @@
 """The pycauchy C ABI (the reference's SWIG boundary) on the HIP engine, checked
 against the public outputs the reference reports for the same calls."""
 import ctypes as ct
+import os
 
 import numpy as np
 import pytest
@@ -109,3 +110,31 @@ def test_pycauchy_reset_and_search_ordering():
         assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.arange(3)) == 0
     finally:
         pycauchy.pycauchy_single_step_shutdown(h)
+
+
+@pytest.mark.gpu
+def test_first_estimator_of_a_process_draws_like_the_reference():
+    """The constructor's rand() draws must come before the process's first HIP call (which
+    may draw from rand() itself): a fresh interpreter whose first GPU action is creating
+    an estimator reproduces the reference's root point and perturbations."""
+    import subprocess
+    import sys
+    code = (
+        "import numpy as np\n"
+        "from cauchyfriendly_amd import workloads as W\n"
+        "from cauchyfriendly_amd.estimator import GpuEstimator\n"
+        "s = W.config0()\n"
+        "e = GpuEstimator(s.num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, seed=s.srand_seed)\n"
+        "r, b = e.random_draws()\n"
+        "print(' '.join(repr(float(v)) for v in np.concatenate([r, b[0]])))\n"
+    )
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], cwd=repo, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    got = np.array([float(v) for v in out.stdout.strip().splitlines()[-1].split()])
+    from oracle import driver as dr
+    s = W.config0()
+    est = dr.RefEstimator(s.num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, seed=s.srand_seed, kind="oracle")
+    exp = np.concatenate([est.root_point(), est.b_pert(0)])
+    est.close()
+    assert np.array_equal(got, exp)
