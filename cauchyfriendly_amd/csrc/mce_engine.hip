@@ -44,12 +44,113 @@ thread_local std::string g_last_error;
 
 int ceil_div(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
-// Allocations outgrown while an engine works on this host thread.  They are released by the
-// engine once its streams are idle (hipFree synchronises the whole device, which would stall
-// every other estimator stepped concurrently); outside an engine they are freed at once.
+// Device blocks of up to kMaxBlock bytes come from a process-wide cache with one free list
+// per power-of-two size.  An estimator that outgrows its buffers, or a new estimator of a
+// batch, takes blocks that others gave back instead of calling hipMalloc, which holds a
+// device-wide lock (thousands of small estimators stepped from many host threads otherwise
+// spend half their time there).  A block is given back only when no queued work can touch
+// it: after the streams of the engine that used it were synchronised.  Larger blocks are
+// allocated and freed directly.
+class BlockCache {
+   public:
+    static constexpr size_t kMinBlock = 256;
+    static constexpr size_t kMaxBlock = size_t(256) << 20;
+    static constexpr size_t kMaxCached = size_t(16) << 30;  // beyond this, blocks given back are freed
+
+    static size_t block_size(size_t bytes) {
+        if (bytes > kMaxBlock) return bytes;
+        size_t s = kMinBlock;
+        while (s < bytes) s <<= 1;
+        return s;
+    }
+    // `bytes` is a block_size() value.
+    void* get(size_t bytes) {
+        if (bytes <= kMaxBlock) {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto& l = free_[cls(bytes)];
+            if (!l.empty()) {
+                void* p = l.back();
+                l.pop_back();
+                cached_ -= bytes;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            trim();
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess)
+            throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e) + "' at hipMalloc(" +
+                                     std::to_string(bytes) + " bytes)");
+        return p;
+    }
+    // The caller guarantees that no queued work reads or writes the block.
+    void put(void* p, size_t bytes) {
+        if (!p) return;
+        if (bytes <= kMaxBlock) {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (cached_ + bytes <= kMaxCached) {
+                free_[cls(bytes)].push_back(p);
+                cached_ += bytes;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+    // Frees every cached block (hipFree synchronises the device).
+    void trim() {
+        std::vector<void*> all;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto& l : free_) {
+                all.insert(all.end(), l.begin(), l.end());
+                l.clear();
+            }
+            cached_ = 0;
+        }
+        for (void* p : all) (void)hipFree(p);
+    }
+
+   private:
+    static int cls(size_t bytes) { return __builtin_ctzll(bytes); }
+    std::mutex mu_;
+    std::vector<void*> free_[64];
+    size_t cached_ = 0;
+};
+
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache();  // never destroyed: engines may outlive static teardown
+    return *c;
+}
+
+// Allocations outgrown while an engine works on this host thread.  The engine gives them
+// back once its streams are idle: blocks of the cache's sizes at the end of the call, larger
+// ones (hipFree synchronises the whole device, which would stall every other estimator
+// stepped concurrently) when they add up to more than a bound or the engine goes.  Outside an
+// engine they are given back at once, after a device synchronisation.
 struct Graveyard {
-    std::vector<void*> ptrs;
-    size_t bytes = 0;
+    std::vector<std::pair<void*, size_t>> blocks;
+    size_t bytes = 0, big_bytes = 0;
+    void add(void* p, size_t b) {
+        blocks.emplace_back(p, b);
+        bytes += b;
+        if (b > BlockCache::kMaxBlock) big_bytes += b;
+    }
+    // Gives back the cache-sized blocks, and the larger ones too if `all`.
+    void release(bool all) {
+        std::vector<std::pair<void*, size_t>> keep;
+        for (auto& [p, b] : blocks) {
+            if (all || b <= BlockCache::kMaxBlock)
+                block_cache().put(p, b);
+            else
+                keep.emplace_back(p, b);
+        }
+        blocks.swap(keep);
+        bytes = big_bytes = 0;
+        for (auto& [p, b] : blocks) bytes += b, big_bytes += b;
+    }
 };
 thread_local Graveyard* t_graveyard = nullptr;
 
@@ -60,31 +161,37 @@ struct GraveyardScope {
 };
 
 // Grow-only device buffer; growing discards the contents.
+// Grow-only device buffer; growing discards the contents.  Its block comes from the block
+// cache; the owner's streams are idle whenever it is destroyed (every engine call ends with
+// its streams synchronised), so the block goes straight back.
 template <class T>
 struct DBuf {
     T* ptr = nullptr;
-    size_t cap = 0;
+    size_t cap = 0, bytes = 0;
     void reserve(size_t n) {
         if (n <= cap) return;
+        size_t nn = std::max(n, 2 * cap);
+        nn = std::max<size_t>(nn, 64);
         if (ptr) {
             if (t_graveyard) {
                 // kernels still in flight may read the old allocation: it stays valid
-                t_graveyard->ptrs.push_back(ptr);
-                t_graveyard->bytes += cap * sizeof(T);
+                t_graveyard->add(ptr, bytes);
             } else {
                 HIPCHK(hipDeviceSynchronize());
-                HIPCHK(hipFree(ptr));
+                block_cache().put(ptr, bytes);
             }
+            ptr = nullptr;
+            cap = bytes = 0;
         }
-        size_t nn = std::max(n, 2 * cap);
-        nn = std::max<size_t>(nn, 64);
-        HIPCHK(hipMalloc(&ptr, nn * sizeof(T)));
-        cap = nn;
+        const size_t b = BlockCache::block_size(nn * sizeof(T));
+        ptr = (T*)block_cache().get(b);
+        bytes = b;
+        cap = b / sizeof(T);
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        block_cache().put(ptr, bytes);
         ptr = nullptr;
-        cap = 0;
+        cap = bytes = 0;
     }
     ~DBuf() { release(); }
 };
@@ -329,15 +436,14 @@ struct Engine {
     Graveyard graveyard;
     static constexpr size_t kGraveyardBytes = size_t(4) << 30;  // released early beyond this
 
-    // Frees the outgrown allocations once this engine's streams are idle.
-    void bury() {
-        if (graveyard.ptrs.empty()) return;
+    // Gives back the outgrown allocations once this engine's streams are idle: every one if
+    // `all`, else the cache-sized ones, and the larger ones beyond kGraveyardBytes.
+    void bury(bool all = true) {
+        if (graveyard.blocks.empty()) return;
         bind();
         HIPCHK(hipStreamSynchronize(st));
         HIPCHK(hipStreamSynchronize(st2));
-        for (void* q : graveyard.ptrs) HIPCHK(hipFree(q));
-        graveyard.ptrs.clear();
-        graveyard.bytes = 0;
+        graveyard.release(all || graveyard.big_bytes > kGraveyardBytes);
     }
 
     Engine(int num_steps, int d_, int cmcc_, int pncc_, int p_, const double* A0_, const double* p0_,
@@ -416,7 +522,7 @@ struct Engine {
         bind();
         (void)hipStreamSynchronize(st);
         (void)hipStreamSynchronize(st2);
-        for (void* q : graveyard.ptrs) (void)hipFree(q);
+        graveyard.release(true);
         if (ev_mu) (void)hipEventDestroy(ev_mu);
         if (ev_c0) (void)hipEventDestroy(ev_c0);
         if (ev_c1) (void)hipEventDestroy(ev_c1);
@@ -703,7 +809,7 @@ struct Engine {
             GraveyardScope gs(&graveyard);
             r = update_terms(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u);
         }
-        if (graveyard.bytes > kGraveyardBytes) bury();
+        bury(false);
         return r;
     }
 
@@ -2539,5 +2645,24 @@ void mce_profile_reset(void* handle) {
 }
 
 const char* mce_last_error(void) { return g_last_error.c_str(); }
+
+long long mce_collect_outputs(void** outs, const int* sizes, long long count, int nout, double* dbuf, int* ibuf) {
+    long long nd = 0, ni = 0;
+    for (long long i = 0; i < count; i++)
+        for (int j = 0; j < nout; j++) {
+            const long long k = i * nout + j;
+            const int n = std::max(sizes[k], 0);
+            if (j == nout - 1) {
+                if (n) memcpy(ibuf + ni, outs[k], n * sizeof(int));
+                ni += n;
+            } else {
+                if (n) memcpy(dbuf + nd, outs[k], n * sizeof(double));
+                nd += n;
+            }
+            free(outs[k]);
+            outs[k] = nullptr;
+        }
+    return nd;
+}
 
 }  // extern "C"

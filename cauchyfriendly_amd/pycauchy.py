@@ -29,6 +29,19 @@ def _arr(x):
     return a, a.ctypes.data_as(_dp), int(a.size)
 
 
+def _packed(arrays):
+    """One contiguous copy of the arrays and, per array, a pointer into it (NULL when empty)
+    and its size; the copy must outlive the call."""
+    flat = [np.asarray(a, dtype=np.float64).reshape(-1) for a in arrays]
+    n = np.array([a.size for a in flat], dtype=np.int64)
+    cat = np.concatenate(flat) if len(flat) else np.zeros(0)
+    if cat.size == 0:
+        cat = np.zeros(1)
+    addr = cat.ctypes.data + 8 * (np.cumsum(n) - n)
+    ptrs = (ct.c_void_p * len(flat))(*[int(a) if k else None for a, k in zip(addr.tolist(), n.tolist())])
+    return cat, ct.cast(ptrs, ct.POINTER(_dp)), (ct.c_int * len(flat))(*n.tolist())
+
+
 def _take_d(ptr, n):
     """Copies a library-malloc'd double array into numpy and frees it."""
     out = np.ctypeslib.as_array(ptr, shape=(max(n, 0),)).copy() if n > 0 else np.zeros(0)
@@ -135,25 +148,31 @@ class _Bindings:
 
     def _batch(self, fname, nout, handles, msmts, controls, extra, num_threads):
         count = len(handles)
-        hs = (ct.c_void_p * count)(*[ct.c_void_p(h) for h in handles])
-        m_ = [_arr(m) for m in msmts]
-        c_ = [_arr(c) for c in controls]
-        mp = (_dp * count)(*[a[1] for a in m_])
-        mn = (ct.c_int * count)(*[a[2] for a in m_])
-        cp = (_dp * count)(*[a[1] if a[2] > 0 else _dp() for a in c_])
-        cn = (ct.c_int * count)(*[a[2] for a in c_])
+        hs = (ct.c_void_p * count)(*handles)
+        mcat, mp, mn = _packed(msmts)
+        ccat, cp, cn = _packed(controls)
         outs = (ct.c_void_p * (nout * count))()
         sizes = (ct.c_int * (nout * count))()
         failed = self._f(fname)(hs, count, mp, mn, cp, cn, *extra, outs, sizes, int(num_threads))
+        # every output copied (and freed) in one native call, then handed out as views
+        sz = np.frombuffer(sizes, dtype=np.int32).reshape(count, nout).clip(min=0)
+        dbuf = np.empty(int(sz[:, :-1].sum()))
+        ibuf = np.empty(int(sz[:, -1].sum()), dtype=np.int32)
+        _native.lib().mce_collect_outputs(outs, sizes, count, nout, dbuf.ctypes.data_as(_dp),
+                                          ibuf.ctypes.data_as(_ip))
+        dends = np.cumsum(sz[:, :-1].reshape(-1)).tolist()
+        iends = np.cumsum(sz[:, -1]).tolist()
         res = []
+        d0 = i0 = k = 0
         for i in range(count):
             r = []
-            for j in range(nout):
-                ptr, n = outs[nout * i + j], sizes[nout * i + j]
-                if j == nout - 1:
-                    r.append(_take_i(ct.cast(ptr, _ip), n))
-                else:
-                    r.append(_take_d(ct.cast(ptr, _dp), n))
+            for _ in range(nout - 1):
+                e = dends[k]
+                r.append(dbuf[d0:e])
+                d0 = e
+                k += 1
+            r.append(ibuf[i0:iends[i]])
+            i0 = iends[i]
             res.append(tuple(r))
         if failed:
             msg = self._last_error() if self._last_error else ""

@@ -292,6 +292,13 @@ void mce_profile_reset(void* handle);
 /* Last error message ("" when none). */
 const char* mce_last_error(void);
 
+/* Host helper for the batched calls; no counterpart in the reference.  Copies the outputs
+ * that a pycauchy_batch_* call returned (outs / sizes of count estimators x nout outputs) into
+ * two caller buffers in order: every output but the last of each estimator into dbuf
+ * (doubles), the last (the int error codes) into ibuf, then frees each output and sets its
+ * pointer to NULL.  dbuf and ibuf must hold the summed sizes.  Returns the doubles written. */
+long long mce_collect_outputs(void** outs, const int* sizes, long long count, int nout, double* dbuf, int* ibuf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,3 +45,57 @@ def test_workloads_are_reproducible():
     from cauchyfriendly_amd import workloads as W
     a, b = W.config0(), W.config0()
     assert (a.zs == b.zs).all() and (a.Phis[0] == b.Phis[0]).all()
+
+
+def test_collect_outputs_copies_in_order_and_frees():
+    """mce_collect_outputs (host code): the batched calls' malloc'd outputs, doubles then the
+    last (int) output of each estimator, land in order in two buffers; each pointer is freed
+    and cleared."""
+    import numpy as np
+    from cauchyfriendly_amd import _native
+    L = _native.lib()
+    libc = ct.CDLL("libc.so.6")
+    libc.malloc.restype = ct.c_void_p
+    libc.malloc.argtypes = [ct.c_size_t]
+    sizes = [[2, 0, 3, 1], [1, 4, 0, 2], [0, 0, 0, 0], [5, 1, 1, 3]]
+    count, nout = len(sizes), len(sizes[0])
+    outs = (ct.c_void_p * (count * nout))()
+    sz = (ct.c_int * (count * nout))()
+    rng = np.random.default_rng(5)
+    want_d, want_i = [], []
+    for i in range(count):
+        for j in range(nout):
+            n = sizes[i][j]
+            k = i * nout + j
+            sz[k] = n
+            if j == nout - 1:
+                v = rng.integers(-1000, 1000, n).astype(np.int32)
+                want_i.append(v)
+            else:
+                v = rng.standard_normal(n)
+                want_d.append(v)
+            outs[k] = libc.malloc(max(v.nbytes, 1))
+            ct.memmove(outs[k], v.ctypes.data, v.nbytes)
+    want_d, want_i = np.concatenate(want_d), np.concatenate(want_i)
+    dbuf = np.full(want_d.size, np.nan)
+    ibuf = np.zeros(want_i.size, dtype=np.int32)
+    nd = L.mce_collect_outputs(outs, sz, count, nout, dbuf.ctypes.data_as(ct.POINTER(ct.c_double)),
+                               ibuf.ctypes.data_as(ct.POINTER(ct.c_int)))
+    assert nd == want_d.size
+    assert np.array_equal(dbuf, want_d) and np.array_equal(ibuf, want_i)
+    assert all(outs[k] is None for k in range(count * nout))
+
+
+def test_batch_inputs_are_packed_with_pointers_and_sizes():
+    import numpy as np
+    from cauchyfriendly_amd.pycauchy import _packed
+    arrays = [np.array([1.0, 2.0]), np.zeros(0), [3.0, 4.0, 5.0], np.array([[6.0]])]
+    cat, ptrs, n = _packed(arrays)
+    assert list(n) == [2, 0, 3, 1]
+    assert not ptrs[1]
+    for k, a in enumerate(arrays):
+        a = np.asarray(a, dtype=np.float64).reshape(-1)
+        if a.size:
+            assert [ptrs[k][i] for i in range(a.size)] == a.tolist()
+    _, ptrs0, n0 = _packed([np.zeros(0), np.zeros(0)])
+    assert list(n0) == [0, 0] and not ptrs0[0] and not ptrs0[1]
