@@ -47,15 +47,18 @@ int ceil_div(int64_t a, int b) { return (int)((a + b - 1) / b); }
 // Device blocks of up to kMaxBlock bytes come from a process-wide cache with one free list
 // per power-of-two size.  An estimator that outgrows its buffers, or a new estimator of a
 // batch, takes blocks that others gave back instead of calling hipMalloc, which holds a
-// device-wide lock (thousands of small estimators stepped from many host threads otherwise
-// spend half their time there).  A block is given back only when no queued work can touch
-// it: after the streams of the engine that used it were synchronised.  Larger blocks are
-// allocated and freed directly.
+// device-wide lock and slows down as live allocations multiply (thousands of small
+// estimators stepped from many host threads otherwise spend most of their time there).
+// Blocks of up to kSlabMax bytes are carved from slabs of several hundred blocks and stay
+// with the process; larger ones are allocated one by one and freed when the cache holds more
+// than kMaxCached.  A block is given back only when no queued work can touch it: after the
+// streams of the engine that used it were synchronised.  Larger blocks bypass the cache.
 class BlockCache {
    public:
     static constexpr size_t kMinBlock = 256;
+    static constexpr size_t kSlabMax = size_t(4) << 20;
     static constexpr size_t kMaxBlock = size_t(256) << 20;
-    static constexpr size_t kMaxCached = size_t(16) << 30;  // beyond this, blocks given back are freed
+    static constexpr size_t kMaxCached = size_t(16) << 30;  // unslabbed blocks beyond this are freed
 
     static size_t block_size(size_t bytes) {
         if (bytes > kMaxBlock) return bytes;
@@ -63,34 +66,39 @@ class BlockCache {
         while (s < bytes) s <<= 1;
         return s;
     }
+    // Bytes of the slab that blocks of `bytes` (a block_size() value <= kSlabMax) are cut from.
+    static size_t slab_size(size_t bytes) {
+        return std::min<size_t>(size_t(256) << 20, std::max<size_t>(size_t(4) << 20, bytes * 512));
+    }
     // `bytes` is a block_size() value.
     void* get(size_t bytes) {
         if (bytes <= kMaxBlock) {
             std::lock_guard<std::mutex> lk(mu_);
             auto& l = free_[cls(bytes)];
+            if (l.empty() && bytes <= kSlabMax) {
+                const size_t sb = slab_size(bytes);
+                char* slab = (char*)raw_alloc(sb, true);
+                slab_bytes_ += sb;
+                for (size_t off = sb; off >= bytes; off -= bytes) l.push_back(slab + off - bytes);
+            }
             if (!l.empty()) {
                 void* p = l.back();
                 l.pop_back();
-                cached_ -= bytes;
+                if (bytes > kSlabMax) cached_ -= bytes;
                 return p;
             }
         }
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipErrorOutOfMemory) {
-            trim();
-            e = hipMalloc(&p, bytes);
-        }
-        if (e != hipSuccess)
-            throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e) + "' at hipMalloc(" +
-                                     std::to_string(bytes) + " bytes)");
-        return p;
+        return raw_alloc(bytes, false);
     }
     // The caller guarantees that no queued work reads or writes the block.
     void put(void* p, size_t bytes) {
         if (!p) return;
         if (bytes <= kMaxBlock) {
             std::lock_guard<std::mutex> lk(mu_);
+            if (bytes <= kSlabMax) {
+                free_[cls(bytes)].push_back(p);
+                return;
+            }
             if (cached_ + bytes <= kMaxCached) {
                 free_[cls(bytes)].push_back(p);
                 cached_ += bytes;
@@ -99,25 +107,39 @@ class BlockCache {
         }
         (void)hipFree(p);
     }
-    // Frees every cached block (hipFree synchronises the device).
-    void trim() {
-        std::vector<void*> all;
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            for (auto& l : free_) {
-                all.insert(all.end(), l.begin(), l.end());
-                l.clear();
-            }
-            cached_ = 0;
-        }
-        for (void* p : all) (void)hipFree(p);
-    }
 
    private:
+    // `locked`: the caller holds mu_.
+    void* raw_alloc(size_t bytes, bool locked) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            if (locked) {
+                trim_unslabbed();
+            } else {
+                std::lock_guard<std::mutex> lk(mu_);
+                trim_unslabbed();
+            }
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess)
+            throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e) + "' at hipMalloc(" +
+                                     std::to_string(bytes) + " bytes)");
+        return p;
+    }
+    // Frees the cached blocks that were allocated one by one (hipFree synchronises the
+    // device); the lists it empties belong to classes above kSlabMax.  Needs mu_.
+    void trim_unslabbed() {
+        for (int c = cls(kSlabMax) + 1; c < 64; c++) {
+            for (void* p : free_[c]) (void)hipFree(p);
+            free_[c].clear();
+        }
+        cached_ = 0;
+    }
     static int cls(size_t bytes) { return __builtin_ctzll(bytes); }
     std::mutex mu_;
     std::vector<void*> free_[64];
-    size_t cached_ = 0;
+    size_t cached_ = 0, slab_bytes_ = 0;
 };
 
 BlockCache& block_cache() {
