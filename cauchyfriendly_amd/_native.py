@@ -132,6 +132,8 @@ def lib():
     L.mce_profile_read.argtypes = [ct.c_void_p, _dp]
     L.mce_profile_reset.argtypes = [ct.c_void_p]
     L.mce_last_error.restype = ct.c_char_p
+    L.mce_step_record.restype = ct.c_int
+    L.mce_step_record.argtypes = [ct.c_void_p, _dp, _ip, ct.c_int]
     L.mce_collect_outputs.restype = ct.c_longlong
     L.mce_collect_outputs.argtypes = [ct.POINTER(ct.c_void_p), _ip, ct.c_longlong, ct.c_int, _dp, _ip]
     _lib = L

@@ -192,7 +192,9 @@ struct DBuf {
     size_t cap = 0, bytes = 0;
     void reserve(size_t n) {
         if (n <= cap) return;
-        size_t nn = std::max(n, 2 * cap);
+        // doubling, except for buffers beyond 1 GiB (the g-tables of a window's largest
+        // updates reach tens of GiB): those get an eighth over what is asked
+        size_t nn = n * sizeof(T) > (size_t(1) << 30) ? n + n / 8 : std::max(n, 2 * cap);
         nn = std::max<size_t>(nn, 64);
         if (ptr) {
             if (t_graveyard) {
@@ -1897,6 +1899,19 @@ struct Handle {
     void (*msmt_model)(CauchyDynamicsUpdateContainer*, double*) = nullptr;
     void (*ext_update)(CauchyDynamicsUpdateContainer*) = nullptr;
     std::vector<double> Phi, Gamma, B, H, beta, gamma, x, zbar;
+    // per scalar update of the last step: fz before term reduction, and the term counts after
+    // MU/MUC and after reduction
+    std::vector<double> step_fz;
+    std::vector<int> step_cnt;
+    void record(int i, const Engine* e) {
+        if ((int)step_fz.size() <= i) {
+            step_fz.resize(i + 1, 0.0);
+            step_cnt.resize(2 * (i + 1), 0);
+        }
+        step_fz[i] = e->diag_mom[2];
+        step_cnt[2 * i] = e->diag_cnt[1];
+        step_cnt[2 * i + 1] = e->diag_cnt[2];
+    }
 };
 
 void set_err(const char* where, const std::exception& e) {
@@ -2142,6 +2157,8 @@ void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, doub
         memcpy(*out_beta, duc.beta, pncc * sizeof(double));
         memcpy(*out_gamma, duc.gamma + start, nm * sizeof(double));
     }
+    h->step_fz.clear();
+    h->step_cnt.clear();
     for (int i = 0; i < nm; i++) {
         const int idx = start + i;
         try {
@@ -2149,6 +2166,7 @@ void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, doub
         } catch (const std::exception& ex) {
             set_err("pycauchy_single_step_ltiv", ex);
         }
+        h->record(i, e);
         (*out_fz)[i] = e->fz.re;
         for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
         for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
@@ -2281,6 +2299,8 @@ void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, do
     memcpy(*out_beta, duc.beta, pncc * sizeof(double));
     const int start = p - nm;
     double* zbar = h->zbar.data();
+    h->step_fz.clear();
+    h->step_cnt.clear();
     for (int i = 0; i < nm; i++) {
         const int idx = start + i;
         // zbar = h(x), then H = dh/dx at x (pycauchy.hpp:628-629); x is x_bar before the
@@ -2298,6 +2318,7 @@ void pycauchy_single_step_nonlin(void* handle, double* msmts, int size_msmts, do
         } catch (const std::exception& ex) {
             set_err("pycauchy_single_step_nonlin", ex);
         }
+        h->record(i, e);
         (*out_fz)[i] = e->fz.re;
         for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
         for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
@@ -2667,6 +2688,17 @@ void mce_profile_reset(void* handle) {
 }
 
 const char* mce_last_error(void) { return g_last_error.c_str(); }
+
+int mce_step_record(void* handle, double* out_fz, int* out_counts, int max_updates) {
+    const Handle* h = (const Handle*)handle;
+    const int n = (int)h->step_fz.size();
+    for (int i = 0; i < std::min(n, max_updates); i++) {
+        out_fz[i] = h->step_fz[i];
+        out_counts[2 * i] = h->step_cnt[2 * i];
+        out_counts[2 * i + 1] = h->step_cnt[2 * i + 1];
+    }
+    return n;
+}
 
 long long mce_collect_outputs(void** outs, const int* sizes, long long count, int nout, double* dbuf, int* ibuf) {
     long long nd = 0, ni = 0;

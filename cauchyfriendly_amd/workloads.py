@@ -374,3 +374,111 @@ def lti_sequences(count: int = 64, steps: int = 6, seed_offset: int = 3) -> List
         out.append(System(f"lti3_seq{j}", 3, 1, 1, base.A0, base.p0, base.b0, base.Phis, base.Gamma, base.beta,
                           base.H, base.gamma, zs, xs, BASE_SEED + seed_offset + 1000 * (j + 1)))
     return out
+
+
+# ------------------------------------------------------------ 7-state low-Earth-orbit windows
+# Two-body gravity with drag whose air density carries a first-order Gauss-Markov
+# perturbation (the seventh state), as in the reference's 7-state satellite model
+# (src/leo_satellite_7state.cpp:93-160 constants, 247-262 dynamics, 289-340 discretisation):
+# Earth's gravitational parameter, a 200 km circular-speed orbit, the density at 200 km and
+# the drag area, coefficient, mass and correlation time of that model.
+LEO_MU = 5.9722e24 * 6.674e-11
+LEO_R0 = 6378.1e3 + 200e3
+LEO_DRAG = 0.5 * 64.0 * 2.0 / 5000.0 * 2.541e-10    # A C_D rho / (2 m)
+LEO_TAU = 21600.0
+
+
+def leo_rhs(x: np.ndarray) -> np.ndarray:
+    """Time derivative of (position, velocity, density perturbation)."""
+    r, v = x[:3], x[3:6]
+    rn, vn = np.linalg.norm(r), np.linalg.norm(v)
+    a = -LEO_MU / rn ** 3 * r - LEO_DRAG * (1.0 + x[6]) * vn * v
+    return np.concatenate([v, a, [-x[6] / LEO_TAU]])
+
+
+def leo_jacobian(x: np.ndarray) -> np.ndarray:
+    """d leo_rhs / dx, in closed form."""
+    r, v = x[:3], x[3:6]
+    rn, vn = np.linalg.norm(r), np.linalg.norm(v)
+    J = np.zeros((7, 7))
+    J[0:3, 3:6] = np.eye(3)
+    J[3:6, 0:3] = -LEO_MU * (np.eye(3) / rn ** 3 - 3.0 * np.outer(r, r) / rn ** 5)
+    J[3:6, 3:6] = -LEO_DRAG * (1.0 + x[6]) * (vn * np.eye(3) + np.outer(v, v) / vn)
+    J[3:6, 6] = -LEO_DRAG * vn * v
+    J[6, 6] = -1.0 / LEO_TAU
+    return J
+
+
+def leo_discretise(J: np.ndarray, dt: float, order: int = 6):
+    """Phi = sum_{i<=order} (J dt)^i / i! and Gamma = (sum_{i<=order} J^i dt^(i+1) / (i+1)!) e_7,
+    the truncated series of the reference's discretisation."""
+    n = J.shape[0]
+    Phi = np.eye(n)
+    Gint = dt * np.eye(n)
+    P = np.eye(n)
+    fact_i, fact_i1 = 1.0, 1.0
+    for i in range(1, order + 1):
+        P = P @ J
+        fact_i *= i
+        fact_i1 *= i + 1
+        Phi = Phi + P * dt ** i / fact_i
+        Gint = Gint + P * dt ** (i + 1) / fact_i1
+    return Phi, Gint[:, 6:7].copy()
+
+
+def _rk4(x: np.ndarray, dt: float, sub: int = 10) -> np.ndarray:
+    h = dt / sub
+    for _ in range(sub):
+        k1 = leo_rhs(x)
+        k2 = leo_rhs(x + 0.5 * h * k1)
+        k3 = leo_rhs(x + 0.5 * h * k2)
+        k4 = leo_rhs(x + h * k3)
+        x = x + h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return x
+
+
+def leo_orbit(steps: int, dt: float = 60.0) -> np.ndarray:
+    """Nominal trajectory (steps x 7): a circular orbit at LEO_R0 whose position and velocity
+    directions are those of the reference model's initial condition, (1, sqrt 2, sqrt 3) / sqrt 6
+    and (-1, -1, 1) / sqrt 3, with the velocity's radial part removed (left in, it makes the
+    orbit eccentric with its perigee below the Earth's surface)."""
+    v0 = np.sqrt(LEO_MU / LEO_R0)
+    rhat = np.array([1.0, np.sqrt(2), np.sqrt(3)]) / np.sqrt(6)
+    u = np.array([-1.0, -1.0, 1.0]) / np.sqrt(3)
+    vhat = u - (u @ rhat) * rhat
+    vhat /= np.linalg.norm(vhat)
+    x = np.concatenate([LEO_R0 * rhat, v0 * vhat, [0.0]])
+    out = [x]
+    for _ in range(steps - 1):
+        x = _rk4(x, dt)
+        out.append(x)
+    return np.array(out)
+
+
+def leo7_windows(count: int = 32, window: int = 6, dt: float = 60.0, seed_offset: int = 4,
+                 beta: float = 1e-3, gamma: float = 5e-2) -> List[System]:
+    """`count` windows of `window` steps staggered one step apart along a low-Earth orbit:
+    window j linearises the dynamics about the nominal orbit at steps j .. j + window - 1
+    (Phi_k per step, Gamma at its first step), measures ranges to two seeded stations
+    (H: their unit line-of-sight vectors at its first step, 2 x 7) and simulates its own
+    error-state trajectory and Cauchy noise from the prior (A0 = I, p0 = 0.72 m and m/s for
+    position and velocity, 0.0039 for the density perturbation, b0 = 0)."""
+    rng = np.random.default_rng(BASE_SEED + seed_offset)
+    stations = 3e6 * rng.standard_normal((2, 3))
+    orbit = leo_orbit(count + window - 1, dt)
+    disc = [leo_discretise(leo_jacobian(x), dt) for x in orbit]
+    p0 = np.array([0.7195] * 6 + [0.0039])
+    beta_v = np.array([beta])
+    gamma_v = np.array([gamma, gamma])
+    out = []
+    for j in range(count):
+        Phis = [disc[j + k][0] for k in range(window)]
+        Gamma = disc[j][1]
+        los = orbit[j, :3][None, :] - stations
+        H = np.zeros((2, 7))
+        H[:, :3] = los / np.linalg.norm(los, axis=1, keepdims=True)
+        wrng = np.random.default_rng(BASE_SEED + seed_offset + 1000 * (j + 1))
+        xs, zs = _simulate(wrng, Phis, Gamma, H, beta_v, gamma_v, p0, np.zeros(7), window)
+        out.append(System(f"leo7_w{j}", 7, 1, 2, np.eye(7), p0, np.zeros(7), Phis, Gamma, beta_v, H, gamma_v,
+                          zs, xs, BASE_SEED + seed_offset + 1000 * (j + 1)))
+    return out

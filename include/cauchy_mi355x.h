@@ -292,6 +292,13 @@ void mce_profile_reset(void* handle);
 /* Last error message ("" when none). */
 const char* mce_last_error(void);
 
+/* The scalar updates of the handle's last pycauchy_single_step_* call (no counterpart in the
+ * reference, whose step returns the fz left after term reduction): out_fz[i] = the measurement
+ * likelihood fz of update i before term reduction (cauchy_estimator.hpp:455-460),
+ * out_counts[2i], [2i+1] = its terms after MU/MUC and after reduction.  Writes at most
+ * max_updates entries; returns the number of updates of that step. */
+int mce_step_record(void* handle, double* out_fz, int* out_counts, int max_updates);
+
 /* Host helper for the batched calls; no counterpart in the reference.  Copies the outputs
  * that a pycauchy_batch_* call returned (outs / sizes of count estimators x nout outputs) into
  * two caller buffers in order: every output but the last of each estimator into dbuf

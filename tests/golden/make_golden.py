@@ -72,6 +72,8 @@ CASES = [
     (lambda: W.lti_system(d=2, steps=10, seed_offset=20, name="lti2_w10"), None, (5,)),
     (lambda: W.lti_system(d=4, steps=7, seed_offset=21, name="lti4_w7"), None, ()),
     (lambda: W.lti_system(d=3, steps=9, seed_offset=10, name="lti3_w9"), None, ()),
+    # the first of the staggered 7-state orbit windows, 4 of its 6 steps (8 updates)
+    (lambda: W.leo7_windows(count=1)[0], 4, ()),
 ]
 
 

@@ -87,3 +87,59 @@ def test_gradient_matches_oracle_central_differences():
         for j in range(len(params)):
             if valid[i, j]:
                 assert abs(grad[i, j] - go[j]) <= 1e-6 * max(1.0, abs(go[j])), f"{s.name} param {params[j]}"
+
+
+def _truncated(s, steps):
+    import dataclasses
+    return dataclasses.replace(s, zs=s.zs[:steps], xs=s.xs[:steps], Phis=list(s.Phis[:steps]))
+
+
+def _oracle_L_scales(s, beta, gamma):
+    """L over every scalar update of an LTI or LTV system on the CPU restatement."""
+    est = dr.RefEstimator(s.num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, seed=s.srand_seed, kind="oracle")
+    L = 0.0
+    structure = []
+    Gt = np.ascontiguousarray(s.Gamma.reshape(-1))
+    for k in range(s.num_steps):
+        for i in range(s.p):
+            r = est.step(s.zs[k, i], s.Phi(k).reshape(-1), Gt, np.asarray(beta), s.H[i], gamma[i])
+            L += np.log(r["fz"].real)
+            structure.append((r["nt_mu"], r["nt_ftr"]))
+    est.close()
+    return L, structure
+
+
+@pytest.mark.gpu
+def test_scale_gradient_of_orbit_window_matches_oracle():
+    """Two range measurements per step on the 7-state orbit (LTV through the dynamics callback):
+    L summed over both updates of every step, and its (beta, gamma_1, gamma_2) central
+    differences, against the CPU restatement's at the same step h."""
+    from cauchyfriendly_amd.likelihood import scale_gradient
+    ws = [_truncated(s, 3) for s in W.leo7_windows(count=2)]
+    L0, grad, valid = scale_gradient(ws, h=H_STEP, num_threads=2, max_live=4)
+    for i, s in enumerate(ws):
+        sc = np.concatenate([s.beta, s.gamma])
+        lo, so = _oracle_L_scales(s, s.beta, s.gamma)
+        assert abs(L0[i] - lo) <= 1e-9 * abs(lo), f"{s.name}: L {L0[i]} != {lo}"
+        for j in range(sc.size):
+            tp, tm = sc.copy(), sc.copy()
+            tp[j] += H_STEP
+            tm[j] -= H_STEP
+            lp, sp = _oracle_L_scales(s, tp[:1], tp[1:])
+            lm, sm = _oracle_L_scales(s, tm[:1], tm[1:])
+            assert bool(valid[i, j]) == (sp == sm == so), f"{s.name} scale {j}: validity"
+            go = (lp - lm) / (2 * H_STEP)
+            if valid[i, j]:
+                assert abs(grad[i, j] - go) <= 1e-5 * max(1.0, abs(go)), f"{s.name} scale {j}: {grad[i, j]} != {go}"
+    assert valid.any()
+
+
+@pytest.mark.gpu
+def test_likelihood_sums_every_update_of_a_step():
+    """p = 2: run_likelihoods adds log fz of both updates of each step (the oracle's L)."""
+    from cauchyfriendly_amd.likelihood import make_estimator, run_likelihoods
+    s = _truncated(W.ltv_system(d=5, steps=4, seed_offset=61, name="ltv5_lik"), 4)
+    L, structure = run_likelihoods([s], lambda i: make_estimator(s), num_threads=1)
+    lo, so = _oracle_L_scales(s, s.beta, s.gamma)
+    assert structure[0] == so and len(so) == 2 * s.num_steps
+    assert abs(L[0] - lo) <= 1e-9 * abs(lo)
