@@ -13,6 +13,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <functional>
+#include <initializer_list>
 #include <mutex>
 #include <cstdio>
 #include <climits>
@@ -427,7 +428,7 @@ struct Engine {
     DBuf<uint64_t> fkeys, fskeys;
     DBuf<uint32_t> fidx, fsidx;
     DBuf<double> fvals;
-    DBuf<int> ecnt, eoff, edges, indeg, inoff, cursor, Fc;
+    DBuf<int> ecnt, eoff, edges, indeg, inoff, cursor, cursor2, round_slots, Fc;
     DBuf<uint8_t> st0, st1, isroot;
     DBuf<uint32_t> memkey, memval, smkey, smval, roots, jiota;
     DBuf<int> memcnt, memoff, inedges, dboff, gbase_dev, wlo;
@@ -587,6 +588,31 @@ struct Engine {
         HIPCHK(f((void*)nullptr, bytes));
         cub_tmp.reserve(bytes + 16);
         HIPCHK(f((void*)cub_tmp.ptr, bytes));
+    }
+
+    struct FillSpec {
+        void* ptr;
+        size_t bytes;
+        int val;
+    };
+    // Up to kFillRegions fills of this engine's stream in one launch.
+    void fill_multi(std::initializer_list<FillSpec> regions) {
+        if (regions.size() > (size_t)kFillRegions) throw std::runtime_error("fill_multi: too many regions");
+        FillArgs f{};
+        int n = 0;
+        int64_t most = 0;
+        for (const FillSpec& r : regions) {
+            if (r.bytes == 0) continue;
+            f.ptr[n] = (uint8_t*)r.ptr;
+            f.bytes[n] = (int64_t)r.bytes;
+            f.val[n] = (uint32_t)(r.val & 0xff);
+            most = std::max(most, (int64_t)r.bytes);
+            n++;
+        }
+        if (n == 0) return;
+        const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (most / 4 + 255) / 256));
+        hipLaunchKernelGGL(k_fill_multi, dim3(blocks, n), dim3(256), 0, st, f);
+        HIPCHK(hipGetLastError());
     }
 
     // Device scalars are read through pinned host memory, one copy and one stream
@@ -891,11 +917,10 @@ struct Engine {
         bt_base.reserve(np);
         btcap.reserve(np);
         TermPool IN = in.view();
-        HIPCHK(hipMemsetAsync(counters.ptr, 0, 16 * sizeof(int), st));
+        fill_multi({{counters.ptr, 16 * sizeof(int), 0}, {slot_base.ptr, sizeof(int), 0}});
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
                                              din_off.ptr, dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemsetAsync(slot_base.ptr, 0, sizeof(int), st));
         scan_i32_to_i32_incl(slots.ptr, slot_base.ptr + 1, np);
         const int nslots = read1(slot_base.ptr + np);
         prof.mark(kPhTp, st);
@@ -1414,8 +1439,19 @@ struct Engine {
             edges.reserve(E);
             inedges.reserve(E);
             cursor.reserve(N);
+            cursor2.reserve(N);
+            indeg.reserve(N);
+            inoff.reserve(N);
+            st0.reserve(N);
+            st1.reserve(N);
+            round_slots.reserve(kRoundSlots);
+            // every buffer the edge passes and the rounds start from, in one launch
+            fill_multi({{cursor.ptr, N * sizeof(int), 0},
+                        {cursor2.ptr, N * sizeof(int), 0},
+                        {indeg.ptr, N * sizeof(int), 0},
+                        {st0.ptr, (size_t)N, kUnknown},
+                        {round_slots.ptr, kRoundSlots * sizeof(int), 0}});
             if (by_pairs) {
-                HIPCHK(hipMemsetAsync(cursor.ptr, 0, N * sizeof(int), st));
                 MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
                                                      N, boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr,
                                                      fgp.ptr, fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr,
@@ -1424,33 +1460,24 @@ struct Engine {
                 MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
                                                      dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
             }
-            indeg.reserve(N);
-            inoff.reserve(N);
-            cursor.reserve(N);
-            HIPCHK(hipMemsetAsync(indeg.ptr, 0, N * sizeof(int), st));
             hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
                                (const int*)nullptr, (int*)nullptr, (int*)nullptr, 0);
             scan_i32_excl(indeg.ptr, inoff.ptr, N);
-            HIPCHK(hipMemsetAsync(cursor.ptr, 0, N * sizeof(int), st));
             hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
-                               inoff.ptr, cursor.ptr, inedges.ptr, 1);
-            st0.reserve(N);
-            st1.reserve(N);
-            HIPCHK(hipMemsetAsync(st0.ptr, kUnknown, N, st));
+                               inoff.ptr, cursor2.ptr, inedges.ptr, 1);
             uint8_t* sa = st0.ptr;
             uint8_t* sb = st1.ptr;
             // Rounds are idempotent once every term is resolved, so several run between
             // reads of the unresolved count (taken from the last round of a batch).
-            int batch = 2;
+            int batch = 2, round = 0;
             for (int done = 0; done <= N + 1;) {
-                for (int r = 0; r < batch; r++) {
-                    HIPCHK(hipMemsetAsync(counters.ptr + 4, 0, sizeof(int), st));
+                for (int r = 0; r < batch; r++, round++) {
                     hipLaunchKernelGGL(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
-                                       counters.ptr + 4);
+                                       round_slots.ptr, round);
                     std::swap(sa, sb);
                 }
                 done += batch;
-                if (read1(counters.ptr + 4) == 0) break;
+                if (read1(round_slots.ptr + (round - 1) % kRoundSlots) == 0) break;
                 batch = std::min(batch * 2, 16);
             }
             hipLaunchKernelGGL(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);

@@ -1460,10 +1460,41 @@ enum { kUnknown = 0, kRoot = 1, kClaimed = 2 };
 
 // One round of root resolution: a term is a root of its chunk's greedy pass when
 // no same-chunk root before it claims it.
+// Up to kFillRegions byte fills in one launch: region blockIdx.y gets `bytes` copies of
+// `val`, by 32-bit words where the pointer allows, then the tail bytes.
+constexpr int kFillRegions = 6;
+struct FillArgs {
+    uint8_t* ptr[kFillRegions];
+    int64_t bytes[kFillRegions];
+    uint32_t val[kFillRegions];
+};
+__global__ void k_fill_multi(FillArgs f) {
+    const int k = blockIdx.y;
+    uint8_t* p = f.ptr[k];
+    const int64_t nb = f.bytes[k];
+    const uint32_t v = f.val[k] & 0xffu;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t words = 0;
+    if (((uintptr_t)p & 3) == 0) {
+        words = nb / 4;
+        uint32_t* w = (uint32_t*)p;
+        const uint32_t v4 = v * 0x01010101u;
+        for (int64_t i = t; i < words; i += stride) w[i] = v4;
+    }
+    for (int64_t i = 4 * words + t; i < nb; i += stride) p[i] = (uint8_t)v;
+}
+
+// Round r counts its unresolved terms in slots[r % kRoundSlots] and clears the next
+// round's slot (no other thread of round r touches it, and round r + 1 starts after round r
+// ends), so the rounds need no fill between them.
+constexpr int kRoundSlots = 64;
 __global__ void k_ftr_round(int n, const int* __restrict__ inoff, const int* __restrict__ indeg,
                             const int* __restrict__ inedges, const uint8_t* __restrict__ st_in,
-                            uint8_t* __restrict__ st_out, int* __restrict__ unresolved) {
+                            uint8_t* __restrict__ st_out, int* __restrict__ slots, int r) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) slots[(r + 1) % kRoundSlots] = 0;
+    int* unresolved = slots + r % kRoundSlots;
     if (j >= n) return;
     uint8_t s = st_in[j];
     if (s == kUnknown) {
