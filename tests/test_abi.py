@@ -99,3 +99,69 @@ def test_batch_inputs_are_packed_with_pointers_and_sizes():
             assert [ptrs[k][i] for i in range(a.size)] == a.tolist()
     _, ptrs0, n0 = _packed([np.zeros(0), np.zeros(0)])
     assert list(n0) == [0, 0] and not ptrs0[0] and not ptrs0[1]
+
+
+def _split_top(params):
+    out, depth, cur = [], 0, ""
+    for ch in params:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
+def _header_functions():
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    text = re.sub(r"typedef struct.*?\}\s*\w+;", "", text, flags=re.S)
+    funcs = {}
+    for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(((?:[^;])*?)\);", text, flags=re.M | re.S):
+        funcs[m.group(1)] = [" ".join(p.split()) for p in _split_top(m.group(2)) if p.strip() not in ("", "void")]
+    return funcs
+
+
+def test_swig_interface_covers_every_wrapped_argument():
+    """include/pycauchy_mi355x.i (no SWIG here to build it): every function it wraps has a
+    typemap for each array, output and callback argument, by the argument names of the C
+    header, and every function it ignores exists."""
+    iface = open(os.path.join(REPO, "include", "pycauchy_mi355x.i")).read()
+    body = re.sub(r"/\*.*?\*/", "", iface, flags=re.S)
+    ignored = set(re.findall(r"%ignore\s+(\w+);", body))
+    funcs = _header_functions()
+    assert ignored <= set(funcs), f"ignored but not declared: {sorted(ignored - set(funcs))}"
+    applies = {}
+    for kind, targets in re.findall(r"%apply\s*\(([^)]*)\)\s*\{(.*?)\};", body, flags=re.S):
+        for t in re.findall(r"\(([^)]*)\)", targets):
+            applies[" ".join(t.split())] = " ".join(kind.split())
+    outputs = set()
+    for targets in re.findall(r"%apply\s+int\s*\*\s*OUTPUT\s*\{(.*?)\};", body, flags=re.S):
+        outputs |= {" ".join(t.split()) for t in targets.split(",")}
+    callbacks = set(re.findall(r"%typemap\(in\)\s*void\s*\(\*(\w+)\)", body))
+    scalar = re.compile(r"^(int|double|bool|long long|void\*) \w+$")
+    wrapped = [f for f in funcs if f not in ignored]
+    assert "pycauchy_single_step_ltiv" in wrapped and "pycauchy_initialize_nonlin" in wrapped
+    for f in wrapped:
+        ps = funcs[f]
+        k = 0
+        while k < len(ps):
+            p = ps[k].replace(" *", "*").replace("* ", "* ")
+            pair = (p + ", " + ps[k + 1].replace(" *", "*")) if k + 1 < len(ps) else None
+            cb = re.match(r"void \(\*(\w+)\)", p)
+            if pair is not None and pair in applies:
+                k += 2
+            elif p in outputs:
+                k += 1
+            elif cb:
+                assert cb.group(1) in callbacks, f"{f}: callback {cb.group(1)} has no typemap"
+                k += 1
+            elif p == "char* log_dir" or scalar.match(p):
+                k += 1
+            else:
+                raise AssertionError(f"{f}: argument '{ps[k]}' has no typemap")

@@ -138,3 +138,41 @@ def test_first_estimator_of_a_process_draws_like_the_reference():
     exp = np.concatenate([est.root_point(), est.b_pert(0)])
     est.close()
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.gpu
+def test_reused_device_blocks_give_identical_results():
+    """Device buffers come from a process-wide block cache, so a new estimator can receive
+    memory holding an earlier estimator's data.  In a fresh process, a 5-state LTV window
+    run on freshly allocated memory, then again after estimators of other shapes have been
+    created, stepped and destroyed, must give bit-identical counts, error bits and moments."""
+    import subprocess
+    import sys
+    code = (
+        "import numpy as np\n"
+        "from cauchyfriendly_amd import workloads as W\n"
+        "from cauchyfriendly_amd.estimator import GpuEstimator\n"
+        "def run(s, steps):\n"
+        "    e = GpuEstimator(steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, seed=s.srand_seed)\n"
+        "    out = []\n"
+        "    for k in range(steps):\n"
+        "        for i in range(s.p):\n"
+        "            r = e.step(s.zs[k, i], s.Phi(k).reshape(-1), s.Gamma.reshape(-1), s.beta, s.H[i], s.gamma[i])\n"
+        "            out.append((r['nt_mu'], r['nt_ftr'], r['err'], complex(r['fz']), np.asarray(r['mean']).tobytes(),\n"
+        "                        np.asarray(r['cov']).tobytes(), np.asarray(r['shape_ftr']).tobytes()))\n"
+        "    e.close()\n"
+        "    return out\n"
+        "s = W.config1(steps=4)\n"
+        "first = run(s, 4)\n"
+        "for d in (2, 3, 4, 5, 3):\n"
+        "    run(W.lti_system(d=d, steps=6, seed_offset=300 + d), 6)\n"
+        "run(W.leo7_windows(count=1, window=3)[0], 3)\n"
+        "live = [GpuEstimator(3, 4, 1, 1, np.eye(4), np.full(4, 0.1), np.zeros(4), seed=9) for _ in range(3)]\n"
+        "again = run(s, 4)\n"
+        "print('SAME' if first == again else 'DIFFERENT', len(first), first[-1][0])\n"
+    )
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], cwd=repo, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stderr[-2000:]
+    last = out.stdout.strip().splitlines()[-1]
+    assert last.startswith("SAME"), last
