@@ -537,8 +537,9 @@ struct Engine {
         dbpert.reserve(8 * MM);
         upload_draws();
         dconst.reserve(1);
-        counters.reserve(16);
+        counters.reserve(32);
         hist.reserve(sr + 1);
+        hist_mx.reserve(sr + 2);
         din_off.reserve(sr + 2);
         reset_terms();
     }
@@ -808,7 +809,7 @@ struct Engine {
         std::vector<double> part((size_t)T * NC * 2);
         HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st2));
         int mu_err = 0;
-        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 17, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st2));
         HIPCHK(hipStreamSynchronize(st));
         moments_pending = false;
@@ -917,7 +918,14 @@ struct Engine {
         bt_base.reserve(np);
         btcap.reserve(np);
         TermPool IN = in.view();
-        fill_multi({{counters.ptr, 16 * sizeof(int), 0}, {slot_base.ptr, sizeof(int), 0}});
+        // counters: [5] root count, [8] enumeration guard, [16] parents to enumerate, [17] MU
+        // errors, [18] groups to sort, [19] rare-path flags; the shape histograms of the gather
+        // (hist) and of the compaction (hist_mx, with the largest g-table after it): all
+        // cleared here, once per update
+        fill_multi({{counters.ptr, 32 * sizeof(int), 0},
+                    {slot_base.ptr, sizeof(int), 0},
+                    {hist.ptr, (sr + 1) * sizeof(int), 0},
+                    {hist_mx.ptr, (sr + 2) * sizeof(int), 0}});
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
                                              din_off.ptr, dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
@@ -930,11 +938,10 @@ struct Engine {
             scan_i64_excl(btcap64.ptr, bt_base.ptr, np);
             barena.reserve((size_t)read_total(bt_base.ptr, btcap.ptr, np) + 1);
             enum_list.reserve(np);
-            HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
             hipLaunchKernelGGL(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
-                               gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr);
+                               gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr + 16);
             HIPCHK(hipGetLastError());
-            const int nenum = read1(counters.ptr);
+            const int nenum = read1(counters.ptr + 16);
             if (nenum > 0) {
                 // 2^MM-bit candidate set plus the 2^(MM-1)-bit parent key set
                 const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
@@ -964,22 +971,21 @@ struct Engine {
         chain_off.reserve(16);
         mom_part.reserve((size_t)T * (1 + d + d * d) * 2);
         TermPool OUT = out.view();
-        HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
         // child rows in registers when the largest shape fits a compile-time bound
         const int mu_km = MM <= 8 ? 8 : MM <= 14 ? 14 : 0;
         if (mu_km == 8) {
             MCE_DISPATCH_D(d, hipLaunchKernelGGL((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
                                                  np, slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr,
-                                                 gv[gcur].ptr, gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 1));
+                                                 gv[gcur].ptr, gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         } else if (mu_km == 14 && d <= 5) {
             MCE_DISPATCH_D(d, if constexpr (DD <= 5) hipLaunchKernelGGL(
                                   (k_mu_r<DD, 14>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                   slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr, keys.ptr,
-                                  chain_key.ptr, counters.ptr + 1));
+                                  chain_key.ptr, counters.ptr + 17));
         } else {
             MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                                  slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr,
-                                                 gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 1));
+                                                 gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         }
         HIPCHK(hipGetLastError());
         prof.mark(kPhMu, st);
@@ -1013,7 +1019,6 @@ struct Engine {
         if (first || skip) collect_moments(first);
 
         if (skip) {
-            HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
             hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> h(sr + 1);
@@ -1037,7 +1042,6 @@ struct Engine {
                 return hipcub::DeviceRadixSort::SortPairs(t, b, keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0,
                                                           32, st);
             });
-            HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
             hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> cnt(sr + 1);
@@ -1163,7 +1167,6 @@ struct Engine {
         surv.reserve(ng);
         gxor.reserve(ng);
         sort_list.reserve(ng);
-        HIPCHK(hipMemsetAsync(counters.ptr, 0, 8 * sizeof(int), st));
         if (prof.on) {
             if (!prof.k0) {
                 HIPCHK(hipEventCreate(&prof.k0));
@@ -1218,7 +1221,7 @@ struct Engine {
         hipLaunchKernelGGL((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
                            grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
                            b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
-                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 2, counters.ptr + 3,          \
+                           gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 18, counters.ptr + 19,          \
                            big_members, defer_k0.ptr);                                                            \
     } while (0)
             // rows held in registers: the smallest compile-time bound >= the largest shape
@@ -1264,7 +1267,7 @@ struct Engine {
         }
         if (ge_mode >= kLookupRank) big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
         int nr_[2];
-        readn(counters.ptr + 2, nr_, 2);
+        readn(counters.ptr + 18, nr_, 2);
         const int nsort = nr_[0];
         const int rare = nr_[1];
         // bits 1 and 2 note parent (old) terms among a group's members, which stage_term
@@ -1293,14 +1296,11 @@ struct Engine {
         MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
                                              surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemsetAsync(hist.ptr, 0, (sr + 1) * sizeof(int), st));
         hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
-                           (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist.ptr);
+                           (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist_mx.ptr);
         std::vector<int> h(sr + 2);
         {
             // histogram and the largest g-table (the next update's parent tables) in one read
-            hist_mx.reserve(sr + 2);
-            HIPCHK(hipMemcpyAsync(hist_mx.ptr, hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
             cub([&](void* t_, size_t& b_) {
                 return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, hist_mx.ptr + sr + 1, ng, st);
             });
@@ -1352,7 +1352,7 @@ struct Engine {
         hipLaunchKernelGGL((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
                            ni, big_coff.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr,    \
                            b2off.ptr, b2n.ptr, gk[gcur].ptr, gv[gcur].ptr, gscale, surv.ptr, gxor.ptr,             \
-                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 3);                                 \
+                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19);                                 \
     } while (0)
         if (mode == kLookupRankVals && MM <= 8) MCE_GMEM(kLookupRankVals, 8);
         else if (mode == kLookupRankVals && MM <= 11) MCE_GMEM(kLookupRankVals, 11);
