@@ -165,3 +165,11 @@ def test_swig_interface_covers_every_wrapped_argument():
                 k += 1
             else:
                 raise AssertionError(f"{f}: argument '{ps[k]}' has no typemap")
+
+
+def test_search_ordering_refuses_more_than_12_states():
+    """pycauchy_set_tr_search_idxs_ordering returns 1, as the reference's does, beyond 12 states
+    (host code only)."""
+    import numpy as np
+    from cauchyfriendly_amd import pycauchy
+    assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.arange(13, dtype=np.int32)) == 1
