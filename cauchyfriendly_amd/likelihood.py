@@ -117,66 +117,96 @@ def log_likelihoods(systems, thetas, num_threads=16):
     return run_likelihoods(systems, make, num_threads)
 
 
-def _central(L, structure, n, P, h):
-    stride = 1 + 2 * P
+SHRINK = (1e-7, 1e-8, 1e-9)
+
+
+def _fd_gradient(systems, base, make_for, h, shrink, num_threads, max_live):
+    """Central differences of L at base[i] (a parameter vector of systems[i]) over every
+    component.  A component whose term structure differs at base +/- h from base's is run
+    again at each step of `shrink` in turn, until the structure holds; it stays invalid
+    otherwise.  Returns (L0, grad, valid, steps, terms generated, estimator runs)."""
+    n = len(systems)
+    P = len(base[0]) if n else 0
+
+    def run(jobs):
+        Ls, sts = run_likelihoods([systems[i] for i, _, _ in jobs], lambda k: make_for(systems[jobs[k][0]], jobs[k][2]),
+                                  num_threads, max_live)
+        return Ls, sts, sum(c[0] for st in sts for c in st)
+
+    jobs = []
+    for i in range(n):
+        jobs.append((i, -1, base[i]))
+        for j in range(P):
+            for sg in (1.0, -1.0):
+                t = base[i].copy()
+                t[j] += sg * h
+                jobs.append((i, j, t))
+    L, st, terms = run(jobs)
+    runs = len(jobs)
     L0 = np.zeros(n)
+    S0 = [None] * n
     grad = np.zeros((n, P))
     valid = np.zeros((n, P), dtype=bool)
+    steps = np.full((n, P), h)
+    stride = 1 + 2 * P
     for i in range(n):
         b = i * stride
-        L0[i] = L[b]
+        L0[i], S0[i] = L[b], st[b]
         for j in range(P):
             lp, lm = b + 1 + 2 * j, b + 2 + 2 * j
             grad[i, j] = (L[lp] - L[lm]) / (2 * h)
-            valid[i, j] = structure[lp] == structure[lm] == structure[b]
+            valid[i, j] = st[lp] == st[lm] == S0[i]
+    for h2 in shrink:
+        todo = [(i, j) for i in range(n) for j in range(P) if not valid[i, j]]
+        if not todo:
+            break
+        jobs = []
+        for i, j in todo:
+            for sg in (1.0, -1.0):
+                t = base[i].copy()
+                t[j] += sg * h2
+                jobs.append((i, j, t))
+        L2, st2, t2 = run(jobs)
+        terms += t2
+        runs += len(jobs)
+        for q, (i, j) in enumerate(todo):
+            if st2[2 * q] == st2[2 * q + 1] == S0[i]:
+                grad[i, j] = (L2[2 * q] - L2[2 * q + 1]) / (2 * h2)
+                valid[i, j] = True
+                steps[i, j] = h2
+    return L0, grad, valid, steps, terms, runs
+
+
+def scale_gradient(systems, h=1e-6, num_threads=16, max_live=None, shrink=SHRINK, details=False):
+    """Central-difference gradient of L for every system over its Cauchy scales
+    (beta_1 .. beta_pncc, gamma_1 .. gamma_p; the systems must share pncc and p), with the
+    step shrunk where the term structure does not hold.  Returns (L, grad, valid), or with
+    details a dict that adds the step of each component and the terms generated."""
+    base = [np.concatenate([np.ravel(s.beta), np.ravel(s.gamma)]).astype(np.float64) for s in systems]
+    if len({b.size for b in base}) > 1:
+        raise ValueError("the systems must share the number of noise scales")
+    L0, grad, valid, steps, terms, runs = _fd_gradient(
+        systems, base, lambda s, t: make_estimator(s, t[:s.pncc], t[s.pncc:]), h, shrink, num_threads, max_live)
+    if details:
+        return {"L": L0, "grad": grad, "valid": valid, "h": steps, "terms": terms, "runs": runs}
     return L0, grad, valid
 
 
-def scale_gradient(systems, h=1e-6, num_threads=16, max_live=None, with_terms=False):
-    """Central-difference gradient of L for every system over its Cauchy scales
-    (beta_1 .. beta_pncc, gamma_1 .. gamma_p; the systems must share pncc and p).
-    Returns (L, grad, valid), and with_terms the terms generated by all the runs."""
-    sys_all, scales = [], []
-    P = None
-    for s in systems:
-        sc = np.concatenate([np.ravel(s.beta), np.ravel(s.gamma)]).astype(np.float64)
-        if P is None:
-            P = sc.size
-        elif sc.size != P:
-            raise ValueError("the systems must share the number of noise scales")
-        sys_all.append(s)
-        scales.append(sc)
-        for j in range(P):
-            for sign in (1.0, -1.0):
-                t = sc.copy()
-                t[j] += sign * h
-                sys_all.append(s)
-                scales.append(t)
-
-    def make(i):
-        s = sys_all[i]
-        return make_estimator(s, scales[i][:s.pncc], scales[i][s.pncc:])
-    L, structure = run_likelihoods(sys_all, make, num_threads, max_live)
-    out = _central(L, structure, len(systems), P or 0, h)
-    if with_terms:
-        return out + (sum(c[0] for st in structure for c in st),)
-    return out
-
-
-def loglik_gradient(systems, h=1e-6, params=None, num_threads=16):
-    """Central-difference gradient of L for every system, over the parameters `params`
-    (indices into PARAM_NAMES; all 14 by default).  Returns (L, grad, valid)."""
+def loglik_gradient(systems, h=1e-6, params=None, num_threads=16, shrink=SHRINK, details=False):
+    """Central-difference gradient of L for every single-noise LTI system over the
+    parameters `params` (indices into PARAM_NAMES; all 14 by default), with the step shrunk
+    where the term structure does not hold.  Returns (L, grad, valid), or with details a
+    dict that adds the step of each component and the terms generated."""
     P = list(range(len(PARAM_NAMES))) if params is None else list(params)
-    sys_all, th_all = [], []
-    for s in systems:
+    full = [theta_of(s) for s in systems]
+
+    def make_for(s, sub):
         th = theta_of(s)
-        sys_all.append(s)
-        th_all.append(th)
-        for p in P:
-            for sign in (1.0, -1.0):
-                t = th.copy()
-                t[p] += sign * h
-                sys_all.append(s)
-                th_all.append(t)
-    L, structure = log_likelihoods(sys_all, th_all, num_threads)
-    return _central(L, structure, len(systems), len(P), h)
+        th[P] = sub
+        beta, gamma, Phi, H = _unpack(th, s.n)
+        return make_estimator(s, beta, gamma, Phi, H)
+    L0, grad, valid, steps, terms, runs = _fd_gradient(systems, [f[P].copy() for f in full], make_for, h, shrink,
+                                                       num_threads, None)
+    if details:
+        return {"L": L0, "grad": grad, "valid": valid, "h": steps, "terms": terms, "runs": runs}
+    return L0, grad, valid
