@@ -1548,6 +1548,7 @@ struct Engine {
     DBuf<int> c2_mc, c2_err;
     DBuf<double> c2_b2, c2_sn, c2_cs, c2_g1r, c2_g2r, c2_px, c2_py;
     DBuf<cplx> c2_g;
+    DBuf<Cell2D> c2_fc;
 
     // Extended-mode finish of a measurement update (cauchy_estimator.hpp:1358-1394): the
     // terms' b-vectors move by -Re(mean) (skipped on a window's last update), the mean
@@ -1701,6 +1702,7 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
         c2_g1r.reserve(cells);
         c2_g2r.reserve(cells);
         c2_g.reserve(cells);
+        c2_fc.reserve(cells);
         c2_err.reserve(1);
         c2_px.reserve(npts);
         c2_py.reserve(npts);
@@ -1720,6 +1722,7 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
         C.g1r = c2_g1r.ptr;
         C.g2r = c2_g2r.ptr;
         C.gc = c2_g.ptr;
+        C.fc = c2_fc.ptr;
         VecArg bn;
         for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
         TermPool IN = in.view();
@@ -1791,7 +1794,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     const int64_t npairs = (int64_t)np * ns;
     const int CM = P.MM + S.MM;
     if (CM > kMaxM) throw std::runtime_error("the relative arrangement would exceed 31 hyperplanes");
-    const double bytes = (double)npairs * (CM * 40.0 + 8.0 * (CM + 1) * 2 + 14 * 8 + 32);
+    const double bytes = (double)npairs * (CM * 72.0 + 8.0 * (CM + 1) * 2 + 14 * 8 + 32);
     if (bytes > 64.0 * (1ull << 30))
         throw std::runtime_error("the relative system's pairs need more than 64 GiB; use estimators with fewer terms");
     GraveyardScope gs(&P.graveyard);
@@ -1804,6 +1807,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     P.c2_g1r.reserve(npairs * CM);
     P.c2_g2r.reserve(npairs * CM);
     P.c2_g.reserve(npairs * CM);
+    P.c2_fc.reserve(npairs * CM);
     P.rs_mom.reserve(npairs * 14);
     P.rs_keep.reserve(npairs);
     P.rs_idx.reserve(npairs);
@@ -1820,6 +1824,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     C.g1r = P.c2_g1r.ptr;
     C.g2r = P.c2_g2r.ptr;
     C.gc = P.c2_g.ptr;
+    C.fc = P.c2_fc.ptr;
     TermPool PI = P.in.view(), SI = S.in.view();
     if (npairs > 0) {
         const unsigned blocks = (unsigned)((npairs + 127) / 128);

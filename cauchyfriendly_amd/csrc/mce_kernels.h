@@ -2560,7 +2560,19 @@ __global__ void __launch_bounds__(kCpdfPoints) k_cpdf1d_grid(const CpdfTerm1D* _
     if (k >= npts) return;
     const double x = xs[k];
     double acc = 0.0;
-    for (int j = 0; j < nt; j++) {
+    // terms in pairs, n0 / d0 + n1 / d1 = (n0 d1 + n1 d0) / (d0 d1): one division per two
+    // terms where d0 d1 (each d = w^2 + (x - s)^2 > 0) stays in range, two otherwise
+    int j = 0;
+    for (; j + 1 < nt; j += 2) {
+        const CpdfTerm1D c0 = sh[j], c1 = sh[j + 1];
+        const double x0 = x - c0.s, x1 = x - c1.s;
+        const double d0 = c0.w2 + x0 * x0, d1 = c1.w2 + x1 * x1;
+        const double n0 = c0.a * x + c0.bb, n1 = c1.a * x + c1.bb;
+        const double dd = d0 * d1;
+        if (dd > 1e-280 && dd < 1e280) acc += (n0 * d1 + n1 * d0) / dd;
+        else acc += n0 / d0 + n1 / d1;
+    }
+    if (j < nt) {
         const CpdfTerm1D c = sh[j];
         double xm = x - c.s;
         xm *= xm;
@@ -2596,6 +2608,13 @@ __global__ void k_sub_b(TermPool P, int n, VecArg dx) {
 // (marg2d_remove_zeros_and_coalign), the cell walls' angles sorted over a full turn,
 // and for each of the m' cells of a half turn its g numerator and the real parts of
 // gamma1, gamma2.  Arrays have a stride of MM cells per term (MM + 1 angles).
+// The point-independent parts of a cell's integral sin(t2 - t1) g / (D1 D2), with
+// Dk = gamma1 cos tk + gamma2 sin tk: Re D1, Re D2 and the numerator.  Only Im gamma1 and
+// Im gamma2 depend on the grid point.
+struct Cell2D {
+    double d1r, d2r, nre, nim;
+};
+
 struct Cpdf2D {
     int MM;
     int* mc;           // cells per term
@@ -2605,7 +2624,13 @@ struct Cpdf2D {
     double* g1r;       // Re gamma1 per cell
     double* g2r;       // Re gamma2 per cell
     cplx* gc;          // g numerator per cell
+    Cell2D* fc;        // point-independent parts per cell
 };
+
+__device__ inline Cell2D cell2d_parts(double s1, double c1, double s2, double c2, double g1re, double g2re, cplx g) {
+    const double sd = s2 * c1 - s1 * c2;
+    return Cell2D{g1re * c1 + g2re * s1, g1re * c2 + g2re * s2, g.re * sd, g.im * sd};
+}
 
 constexpr double kIntegralGammaEps = 1e-8;  // INTEGRAL_GAMMA_EPS, cauchy_constants.hpp:76
 constexpr int kZeroHpMarker = 32;           // ZERO_HP_MARKER_VALUE, cpdf_ndim.hpp:371
@@ -2773,6 +2798,7 @@ __global__ void k_cpdf2d_terms(TermPool P, int n, int i1, int i2, VecArg bar_nu,
         C.g1r[co + c] = g1re;
         C.g2r[co + c] = g2re;
         C.gc[co + c] = g;
+        C.fc[co + c] = cell2d_parts(s1, c1, s2, c2, g1re, g2re, g);
         s1 = s2;
         c1 = c2;
     }
@@ -2810,9 +2836,22 @@ __global__ void __launch_bounds__(kCpdfPoints) k_cpdf2d_grid(Cpdf2D C, int n, co
         const double* sn = C.sn + co + t;
         const double* cs = C.cs + co + t;
         double term = 0.0;
-        for (int c = 0; c < u; c++)
-            term += cpdf2d_cell(sn[c], cs[c], sn[c + 1], cs[c + 1], C.g1r[co + c], C.g2r[co + c], g1im, g2im,
-                                C.gc[co + c], check_g1, singular);
+        for (int c = 0; c < u; c++) {
+            const double s1 = sn[c], c1 = cs[c], s2 = sn[c + 1], c2 = cs[c + 1];
+            if (check_g1 && fabs(C.g1r[co + c]) < kIntegralGammaEps) {
+                term += cpdf2d_cell(s1, c1, s2, c2, C.g1r[co + c], C.g2r[co + c], g1im, g2im, C.gc[co + c], true,
+                                    singular);
+                continue;
+            }
+            // Re(n / (D1 D2)) from the cell's point-independent parts: one real division where
+            // |D1 D2|^2 stays in range, the complex quotient otherwise
+            const Cell2D f = C.fc[co + c];
+            const double d1i = g1im * c1 + g2im * s1, d2i = g1im * c2 + g2im * s2;
+            const cplx den = cmake(f.d1r * f.d2r - d1i * d2i, f.d1r * d2i + d1i * f.d2r);
+            const double n2 = den.re * den.re + den.im * den.im;
+            if (n2 > 1e-280 && n2 < 1e280) term += (f.nre * den.re + f.nim * den.im) / n2;
+            else term += cdiv(cmake(f.nre, f.nim), den).re;
+        }
         acc += term;
     }
     part[(int64_t)chunk * npts + k] = acc;
@@ -3034,6 +3073,7 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         C.g1r[co + c] = g1re;
         C.g2r[co + c] = g2re;
         C.gc[co + c] = g;
+        C.fc[co + c] = cell2d_parts(s1, c1, s2, c2, g1re, g2re, g);
         // the approximation's bound on the cell's contribution
         const double den1 = g1re * (g1re * c1 + g2re * s1);
         const double den2 = g1re * (g1re * c2 + g2re * s2);
