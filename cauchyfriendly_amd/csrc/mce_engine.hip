@@ -369,11 +369,16 @@ constexpr size_t kStageBytes = 4096;
 struct ThreadStreams {
     hipStream_t st = nullptr, st2 = nullptr;
     uint8_t* stage = nullptr;
+    // a pinned, device-mapped slot that kernels write scan totals into directly
+    int64_t* total = nullptr;
+    int64_t* total_dev = nullptr;
     void init() {
         if (st) return;
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHK(hipHostMalloc((void**)&stage, kStageBytes, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void**)&total, 64, hipHostMallocMapped));
+        HIPCHK(hipHostGetDevicePointer((void**)&total_dev, total, 0));
     }
 };
 thread_local ThreadStreams t_streams;
@@ -444,6 +449,8 @@ struct Engine {
     DBuf<unsigned long long> bytes_acc;
     DBuf<int> pstat;
     uint8_t* h_stage = nullptr;  // pinned staging for device-to-host scalar reads (the thread's)
+    int64_t* h_total = nullptr;      // the thread's mapped total slot, host and device views
+    int64_t* h_total_dev = nullptr;
 
     // Takes the calling thread's streams and staging buffer; every entry point starts here.
     void bind() {
@@ -451,6 +458,8 @@ struct Engine {
         st = t_streams.st;
         st2 = t_streams.st2;
         h_stage = t_streams.stage;
+        h_total = t_streams.total;
+        h_total_dev = t_streams.total_dev;
     }
     DBuf<int64_t> d_scal;        // device scalars (scan totals)
     DBuf<int> defer_k0, big_icnt, big_ioff, big_hcnt, big_hoff, hist_mx;
@@ -643,10 +652,18 @@ struct Engine {
     void enqueue_total(const A* off, const B* cnt, int n, int slot) {
         hipLaunchKernelGGL((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
     }
+    // off[n-1] + cnt[n-1] read by the host: the kernel writes it into the mapped host slot (and
+    // d_scal[0]), so no copy is launched.  The slot holds -1 until the kernel's value arrives;
+    // should it not have arrived when the stream is done, d_scal[0] is copied as before.
     template <class A, class B>
     int64_t read_total(const A* off, const B* cnt, int n) {
-        enqueue_total(off, cnt, n, 0);
-        return read1(d_scal.ptr);
+        volatile int64_t* slot = h_total;
+        *slot = -1;
+        hipLaunchKernelGGL((k_scan_total_mapped<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr, h_total_dev);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        const int64_t v = *slot;
+        return v >= 0 ? v : read1(d_scal.ptr);
     }
 
     // Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.

@@ -2433,6 +2433,15 @@ __global__ void k_scan_total(const A* __restrict__ off, const B* __restrict__ cn
     out[0] = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
 }
 
+// The same total, also stored into a device-mapped host slot.
+template <class A, class B>
+__global__ void k_scan_total_mapped(const A* __restrict__ off, const B* __restrict__ cnt, int n,
+                                    int64_t* __restrict__ out, int64_t* host_out) {
+    const int64_t v = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
+    out[0] = v;
+    *(volatile int64_t*)host_out = v;
+}
+
 __global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (int64_t)src[i];
