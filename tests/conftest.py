@@ -18,8 +18,8 @@ def pytest_configure(config):
 
 
 # fixtures of other record types (window bank, window re-initialisation, extended mode, densities,
-# relative systems)
-NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_", "rsys_")
+# relative systems, deterministic transforms)
+NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_", "rsys_", "transform_")
 
 
 def golden_names():
