@@ -2,7 +2,7 @@
 
 Run in the container that holds the reference tree (after `make -C oracle ref`):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture name ...]
 
 For every system it records the inputs and, per scalar measurement update, the
 reference estimator's outputs: fz before term reduction, the public fz, x_hat and
@@ -74,10 +74,14 @@ CASES = [
     (lambda: W.lti_system(d=3, steps=9, seed_offset=10, name="lti3_w9"), None, ()),
     # the first of the staggered 7-state orbit windows, 4 of its 6 steps (8 updates)
     (lambda: W.leo7_windows(count=1)[0], 4, ()),
+    # the largest state dimensions the engine takes (6 and its maximum, 8)
+    (lambda: W.lti_system(d=6, steps=6, seed_offset=106, name="lti6_w6"), None, (1,)),
+    (lambda: W.lti_system(d=8, steps=5, seed_offset=108, name="lti8_w5"), None, (0,)),
 ]
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or only those named on the command line."""
     if not dr.available("ref"):
         raise SystemExit("build the reference first: make -C oracle ref")
     systems = [(f, s, t) for f, s, t in CASES]
@@ -87,6 +91,8 @@ def main():
         systems.append((lambda sy=sy: sy, None, (2,) if sy.name == "ref_3state_3msmts_horth" else ()))
     for factory, steps, tu in systems:
         sysobj = factory()
+        if only and sysobj.name not in only:
+            continue
         rec = record(sysobj, steps, tu)
         path = os.path.join(HERE, f"{sysobj.name}.json")
         with open(path, "w") as fh:
@@ -96,4 +102,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))
