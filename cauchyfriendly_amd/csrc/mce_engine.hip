@@ -403,6 +403,12 @@ struct Engine {
     std::vector<cplx> mean, cov, last_mean, last_cov;
     double gscale = 0;
     std::vector<int> in_count;  // per-shape term counts of the parent pool
+    // largest hyperplane count among the parent pool's terms (0 when it is empty)
+    int largest_shape() const {
+        for (int m = (int)in_count.size() - 1; m > 0; m--)
+            if (in_count[m] > 0) return m;
+        return 0;
+    }
     // diagnostics of the last update
     bool have_diag = false;
     std::vector<double> diag_mom;
@@ -1857,9 +1863,22 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     TermPool PI = P.in.view(), SI = S.in.view();
     if (npairs > 0) {
         const unsigned blocks = (unsigned)((npairs + 127) / 128);
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_rsys_pairs<DD>, dim3(blocks), dim3(128), 0, st, PI, np,
-                                             P.gk[P.gcur].ptr, P.gv[P.gcur].ptr, SI, ns, S.gk[S.gcur].ptr,
-                                             S.gv[S.gcur].ptr, R, C, P.rs_mom.ptr, P.rs_keep.ptr));
+        // per-lane arrays sized by the pairs' largest row count: the smaller the bound,
+        // the smaller each lane's private memory
+        const int KM = P.largest_shape() + S.largest_shape();
+        HIPCHK(hipMemsetAsync(P.c2_err.ptr, 0, sizeof(int), st));
+#define MCE_RSYS(K_)                                                                                             \
+    MCE_DISPATCH_D(d, hipLaunchKernelGGL((k_rsys_pairs<DD, K_>), dim3(blocks), dim3(128), 0, st, PI, np,         \
+                                         P.gk[P.gcur].ptr, P.gv[P.gcur].ptr, SI, ns, S.gk[S.gcur].ptr,           \
+                                         S.gv[S.gcur].ptr, R, C, P.rs_mom.ptr, P.rs_keep.ptr, P.c2_err.ptr))
+        if (KM <= 16) {
+            MCE_RSYS(16);
+        } else if (KM <= 24) {
+            MCE_RSYS(24);
+        } else {
+            MCE_RSYS(kMaxM);
+        }
+#undef MCE_RSYS
         hipLaunchKernelGGL(k_colsum_part, dim3(kColSlices, 14), dim3(256), 0, st, P.rs_mom.ptr, npairs,
                            P.rs_part.ptr);
         hipLaunchKernelGGL(k_colsum_finish, dim3(1), dim3(64), 0, st, P.rs_part.ptr, 14, P.rs_sums.ptr);
@@ -1870,12 +1889,14 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         HIPCHK(hipGetLastError());
     }
     double sums[14] = {0};
-    int nkeep = 0;
+    int nkeep = 0, pair_err = 0;
     if (npairs > 0) {
         HIPCHK(hipMemcpyAsync(sums, P.rs_sums.ptr, sizeof(sums), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&nkeep, P.rs_cnt.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&pair_err, P.c2_err.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    if (pair_err) throw std::runtime_error("a relative term has more rows than the pair kernel was launched for");
     // finalize_rel_sys_moments (cauchy_prediction.hpp:288-304)
     nf = cmake(sums[0], sums[1]);
     const cplx ifz = cmake(0.0, nf.re);

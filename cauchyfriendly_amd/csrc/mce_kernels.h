@@ -2886,20 +2886,29 @@ struct RsysArgs {
 
 __device__ inline double sgn_d(double x) { return (double)((x > 0) - (x < 0)); }
 
-template <int D>
+template <int D, int K>
 __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pkeys, const cplx* __restrict__ pvals,
                              TermPool S, int ns, const uint32_t* __restrict__ skeys, const cplx* __restrict__ svals,
-                             RsysArgs R, Cpdf2D C, double* __restrict__ mom, uint8_t* __restrict__ keep) {
+                             RsysArgs R, Cpdf2D C, double* __restrict__ mom, uint8_t* __restrict__ keep,
+                             int* __restrict__ err) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)np * ns) return;
     const int tp = (int)(q / ns), ts = (int)(q % ns);
     const int mp = P.m[tp], ms = S.m[ts];
     const int m = mp + ms;
+    // K bounds the rows of every pair (the two pools' largest shapes); a pair above it
+    // is reported, never evaluated
+    if (m > K) {
+        atomicOr(err, 2);
+        keep[q] = 0;
+        return;
+    }
     const double* Ap = P.A + (int64_t)tp * P.MM * D;
     const double* As = S.A + (int64_t)ts * S.MM * D;
-    double a2[2 * kMaxM], p2[kMaxM], wa2[2 * kMaxM], wp2[kMaxM], th[2 * kMaxM];
-    int cmap[kMaxM], csm[kMaxM], fidx[kMaxM];
-    bool F[kMaxM];
+    // per-lane arrays, sized by K: the merged rows are compacted into a2 / p2 in place
+    double a2[2 * K], p2[K], th[2 * K];
+    int8_t cmap[K], csm[K], fidx[K];
+    bool F[K];
     // A_2d = A_rel Trel^T (matmatmul with the second operand transposed), p_2d = p_rel
     for (int i = 0; i < m; i++) {
         const bool pr = i < mp;
@@ -2936,57 +2945,20 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
             F[r] = true;
         }
     }
-    int u = 0;
-    for (int r = 0; r < m; r++) {
-        if (!F[r]) continue;
-        cmap[r] = u;
-        csm[r] = 1;
-        fidx[r] = r;
-        wp2[u] = p2[r];
-        for (int j = r + 1; j < m; j++) {
-            if (!F[j]) continue;
-            const bool pos = (fabs(a2[2 * r] - a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] - a2[2 * j + 1]) < kCoalignEps);
-            if (pos) {
-                cmap[j] = u;
-                csm[j] = 1;
-                F[j] = false;
-                fidx[j] = r;
-                wp2[u] += p2[j];
-                continue;
-            }
-            const bool neg = (fabs(a2[2 * r] + a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] + a2[2 * j + 1]) < kCoalignEps);
-            if (neg) {
-                cmap[j] = u;
-                csm[j] = -1;
-                F[j] = false;
-                fidx[j] = r;
-                wp2[u] += p2[j];
-            }
-        }
-        u++;
-    }
-    {
-        int k = 0;
-        for (int r = 0; r < m; r++)
-            if (fidx[r] == r) {
-                wa2[2 * k] = a2[2 * r];
-                wa2[2 * k + 1] = a2[2 * r + 1];
-                k++;
-            }
-    }
     const uint32_t* pk = pkeys + P.gp_off[tp];
     const cplx* pv = pvals + P.gp_off[tp];
     const int pn = P.gp_n[tp];
     const uint32_t* sk = skeys + S.gp_off[ts];
     const cplx* sv_ = svals + S.gp_off[ts];
     const int sn_ = S.gp_n[ts];
-    // moment contribution: the cell of bar_nu (Trel bar_nu in 2D, bar_nu on zero rows)
+    // moment contribution: the cell of bar_nu (Trel bar_nu in 2D, bar_nu on zero rows);
+    // taken before the merge, which leaves a2 and p2 as they are until the compaction
     {
         double ty0 = 0.0, ty1 = 0.0;
         int encp = 0, encs = 0;
         for (int i = 0; i < m; i++) {
             double ds;
-            if (cmap[i] != kZeroHpMarker) {
+            if (F[i]) {
                 ds = a2[2 * i] * R.nu2[0] + a2[2 * i + 1] * R.nu2[1];
             } else {
                 const bool pr = i < mp;
@@ -3016,9 +2988,48 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
 #pragma unroll
         for (int c = 0; c < 14; c++) mom[c * npq + q] = v[c];
     }
+    int u = 0;
+    for (int r = 0; r < m; r++) {
+        if (!F[r]) continue;
+        cmap[r] = u;
+        csm[r] = 1;
+        fidx[r] = r;
+        double w = p2[r];
+        for (int j = r + 1; j < m; j++) {
+            if (!F[j]) continue;
+            const bool pos = (fabs(a2[2 * r] - a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] - a2[2 * j + 1]) < kCoalignEps);
+            if (pos) {
+                cmap[j] = u;
+                csm[j] = 1;
+                F[j] = false;
+                fidx[j] = r;
+                w += p2[j];
+                continue;
+            }
+            const bool neg = (fabs(a2[2 * r] + a2[2 * j]) < kCoalignEps) && (fabs(a2[2 * r + 1] + a2[2 * j + 1]) < kCoalignEps);
+            if (neg) {
+                cmap[j] = u;
+                csm[j] = -1;
+                F[j] = false;
+                fidx[j] = r;
+                w += p2[j];
+            }
+        }
+        p2[u] = w;  // u <= r: no later read of the merge touches index u
+        u++;
+    }
+    {
+        int k = 0;
+        for (int r = 0; r < m; r++)
+            if (fidx[r] == r) {  // k <= r
+                a2[2 * k] = a2[2 * r];
+                a2[2 * k + 1] = a2[2 * r + 1];
+                k++;
+            }
+    }
     // cell data over a half turn
     for (int r = 0; r < u; r++) {
-        const double x0 = wa2[2 * r], x1 = wa2[2 * r + 1];
+        const double x0 = a2[2 * r], x1 = a2[2 * r + 1];
         double px, py;
         if (fabs(x0) < fabs(x1)) {
             px = 1.0;
@@ -3051,7 +3062,7 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         const double ptx = cos(t), pty = sin(t);
         uint32_t negb = 0;
         for (int j = 0; j < u; j++)
-            if (!(wa2[2 * j] * ptx + wa2[2 * j + 1] * pty > 0)) negb |= 1u << j;
+            if (!(a2[2 * j] * ptx + a2[2 * j + 1] * pty > 0)) negb |= 1u << j;
         int enc = 0;
         for (int j = 0; j < m; j++) {
             if (cmap[j] == kZeroHpMarker) {
@@ -3072,8 +3083,8 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         double g1re = 0.0, g2re = 0.0;
         for (int j = 0; j < u; j++) {
             const double svd = ((negb >> j) & 1u) ? -1.0 : 1.0;
-            g1re -= (wa2[2 * j] * wp2[j]) * svd;
-            g2re -= (wa2[2 * j + 1] * wp2[j]) * svd;
+            g1re -= (a2[2 * j] * p2[j]) * svd;
+            g2re -= (a2[2 * j + 1] * p2[j]) * svd;
         }
         const double s1 = sin(th[c]), c1 = cos(th[c]);
         const double s2 = sin(th[c + 1]), c2 = cos(th[c + 1]);
