@@ -3027,7 +3027,8 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
                 k++;
             }
     }
-    // cell data over a half turn
+    // cell data over a half turn; rowof[i] is the compacted row whose line lies at th[i]
+    int8_t rowof[2 * K];
     for (int r = 0; r < u; r++) {
         const double x0 = a2[2 * r], x1 = a2[2 * r + 1];
         double px, py;
@@ -3042,52 +3043,88 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         if (t1 < 0) t1 += kPi;
         th[2 * r] = t1;
         th[2 * r + 1] = t1 + kPi;
+        rowof[2 * r] = rowof[2 * r + 1] = (int8_t)r;
     }
     for (int a = 1; a < 2 * u; a++) {
         const double v = th[a];
+        const int8_t rv = rowof[a];
         int b = a - 1;
         while (b >= 0 && th[b] > v) {
             th[b + 1] = th[b];
+            rowof[b + 1] = rowof[b];
             b--;
         }
         th[b + 1] = v;
+        rowof[b + 1] = rv;
+    }
+    // The key bits of the original rows as a function of the compacted rows' sides: a zero
+    // row's bit is fixed by bar_nu; row j on compacted row k = cmap[j] has its bit set when
+    // k's side is negative and csm[j] = 1, or positive and csm[j] = -1.  With every side
+    // positive the bits are enc_fix | enc_pos; turning row k negative flips the bits hit[k].
+    uint32_t hit[K];
+    for (int k = 0; k < u; k++) hit[k] = 0;
+    int enc_fix = 0, enc_pos = 0;
+    for (int j = 0; j < m; j++) {
+        if (cmap[j] == kZeroHpMarker) {
+            const bool pr = j < mp;
+            const double* ar = pr ? Ap + j * D : As + (j - mp) * D;
+            const double sg = pr ? -1.0 : 1.0;
+            double ip = 0.0;
+            for (int k = 0; k < D; k++) ip += (ar[k] * sg) * R.nu[k];
+            if (ip < 0) enc_fix |= 1 << j;
+        } else {
+            hit[cmap[j]] |= 1u << j;
+            if (csm[j] < 0) enc_pos |= 1 << j;
+        }
+    }
+    // sides at the first cell's midpoint, then one crossing per cell: cell c lies between
+    // th[c] and th[c + 1], so moving into it crosses the line at th[c] and flips its row.
+    // Rows closer in angle than the merge tolerance were merged, so every midpoint is far
+    // from every line compared with the rounding of its dot products, and the flips give
+    // the sides a dot product at each midpoint gives.
+    uint32_t negb = 0;
+    int encv = 0;
+    if (u > 0) {
+        const double t = (th[1] + th[0]) / 2.0;
+        const double ptx = cos(t), pty = sin(t);
+        for (int j = 0; j < u; j++)
+            if (!(a2[2 * j] * ptx + a2[2 * j + 1] * pty > 0)) {
+                negb |= 1u << j;
+                encv ^= (int)hit[j];
+            }
+    }
+    // the g-integral terms a_j p_j (the products the sums below take, formed once)
+    for (int j = 0; j < u; j++) {
+        a2[2 * j] *= p2[j];
+        a2[2 * j + 1] *= p2[j];
     }
     const int MM = C.MM;
     const int64_t co = q * MM;
     const int mpmask = (1 << mp) - 1;
     double total = 0.0;
     bool keep_term = false;
+    double s_hi = 0.0, c_hi = 0.0;  // sin / cos of the previous cell's upper edge
     for (int c = 0; c < u; c++) {
-        const double t = (th[c + 1] + th[c]) / 2.0;
-        const double ptx = cos(t), pty = sin(t);
-        uint32_t negb = 0;
-        for (int j = 0; j < u; j++)
-            if (!(a2[2 * j] * ptx + a2[2 * j + 1] * pty > 0)) negb |= 1u << j;
-        int enc = 0;
-        for (int j = 0; j < m; j++) {
-            if (cmap[j] == kZeroHpMarker) {
-                const bool pr = j < mp;
-                const double* ar = pr ? Ap + j * D : As + (j - mp) * D;
-                const double sg = pr ? -1.0 : 1.0;
-                double ip = 0.0;
-                for (int k = 0; k < D; k++) ip += (ar[k] * sg) * R.nu[k];
-                if (ip < 0) enc |= 1 << j;
-            } else {
-                const int svv = ((negb >> cmap[j]) & 1u) ? -1 : 1;
-                if (svv * csm[j] < 0) enc |= 1 << j;
-            }
+        if (c > 0) {
+            const int k = rowof[c];
+            negb ^= 1u << k;
+            encv ^= (int)hit[k];
         }
+        const int enc = enc_fix | (enc_pos ^ encv);
         const cplx gp = lookup_num(enc & mpmask, 1 << (mp - 1), mpmask, pk, pv, pn);
         const cplx gs = lookup_num(enc >> mp, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
         const cplx g = cmul(gp, gs);
         double g1re = 0.0, g2re = 0.0;
         for (int j = 0; j < u; j++) {
             const double svd = ((negb >> j) & 1u) ? -1.0 : 1.0;
-            g1re -= (a2[2 * j] * p2[j]) * svd;
-            g2re -= (a2[2 * j + 1] * p2[j]) * svd;
+            g1re -= a2[2 * j] * svd;
+            g2re -= a2[2 * j + 1] * svd;
         }
-        const double s1 = sin(th[c]), c1 = cos(th[c]);
+        // a cell's lower edge is the previous cell's upper edge: same argument, same value
+        const double s1 = c > 0 ? s_hi : sin(th[c]), c1 = c > 0 ? c_hi : cos(th[c]);
         const double s2 = sin(th[c + 1]), c2 = cos(th[c + 1]);
+        s_hi = s2;
+        c_hi = c2;
         C.sn[co + q + c] = s1;
         C.cs[co + q + c] = c1;
         C.g1r[co + c] = g1re;
@@ -3101,9 +3138,9 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
         const double gr = hypot(g.re, g.im);
         total += fabs((gr * s2) / den2) + fabs((gr * s1) / den1);
     }
-    if (u > 0) {
-        C.sn[co + q + u] = sin(th[u]);
-        C.cs[co + q + u] = cos(th[u]);
+    if (u > 0) {  // th[u], the last cell's upper edge
+        C.sn[co + q + u] = s_hi;
+        C.cs[co + q + u] = c_hi;
     }
     C.mc[q] = u;
     C.b2[2 * q] = b2[0];
