@@ -3104,6 +3104,10 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
     double total = 0.0;
     bool keep_term = false;
     double s_hi = 0.0, c_hi = 0.0;  // sin / cos of the previous cell's upper edge
+    // a crossing usually changes the key of one estimator's term only: the other
+    // term's g value is the previous cell's (same key, same table)
+    int key_p = -1, key_s = -1;
+    cplx gp = cmake(0, 0), gs = cmake(0, 0);
     for (int c = 0; c < u; c++) {
         if (c > 0) {
             const int k = rowof[c];
@@ -3111,8 +3115,14 @@ __global__ void k_rsys_pairs(TermPool P, int np, const uint32_t* __restrict__ pk
             encv ^= (int)hit[k];
         }
         const int enc = enc_fix | (enc_pos ^ encv);
-        const cplx gp = lookup_num(enc & mpmask, 1 << (mp - 1), mpmask, pk, pv, pn);
-        const cplx gs = lookup_num(enc >> mp, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
+        if ((enc & mpmask) != key_p) {
+            key_p = enc & mpmask;
+            gp = lookup_num(key_p, 1 << (mp - 1), mpmask, pk, pv, pn);
+        }
+        if ((enc >> mp) != key_s) {
+            key_s = enc >> mp;
+            gs = lookup_num(key_s, 1 << (ms - 1), (1 << ms) - 1, sk, sv_, sn_);
+        }
         const cplx g = cmul(gp, gs);
         double g1re = 0.0, g2re = 0.0;
         for (int j = 0; j < u; j++) {
