@@ -144,3 +144,16 @@ def test_gtable_lookup_modes_match_reference(name, mode, monkeypatch):
     _, out, _ = run_gpu(s, rec["num_steps"])
     for u, (got, exp) in enumerate(zip(out, rec["updates"])):
         compare_update(got, exp, TOL, where=f"{name} mode {mode} update {u}")
+
+
+@pytest.mark.parametrize("lanes", [128, 256])
+@pytest.mark.parametrize("name", ["ltv5_p2", "lti8_w5"])
+def test_enumerator_lanes_match_reference(name, lanes, monkeypatch):
+    """The TP cell enumerator gives the reference's B-tables at every supported lane count
+    per parent (64 is the default the other tests run)."""
+    monkeypatch.setenv("MCE_ENUM_THREADS", str(lanes))
+    rec = load_golden(name)
+    s = W.System.from_dict(rec["system"])
+    _, out, _ = run_gpu(s, rec["num_steps"])
+    for u, (got, exp) in enumerate(zip(out, rec["updates"])):
+        compare_update(got, exp, TOL, where=f"{name} lanes {lanes} update {u}")
