@@ -72,6 +72,22 @@ def declare_pycauchy(L, sym=lambda name: name):
         getattr(L, sym("pycauchy_batch_single_step_nonlin")).argtypes = [ct.POINTER(ct.c_void_p), ct.c_int, _dpp,
                                                                          _ip, _dpp, _ip, _ip,
                                                                          ct.POINTER(ct.c_void_p), _ip, ct.c_int]
+    if hasattr(L, sym("pycauchy_step")):  # the window-manager set (pycauchy.hpp:80-221)
+        tail = [ct.c_bool] * 3 + [ct.c_char_p, ct.c_double, ct.c_int, _dp, ct.c_int]
+        getattr(L, sym("pycauchy_initialize_lti_window_manager")).restype = None
+        getattr(L, sym("pycauchy_initialize_lti_window_manager")).argtypes = [ct.c_int, ct.c_int] + \
+            [_dp, ct.c_int] * 9 + tail
+        getattr(L, sym("pycauchy_initialize_ltv_window_manager")).restype = None
+        getattr(L, sym("pycauchy_initialize_ltv_window_manager")).argtypes = [ct.c_int, ct.c_int] + \
+            [_dp, ct.c_int] * 9 + [ct.c_void_p] + tail
+        getattr(L, sym("pycauchy_initialize_nonlin_window_manager")).restype = None
+        getattr(L, sym("pycauchy_initialize_nonlin_window_manager")).argtypes = [ct.c_int, ct.c_int] + \
+            [_dp, ct.c_int] * 6 + [ct.c_void_p] * 3 + [ct.c_int] + tail
+        getattr(L, sym("pycauchy_step")).restype = None
+        getattr(L, sym("pycauchy_step")).argtypes = [_dp, ct.c_int, _dp, ct.c_int, _dp, _dpp, _ip, _dpp, _ip, _dp, _dp,
+                                                     _dp, _ip, _ip]
+        getattr(L, sym("pycauchy_shutdown")).restype = None
+        getattr(L, sym("pycauchy_shutdown")).argtypes = []
     getattr(L, sym("pycauchy_get_marginal_1D_pointwise_cpdf")).restype = None
     getattr(L, sym("pycauchy_get_marginal_1D_pointwise_cpdf")).argtypes = [ct.c_void_p, ct.c_int] + [ct.c_double] * 3 + \
         [ct.c_char_p, _dpp, _ip, _ip]
@@ -136,6 +152,11 @@ def lib():
     L.mce_step_record.argtypes = [ct.c_void_p, _dp, _ip, ct.c_int]
     L.mce_collect_outputs.restype = ct.c_longlong
     L.mce_collect_outputs.argtypes = [ct.POINTER(ct.c_void_p), _ip, ct.c_longlong, ct.c_int, _dp, _ip]
+    L.mce_estimator_enable_tangents.restype = ct.c_int
+    L.mce_estimator_enable_tangents.argtypes = [ct.c_void_p, ct.c_int]
+    L.mce_estimator_step_tangent.restype = ct.c_int
+    L.mce_estimator_step_tangent.argtypes = [ct.c_void_p, ct.c_double, _dp, _dp, _dp, _dp, ct.c_double, _dp, _dp,
+                                             _dp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 

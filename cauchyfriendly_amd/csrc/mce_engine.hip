@@ -27,6 +27,7 @@
 #include "../../include/cauchy_mi355x.h"
 #include "mce_host_linalg.h"
 #include "mce_kernels.h"
+#include "mce_tangent.h"
 
 using namespace mce;
 
@@ -54,6 +55,79 @@ int enum_threads_from_env() {
     } while (0)
 
 int ceil_div(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+// LDS of one workgroup on gfx950 (static and dynamic together).
+constexpr size_t kLdsPerGroup = 160 * 1024;
+
+// MCE_CHECK_LAUNCH=1: every launch is followed by an error check and a synchronisation of
+// its stream, so a failing dispatch is named by its kernel.
+bool check_launches() {
+    static const bool on = [] {
+        const char* v = getenv("MCE_CHECK_LAUNCH");
+        return v && v[0] && v[0] != '0';
+    }();
+    return on;
+}
+
+// Validates a launch on the host before anything reaches the queue.  An empty grid has
+// nothing to do and is skipped, so no dispatch packet is ever formed for it; a block or an
+// LDS request outside the device's limits throws.
+bool launch_ok(const dim3& g, const dim3& b, size_t lds, const char* name) {
+    if (g.x == 0 || g.y == 0 || g.z == 0) return false;
+    const uint64_t threads = (uint64_t)b.x * b.y * b.z;
+    if (threads == 0 || threads > 1024 || g.y > 65535 || g.z > 65535 || g.x > 0x7fffffffu)
+        throw std::runtime_error(std::string("launch of ") + name + ": grid " + std::to_string(g.x) + "x" +
+                                 std::to_string(g.y) + "x" + std::to_string(g.z) + ", block " +
+                                 std::to_string(threads) + " outside the device's limits");
+    if (lds > kLdsPerGroup)
+        throw std::runtime_error(std::string("launch of ") + name + ": " + std::to_string(lds) +
+                                 " B of LDS requested, the device has " + std::to_string(kLdsPerGroup));
+    return true;
+}
+
+void launch_done(hipStream_t s, const char* name) {
+    if (!check_launches()) return;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error '") + hipGetErrorString(e) + "' in " + name);
+}
+
+#define MCE_LAUNCH(K_, G_, B_, L_, S_, ...)                      \
+    do {                                                         \
+        const dim3 g_ = (G_), b_ = (B_);                         \
+        const size_t l_ = (L_);                                  \
+        if (launch_ok(g_, b_, l_, #K_)) {                        \
+            hipLaunchKernelGGL(K_, g_, b_, l_, S_, __VA_ARGS__); \
+            launch_done(S_, #K_);                                \
+        }                                                        \
+    } while (0)
+
+// Lets a kernel take more than the default 64 KiB of dynamic LDS.  The attribute belongs to
+// the kernel symbol, which every engine shares; it is raised once per kernel, to everything
+// its static LDS leaves, under a lock, and never again: host threads that launch the same
+// kernel concurrently never see it change under them.
+void allow_lds(const void* fn, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> raised;  // kernel, its static LDS
+    std::lock_guard<std::mutex> lk(mu);
+    size_t stat = 0;
+    auto it = std::find_if(raised.begin(), raised.end(), [&](const auto& r) { return r.first == fn; });
+    if (it != raised.end()) {
+        stat = it->second;
+    } else {
+        hipFuncAttributes at;
+        HIPCHK(hipFuncGetAttributes(&at, fn));
+        stat = at.sharedSizeBytes;
+    }
+    if (stat + bytes > kLdsPerGroup)
+        throw std::runtime_error("kernel needs " + std::to_string(bytes) + " B of dynamic LDS beside " +
+                                 std::to_string(stat) + " B static; the device has " + std::to_string(kLdsPerGroup));
+    if (it != raised.end()) return;
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsPerGroup - stat)));
+    raised.emplace_back(fn, stat);
+}
 
 // Device blocks of up to kMaxBlock bytes come from a process-wide cache with one free list
 // per power-of-two size.  An estimator that outgrows its buffers, or a new estimator of a
@@ -298,6 +372,38 @@ struct PoolStore {
     }
 };
 
+// Tangents of a term pool in tangent mode (P directions), grown with it; growing discards
+// the contents, as for the pool.
+struct TanStore {
+    int P = 0, MM = 0, d = 0;
+    int64_t cap = 0;
+    DBuf<double> A, p, q, b, c, dv;
+    void reserve(int64_t n) {
+        if (n <= cap) return;
+        const int64_t nn = std::max<int64_t>(n, cap + cap / 2);
+        A.reserve((size_t)nn * P * MM * d);
+        p.reserve((size_t)nn * P * MM);
+        q.reserve((size_t)nn * P * MM);
+        b.reserve((size_t)nn * P * d);
+        c.reserve((size_t)nn * P);
+        dv.reserve((size_t)nn * P);
+        cap = nn;
+    }
+    TanPool view() const {
+        TanPool t;
+        t.A = A.ptr;
+        t.p = p.ptr;
+        t.q = q.ptr;
+        t.b = b.ptr;
+        t.c = c.ptr;
+        t.dv = dv.ptr;
+        t.P = P;
+        t.MM = MM;
+        t.d = d;
+        return t;
+    }
+};
+
 // ---------------------------------------------------------------- host numerics
 unsigned long long binom(int n, int k) {
     unsigned long long r = 1;
@@ -435,6 +541,18 @@ struct Engine {
     PoolStore in, out;
     DBuf<uint32_t> gk[2];
     DBuf<cplx> gv[2];
+    // ---- tangent mode (P directions): the pools' tangents, the g-tables' tangents
+    // ([cell][direction], same offsets as gv), the update's seeds and results
+    int ntan = 0;
+    TanStore tin, tout;
+    DBuf<cplx> dgv[2];
+    DBuf<cplx> dg_slot, dfz_dev;
+    DBuf<double> dgscale_dev;
+    DBuf<StepTan> dstan;
+    DBuf<uint8_t> mem_add;
+    StepTan htan;                // seeds of the update being stepped
+    bool tan_seeded = false;
+    std::vector<cplx> dfz_tan;   // d fz of the last update, per direction
     int gcur = 0;
     DBuf<uint32_t> barena, b2arena, tmpk;
     DBuf<cplx> scratch, tmpv;
@@ -608,7 +726,37 @@ struct Engine {
         Nt = 1;
         master_step = 0;
         errs = 0;
+        if (ntan) zero_prior_tangents();
     }
+
+    // The first term (A0, p0, b0) carries no tangent.
+    void zero_prior_tangents() {
+        tin.reserve(1);
+        std::vector<double> z((size_t)ntan * MM * d, 0.0);
+        copy_sync(tin.A.ptr, z.data(), (size_t)ntan * MM * d * sizeof(double), hipMemcpyHostToDevice);
+        copy_sync(tin.p.ptr, z.data(), (size_t)ntan * MM * sizeof(double), hipMemcpyHostToDevice);
+        copy_sync(tin.b.ptr, z.data(), (size_t)ntan * d * sizeof(double), hipMemcpyHostToDevice);
+    }
+
+    // Tangent mode with P directions, from the first update of a window on (the first
+    // update after construction or a reset).
+    void enable_tangents(int P) {
+        if (P < 1 || P > kMaxTan) throw std::runtime_error("tangent directions outside 1.." + std::to_string(kMaxTan));
+        if (master_step != 0) throw std::runtime_error("tangents are enabled before a window's first update");
+        bind();
+        GraveyardScope gs(&graveyard);
+        ntan = P;
+        for (TanStore* t : {&tin, &tout}) {
+            t->P = P;
+            t->MM = MM;
+            t->d = d;
+            t->cap = 0;
+        }
+        zero_prior_tangents();
+        memset(&htan, 0, sizeof(htan));
+        dfz_tan.assign(P, cmake(0, 0));
+    }
+    std::vector<double> tan_dbeta;  // raw process-scale seeds of the update, ntan x pncc
 
     template <class F>
     void cub(F f) {
@@ -639,7 +787,7 @@ struct Engine {
         }
         if (n == 0) return;
         const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (most / 4 + 255) / 256));
-        hipLaunchKernelGGL(k_fill_multi, dim3(blocks, n), dim3(256), 0, st, f);
+        MCE_LAUNCH(k_fill_multi, dim3(blocks, n), dim3(256), 0, st, f);
         HIPCHK(hipGetLastError());
     }
 
@@ -668,7 +816,7 @@ struct Engine {
     // off[n-1] + cnt[n-1] (the total of an exclusive scan) into device scalar `slot`.
     template <class A, class B>
     void enqueue_total(const A* off, const B* cnt, int n, int slot) {
-        hipLaunchKernelGGL((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
+        MCE_LAUNCH((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
     }
     // off[n-1] + cnt[n-1] read by the host: the kernel writes it into the mapped host slot (and
     // d_scal[0]), so no copy is launched.  The slot holds -1 until the kernel's value arrives;
@@ -677,7 +825,7 @@ struct Engine {
     int64_t read_total(const A* off, const B* cnt, int n) {
         volatile int64_t* slot = h_total;
         *slot = -1;
-        hipLaunchKernelGGL((k_scan_total_mapped<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr, h_total_dev);
+        MCE_LAUNCH((k_scan_total_mapped<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr, h_total_dev);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         const int64_t v = *slot;
@@ -685,17 +833,24 @@ struct Engine {
     }
 
     // Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.
-    int precoalign(const double* Gamma, const double* beta, StepConst& sc) {
+    // In tangent mode `dbeta` (ntan x pncc) is carried through the same scaling and merges
+    // into tan->beta.
+    int precoalign(const double* Gamma, const double* beta, StepConst& sc, const double* dbeta = nullptr,
+                   StepTan* tan = nullptr) {
         std::vector<double> G(pncc * d), bt(pncc);
+        const int NP = (dbeta && tan) ? ntan : 0;
+        std::vector<double> dbt((size_t)NP * pncc);
         for (int i = 0; i < pncc; i++) {
             for (int j = 0; j < d; j++) G[i * d + j] = Gamma[j * pncc + i];
             bt[i] = beta[i];
+            for (int k = 0; k < NP; k++) dbt[k * pncc + i] = dbeta[k * pncc + i];
         }
         for (int i = 0; i < pncc; i++) {
             double nf = 0;
             for (int j = 0; j < d; j++) nf += fabs(G[i * d + j]);
             for (int j = 0; j < d; j++) G[i * d + j] /= nf;
             bt[i] *= nf;
+            for (int k = 0; k < NP; k++) dbt[k * pncc + i] *= nf;
         }
         std::vector<char> F(pncc, 1);
         for (int i = 0; i < pncc - 1; i++) {
@@ -711,6 +866,7 @@ struct Engine {
                 if (pos || neg) {
                     F[j] = 0;
                     bt[i] += bt[j];
+                    for (int k = 0; k < NP; k++) dbt[k * pncc + i] += dbt[k * pncc + j];
                 }
             }
         }
@@ -723,6 +879,7 @@ struct Engine {
                     if (u < j) {
                         memcpy(&G[u * d], &G[j * d], d * sizeof(double));
                         bt[u] = bt[j];
+                        for (int k = 0; k < NP; k++) dbt[k * pncc + u] = dbt[k * pncc + j];
                     }
                     u++;
                 }
@@ -730,6 +887,7 @@ struct Engine {
         for (int i = 0; i < cnt; i++) {
             for (int j = 0; j < d; j++) sc.G[i * d + j] = G[i * d + j];
             sc.beta[i] = bt[i];
+            for (int k = 0; k < NP; k++) tan->beta[k][i] = dbt[k * pncc + i];
         }
         return cnt;
     }
@@ -837,6 +995,20 @@ struct Engine {
 
     // Joins the moment chains, adds the per-thread chunks in thread order
     // (cauchy_estimator.hpp:886-895) and finalises the moments.
+    // Tangent mode, once the update's fz (and so its g scale 1 / (2 pi Re fz)) is known:
+    // d fz per direction, and the tangent of the g scale, -gscale d(Re fz) / Re fz.
+    std::vector<double> dgscale_host;
+    void tan_after_moments() {
+        dfz_tan.assign(ntan, cmake(0, 0));
+        HIPCHK(hipMemcpyAsync(dfz_tan.data(), dfz_dev.ptr, ntan * sizeof(cplx), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        dgscale_host.assign(ntan, 0.0);
+        for (int k = 0; k < ntan; k++) dgscale_host[k] = -gscale * dfz_tan[k].re / fz.re;
+        dgscale_dev.reserve(ntan);
+        HIPCHK(hipMemcpyAsync(dgscale_dev.ptr, dgscale_host.data(), ntan * sizeof(double), hipMemcpyHostToDevice,
+                              st));
+    }
+
     void collect_moments(bool first) {
         if (!moments_pending) return;
         const int NC = 1 + d + d * d;
@@ -874,6 +1046,7 @@ struct Engine {
         } else {
             finalize_and_check();
         }
+        if (ntan) tan_after_moments();
         diag_mom[2] = fz.re;
         diag_mom[3] = fz.im;
         for (int i = 0; i < d; i++) {
@@ -921,7 +1094,8 @@ struct Engine {
         memset(&sc, 0, sizeof(sc));
         for (int i = 0; i < d * d; i++) sc.Phi[i] = Phi ? Phi[i] : 0.0;
         for (int i = 0; i < d; i++) sc.H[i] = Hrow[i];
-        sc.ng = with_tp ? precoalign(Gamma, beta, sc) : 0;
+        if (ntan && !tan_seeded) throw std::runtime_error("tangent mode: step with mce_estimator_step_tangent");
+        sc.ng = with_tp ? precoalign(Gamma, beta, sc, ntan ? tan_dbeta.data() : nullptr, ntan ? &htan : nullptr) : 0;
         if (with_tp && cmcc > 0 && Bm && u)
             for (int i = 0; i < d; i++) {
                 double s = 0.0;
@@ -938,8 +1112,18 @@ struct Engine {
         sc.first = first;
         sc.T = T;
         sc.threaded = threaded;
-        for (int i = 0; i < d; i++) sc.order[i] = g_search_order[i];
+        for (int i = 0; i < d; i++) {
+            sc.order[i] = g_search_order[i];
+            // the reduction reads b[order[i]] of every term
+            if (sc.order[i] < 0 || sc.order[i] >= d)
+                throw std::runtime_error("term-reduction search ordering names axis " + std::to_string(sc.order[i]) +
+                                         " of a " + std::to_string(d) + "-state estimator");
+        }
         HIPCHK(hipMemcpyAsync(dconst.ptr, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        if (ntan) {
+            dstan.reserve(1);
+            HIPCHK(hipMemcpyAsync(dstan.ptr, &htan, sizeof(StepTan), hipMemcpyHostToDevice, st));
+        }
         std::vector<int> in_off(sr + 1, 0);
         for (int m = 0; m < sr; m++) in_off[m + 1] = in_off[m] + in_count[m];
         HIPCHK(hipMemcpyAsync(din_off.ptr, in_off.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
@@ -961,7 +1145,11 @@ struct Engine {
                     {slot_base.ptr, sizeof(int), 0},
                     {hist.ptr, (sr + 1) * sizeof(int), 0},
                     {hist_mx.ptr, (sr + 2) * sizeof(int), 0}});
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
+        // tangents first: k_tp_tan reads the parents before k_tp propagates them in place
+        if (ntan)
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp_tan<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, tin.view(), np,
+                                         dconst.ptr, dstan.ptr));
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
                                              din_off.ptr, dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
         scan_i32_to_i32_incl(slots.ptr, slot_base.ptr + 1, np);
@@ -973,22 +1161,35 @@ struct Engine {
             scan_i64_excl(btcap64.ptr, bt_base.ptr, np);
             barena.reserve((size_t)read_total(bt_base.ptr, btcap.ptr, np) + 1);
             enum_list.reserve(np);
-            hipLaunchKernelGGL(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
+            MCE_LAUNCH(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
                                gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr + 16);
             HIPCHK(hipGetLastError());
-            const int nenum = read1(counters.ptr + 16);
+            int ec[6];  // parents to enumerate, -, -, -, their largest shape and parent hyperplane count
+            readn(counters.ptr + 16, ec, 6);
+            const int nenum = ec[0];
             if (nenum > 0) {
-                // 2^MM-bit candidate set plus the 2^(MM-1)-bit parent key set
-                const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
-                const size_t lds = (size_t)words * 4 + (size_t)std::max(1, words / 2) * 4;
-                if (lds > 160 * 1024)
+                // the 2^m-bit candidate set of the largest shape, then the parent key set of
+                // 2^(phc-1) bits; when both do not fit beside the kernel's static LDS (m = 20
+                // with phc = 19), parent keys are binary-searched in global memory instead
+                const int mt = ec[4], ph = ec[5];
+                if (mt < 1 || mt > MM || ph < 1 || ph >= mt)
+                    throw std::runtime_error("cell enumerator: parent shapes outside the estimator's range");
+                const int words = mt >= 5 ? (1 << (mt - 5)) : 1;
+                const int pwords = ph - 1 >= 5 ? (1 << (ph - 6)) : 1;
+                const size_t enum_static = 256 * sizeof(int);
+                int psearch = 0;
+                size_t lds = (size_t)(words + pwords) * 4;
+                if (lds + enum_static > kLdsPerGroup) {
+                    psearch = 1;
+                    lds = (size_t)words * 4;
+                }
+                if (lds + enum_static > kLdsPerGroup)
                     throw std::runtime_error("hyperplane count too large for the cell enumerator's LDS sign-vector set");
                 MCE_DISPATCH_D(d, {
-                    HIPCHK(hipFuncSetAttribute((const void*)k_btab_enum<DD>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                    hipLaunchKernelGGL(k_btab_enum<DD>, dim3(nenum), dim3(enum_threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
-                                       dcombos.ptr, dcombo_off.ptr, dcombo_cnt.ptr, gk[gcur].ptr, barena.ptr,
-                                       btcap.ptr, counters.ptr + 8);
+                    allow_lds((const void*)k_btab_enum<DD>, lds);
+                    MCE_LAUNCH(k_btab_enum<DD>, dim3(nenum), dim3(enum_threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
+                               dcombos.ptr, dcombo_off.ptr, dcombo_cnt.ptr, gk[gcur].ptr, barena.ptr, btcap.ptr,
+                               counters.ptr + 8, words, psearch);
                 });
                 HIPCHK(hipGetLastError());
             }
@@ -1009,29 +1210,38 @@ struct Engine {
         // child rows in registers when the largest shape fits a compile-time bound
         const int mu_km = MM <= 8 ? 8 : MM <= 14 ? 14 : 0;
         if (mu_km == 8) {
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
+            MCE_DISPATCH_D(d, MCE_LAUNCH((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
                                                  np, slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr,
                                                  gv[gcur].ptr, gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         } else if (mu_km == 14 && d <= 5) {
-            MCE_DISPATCH_D(d, if constexpr (DD <= 5) hipLaunchKernelGGL(
+            MCE_DISPATCH_D(d, if constexpr (DD <= 5) MCE_LAUNCH(
                                   (k_mu_r<DD, 14>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                   slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr, keys.ptr,
                                   chain_key.ptr, counters.ptr + 17));
         } else {
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                                  slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr,
                                                  gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         }
         HIPCHK(hipGetLastError());
+        if (ntan) {
+            tout.reserve(nslots);
+            dg_slot.reserve((size_t)nslots * ntan);
+            dfz_dev.reserve(ntan);
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mu_tan<DD>, dim3(ceil_div(nslots, 128)), dim3(128), 0, st, IN, tin.view(),
+                                         np, slot_base.ptr, OUT, tout.view(), nslots, dconst.ptr, dstan.ptr,
+                                         gk[gcur].ptr, gv[gcur].ptr, dgv[gcur].ptr, dg_slot.ptr));
+            MCE_LAUNCH(k_tan_colsum, dim3(ntan), dim3(256), 0, st, dg_slot.ptr, nslots, ntan, dfz_dev.ptr);
+        }
         prof.mark(kPhMu, st);
 
         // ---- moments: slots in reference-thread chain order, then the chains on st2
-        hipLaunchKernelGGL(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+        MCE_LAUNCH(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
         cub([&](void* t_, size_t& b_) {
             return hipcub::DeviceRadixSort::SortPairs(t_, b_, chain_key.ptr, chain_skey.ptr, iota.ptr, chain_perm.ptr,
                                                       nslots, 0, 4, st);
         });
-        hipLaunchKernelGGL(k_chain_off, dim3(1), dim3(16), 0, st, chain_skey.ptr, nslots, T, chain_off.ptr);
+        MCE_LAUNCH(k_chain_off, dim3(1), dim3(16), 0, st, chain_skey.ptr, nslots, T, chain_off.ptr);
         HIPCHK(hipEventRecord(ev_mu, st));
         HIPCHK(hipStreamWaitEvent(st2, ev_mu, 0));
         if (prof.on) HIPCHK(hipEventRecord(ev_c0, st2));
@@ -1039,10 +1249,10 @@ struct Engine {
             int ncp = 0, nyb = 0;
             MCE_DISPATCH_D(d, (nyb = mom_blocks<DD>(), ncp = nyb * kSumCW / 2));
             mom_x.reserve((size_t)nyb * nslots * kSumCW);
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_terms<DD>, dim3(ceil_div((int64_t)nslots * ncp, 256)), dim3(256),
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mom_terms<DD>, dim3(ceil_div((int64_t)nslots * ncp, 256)), dim3(256),
                                                  0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr, T,
                                                  nslots, mom_x.ptr));
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
                                                  chain_off.ptr, T, mom_part.ptr));
         }
         HIPCHK(hipGetLastError());
@@ -1054,7 +1264,7 @@ struct Engine {
         if (first || skip) collect_moments(first);
 
         if (skip) {
-            hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+            MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> h(sr + 1);
             HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1072,12 +1282,12 @@ struct Engine {
             iota.reserve(nslots);
             perm.reserve(nslots);
             skeys.reserve(nslots);
-            hipLaunchKernelGGL(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+            MCE_LAUNCH(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
             cub([&](void* t, size_t& b) {
                 return hipcub::DeviceRadixSort::SortPairs(t, b, keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0,
                                                           32, st);
             });
-            hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+            MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> cnt(sr + 1);
             HIPCHK(hipMemcpyAsync(cnt.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1135,8 +1345,14 @@ struct Engine {
         in.reserve(n);
         TermPool OUT = out.view();
         TermPool NEXT = in.view();
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_gtable_first<DD>, dim3(1), dim3(64), 0, st, OUT, n, perm.ptr, gscale,
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_gtable_first<DD>, dim3(1), dim3(64), 0, st, OUT, n, perm.ptr, gscale,
                                              gk[nx].ptr, gv[nx].ptr, NEXT));
+        if (ntan) {
+            dgv[nx].reserve((size_t)n * cells * ntan);
+            tin.reserve(n);
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_gtable_first_tan<DD>, dim3(ceil_div(n, 64)), dim3(64), 0, st, OUT,
+                                         tout.view(), n, perm.ptr, gscale, dgscale_dev.ptr, dgv[nx].ptr, tin.view()));
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         gcur = nx;
@@ -1168,7 +1384,7 @@ struct Engine {
         b2n.reserve(ng);
         b2n64.reserve(ng);
         goff.reserve(ng);
-        hipLaunchKernelGGL(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
+        MCE_LAUNCH(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
                            (int*)b2n.ptr);
         widen(b2n.ptr, ub64.ptr, ng);
         scan_i64_excl(ub64.ptr, b2off.ptr, ng);
@@ -1181,9 +1397,8 @@ struct Engine {
             int wpb = 4;
             while (wpb > 1 && (size_t)wpb * words * 4 > 65536) wpb--;
             const size_t lds = (size_t)wpb * words * 4;
-            HIPCHK(hipFuncSetAttribute((const void*)k_group_btab, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)std::max<size_t>(lds, 4)));
-            hipLaunchKernelGGL(k_group_btab, dim3(ceil_div(ngb, wpb)), dim3(64 * wpb), lds, st, OUT, gbase_of[m], ngb,
+            allow_lds((const void*)k_group_btab, lds);
+            MCE_LAUNCH(k_group_btab, dim3(ceil_div(ngb, wpb)), dim3(64 * wpb), lds, st, OUT, gbase_of[m], ngb,
                                m, d, grp_root.ptr, barena.ptr, b2off.ptr, ub64.ptr, b2arena.ptr, b2n.ptr, words,
                                counters.ptr + 8);
             HIPCHK(hipGetLastError());
@@ -1199,6 +1414,11 @@ struct Engine {
         gk[nx].reserve((size_t)cells_total + 1);
         gv[nx].reserve((size_t)cells_total + 1);
         scratch.reserve((size_t)cells_total + 1);
+        if (ntan) {
+            // which members each group adds, written by k_group_eval / k_group_members
+            mem_add.reserve(std::max(total, 1));
+            fill_multi({{mem_add.ptr, (size_t)std::max(total, 1), 0}});
+        }
         surv.reserve(ng);
         gxor.reserve(ng);
         sort_list.reserve(ng);
@@ -1251,13 +1471,12 @@ struct Engine {
             defer_k0.reserve(std::max(ng, 1));
 #define MCE_GEVAL(M_, K_)                                                                                        \
     do {                                                                                                          \
-        HIPCHK(hipFuncSetAttribute((const void*)k_group_eval<M_, K_>,                                             \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 16)));  \
-        hipLaunchKernelGGL((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
+        allow_lds((const void*)k_group_eval<M_, K_>, lds);                                                        \
+        MCE_LAUNCH((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
                            grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
                            b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
                            gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 18, counters.ptr + 19,          \
-                           big_members, defer_k0.ptr);                                                            \
+                           big_members, defer_k0.ptr, ntan ? mem_add.ptr : nullptr);                             \
     } while (0)
             // rows held in registers: the smallest compile-time bound >= the largest shape
             if (mode == kLookupRankVals && MM <= 8) MCE_GEVAL(kLookupRankVals, 8);
@@ -1279,7 +1498,7 @@ struct Engine {
             HIPCHK(hipEventRecord(prof.k1, st));
             bytes_acc.reserve(1);
             HIPCHK(hipMemsetAsync(bytes_acc.ptr, 0, sizeof(unsigned long long), st));
-            hipLaunchKernelGGL(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
+            MCE_LAUNCH(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
                                grp_mcnt.ptr, b2n.ptr, surv.ptr, defer_k0.ptr,
                                ge_mode >= kLookupRank ? big_group_members : INT_MAX, bytes_acc.ptr);
             const unsigned long long bytes = read1(bytes_acc.ptr);
@@ -1313,25 +1532,36 @@ struct Engine {
             tmpv.reserve((size_t)cells_total + 1);
             const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
             const size_t lds = (size_t)words * 4;
-            HIPCHK(hipFuncSetAttribute((const void*)k_group_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
-            hipLaunchKernelGGL(k_group_sort, dim3(nsort), dim3(256), lds, st, nsort, sort_list.ptr, grp_m.ptr, b2n.ptr,
+            allow_lds((const void*)k_group_sort, lds);
+            MCE_LAUNCH(k_group_sort, dim3(nsort), dim3(256), lds, st, nsort, sort_list.ptr, grp_m.ptr, b2n.ptr,
                                goff.ptr, gk[nx].ptr, gv[nx].ptr, tmpk.ptr, tmpv.ptr);
             HIPCHK(hipGetLastError());
+        }
+        if (ntan) {
+            dgv[nx].reserve(((size_t)cells_total + 1) * ntan);
+            MCE_LAUNCH(k_group_tan, dim3(ng), dim3(64), 0, st, OUT, tout.view(), ng, d, grp_moff.ptr, grp_mcnt.ptr,
+                       grp_m.ptr, mem_slots.ptr, mem_add.ptr, surv.ptr, gxor.ptr, b2arena.ptr, b2off.ptr, b2n.ptr,
+                       goff.ptr, gk[gcur].ptr, gv[gcur].ptr, dgv[gcur].ptr, gk[nx].ptr, gscale, dgscale_dev.ptr,
+                       dgv[nx].ptr);
         }
         prof.mark(kPhGEval, st);
 
         // ---- survivors become the next parents (shape ascending, root index ascending)
         sflag.reserve(ng);
         spos.reserve(ng);
-        hipLaunchKernelGGL(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
+        MCE_LAUNCH(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
         scan_i32_excl(sflag.ptr, spos.ptr, ng);
         in.reserve(std::max(ng, 1));  // survivors <= groups; their count comes with the histogram
         TermPool NEXT = in.view();
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
                                              surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
+        if (ntan) {
+            tin.reserve(std::max(ng, 1));
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_compact_tan<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, tout.view(), ng,
+                                         surv.ptr, spos.ptr, grp_m.ptr, tin.view()));
+        }
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
+        MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
                            (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist_mx.ptr);
         std::vector<int> h(sr + 2);
         {
@@ -1363,7 +1593,7 @@ struct Engine {
         big_hoff.reserve(ng);
         big_ccnt.reserve(ng);
         big_coff.reserve(ng);
-        hipLaunchKernelGGL(k_big_prep, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, big_group_members, grp_mcnt.ptr,
+        MCE_LAUNCH(k_big_prep, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, big_group_members, grp_mcnt.ptr,
                            b2n.ptr, big_icnt.ptr, big_ccnt.ptr, big_hcnt.ptr);
         scan_i32_excl(big_icnt.ptr, big_ioff.ptr, ng);
         scan_i32_excl(big_hcnt.ptr, big_hoff.ptr, ng);
@@ -1382,12 +1612,11 @@ struct Engine {
         TermPool OUT = out.view();
 #define MCE_GMEM(M_, K_)                                                                                         \
     do {                                                                                                          \
-        HIPCHK(hipFuncSetAttribute((const void*)k_group_members<M_, K_>,                                          \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(lds, 16)));  \
-        hipLaunchKernelGGL((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
+        allow_lds((const void*)k_group_members<M_, K_>, lds);                                                     \
+        MCE_LAUNCH((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
                            ni, big_coff.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr,    \
                            b2off.ptr, b2n.ptr, gk[gcur].ptr, gv[gcur].ptr, gscale, surv.ptr, gxor.ptr,             \
-                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19);                                 \
+                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19, ntan ? mem_add.ptr : nullptr);   \
     } while (0)
         if (mode == kLookupRankVals && MM <= 8) MCE_GMEM(kLookupRankVals, 8);
         else if (mode == kLookupRankVals && MM <= 11) MCE_GMEM(kLookupRankVals, 11);
@@ -1401,7 +1630,7 @@ struct Engine {
         else MCE_GMEM(kLookupRank, 32);
 #undef MCE_GMEM
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_group_sum, dim3(nh), dim3(64), 0, st, ng, big_hoff.ptr, nh, big_ioff.ptr, big_coff.ptr,
+        MCE_LAUNCH(k_group_sum, dim3(nh), dim3(64), 0, st, ng, big_hoff.ptr, nh, big_ioff.ptr, big_coff.ptr,
                            grp_mcnt.ptr, b2n.ptr, goff.ptr, surv.ptr, defer_k0.ptr, contrib.ptr, cneg.ptr, gv[nx].ptr);
         HIPCHK(hipGetLastError());
     }
@@ -1414,6 +1643,9 @@ struct Engine {
         gbase_of.assign(sr + 1, 0);
         if (N == 0) return 0;
         const int a0 = g_search_order[0];
+        if (a0 < 0 || a0 >= d)
+            throw std::runtime_error("term-reduction search ordering names axis " + std::to_string(a0) + " of a " +
+                                     std::to_string(d) + "-state estimator");
         h_boff = off;
         dboff.reserve(sr + 1);
         HIPCHK(hipMemcpyAsync(dboff.ptr, h_boff.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
@@ -1429,24 +1661,24 @@ struct Engine {
         eoff.reserve(N);
         const dim3 g1(ceil_div(N, 256)), b1(256);
         // value order on the first search axis, then (stable) bucket order
-        hipLaunchKernelGGL(k_ftr_keys, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, fkeys.ptr, fidx.ptr);
+        MCE_LAUNCH(k_ftr_keys, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, fkeys.ptr, fidx.ptr);
         cub([&](void* t, size_t& b) {
             return hipcub::DeviceRadixSort::SortPairs(t, b, fkeys.ptr, fskeys.ptr, fidx.ptr, fsidx.ptr, N, 0, 64, st);
         });
-        hipLaunchKernelGGL(k_ftr_bkeys, g1, b1, 0, st, N, boff, sr, fsidx.ptr, bkey.ptr);
+        MCE_LAUNCH(k_ftr_bkeys, g1, b1, 0, st, N, boff, sr, fsidx.ptr, bkey.ptr);
         int bbits = 1;
         while ((1 << bbits) < sr) bbits++;
         cub([&](void* t, size_t& b) {
             return hipcub::DeviceRadixSort::SortPairs(t, b, bkey.ptr, bskey.ptr, fsidx.ptr, fidx.ptr, N, 0, bbits, st);
         });
         const uint32_t* sidx = fidx.ptr;
-        hipLaunchKernelGGL(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
+        MCE_LAUNCH(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
         // candidate windows on the first search axis; the predicate runs one thread per
         // (term, window slot) pair unless the pairs would not fit, then one per term
         wlo.reserve(N);
         wcnt.reserve(N);
         woff.reserve(N);
-        hipLaunchKernelGGL(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr);
+        MCE_LAUNCH(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr);
         scan_i64_excl(wcnt.ptr, woff.ptr, N);
         const int64_t npairs = read_total(woff.ptr, wcnt.ptr, N);
         const bool by_pairs = npairs < ((int64_t)1 << 31);
@@ -1456,20 +1688,20 @@ struct Engine {
             fgp.reserve((size_t)N * MM);
             fgA.reserve((size_t)N * MM);
             pflag.reserve((size_t)npairs);
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_gather<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, sidx,
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_gather<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, sidx,
                                                  fgb.ptr, fgp.ptr, fgA.ptr));
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs, N,
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs, N,
                                                  boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr, fgp.ptr,
                                                  fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr, edges.ptr, 0));
         } else if (!by_pairs) {
-            MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
                                                  dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
         }
         HIPCHK(hipGetLastError());
         scan_i32_excl(ecnt.ptr, eoff.ptr, N);
         const int E = (int)read_total(eoff.ptr, ecnt.ptr, N);
         Fc.reserve((size_t)8 * N);
-        hipLaunchKernelGGL(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
+        MCE_LAUNCH(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
         if (E > 0) {
             edges.reserve(E);
             inedges.reserve(E);
@@ -1487,18 +1719,18 @@ struct Engine {
                         {st0.ptr, (size_t)N, kUnknown},
                         {round_slots.ptr, kRoundSlots * sizeof(int), 0}});
             if (by_pairs) {
-                MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
+                MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
                                                      N, boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr,
                                                      fgp.ptr, fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr,
                                                      edges.ptr, 1));
             } else {
-                MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+                MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
                                                      dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
             }
-            hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+            MCE_LAUNCH(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
                                (const int*)nullptr, (int*)nullptr, (int*)nullptr, 0);
             scan_i32_excl(indeg.ptr, inoff.ptr, N);
-            hipLaunchKernelGGL(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+            MCE_LAUNCH(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
                                inoff.ptr, cursor2.ptr, inedges.ptr, 1);
             uint8_t* sa = st0.ptr;
             uint8_t* sb = st1.ptr;
@@ -1507,7 +1739,7 @@ struct Engine {
             int batch = 2, round = 0;
             for (int done = 0; done <= N + 1;) {
                 for (int r = 0; r < batch; r++, round++) {
-                    hipLaunchKernelGGL(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
+                    MCE_LAUNCH(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
                                        round_slots.ptr, round);
                     std::swap(sa, sb);
                 }
@@ -1515,10 +1747,10 @@ struct Engine {
                 if (read1(round_slots.ptr + (round - 1) % kRoundSlots) == 0) break;
                 batch = std::min(batch * 2, 16);
             }
-            hipLaunchKernelGGL(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);
+            MCE_LAUNCH(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);
             if (N > 10000) {
                 for (int lvl = 0; lvl < 3; lvl++)
-                    hipLaunchKernelGGL(k_ftr_merge, dim3(ceil_div(N, 256), 8 >> (lvl + 1)), b1, 0, st, N, boff, sr,
+                    MCE_LAUNCH(k_ftr_merge, dim3(ceil_div(N, 256), 8 >> (lvl + 1)), b1, 0, st, N, boff, sr,
                                        Fc.ptr, lvl);
             }
             HIPCHK(hipGetLastError());
@@ -1533,8 +1765,8 @@ struct Engine {
         jiota.reserve(N);
         memcnt.reserve(N + 1);
         memoff.reserve(N + 1);
-        hipLaunchKernelGGL(k_ftr_rootflags, g1, b1, 0, st, N, boff, sr, Fc.ptr, isroot.ptr, memkey.ptr, memval.ptr);
-        hipLaunchKernelGGL(k_iota, g1, b1, 0, st, jiota.ptr, N);
+        MCE_LAUNCH(k_ftr_rootflags, g1, b1, 0, st, N, boff, sr, Fc.ptr, isroot.ptr, memkey.ptr, memval.ptr);
+        MCE_LAUNCH(k_iota, g1, b1, 0, st, jiota.ptr, N);
         cub([&](void* t, size_t& b) {
             return hipcub::DeviceSelect::Flagged(t, b, jiota.ptr, isroot.ptr, roots.ptr, counters.ptr + 5, N, st);
         });
@@ -1546,12 +1778,12 @@ struct Engine {
         });
         gbase_dev.reserve(sr + 1);
         const int* d_nroots = counters.ptr + 5;
-        hipLaunchKernelGGL(k_ftr_gbase, dim3(1), dim3(64), 0, st, d_nroots, roots.ptr, boff, sr, gbase_dev.ptr);
+        MCE_LAUNCH(k_ftr_gbase, dim3(1), dim3(64), 0, st, d_nroots, roots.ptr, boff, sr, gbase_dev.ptr);
         HIPCHK(hipMemsetAsync(memcnt.ptr, 0, (N + 1) * sizeof(int), st));
-        hipLaunchKernelGGL(k_ftr_memcount, g1, b1, 0, st, N, d_nroots, smkey.ptr, memcnt.ptr);
-        hipLaunchKernelGGL(k_ftr_memslots, g1, b1, 0, st, N, d_nroots, smval.ptr, perm.ptr, mem_slots.ptr);
+        MCE_LAUNCH(k_ftr_memcount, g1, b1, 0, st, N, d_nroots, smkey.ptr, memcnt.ptr);
+        MCE_LAUNCH(k_ftr_memslots, g1, b1, 0, st, N, d_nroots, smval.ptr, perm.ptr, mem_slots.ptr);
         scan_i32_excl(memcnt.ptr, memoff.ptr, N);
-        hipLaunchKernelGGL(k_ftr_groups, g1, b1, 0, st, d_nroots, roots.ptr, perm.ptr, boff, sr, memcnt.ptr,
+        MCE_LAUNCH(k_ftr_groups, g1, b1, 0, st, d_nroots, roots.ptr, perm.ptr, boff, sr, memcnt.ptr,
                            memoff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr);
         HIPCHK(hipGetLastError());
         // group offsets per shape; the last one is the root count
@@ -1568,6 +1800,9 @@ struct Engine {
     // Fills xy with (x, f(x)) pairs; returns 1 (no points) after a window's last update,
     // whose g-tables are not built.
     int cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy);
+    // Checks of the grid; returns the point count, or 0 after the window's last
+    // update (whose g-tables were not built).
+    int cpdf_1d_points(int marg, double lo, double hi, double res);
     // 2D marginal density of states (i1, i2) on the grid of CauchyCPDFGridDispatcher2D
     // (cpdf_ndim.hpp:1816-1900): x runs fastest; fills xyz with (x, y, f) triples.
     int cpdf_2d(int i1, int i2, double lox, double hix, double resx, double loy, double hiy, double resy,
@@ -1621,10 +1856,11 @@ struct Engine {
 
 void Engine::widen(const int* src, int64_t* dst, int n) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_widen, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, n);
+    MCE_LAUNCH(k_widen, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, n);
 }
 
 void Engine::deterministic_transform(const double* Trans, const double* bias) {
+    if (ntan) throw std::runtime_error("the deterministic transform carries no tangents");
     bind();
     DBuf<double> dT;
     dT.reserve(d * d + d);
@@ -1632,24 +1868,29 @@ void Engine::deterministic_transform(const double* Trans, const double* bias) {
     const bool shift = !skip && bias != nullptr;
     if (shift) copy_sync(dT.ptr + d * d, bias, d * sizeof(double), hipMemcpyHostToDevice);
     TermPool IN = in.view();
-    MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_det_tp<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dT.ptr,
+    MCE_DISPATCH_D(d, MCE_LAUNCH(k_det_tp<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dT.ptr,
                                          shift ? dT.ptr + d * d : nullptr));
     HIPCHK(hipStreamSynchronize(st));
 }
 
-int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
+int Engine::cpdf_1d_points(int marg, double lo, double hi, double res) {
     if (master_step == 0) throw std::runtime_error("the density needs at least one measurement update");
     if (marg < 0 || marg >= d) throw std::runtime_error("marginal index outside the state");
     if (!(hi > lo) || !(res > 0)) throw std::runtime_error("the grid needs high > low and resolution > 0");
     if (master_step == n_mu) {
         fprintf(stderr, "[cauchy_mi355x] the density cannot be evaluated after the window's last update: its g-tables "
                         "were not built\n");
-        return 1;
+        return 0;
     }
     if ((hi - lo) / res > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
+    return (int)((hi - lo + res - 1e-15) / res) + 1;
+}
+
+int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
+    const int npts = cpdf_1d_points(marg, lo, hi, res);
+    if (npts == 0) return 1;
     draw_bar_nu();
     bind();  // after the draws: a thread's first stream creation must not come between them
-    const int npts = (int)((hi - lo + res - 1e-15) / res) + 1;
     std::vector<double> xs(npts), ys(npts, 0.0);
     for (int i = 0; i < npts; i++) xs[i] = std::min(lo + i * res, hi);
     const double norm = fz.re;
@@ -1666,14 +1907,14 @@ int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<doub
         VecArg bn;
         for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
         TermPool IN = in.view();
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_cpdf1d_terms<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt,
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_cpdf1d_terms<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt,
                                              marg, bn, gk[gcur].ptr, gv[gcur].ptr, xs[0], cpdf_direct.ptr,
                                              cpdf_cache.ptr));
-        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
+        MCE_LAUNCH(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
         if (npts > 1) {
-            hipLaunchKernelGGL(k_cpdf1d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
+            MCE_LAUNCH(k_cpdf1d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
                                cpdf_cache.ptr, Nt, cpdf_x.ptr + 1, npts - 1, cpdf_part.ptr);
-            hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
+            MCE_LAUNCH(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
                                nchunks, npts - 1, norm, cpdf_y.ptr + 1);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(ys.data() + 1, cpdf_y.ptr + 1, (npts - 1) * sizeof(double), hipMemcpyDeviceToHost,
@@ -1761,15 +2002,15 @@ int Engine::cpdf_2d(int i1, int i2, double lox, double hix, double resx, double 
         VecArg bn;
         for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
         TermPool IN = in.view();
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_cpdf2d_terms<DD>, dim3(ceil_div(Nt, 128)), dim3(128), 0, st, IN, Nt,
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_cpdf2d_terms<DD>, dim3(ceil_div(Nt, 128)), dim3(128), 0, st, IN, Nt,
                                              i1, i2, bn, gk[gcur].ptr, gv[gcur].ptr, xs[0], ys[0], C,
                                              cpdf_direct.ptr, c2_err.ptr));
-        hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
+        MCE_LAUNCH(k_sum_fixed, dim3(1), dim3(1024), 0, st, cpdf_direct.ptr, Nt, cpdf_sum.ptr);
         if (npts > 1) {
-            hipLaunchKernelGGL(k_cpdf2d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
+            MCE_LAUNCH(k_cpdf2d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st,
                                C, Nt, (const int*)nullptr, c2_px.ptr + 1, c2_py.ptr + 1, npts - 1, cpdf_part.ptr,
                                c2_err.ptr);
-            hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
+            MCE_LAUNCH(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, st, cpdf_part.ptr,
                                nchunks, npts - 1, 1.0, cpdf_y.ptr + 1);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(zs.data() + 1, cpdf_y.ptr + 1, (npts - 1) * sizeof(double), hipMemcpyDeviceToHost,
@@ -1868,7 +2109,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         const int KM = P.largest_shape() + S.largest_shape();
         HIPCHK(hipMemsetAsync(P.c2_err.ptr, 0, sizeof(int), st));
 #define MCE_RSYS(K_)                                                                                             \
-    MCE_DISPATCH_D(d, hipLaunchKernelGGL((k_rsys_pairs<DD, K_>), dim3(blocks), dim3(128), 0, st, PI, np,         \
+    MCE_DISPATCH_D(d, MCE_LAUNCH((k_rsys_pairs<DD, K_>), dim3(blocks), dim3(128), 0, st, PI, np,         \
                                          P.gk[P.gcur].ptr, P.gv[P.gcur].ptr, SI, ns, S.gk[S.gcur].ptr,           \
                                          S.gv[S.gcur].ptr, R, C, P.rs_mom.ptr, P.rs_keep.ptr, P.c2_err.ptr))
         if (KM <= 16) {
@@ -1879,9 +2120,9 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
             MCE_RSYS(kMaxM);
         }
 #undef MCE_RSYS
-        hipLaunchKernelGGL(k_colsum_part, dim3(kColSlices, 14), dim3(256), 0, st, P.rs_mom.ptr, npairs,
+        MCE_LAUNCH(k_colsum_part, dim3(kColSlices, 14), dim3(256), 0, st, P.rs_mom.ptr, npairs,
                            P.rs_part.ptr);
-        hipLaunchKernelGGL(k_colsum_finish, dim3(1), dim3(64), 0, st, P.rs_part.ptr, 14, P.rs_sums.ptr);
+        MCE_LAUNCH(k_colsum_finish, dim3(1), dim3(64), 0, st, P.rs_part.ptr, 14, P.rs_sums.ptr);
         P.cub([&](void* t, size_t& b) {
             return hipcub::DeviceSelect::Flagged(t, b, hipcub::CountingInputIterator<int>(0), P.rs_keep.ptr,
                                                  P.rs_idx.ptr, P.rs_cnt.ptr, (int)npairs, st);
@@ -1933,9 +2174,9 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         HIPCHK(hipMemcpyAsync(P.c2_px.ptr, px.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(P.c2_py.ptr, py.data(), npts * sizeof(double), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(P.c2_err.ptr, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_cpdf2d_grid, dim3(ceil_div(npts, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st, C,
+        MCE_LAUNCH(k_cpdf2d_grid, dim3(ceil_div(npts, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, st, C,
                            nkeep, P.rs_idx.ptr, P.c2_px.ptr, P.c2_py.ptr, npts, P.cpdf_part.ptr, P.c2_err.ptr);
-        hipLaunchKernelGGL(k_cpdf1d_finish, dim3(ceil_div(npts, 256)), dim3(256), 0, st, P.cpdf_part.ptr, nchunks, npts,
+        MCE_LAUNCH(k_cpdf1d_finish, dim3(ceil_div(npts, 256)), dim3(256), 0, st, P.cpdf_part.ptr, nchunks, npts,
                            1.0, P.cpdf_y.ptr);
         HIPCHK(hipGetLastError());
         int singular = 0;
@@ -1957,12 +2198,13 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
 }
 
 void Engine::finalize_extended_moments(double* x_bar) {
+    if (ntan) throw std::runtime_error("the extended-mode finish carries no tangents");
     bind();
     if (!skip) {
         VecArg dx;
         for (int i = 0; i < d; i++) dx.v[i] = mean[i].re;
         TermPool IN = in.view();
-        MCE_DISPATCH_D(d, hipLaunchKernelGGL(k_sub_b<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dx));
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_sub_b<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dx));
         HIPCHK(hipGetLastError());
     }
     for (int i = 0; i < d; i++) mean[i].re += x_bar[i];
@@ -2605,6 +2847,17 @@ int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering) {
         fprintf(stderr, "[ERROR pycauchy_set_tr_search_idxs_ordering:] Cannot run estimation problems over 12 dimensions currently!\n");
         return 1;
     }
+    // every entry becomes an axis index into a term's b vector: refuse what cannot be one
+    if (size_ordering < 0 || (size_ordering > 0 && !ordering)) {
+        fprintf(stderr, "[ERROR pycauchy_set_tr_search_idxs_ordering:] invalid ordering array\n");
+        return 1;
+    }
+    for (int i = 0; i < size_ordering; i++)
+        if (ordering[i] < 0 || ordering[i] >= 12) {
+            fprintf(stderr, "[ERROR pycauchy_set_tr_search_idxs_ordering:] axis %d at position %d is outside 0..11\n",
+                    ordering[i], i);
+            return 1;
+        }
     for (int i = 0; i < size_ordering; i++) g_search_order[i] = ordering[i];
     return 0;
 }
@@ -2671,6 +2924,53 @@ int mce_estimator_step(void* handle, double msmt, double* Phi, double* Gamma, do
         set_err("mce_estimator_step", ex);
         return -2;
     }
+}
+
+int mce_estimator_enable_tangents(void* handle, int num_directions) {
+    if (!handle) return -1;
+    try {
+        ((Handle*)handle)->eng->enable_tangents(num_directions);
+        return 0;
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_enable_tangents", ex);
+        return -1;
+    }
+}
+
+int mce_estimator_step_tangent(void* handle, double msmt, double* Phi, double* Gamma, double* beta, double* H,
+                               double gamma, double* B, double* u, const double* dPhi, const double* dbeta,
+                               const double* dH, const double* dgamma, double* dfz) {
+    Engine* e = ((Handle*)handle)->eng;
+    if (!e->ntan) {
+        set_err("mce_estimator_step_tangent", std::runtime_error("tangent mode is not enabled"));
+        return -2;
+    }
+    if ((e->errs & (1 << 9)) || e->master_step == e->n_mu) return -1;
+    const int P = e->ntan, d = e->d;
+    memset(&e->htan, 0, sizeof(e->htan));
+    for (int k = 0; k < P; k++) {
+        for (int i = 0; i < d * d; i++) e->htan.Phi[k][i] = dPhi ? dPhi[(size_t)k * d * d + i] : 0.0;
+        for (int i = 0; i < d; i++) e->htan.H[k][i] = dH ? dH[(size_t)k * d + i] : 0.0;
+        e->htan.gamma[k] = dgamma ? dgamma[k] : 0.0;
+    }
+    e->tan_dbeta.assign((size_t)P * e->pncc, 0.0);
+    if (dbeta)
+        for (size_t i = 0; i < e->tan_dbeta.size(); i++) e->tan_dbeta[i] = dbeta[i];
+    e->tan_seeded = true;
+    int rc;
+    try {
+        rc = e->update(msmt, Phi, Gamma, beta, H, gamma, B, u);
+    } catch (const std::exception& ex) {
+        e->tan_seeded = false;
+        set_err("mce_estimator_step_tangent", ex);
+        return -2;
+    }
+    e->tan_seeded = false;
+    for (int k = 0; k < P; k++) {
+        dfz[2 * k] = e->dfz_tan[k].re;
+        dfz[2 * k + 1] = e->dfz_tan[k].im;
+    }
+    return rc;
 }
 
 int mce_estimator_moments(void* handle, double* fz, double* mean, double* cov) {
@@ -2802,3 +3102,5 @@ long long mce_collect_outputs(void** outs, const int* sizes, long long count, in
 }
 
 }  // extern "C"
+
+#include "mce_window_bank.h"

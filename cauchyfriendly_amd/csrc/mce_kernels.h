@@ -349,6 +349,10 @@ __global__ void k_btab_simple(TermPool P, int n, const int64_t* __restrict__ bt_
         case kModeEnum: {
             const int slot = atomicAdd(enum_count, 1);
             enum_list[slot] = i;
+            // the enumerator's LDS is sized by the largest shape and parent hyperplane
+            // count among the parents it visits
+            atomicMax(enum_count + 4, P.m[i]);
+            atomicMax(enum_count + 5, P.phc[i]);
             P.bt_n[i] = 0;
         } break;
         default:
@@ -360,12 +364,16 @@ __global__ void k_btab_simple(TermPool P, int n, const int64_t* __restrict__ bt_
 // Cells of the time-propagated arrangement by vertex enumeration,
 // cell_enumeration.hpp:676-855.  One block per parent; the sign vectors seen are
 // an LDS bitset of 2^m bits, so the output comes out sorted and de-duplicated.
+// The parent's g-table keys are a second LDS bitset, `pofs` words after the first,
+// or, when `psearch` is set (both sets would not fit in LDS: 2^20 + 2^18 bits), are
+// looked up by binary search in the parent's sorted keys in global memory.
 template <int D>
 __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __restrict__ list,
                                                    const double* __restrict__ bpert, const uint8_t* __restrict__ combos,
                                                    const int64_t* __restrict__ combo_off, const int* __restrict__ combo_cnt,
                                                    const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ Barena,
-                                                   const int* __restrict__ btcap, int* __restrict__ err) {
+                                                   const int* __restrict__ btcap, int* __restrict__ err, int pofs,
+                                                   int psearch) {
     extern __shared__ uint32_t bits[];
     __shared__ int scan_buf[256];
     const int i = list[blockIdx.x];
@@ -373,23 +381,23 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
     const int m = P.m[i];
     const int phc = P.phc[i];
     const int W = m >= 5 ? (1 << (m - 5)) : 1;
-    // bitset of the parent's g-table keys (all below 2^(phc-1)), after the 2^MM-bit
-    // set of candidate sign vectors
-    uint32_t* pbits = bits + (MM >= 5 ? (1 << (MM - 5)) : 1);
+    // bitset of the parent's g-table keys (all below 2^(phc-1)), after the 2^m-bit
+    // sets of candidate sign vectors of every parent of the launch
+    uint32_t* pbits = bits + pofs;
     const uint32_t ptop = 1u << (phc - 1);
-    const int PW = (int)((ptop + 31u) >> 5);
+    const int PW = psearch ? 0 : (int)((ptop + 31u) >> 5);
     for (int w = threadIdx.x; w < W; w += blockDim.x) bits[w] = 0;
     for (int w = threadIdx.x; w < PW; w += blockDim.x) pbits[w] = 0;
     __syncthreads();
-    {
-        const uint32_t* gk = gkeys + P.gp_off[i];
-        const int gn = P.gp_n[i];
+    const uint32_t* gk = gkeys + P.gp_off[i];
+    const int gn = P.gp_n[i];
+    // a parent key at or above 2^(phc-1) (a coaligned child's cell kept with bit phc-1
+    // set) can never equal a looked-up sign vector, which is reversed below it
+    if (!psearch)
         for (int q = threadIdx.x; q < gn; q += blockDim.x) {
             const uint32_t key = gk[q];
             if (key < ptop) atomicOr(&pbits[key >> 5], 1u << (key & 31));
-            else atomicOr(err, 4);
         }
-    }
     const double* A = P.A + (int64_t)i * MM * D;
     const double* bp = bpert + (int64_t)P.tid[i] * MM;
     const uint8_t* cmb = combos + combo_off[m];
@@ -437,7 +445,19 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
             x &= x - 1;
             uint32_t psv = ((uint32_t)(w * 32 + bt)) & rev_phc;
             if (psv & top_phc) psv ^= rev_phc;
-            if ((pbits[psv >> 5] >> (psv & 31)) & 1u) keep |= (1u << bt);
+            bool in_parent;
+            if (!psearch) {
+                in_parent = (pbits[psv >> 5] >> (psv & 31)) & 1u;
+            } else {
+                int lo = 0, hi = gn;  // first key >= psv
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (gk[mid] < psv) lo = mid + 1;
+                    else hi = mid;
+                }
+                in_parent = lo < gn && gk[lo] == psv;
+            }
+            if (in_parent) keep |= (1u << bt);
         }
         bits[w] = keep;
     }
@@ -1888,14 +1908,16 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         for (int i = lane; i < T.pn; i += kWave) lds[i] = src[i];
         T.pkeys = lds;
     } else if (MODE >= kLookupRank) {
-        // keys are sorted ascending and below klim; W words cover them
+        // keys are sorted ascending; W words cover those below klim.  A key at or above
+        // klim (a coaligned child's cell whose bit m-1 was left set, kept as a parent key)
+        // sorts after every key below it and is never looked up (lookups reverse their
+        // key below klim, eval_gs.hpp:116-151), so it only ends the prefix counts.
         const uint32_t* src = gk_prev + go;
         const int W = (int)((T.klim + 31u) >> 5);
         uint32_t* bits = lds;
         uint32_t* pre = lds + wmax;
         for (int w = lane; w < W; w += kWave) bits[w] = 0;
         __syncthreads();
-        bool bad = false;
         // eight keys (and, with LDS values, eight values) per lane per round; all
         // loads of a round are issued before the LDS updates
         constexpr int U = 8;
@@ -1918,7 +1940,6 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
                 if (MODE == kLookupRankVals) dst[i] = vv[u];
                 const uint32_t key = kc[u];
                 if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
-                else bad = true;
                 // pre[x] = index of the first key in word >= x, filled for the words
                 // between the previous key's word and this key's word
                 const int w = min((int)(key >> 5), W - 1);
@@ -1929,7 +1950,6 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         const int lastw = T.pn ? min((int)(src[T.pn - 1] >> 5), W - 1) : -1;
         for (int w = lane; w < W; w += kWave)
             if (w > lastw) pre[w] = T.pn;
-        if (bad) atomicOr(rare, 4);
         T.rbits = bits;
         T.rpre = pre;
         if (MODE == kLookupRankVals) {
@@ -1968,7 +1988,8 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                                                   cplx* __restrict__ scratch, double gscale, int* __restrict__ surv,
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
                                                   int* __restrict__ sort_count, int* __restrict__ rare,
-                                                  int big_members, int* __restrict__ defer_k0) {
+                                                  int big_members, int* __restrict__ defer_k0,
+                                                  uint8_t* __restrict__ mem_add) {
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
@@ -1990,7 +2011,10 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     int r = grp_root[g];
     uint32_t X = 0;
     if constexpr (MODE >= kLookupRank) {
-        // bitset of the group's B-table keys (all below 2^(m-1)), for add_gtables
+        // bitset of the group's B-table keys below 2^(m-1), for add_gtables.  A coaligned
+        // child's projected cells are not re-normalised (cell_enumeration.hpp:206-371), so
+        // a key can have bit m-1 set; add_gtables only ever asks for keys below 2^(m-1)
+        // (flattening.hpp:297-312), so such a key is never found and is left out.
         uint32_t* bset = lds_dyn + 2 * wmax;
         const int WB = (int)((top + 31u) >> 5);
         for (int w = lane; w < WB; w += kWave) bset[w] = 0;
@@ -1998,7 +2022,6 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
         for (int i = lane; i < n; i += kWave) {
             const uint32_t key = B[i];
             if (key < top) atomicOr(&bset[key >> 5], 1u << (key & 31));
-            else atomicOr(rare, 4);
         }
         // made visible by the barriers of stage_term
     }
@@ -2080,6 +2103,7 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                     if (i < n) v = eval_cell<MODE, KM>(Tk, qr, cm_s, cs_s, B[i] ^ X ^ sh);
                     sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
                 }
+                if (mem_add && lane == 0) mem_add[grp_moff[g] + k] = sm ? 0 : 1;
                 if (sm) continue;
                 // add_gtables: root cell bi takes the member's value at bi ^ sigma
                 // (reversed and conjugated when that key is in the lower half), which
@@ -2113,6 +2137,7 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
                     }
                     if (sm) sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
                 }
+                if (mem_add && lane == 0) mem_add[grp_moff[g] + k] = sm ? 0 : 1;
                 if (sm) continue;
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __syncthreads();
@@ -2190,7 +2215,8 @@ __global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng
                                                      const cplx* __restrict__ gv_prev, double gscale,
                                                      const int* __restrict__ surv, const int* __restrict__ gxor,
                                                      const int* __restrict__ defer_k0, cplx* __restrict__ contrib,
-                                                     uint8_t* __restrict__ cneg, int* __restrict__ rare) {
+                                                     uint8_t* __restrict__ cneg, int* __restrict__ rare,
+                                                     uint8_t* __restrict__ mem_add) {
     static_assert(MODE >= kLookupRank, "parallel member sums use the rank lookup");
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     double qr[KM];
@@ -2234,6 +2260,7 @@ __global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng
         sm = wave_all(i >= n || !((psk * hypot(v.re, v.im)) > kApproxEps));
     }
     if (lane == 0) cneg[item] = sm ? 1 : 0;
+    if (mem_add && lane == 0) mem_add[grp_moff[g] + k] = sm ? 0 : 1;
     if (sm) return;
     cplx* out = contrib + coff[g] + (int64_t)k * n;
     for (int i = lane; i < n; i += kWave) {

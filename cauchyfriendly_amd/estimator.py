@@ -52,12 +52,31 @@ class GpuEstimator:
         _native.lib().mce_get_random_draws(self.h, r.ctypes.data_as(_dp), b.ctypes.data_as(_dp))
         return r, b.reshape(8, -1)
 
-    def step(self, z, Phi, Gamma, beta, H_row, gamma, B=None, u=None):
+    def enable_tangents(self, num_directions):
+        """Tangent mode: every later step also carries num_directions derivative directions
+        (before the window's first update)."""
+        L = _native.lib()
+        if L.mce_estimator_enable_tangents(self.h, int(num_directions)) != 0:
+            raise RuntimeError("mce_estimator_enable_tangents failed: " + L.mce_last_error().decode())
+        self.ntan = int(num_directions)
+
+    def step(self, z, Phi, Gamma, beta, H_row, gamma, B=None, u=None, tangents=None):
+        """One scalar update.  In tangent mode, tangents = (dPhi (P, d, d), dbeta (P, pncc),
+        dH (P, d), dgamma (P,)), the update's seed directions, and the record gains "dfz":
+        d fz / d theta (complex, P) of the pre-reduction fz."""
         L = _native.lib()
         Phi_, Gam_, beta_, H_ = _d(Phi), _d(Gamma), _d(beta), _d(H_row)
         B_ = _d(B) if B is not None else (None, None)
         u_ = _d(u) if u is not None else (None, None)
-        rc = L.mce_estimator_step(self.h, float(z), Phi_[1], Gam_[1], beta_[1], H_[1], float(gamma), B_[1], u_[1])
+        dfz = None
+        if tangents is None:
+            rc = L.mce_estimator_step(self.h, float(z), Phi_[1], Gam_[1], beta_[1], H_[1], float(gamma), B_[1], u_[1])
+        else:
+            dP, db, dH, dg = (_d(a) for a in tangents)
+            out = np.zeros(2 * self.ntan)
+            rc = L.mce_estimator_step_tangent(self.h, float(z), Phi_[1], Gam_[1], beta_[1], H_[1], float(gamma), B_[1],
+                                              u_[1], dP[1], db[1], dH[1], dg[1], out.ctypes.data_as(_dp))
+            dfz = out[0::2] + 1j * out[1::2]
         if rc == -2:
             raise RuntimeError("mce_estimator_step failed: " + L.mce_last_error().decode())
         d, sr = self.d, self.shape_range
@@ -80,6 +99,7 @@ class GpuEstimator:
             "threads": int(cnt[3]),
             "shape_mu": cnt[4:4 + sr].copy(),
             "shape_ftr": cnt[4 + sr:4 + 2 * sr].copy(),
+            "dfz": dfz,
         }
 
     def terms(self, with_tables=True):

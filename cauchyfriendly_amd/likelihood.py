@@ -210,3 +210,114 @@ def loglik_gradient(systems, h=1e-6, params=None, num_threads=16, shrink=SHRINK,
     if details:
         return {"L": L0, "grad": grad, "valid": valid, "h": steps, "terms": terms, "runs": runs}
     return L0, grad, valid
+
+
+# ---------------------------------------------------------------- tangent-kernel gradient
+# The gradient of L by the engine's tangent kernels: one forward pass per window carries P
+# derivative directions (mce_estimator_step_tangent), with the term tree's discrete
+# structure that of the nominal pass.  Nothing is stored between steps: every step
+# recomputes the forward with its tangents.  Directions are ("beta", j), ("gamma", i),
+# ("Phi", r, c) (the same perturbation of Phi at every step) and ("H", i, j).
+
+def theta_directions(s):
+    """Directions of PARAM_NAMES for a single-noise LTI system: beta, gamma, Phi, H."""
+    n = s.n
+    return [("beta", 0), ("gamma", 0)] + [("Phi", r, c) for r in range(n) for c in range(n)] + \
+        [("H", 0, j) for j in range(n)]
+
+
+def scale_directions(s):
+    """The Cauchy scales: beta_1 .. beta_pncc, gamma_1 .. gamma_p."""
+    return [("beta", j) for j in range(s.pncc)] + [("gamma", i) for i in range(s.p)]
+
+
+def _seeds(s, dirs, row):
+    """(dPhi, dbeta, dH, dgamma) of measurement row `row` for the directions."""
+    P, n = len(dirs), s.n
+    dPhi = np.zeros((P, n, n))
+    dbeta = np.zeros((P, s.pncc))
+    dH = np.zeros((P, n))
+    dgamma = np.zeros(P)
+    for k, dr in enumerate(dirs):
+        if dr[0] == "beta":
+            dbeta[k, dr[1]] = 1.0
+        elif dr[0] == "gamma":
+            dgamma[k] = 1.0 if dr[1] == row else 0.0
+        elif dr[0] == "Phi":
+            dPhi[k, dr[1], dr[2]] = 1.0
+        elif dr[0] == "H":
+            if dr[1] == row:
+                dH[k, dr[2]] = 1.0
+        else:
+            raise ValueError(f"unknown direction {dr}")
+    return dPhi, dbeta, dH, dgamma
+
+
+def window_loglik_tangent(s, dirs, beta=None, gamma=None, Phi=None, H=None):
+    """(L, dL/dtheta along dirs, structure) of one window of system s: L = sum over every
+    scalar update of log Re fz (fz before term reduction), computed on the GPU with the
+    tangent kernels.  beta / gamma / Phi / H replace the system's own (Phi: one matrix for
+    every step); structure is the (terms after MU/MUC, after reduction) of each update."""
+    from .estimator import GpuEstimator
+    beta = np.ravel(s.beta) if beta is None else np.ravel(beta)
+    gamma = np.ravel(s.gamma) if gamma is None else np.ravel(gamma)
+    H = s.H if H is None else np.asarray(H).reshape(s.p, s.n)
+    est = GpuEstimator(s.num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, cmcc=s.cmcc, seed=s.srand_seed)
+    try:
+        est.enable_tangents(len(dirs))
+        seeds = [_seeds(s, dirs, i) for i in range(s.p)]
+        Gt = np.ascontiguousarray(s.Gamma.reshape(-1))
+        L, dL, structure = 0.0, np.zeros(len(dirs)), []
+        for k in range(s.num_steps):
+            Phik = s.Phi(k) if Phi is None else np.asarray(Phi)
+            for i in range(s.p):
+                r = est.step(s.zs[k, i], Phik.reshape(-1), Gt, beta, H[i], gamma[i], tangents=seeds[i])
+                fz = r["fz"].real
+                L += float(np.log(fz))
+                dL += r["dfz"].real / fz
+                structure.append((r["nt_mu"], r["nt_ftr"]))
+        return L, dL, structure
+    finally:
+        est.close()
+
+
+def tangent_gradients(systems, thetas=None, dirs_of=theta_directions, num_threads=16):
+    """L and dL/dtheta of every window by the tangent kernels, the windows stepped from
+    num_threads host threads.  thetas[i] (optional) gives system i's (beta, gamma, Phi, H)
+    vector in PARAM_NAMES order (single-noise LTI systems).  Returns (L, grad)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        s = systems[i]
+        dirs = dirs_of(s)
+        if thetas is None:
+            L, g, _ = window_loglik_tangent(s, dirs)
+        else:
+            beta, gamma, Phi, H = _unpack(np.asarray(thetas[i], dtype=np.float64), s.n)
+            L, g, _ = window_loglik_tangent(s, dirs, beta, gamma, Phi, H)
+        return L, g
+
+    with ThreadPoolExecutor(max_workers=max(1, num_threads)) as ex:
+        res = list(ex.map(one, range(len(systems))))
+    return np.array([r[0] for r in res]), np.array([r[1] for r in res])
+
+
+def window_loglikelihood(theta, systems, num_threads=16):
+    """L(theta) of a batch of single-noise LTI windows as a differentiable torch function:
+    theta (batch, 2 + n^2 + n) float64 in PARAM_NAMES order -> L (batch,).  Its backward
+    multiplies by the gradient that the forward's tangent kernels computed."""
+    import torch
+
+    class _WindowLogLikelihood(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, th):
+            L, G = tangent_gradients(systems, th.detach().cpu().numpy(), num_threads=num_threads)
+            ctx.save_for_backward(torch.from_numpy(G).to(th.dtype))
+            return torch.from_numpy(L).to(th.dtype)
+
+        @staticmethod
+        def backward(ctx, gL):
+            (G,) = ctx.saved_tensors
+            return gL[:, None] * G
+
+    return _WindowLogLikelihood.apply(theta)

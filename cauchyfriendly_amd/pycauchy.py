@@ -281,6 +281,65 @@ class _Bindings:
                                            float(gamma), *o))
 
 
+    # ---- the window manager (pycauchy.hpp:80-221): one bank per process
+    def _wm_tail(self, debug_print, log_seq, log_full, log_dir, dt, init_step, win_var_boost):
+        w = _arr(win_var_boost if win_var_boost is not None else np.zeros(0))
+        self._wm_keep = w
+        ld = None if log_dir is None else str(log_dir).encode()
+        return [bool(debug_print), bool(log_seq), bool(log_full), ld, float(dt), int(init_step), w[1], w[2]]
+
+    @staticmethod
+    def _fptr(f):
+        return ct.c_void_p(f if isinstance(f, int) else ct.cast(f, ct.c_void_p).value)
+
+    def pycauchy_initialize_lti_window_manager(self, num_windows, num_sim_steps, A0, p0, b0, Phi, Gamma, B, beta, H,
+                                               gamma, debug_print, log_seq, log_full, log_dir, dt, init_step,
+                                               win_var_boost):
+        args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
+        flat = [x for a in args for x in (a[1], a[2])]
+        self._f("pycauchy_initialize_lti_window_manager")(
+            int(num_windows), int(num_sim_steps), *flat,
+            *self._wm_tail(debug_print, log_seq, log_full, log_dir, dt, init_step, win_var_boost))
+
+    def pycauchy_initialize_ltv_window_manager(self, num_windows, num_sim_steps, A0, p0, b0, Phi, Gamma, B, beta, H,
+                                               gamma, f_duc_ptr, debug_print, log_seq, log_full, log_dir, dt,
+                                               init_step, win_var_boost):
+        args = [_arr(v) for v in (A0, p0, b0, Phi, Gamma, B, beta, H, gamma)]
+        flat = [x for a in args for x in (a[1], a[2])]
+        self._f("pycauchy_initialize_ltv_window_manager")(
+            int(num_windows), int(num_sim_steps), *flat, self._fptr(f_duc_ptr),
+            *self._wm_tail(debug_print, log_seq, log_full, log_dir, dt, init_step, win_var_boost))
+
+    def pycauchy_initialize_nonlin_window_manager(self, num_windows, num_sim_steps, x0, A0, p0, b0, beta, gamma,
+                                                  f_duc_ptr1, f_duc_ptr2, f_duc_ptr3, cmcc, debug_print, log_seq,
+                                                  log_full, log_dir, dt, init_step, win_var_boost):
+        args = [_arr(v) for v in (x0, A0, p0, b0, beta, gamma)]
+        flat = [x for a in args for x in (a[1], a[2])]
+        self._f("pycauchy_initialize_nonlin_window_manager")(
+            int(num_windows), int(num_sim_steps), *flat, self._fptr(f_duc_ptr1), self._fptr(f_duc_ptr2),
+            self._fptr(f_duc_ptr3), int(cmcc),
+            *self._wm_tail(debug_print, log_seq, log_full, log_dir, dt, init_step, win_var_boost))
+
+    def pycauchy_step(self, msmts, controls):
+        """(fz, xhat, Phat, cerr_fz, cerr_xhat, cerr_Phat, win_idx, err_code) of the bank's best
+        window after this measurement (pycauchy.hpp:184-214)."""
+        m_, c_ = _arr(msmts), _arr(controls if controls is not None else np.zeros(0))
+        fz, cfz, cx, cP = ct.c_double(), ct.c_double(), ct.c_double(), ct.c_double()
+        x, P, sx, sP = _dp(), _dp(), ct.c_int(0), ct.c_int(0)
+        win, err = ct.c_int(0), ct.c_int(0)
+        self._f("pycauchy_step")(m_[1], m_[2], c_[1] if c_[2] > 0 else None, c_[2], ct.byref(fz), ct.byref(x),
+                                 ct.byref(sx), ct.byref(P), ct.byref(sP), ct.byref(cfz), ct.byref(cx), ct.byref(cP),
+                                 ct.byref(win), ct.byref(err))
+        if sx.value == 0:
+            msg = self._last_error() if self._last_error else ""
+            raise RuntimeError(f"pycauchy_step failed: {msg}")
+        return (fz.value, _take_d(x, sx.value), _take_d(P, sP.value), cfz.value, cx.value, cP.value, win.value,
+                err.value)
+
+    def pycauchy_shutdown(self):
+        self._f("pycauchy_shutdown")()
+
+
 _NAMES = [n for n in dir(_Bindings) if n.startswith("pycauchy_")]
 
 

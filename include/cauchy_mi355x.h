@@ -206,6 +206,53 @@ void pycauchy_single_step_shutdown(void* handle);
  * Returns 1 on error (more than 12 entries), 0 otherwise. */
 int pycauchy_set_tr_search_idxs_ordering(int* ordering, int size_ordering);
 
+/* ---- the window manager (one bank per process) -------------------------------------- */
+
+/* Replace pycauchy_initialize_{lti,ltv,nonlin}_window_manager, scripts/swig/cauchy/
+ * pycauchy.hpp:80-182: a bank of num_windows windows for num_sim_steps measurements, with
+ * the reference's arguments (sizes as for pycauchy_initialize_lti; win_var_boost, n entries,
+ * is added to the diagonal of the restart covariance; size_wvb == 0 for none).  The
+ * reference forks one process per window (SlidingWindowManager, include/cauchy_windows.hpp:
+ * 729-1705); here every window is an estimator of this library in the calling process, with
+ * the same schedule, window selection, Speyer restarts and outputs, and the caller's rand()
+ * state left as it was.  log_seq / log_full / log_dir are accepted and no log is written.
+ * On failure no bank is made (message in mce_last_error). */
+void pycauchy_initialize_lti_window_manager(int num_windows, int num_sim_steps, double* A0, int size_A0, double* p0,
+                                            int size_p0, double* b0, int size_b0, double* Phi, int size_Phi,
+                                            double* Gamma, int size_Gamma, double* B, int size_B, double* beta,
+                                            int size_beta, double* H, int size_H, double* gamma, int size_gamma,
+                                            bool debug_print, bool log_seq, bool log_full, char* log_dir, double dt,
+                                            int init_step, double* win_var_boost, int size_wvb);
+void pycauchy_initialize_ltv_window_manager(int num_windows, int num_sim_steps, double* A0, int size_A0, double* p0,
+                                            int size_p0, double* b0, int size_b0, double* Phi, int size_Phi,
+                                            double* Gamma, int size_Gamma, double* B, int size_B, double* beta,
+                                            int size_beta, double* H, int size_H, double* gamma, int size_gamma,
+                                            void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                                            bool debug_print, bool log_seq, bool log_full, char* log_dir, double dt,
+                                            int init_step, double* win_var_boost, int size_wvb);
+void pycauchy_initialize_nonlin_window_manager(int num_windows, int num_sim_steps, double* x0, int size_x0,
+                                               double* A0, int size_A0, double* p0, int size_p0, double* b0,
+                                               int size_b0, double* beta, int size_beta, double* gamma,
+                                               int size_gamma,
+                                               void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*),
+                                               void (*f_nonlinear_msmt_model)(CauchyDynamicsUpdateContainer*,
+                                                                              double*),
+                                               void (*f_extended_msmt_update_callback)(CauchyDynamicsUpdateContainer*),
+                                               int cmcc, bool debug_print, bool log_seq, bool log_full, char* log_dir,
+                                               double dt, int init_step, double* win_var_boost, int size_wvb);
+
+/* Replaces pycauchy_step, scripts/swig/cauchy/pycauchy.hpp:184-214: one measurement (p values,
+ * cmcc controls) into the bank; the best window's fz, x_hat (malloc'd, n), P_hat (malloc'd,
+ * n x n), their largest imaginary parts, its index and its error bits.  On error the arrays
+ * are NULL with size 0 and the index -1. */
+void pycauchy_step(double* msmts, int size_msmts, double* controls, int size_controls, double* out_swm_fz,
+                   double** out_xhat, int* size_out_xhat, double** out_Phat, int* size_out_Phat,
+                   double* out_swm_cerr_fz, double* out_swm_cerr_xhat, double* out_swm_cerr_Phat,
+                   int* out_swm_win_idx, int* out_swm_err_code);
+
+/* Replaces pycauchy_shutdown, scripts/swig/cauchy/pycauchy.hpp:216-221: ends the bank. */
+void pycauchy_shutdown(void);
+
 /* Replaces pycauchy_speyers_window_init, scripts/swig/cauchy/pycauchy.hpp:973-993: the
  * initial terms (A0 n x n rows, p0, b0; malloc'd) of a window restarted about the estimate
  * (xhat, Phat) after scalar measurement z = H x + v, v ~ Cauchy(0, gamma)
@@ -249,6 +296,25 @@ void mce_estimator_destroy(void* handle);
  * b -= Re(mean) for every term (not after a window's last update), mean += x_bar,
  * x_bar <- Re(mean).  Returns 0, or -1 on a device error. */
 int mce_estimator_finalize_extended_moments(void* handle, double* x_bar);
+
+/* ---- derivatives (no reference counterpart) ----------------------------------------- */
+
+/* Tangent mode for num_directions (1..16) derivative directions, before a window's first
+ * update (after mce_estimator_create or mce_estimator_reset).  Every later step carries the
+ * derivatives of the terms and g-tables along the directions, with the term structure of
+ * the nominal pass.  Returns 0, or -1 (message in mce_last_error). */
+int mce_estimator_enable_tangents(void* handle, int num_directions);
+
+/* mce_estimator_step in tangent mode.  Seeds of this update per direction k < P:
+ * dPhi (P x d x d), dbeta (P x pncc, before the pre-coalignment of Gamma's columns),
+ * dH (P x d, the derivative of this update's row H) and dgamma (P); a NULL seed is zero.
+ * Writes dfz (2P doubles, (re, im) per direction): the derivative of this update's
+ * measurement likelihood fz before term reduction (the moment sum the reference's
+ * cache_moments forms, include/cauchy_estimator.hpp:307-338).  Returns as
+ * mce_estimator_step; -2 on an error. */
+int mce_estimator_step_tangent(void* handle, double msmt, double* Phi, double* Gamma, double* beta, double* H,
+                               double gamma, double* B, double* u, const double* dPhi, const double* dbeta,
+                               const double* dH, const double* dgamma, double* dfz);
 
 /* ---- diagnostics (no reference counterpart) ---------------------------------------- */
 

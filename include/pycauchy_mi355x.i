@@ -39,7 +39,7 @@ import_array();
     (double* Gamma, int size_Gamma), (double* B, int size_B), (double* beta, int size_beta),
     (double* H, int size_H), (double* gamma, int size_gamma), (double* xbar, int size_xbar),
     (double* xhat, int size_xhat), (double* Phat, int size_Phat), (double* Trans, int size_Trans),
-    (double* bias, int size_bias), (double* Trel, int size_Trel)
+    (double* bias, int size_bias), (double* Trel, int size_Trel), (double* win_var_boost, int size_wvb)
 };
 %apply (int* IN_ARRAY1, int DIM1) { (int* ordering, int size_ordering) };
 
@@ -62,6 +62,8 @@ import_array();
 
 /* scalar outputs */
 %apply int* OUTPUT { int* out_num_gridx, int* out_num_gridy };
+%apply double* OUTPUT { double* out_swm_fz, double* out_swm_cerr_xhat, double* out_swm_cerr_Phat, double* out_swm_cerr_fz };
+%apply int* OUTPUT { int* out_swm_win_idx, int* out_swm_err_code };
 
 /* callbacks: the address of a ctypes function object */
 %typemap(in) void (*f_dyn_update_callback)(CauchyDynamicsUpdateContainer*) {
@@ -84,6 +86,8 @@ import_array();
 %ignore mce_estimator_reset;
 %ignore mce_estimator_destroy;
 %ignore mce_estimator_finalize_extended_moments;
+%ignore mce_estimator_enable_tangents;
+%ignore mce_estimator_step_tangent;
 %ignore mce_last_update;
 %ignore mce_shape_range;
 %ignore mce_terms_generated;

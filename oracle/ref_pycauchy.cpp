@@ -107,4 +107,44 @@ void refpy_get_reinitialization_statistics(void* h, double z, double* xhat, int 
     pycauchy_get_reinitialization_statistics(h, z, xhat, sx, Phat, sP, H, sH, gamma, oA, sA, op, sp, ob, sb);
 }
 
+// The reference's forked window manager (pycauchy.hpp:80-221): every window a child process.
+void refpy_initialize_lti_window_manager(int nw, int nsim, double* A0, int sA0, double* p0, int sp0, double* b0,
+                                         int sb0, double* Phi, int sPhi, double* Gamma, int sGamma, double* B, int sB,
+                                         double* beta, int sbeta, double* H, int sH, double* gamma, int sgamma,
+                                         bool dbg, bool lseq, bool lfull, char* ldir, double dt, int init_step,
+                                         double* wvb, int swvb) {
+    pycauchy_initialize_lti_window_manager(nw, nsim, A0, sA0, p0, sp0, b0, sb0, Phi, sPhi, Gamma, sGamma, B, sB, beta,
+                                           sbeta, H, sH, gamma, sgamma, dbg, lseq, lfull, ldir, dt, init_step, wvb,
+                                           swvb);
+}
+
+void refpy_initialize_ltv_window_manager(int nw, int nsim, double* A0, int sA0, double* p0, int sp0, double* b0,
+                                         int sb0, double* Phi, int sPhi, double* Gamma, int sGamma, double* B, int sB,
+                                         double* beta, int sbeta, double* H, int sH, double* gamma, int sgamma,
+                                         void (*cb)(CauchyDynamicsUpdateContainer*), bool dbg, bool lseq, bool lfull,
+                                         char* ldir, double dt, int init_step, double* wvb, int swvb) {
+    pycauchy_initialize_ltv_window_manager(nw, nsim, A0, sA0, p0, sp0, b0, sb0, Phi, sPhi, Gamma, sGamma, B, sB, beta,
+                                           sbeta, H, sH, gamma, sgamma, cb, dbg, lseq, lfull, ldir, dt, init_step, wvb,
+                                           swvb);
+}
+
+void refpy_initialize_nonlin_window_manager(int nw, int nsim, double* x0, int sx0, double* A0, int sA0, double* p0,
+                                            int sp0, double* b0, int sb0, double* beta, int sbeta, double* gamma,
+                                            int sgamma, void (*f_dyn)(CauchyDynamicsUpdateContainer*),
+                                            void (*f_msmt)(CauchyDynamicsUpdateContainer*, double*),
+                                            void (*f_ext)(CauchyDynamicsUpdateContainer*), int cmcc, bool dbg,
+                                            bool lseq, bool lfull, char* ldir, double dt, int init_step, double* wvb,
+                                            int swvb) {
+    pycauchy_initialize_nonlin_window_manager(nw, nsim, x0, sx0, A0, sA0, p0, sp0, b0, sb0, beta, sbeta, gamma,
+                                              sgamma, f_dyn, f_msmt, f_ext, cmcc, dbg, lseq, lfull, ldir, dt,
+                                              init_step, wvb, swvb);
+}
+
+void refpy_step(double* msmts, int sm, double* controls, int sc, double* fz, double** x, int* sx, double** P, int* sP,
+                double* cfz, double* cx, double* cP, int* win, int* err) {
+    pycauchy_step(msmts, sm, controls, sc, fz, x, sx, P, sP, cfz, cx, cP, win, err);
+}
+
+void refpy_shutdown() { pycauchy_shutdown(); }
+
 }  // extern "C"

@@ -141,7 +141,7 @@ def test_swig_interface_covers_every_wrapped_argument():
         for t in re.findall(r"\(([^)]*)\)", targets):
             applies[" ".join(t.split())] = " ".join(kind.split())
     outputs = set()
-    for targets in re.findall(r"%apply\s+int\s*\*\s*OUTPUT\s*\{(.*?)\};", body, flags=re.S):
+    for targets in re.findall(r"%apply\s+(?:int|double)\s*\*\s*OUTPUT\s*\{(.*?)\};", body, flags=re.S):
         outputs |= {" ".join(t.split()) for t in targets.split(",")}
     callbacks = set(re.findall(r"%typemap\(in\)\s*void\s*\(\*(\w+)\)", body))
     scalar = re.compile(r"^(int|double|bool|long long|void\*) \w+$")
@@ -173,3 +173,19 @@ def test_search_ordering_refuses_more_than_12_states():
     import numpy as np
     from cauchyfriendly_amd import pycauchy
     assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.arange(13, dtype=np.int32)) == 1
+
+
+def test_search_ordering_refuses_axes_outside_0_to_11():
+    """Each entry of the ordering becomes an axis index into the terms' b vectors, so the
+    setter refuses a negative axis or one beyond the 12 supported states and leaves the
+    ordering unchanged (host code only); an engine also refuses an axis at or beyond its
+    own state dimension before it reduces."""
+    import numpy as np
+    from cauchyfriendly_amd import pycauchy
+    default = np.arange(12, dtype=np.int32)
+    try:
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.array([0, -1, 2], dtype=np.int32)) == 1
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.array([12, 0], dtype=np.int32)) == 1
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.array([2, 0, 1], dtype=np.int32)) == 0
+    finally:
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(default) == 0

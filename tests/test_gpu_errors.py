@@ -64,6 +64,32 @@ def test_initializer_refuses_windows_beyond_the_hyperplane_limits():
     pycauchy.pycauchy_single_step_shutdown(h)
 
 
+def test_window_at_the_hyperplane_limit_steps_like_the_oracle():
+    """The 20-hyperplane window the initializer accepts (4 states, two process noises, 9
+    steps) steps to its end: row strides, LDS sets and g-table lookups are sized for 20
+    hyperplanes.  Every update equals the CPU restatement (counts and error bits exactly,
+    moments within 1e-9)."""
+    from conftest import rel_err
+    from cauchyfriendly_amd.estimator import GpuEstimator
+    from oracle import driver
+    n, pncc, steps = 4, 2, 9
+    zs = np.random.default_rng(5).standard_normal((steps, 1))
+    s = W.explicit_system("mm20", np.eye(n), np.ones((n, pncc)), np.ones((1, n)), np.full(pncc, 0.1),
+                          np.array([0.2]), np.eye(n), np.full(n, 0.1), np.zeros(n), zs)
+    print("case mm20: 4 states, pncc 2, 9 steps", flush=True)
+    est = GpuEstimator(steps, n, pncc, 1, s.A0, s.p0, s.b0, seed=s.srand_seed)
+    assert est.shape_range == 21
+    got = [est.step(s.zs[k, 0], s.Phi(k).reshape(-1), s.Gamma.reshape(-1), s.beta, s.H[0], s.gamma[0])
+           for k in range(steps)]
+    est.close()
+    _, exp = driver.run_system(s, kind="oracle")
+    assert len(got) == len(exp) == steps
+    for u, (a, b) in enumerate(zip(got, exp)):
+        assert (a["nt_mu"], a["nt_ftr"], a["err"]) == (b["nt_mu"], b["nt_ftr"], b["err"]), f"update {u}"
+        assert np.array_equal(a["shape_mu"], b["shape_mu"]) and np.array_equal(a["shape_ftr"], b["shape_ftr"])
+        assert max(rel_err(a["fz"], b["fz"]), rel_err(a["mean"], b["mean"]), rel_err(a["cov"], b["cov"])) <= 1e-9
+
+
 def test_density_grid_beyond_1e8_points_returns_no_points():
     from cauchyfriendly_amd.cauchy_estimator import PyCauchyEstimator
     s = W.config0()
@@ -78,3 +104,19 @@ def test_density_grid_beyond_1e8_points_returns_no_points():
     X, Y = e.get_marginal_1D_pointwise_cpdf(0, -1.0, 1.0, 0.5)    # a sane grid afterwards
     assert X.size == 5 and np.all(Y > 0)
     e.shutdown()
+
+
+def test_search_ordering_beyond_the_state_dimension_is_refused_by_the_step():
+    """An ordering accepted by the setter (axes below 12) that names an axis a 3-state
+    estimator does not have makes its step fail with a message; nothing reads past a term's
+    b vector."""
+    from cauchyfriendly_amd.estimator import GpuEstimator
+    s = W.config0()
+    try:
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.array([5, 0, 1], dtype=np.int32)) == 0
+        est = GpuEstimator(s.num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, seed=s.srand_seed)
+        with pytest.raises(RuntimeError, match="search ordering names axis 5"):
+            est.step(s.zs[0, 0], s.Phi(0).reshape(-1), s.Gamma.reshape(-1), s.beta, s.H[0], s.gamma[0])
+        est.close()
+    finally:
+        assert pycauchy.pycauchy_set_tr_search_idxs_ordering(np.arange(12, dtype=np.int32)) == 0

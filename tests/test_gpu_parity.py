@@ -13,13 +13,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-9
 
 
-def run_gpu(s, num_steps, term_updates=()):
+def run_gpu(s, num_steps, term_updates=(), run_steps=None):
+    """An estimator for a window of num_steps, stepped for run_steps of them (all by default)."""
     est = GpuEstimator(num_steps, s.n, s.pncc, s.p, s.A0, s.p0, s.b0, cmcc=s.cmcc, seed=s.srand_seed)
     Gt = np.ascontiguousarray(s.Gamma.reshape(-1))
     B = None if s.B is None else np.ascontiguousarray(s.B.reshape(-1))
     out, terms = [], {}
     u = 0
-    for k in range(num_steps):
+    for k in range(num_steps if run_steps is None else run_steps):
         for i in range(s.p):
             out.append(est.step(s.zs[k, i], s.Phi(k).reshape(-1), Gt, s.beta, s.H[i], s.gamma[i], B, s.u(k)))
             if str(u) in term_updates:
@@ -33,7 +34,7 @@ def test_gpu_matches_reference_fixture(name):
     rec = load_golden(name)
     s = W.System.from_dict(rec["system"])
     print(f"case {name}: {rec['num_steps']} steps", flush=True)
-    est, out, terms = run_gpu(s, rec["num_steps"], tuple(rec["terms"].keys()))
+    est, out, terms = run_gpu(s, rec["num_steps"], tuple(rec["terms"].keys()), rec.get("run_steps"))
     worst = 0.0
     for u, (got, exp) in enumerate(zip(out, rec["updates"])):
         worst = max(worst, compare_update(got, exp, TOL, f"{name} update {u}"))

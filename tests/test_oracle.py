@@ -18,7 +18,7 @@ def run_oracle(rec, term_updates=()):
     B = None if s.B is None else np.ascontiguousarray(s.B.reshape(-1))
     out, terms = [], {}
     u = 0
-    for k in range(rec["num_steps"]):
+    for k in range(rec.get("run_steps", rec["num_steps"])):
         for i in range(s.p):
             out.append(est.step(s.zs[k, i], s.Phi(k).reshape(-1), Gt, s.beta, s.H[i], s.gamma[i], B, s.u(k)))
             if str(u) in term_updates:
