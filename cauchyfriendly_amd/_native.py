@@ -157,6 +157,9 @@ def lib():
     L.mce_estimator_step_tangent.restype = ct.c_int
     L.mce_estimator_step_tangent.argtypes = [ct.c_void_p, ct.c_double, _dp, _dp, _dp, _dp, ct.c_double, _dp, _dp,
                                              _dp, _dp, _dp, _dp, _dp]
+    L.mce_cpdf1d_noalloc.restype = ct.c_int
+    L.mce_cpdf1d_noalloc.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_void_p,
+                                     ct.c_void_p, ct.c_int, ct.c_void_p, _ip]
     _lib = L
     return L
 

@@ -1800,7 +1800,11 @@ struct Engine {
     // Fills xy with (x, f(x)) pairs; returns 1 (no points) after a window's last update,
     // whose g-tables are not built.
     int cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy);
-    // Checks of the grid; returns the point count, or 0 after the window's last
+    // The same density written to device arrays xs / ys (capacity points each) on `s`,
+    // with every intermediate in `work` (work_bytes): no device allocation.
+    int cpdf_1d_into(int marg, double lo, double hi, double res, double* xs_dev, double* ys_dev, int capacity,
+                     hipStream_t s, char* work, size_t work_bytes, int* npts_out);
+    // Checks shared by both forms; returns the point count, or 0 after the window's last
     // update (whose g-tables were not built).
     int cpdf_1d_points(int marg, double lo, double hi, double res);
     // 2D marginal density of states (i1, i2) on the grid of CauchyCPDFGridDispatcher2D
@@ -1884,6 +1888,55 @@ int Engine::cpdf_1d_points(int marg, double lo, double hi, double res) {
     }
     if ((hi - lo) / res > 1e8) throw std::runtime_error("the grid would exceed 1e8 points");
     return (int)((hi - lo + res - 1e-15) / res) + 1;
+}
+
+int Engine::cpdf_1d_into(int marg, double lo, double hi, double res, double* xs_dev, double* ys_dev, int capacity,
+                         hipStream_t s, char* work, size_t work_bytes, int* npts_out) {
+    *npts_out = 0;
+    const int npts = cpdf_1d_points(marg, lo, hi, res);
+    if (npts == 0) return 1;
+    if (!xs_dev || !ys_dev || capacity < npts)
+        throw std::runtime_error("the grid has " + std::to_string(npts) + " points; the output arrays hold " +
+                                 std::to_string(capacity));
+    // working memory: direct values and cached terms (40 B per term), the chunk partial
+    // sums (8 B per chunk of terms and grid point) and the direct sum
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const int nchunks = ceil_div(Nt, kCpdfChunk);
+    const size_t b_direct = up((size_t)Nt * sizeof(double)), b_cache = up((size_t)Nt * sizeof(CpdfTerm1D));
+    const size_t b_part = up((size_t)nchunks * (npts - 1) * sizeof(double));
+    const size_t need = b_direct + b_cache + b_part + 256;
+    if (need > work_bytes)
+        throw std::runtime_error("the density needs " + std::to_string(need) + " B of working memory (" +
+                                 std::to_string(Nt) + " terms, " + std::to_string(npts) +
+                                 " points); the no-allocation form has " + std::to_string(work_bytes));
+    draw_bar_nu();
+    double* direct = (double*)work;
+    CpdfTerm1D* cache = (CpdfTerm1D*)(work + b_direct);
+    double* part = (double*)(work + b_direct + b_cache);
+    double* dsum = (double*)(work + b_direct + b_cache + b_part);
+    MCE_LAUNCH(k_cpdf1d_xs, dim3(ceil_div(npts, 256)), dim3(256), 0, s, xs_dev, npts, lo, res, hi);
+    const double norm = fz.re;
+    if (Nt > 0) {
+        const double x0 = std::min(lo + 0 * res, hi);  // the first abscissa, formed as the host grid forms it
+        VecArg bn;
+        for (int i = 0; i < d; i++) bn.v[i] = bar_nu[i];
+        TermPool IN = in.view();
+        MCE_DISPATCH_D(d, MCE_LAUNCH(k_cpdf1d_terms<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, s, IN, Nt, marg, bn,
+                                     gk[gcur].ptr, gv[gcur].ptr, x0, direct, cache));
+        MCE_LAUNCH(k_sum_fixed, dim3(1), dim3(1024), 0, s, direct, Nt, dsum);
+        if (npts > 1) {
+            MCE_LAUNCH(k_cpdf1d_grid, dim3(ceil_div(npts - 1, kCpdfPoints), nchunks), dim3(kCpdfPoints), 0, s, cache,
+                       Nt, xs_dev + 1, npts - 1, part);
+            MCE_LAUNCH(k_cpdf1d_finish, dim3(ceil_div(npts - 1, 256)), dim3(256), 0, s, part, nchunks, npts - 1, norm,
+                       ys_dev + 1);
+        }
+        MCE_LAUNCH(k_cpdf1d_first, dim3(1), dim3(64), 0, s, dsum, norm, ys_dev, 0);
+    } else {
+        MCE_LAUNCH(k_cpdf1d_first, dim3(ceil_div(npts, 256)), dim3(256), 0, s, dsum, norm, ys_dev, npts);
+    }
+    HIPCHK(hipGetLastError());
+    *npts_out = npts;
+    return 0;
 }
 
 int Engine::cpdf_1d(int marg, double lo, double hi, double res, std::vector<double>& xy) {
@@ -2241,6 +2294,33 @@ struct Handle {
 void set_err(const char* where, const std::exception& e) {
     g_last_error = std::string(where) + ": " + e.what();
     fprintf(stderr, "[cauchy_mi355x] %s\n", g_last_error.c_str());
+}
+
+// ---------------------------------------------------------------- no-allocation working memory
+// The working memory of mce_cpdf1d_noalloc, for callers that cannot allocate device memory:
+// one array per GPU, in the library's code object, held until the process ends.  Every
+// call on a GPU shares its array, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocBytes = size_t(128) << 20;
+__device__ __align__(256) char g_noalloc_scratch[kNoAllocBytes];
+
+// Device address of this GPU's array, looked up once per device on its first use (the
+// array's name is a host-side stand-in, never a device address).
+char* noalloc_scratch() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    static std::mutex mu;
+    static char* ptr[64] = {};
+    if (dev < 0 || dev >= 64) throw std::runtime_error("device index beyond 63");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_noalloc_scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress of the no-allocation array: ") +
+                                     hipGetErrorString(e));
+        ptr[dev] = (char*)p;
+    }
+    return ptr[dev];
 }
 
 // Helper threads for the batch calls.  They live as long as the process (each keeps its
@@ -2707,6 +2787,21 @@ void pycauchy_get_marginal_1D_pointwise_cpdf(void* handle, int marg_idx1, double
     *size_out_cpdf_data = (int)xy.size();
     *out_cpdf_data = mallocd<double>(xy.size());
     if (!xy.empty()) memcpy(*out_cpdf_data, xy.data(), xy.size() * sizeof(double));
+}
+
+int mce_cpdf1d_noalloc(void* handle, int marg_idx, double gridx_low, double gridx_high, double gridx_resolution,
+                       double* out_x, double* out_f, int capacity, void* stream, int* out_num_gridx) {
+    if (out_num_gridx) *out_num_gridx = 0;
+    if (!handle || !out_num_gridx) return -1;
+    try {
+        char* work = noalloc_scratch();
+        return ((Handle*)handle)->eng->cpdf_1d_into(marg_idx, gridx_low, gridx_high, gridx_resolution, out_x, out_f,
+                                                    capacity, (hipStream_t)stream, work, kNoAllocBytes,
+                                                    out_num_gridx);
+    } catch (const std::exception& ex) {
+        set_err("mce_cpdf1d_noalloc", ex);
+        return -1;
+    }
 }
 
 void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,

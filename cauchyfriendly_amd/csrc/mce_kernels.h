@@ -2628,6 +2628,27 @@ __global__ void k_cpdf1d_finish(const double* __restrict__ part, int nchunks, in
 }
 
 
+// Grid abscissae of the 1D density, x_k = min(low + k res, high), as the host forms them
+// (no multiply-add is fused), for callers that keep the grid in device memory.
+__global__ void k_cpdf1d_xs(double* __restrict__ xs, int npts, double lo, double res, double hi) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    const double v = lo + (double)k * res;
+    xs[k] = hi < v ? hi : v;
+}
+
+// The density at the first grid point from the direct sum, (sum / 2 pi) / norm, or
+// zero everywhere when the estimator holds no terms.
+__global__ void k_cpdf1d_first(const double* __restrict__ direct_sum, double norm, double* __restrict__ y,
+                               int zero_n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero_n > 0) {
+        if (k < zero_n) y[k] = 0.0;
+        return;
+    }
+    if (k == 0) y[0] = direct_sum[0] * kInvTwoPi / norm;
+}
+
 // b <- b - dx for every term (cauchy_estimator.hpp:1371-1380, sub_vecs).
 template <int D>
 __global__ void k_sub_b(TermPool P, int n, VecArg dx) {

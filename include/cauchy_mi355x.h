@@ -133,6 +133,19 @@ void pycauchy_get_marginal_1D_pointwise_cpdf(void* handle, int marg_idx1, double
                                              double gridx_resolution, char* log_dir, double** out_cpdf_data,
                                              int* size_out_cpdf_data, int* out_num_gridx);
 
+/* No counterpart in the reference: pycauchy_get_marginal_1D_pointwise_cpdf for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The grid's x values and densities go to the caller's device arrays
+ * out_x and out_f (capacity points each), launched on `stream` (a hipStream_t; NULL for the
+ * null stream) without waiting: they are ready once the stream reaches this point.  Every
+ * intermediate (40 B per term, 8 B per 1024-term chunk and grid point) lives in one static
+ * 128 MiB device array per GPU, held until the process ends; a larger request, or a grid
+ * beyond `capacity`, is refused.  Calls on one GPU share that array: calls that overlap on one
+ * GPU race.  Returns 0 and the point count in out_num_gridx; 1 after a window's last update
+ * (as the allocating form, no points); -1 on error (message in mce_last_error). */
+int mce_cpdf1d_noalloc(void* handle, int marg_idx, double gridx_low, double gridx_high, double gridx_resolution,
+                       double* out_x, double* out_f, int capacity, void* stream, int* out_num_gridx);
+
 /* Replaces pycauchy_get_1D_pointwise_cpdf, scripts/swig/cauchy/pycauchy.hpp:934-945 (state 0). */
 void pycauchy_get_1D_pointwise_cpdf(void* handle, double gridx_low, double gridx_high, double gridx_resolution,
                                     char* log_dir, double** out_cpdf_data, int* size_out_cpdf_data,

@@ -86,6 +86,7 @@ import_array();
 %ignore mce_estimator_reset;
 %ignore mce_estimator_destroy;
 %ignore mce_estimator_finalize_extended_moments;
+%ignore mce_cpdf1d_noalloc;
 %ignore mce_estimator_enable_tangents;
 %ignore mce_estimator_step_tangent;
 %ignore mce_last_update;
