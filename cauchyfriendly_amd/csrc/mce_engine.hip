@@ -560,7 +560,7 @@ struct Engine {
     DBuf<double> dbpert, gy, mom_part, mom_out, mom_x;
     DBuf<uint8_t> dcombos;
     DBuf<int64_t> dcombo_off;
-    DBuf<int> dcombo_cnt, dcen_half, din_off;
+    DBuf<int> dcombo_cnt, dcen_half;
     DBuf<int> slots, slot_base, enum_list, counters, hist;
     DBuf<int64_t> btcap64, bt_base;
     DBuf<int> btcap;
@@ -685,7 +685,6 @@ struct Engine {
         counters.reserve(32);
         hist.reserve(sr + 1);
         hist_mx.reserve(sr + 2);
-        din_off.reserve(sr + 2);
         reset_terms();
     }
 
@@ -983,15 +982,63 @@ struct Engine {
         default: throw std::runtime_error("unsupported state dimension"); \
     }
 
-    void scan_i32_to_i32_incl(const int* in_, int* out_, int n) {
+    // Scans.  Up to kSmallScan items take one workgroup and one launch (k_small_scan, which
+    // also widens and forms the total); larger inputs the library's device scan.  A total is
+    // read back through the thread's device-mapped host slot, as read_total does.
+    int64_t mapped_total() {
+        volatile int64_t* slot = h_total;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        const int64_t v = *slot;
+        return v >= 0 ? v : read1(d_scal.ptr);
+    }
+    // Inclusive scan of n ints; returns the total (the last inclusive value).
+    int scan_i32_incl_total(const int* in_, int* out_, int n) {
+        if (n <= kSmallScan) {
+            *(volatile int64_t*)h_total = -1;
+            MCE_LAUNCH((k_small_scan<int, int, true>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n,
+                       (int*)nullptr, d_scal.ptr, h_total_dev);
+            return (int)mapped_total();
+        }
         cub([&](void* t, size_t& b) { return hipcub::DeviceScan::InclusiveSum(t, b, in_, out_, n, st); });
+        return n > 0 ? read1(out_ + n - 1) : 0;
     }
-    void scan_i64_excl(const int64_t* in_, int64_t* out_, int n) {
+    // Exclusive scan of n ints widened to 64 bits (the widened input kept in `wide`); returns
+    // the total.
+    int64_t widen_scan_total(const int* in_, int64_t* wide, int64_t* out_, int n) {
+        if (n <= kSmallScan) {
+            *(volatile int64_t*)h_total = -1;
+            MCE_LAUNCH((k_small_scan<int, int64_t, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n,
+                       wide, d_scal.ptr, h_total_dev);
+            return mapped_total();
+        }
+        widen(in_, wide, n);
+        scan_i64_excl(wide, out_, n);
+        return read_total(out_, wide, n);
+    }
+    template <class T>
+    void scan_excl(const T* in_, T* out_, int n) {
+        if (n <= kSmallScan) {
+            MCE_LAUNCH((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
+                       (int64_t*)nullptr, (int64_t*)nullptr);
+            return;
+        }
         cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
     }
-    void scan_i32_excl(const int* in_, int* out_, int n) {
-        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
+    // Exclusive scan with its total.
+    template <class T>
+    int64_t scan_excl_total(const T* in_, T* out_, int n) {
+        if (n <= kSmallScan) {
+            *(volatile int64_t*)h_total = -1;
+            MCE_LAUNCH((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
+                       d_scal.ptr, h_total_dev);
+            return mapped_total();
+        }
+        scan_excl(in_, out_, n);
+        return read_total(out_, in_, n);
     }
+    void scan_i64_excl(const int64_t* in_, int64_t* out_, int n) { scan_excl(in_, out_, n); }
+    void scan_i32_excl(const int* in_, int* out_, int n) { scan_excl(in_, out_, n); }
 
     // Joins the moment chains, adds the per-thread chunks in thread order
     // (cauchy_estimator.hpp:886-895) and finalises the moments.
@@ -1013,12 +1060,11 @@ struct Engine {
         if (!moments_pending) return;
         const int NC = 1 + d + d * d;
         const int T = cur_T;
-        std::vector<double> part((size_t)T * NC * 2);
+        // the partial sums, then the measurement update's error bits (k_mom_sum appends them)
+        std::vector<double> part((size_t)T * NC * 2 + 1);
         HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st2));
-        int mu_err = 0;
-        HIPCHK(hipMemcpyAsync(&mu_err, counters.ptr + 17, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st2));
-        HIPCHK(hipStreamSynchronize(st));
+        const int mu_err = (int)part[(size_t)T * NC * 2];
         moments_pending = false;
         if (prof.on) {
             float t = 0;
@@ -1119,14 +1165,14 @@ struct Engine {
                 throw std::runtime_error("term-reduction search ordering names axis " + std::to_string(sc.order[i]) +
                                          " of a " + std::to_string(d) + "-state estimator");
         }
+        // parents' shape bucket offsets travel with the constants (one upload)
+        sc.in_off[0] = 0;
+        for (int m = 0; m < sr; m++) sc.in_off[m + 1] = sc.in_off[m] + in_count[m];
         HIPCHK(hipMemcpyAsync(dconst.ptr, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
         if (ntan) {
             dstan.reserve(1);
             HIPCHK(hipMemcpyAsync(dstan.ptr, &htan, sizeof(StepTan), hipMemcpyHostToDevice, st));
         }
-        std::vector<int> in_off(sr + 1, 0);
-        for (int m = 0; m < sr; m++) in_off[m + 1] = in_off[m] + in_count[m];
-        HIPCHK(hipMemcpyAsync(din_off.ptr, in_off.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
         const int np = Nt;
         prof.begin(st);
 
@@ -1150,16 +1196,13 @@ struct Engine {
             MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp_tan<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, tin.view(), np,
                                          dconst.ptr, dstan.ptr));
         MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
-                                             din_off.ptr, dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
+                                             dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
-        scan_i32_to_i32_incl(slots.ptr, slot_base.ptr + 1, np);
-        const int nslots = read1(slot_base.ptr + np);
+        const int nslots = scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
         prof.mark(kPhTp, st);
         if (!skip && !first) {
             // B arena offsets (64-bit)
-            widen(btcap.ptr, btcap64.ptr, np);
-            scan_i64_excl(btcap64.ptr, bt_base.ptr, np);
-            barena.reserve((size_t)read_total(bt_base.ptr, btcap.ptr, np) + 1);
+            barena.reserve((size_t)widen_scan_total(btcap.ptr, btcap64.ptr, bt_base.ptr, np) + 1);
             enum_list.reserve(np);
             MCE_LAUNCH(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
                                gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr + 16);
@@ -1205,7 +1248,7 @@ struct Engine {
         chain_perm.reserve(nslots);
         iota.reserve(nslots);
         chain_off.reserve(16);
-        mom_part.reserve((size_t)T * (1 + d + d * d) * 2);
+        mom_part.reserve((size_t)T * (1 + d + d * d) * 2 + 1);
         TermPool OUT = out.view();
         // child rows in registers when the largest shape fits a compile-time bound
         const int mu_km = MM <= 8 ? 8 : MM <= 14 ? 14 : 0;
@@ -1253,7 +1296,7 @@ struct Engine {
                                                  0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr, T,
                                                  nslots, mom_x.ptr));
             MCE_DISPATCH_D(d, MCE_LAUNCH(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
-                                                 chain_off.ptr, T, mom_part.ptr));
+                                                 chain_off.ptr, T, mom_part.ptr, counters.ptr + 17));
         }
         HIPCHK(hipGetLastError());
         if (prof.on) HIPCHK(hipEventRecord(ev_c1, st2));
@@ -1282,10 +1325,10 @@ struct Engine {
             iota.reserve(nslots);
             perm.reserve(nslots);
             skeys.reserve(nslots);
-            MCE_LAUNCH(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+            // iota still holds 0..nslots-1: the chain sort only read it
             cub([&](void* t, size_t& b) {
                 return hipcub::DeviceRadixSort::SortPairs(t, b, keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0,
-                                                          32, st);
+                                                          kGatherKeyBits, st);
             });
             MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
@@ -1386,9 +1429,7 @@ struct Engine {
         goff.reserve(ng);
         MCE_LAUNCH(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
                            (int*)b2n.ptr);
-        widen(b2n.ptr, ub64.ptr, ng);
-        scan_i64_excl(ub64.ptr, b2off.ptr, ng);
-        const int64_t ub_total = read_total(b2off.ptr, ub64.ptr, ng);
+        const int64_t ub_total = widen_scan_total(b2n.ptr, ub64.ptr, b2off.ptr, ng);
         b2arena.reserve((size_t)ub_total + 1);
         for (int m = 1; m < sr; m++) {
             const int ngb = gbase_of[m + 1] - gbase_of[m];
@@ -1403,9 +1444,7 @@ struct Engine {
                                counters.ptr + 8);
             HIPCHK(hipGetLastError());
         }
-        widen(b2n.ptr, b2n64.ptr, ng);
-        scan_i64_excl(b2n64.ptr, goff.ptr, ng);
-        const int64_t cells_total = read_total(goff.ptr, b2n64.ptr, ng);
+        const int64_t cells_total = widen_scan_total(b2n.ptr, b2n64.ptr, goff.ptr, ng);
         prof.mark(kPhGBtab, st);
 
         // ---- group g-tables need the update's g scale 1/(2 pi Re fz)
@@ -1678,11 +1717,9 @@ struct Engine {
         wlo.reserve(N);
         wcnt.reserve(N);
         woff.reserve(N);
-        MCE_LAUNCH(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr);
-        scan_i64_excl(wcnt.ptr, woff.ptr, N);
-        const int64_t npairs = read_total(woff.ptr, wcnt.ptr, N);
+        MCE_LAUNCH(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr, ecnt.ptr);
+        const int64_t npairs = scan_excl_total(wcnt.ptr, woff.ptr, N);
         const bool by_pairs = npairs < ((int64_t)1 << 31);
-        HIPCHK(hipMemsetAsync(ecnt.ptr, 0, N * sizeof(int), st));
         if (by_pairs && npairs > 0) {
             fgb.reserve((size_t)N * d);
             fgp.reserve((size_t)N * MM);
@@ -1698,8 +1735,7 @@ struct Engine {
                                                  dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
         }
         HIPCHK(hipGetLastError());
-        scan_i32_excl(ecnt.ptr, eoff.ptr, N);
-        const int E = (int)read_total(eoff.ptr, ecnt.ptr, N);
+        const int E = (int)scan_excl_total(ecnt.ptr, eoff.ptr, N);
         Fc.reserve((size_t)8 * N);
         MCE_LAUNCH(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
         if (E > 0) {

@@ -19,6 +19,10 @@
 namespace mce {
 
 constexpr int kWave = 64;
+// Gather key of a slot that holds no term: above every term's ((m T + tid) 2 + child), which
+// stays below 336 for shapes up to 20 hyperplanes and T <= 8, so the sort needs 10 bits.
+constexpr uint32_t kGatherKeyNone = 0x3ffu;
+constexpr int kGatherKeyBits = 10;
 // Default: groups with more members than this add them with one wavefront per member.
 constexpr int kBigGroupMembers = 16;
 
@@ -35,6 +39,7 @@ struct StepConst {
     double msmt, gamma, gscale;
     int d, ng, cmcc, with_tp, skip, first, T, threaded;
     int order[kMaxD];          // fast-term-reduction axis order
+    int in_off[kMaxM + 2];     // parents' shape bucket offsets (bucket m: [in_off[m], in_off[m+1]))
 };
 
 // ------------------------------------------------------------------ small dense linear algebra
@@ -227,9 +232,8 @@ __device__ inline bool wave_all(bool pred) { return __all(pred); }
 // cauchy_term.hpp:435-531 (time_prop, normalize_hps, tp_coalign), with the B-table
 // mode decision of cauchy_estimator.hpp:678-708 (serial) / 1614-1637 (threaded).
 template <int D>
-__global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const int* __restrict__ in_off,
-                     const int* __restrict__ cen_half, int* __restrict__ slots, int* __restrict__ btcap,
-                     int* __restrict__ err) {
+__global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const int* __restrict__ cen_half,
+                     int* __restrict__ slots, int* __restrict__ btcap, int* __restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int MM = P.MM;
@@ -287,7 +291,7 @@ __global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const 
         P.m[i] = m;
     }
     int tid = 0;
-    if (sc->threaded) tid = (i - in_off[phc]) % sc->T;
+    if (sc->threaded) tid = (i - sc->in_off[phc]) % sc->T;
     P.tid[i] = tid;
     uint8_t mode;
     int cap;
@@ -688,7 +692,7 @@ __global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, Ter
     double* g_out = gy + (int64_t)s * (2 + 2 * D);
     if (t < m && !((integ >> t) & 1u)) {
         OUT.flags[s] = 0;
-        keys_out[s] = 0xffffffffu;
+        keys_out[s] = kGatherKeyNone;
         chain_key[s] = 15u;
         for (int k = 0; k < 2 + 2 * D; k++) g_out[k] = 0.0;
         return;
@@ -861,7 +865,7 @@ __global__ __launch_bounds__(256, 4) void k_mu_r(TermPool IN, int np, const int*
     double* g_out = gy + (int64_t)s * (2 + 2 * D);
     if (t < m && !((integ >> t) & 1u)) {
         OUT.flags[s] = 0;
-        keys_out[s] = 0xffffffffu;
+        keys_out[s] = kGatherKeyNone;
         chain_key[s] = 15u;
         for (int kk = 0; kk < 2 + 2 * D; kk++) g_out[kk] = 0.0;
         return;
@@ -1094,8 +1098,10 @@ __global__ void k_mom_terms(const double* __restrict__ gy, const uint8_t* __rest
 // loads of a tile are in flight while the tiles before it are added.  part[t][c2]
 // receives the sum of chain t.
 template <int D>
+// part[T * NC2] also receives the measurement update's error bits (*mu_err), so one copy
+// brings both back.
 __global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, const int* __restrict__ chain_off,
-                                                 int T, double* __restrict__ part) {
+                                                 int T, double* __restrict__ part, const int* __restrict__ mu_err) {
     constexpr int NC2 = 2 * (1 + D + D * D);
     constexpr int TE = kSumTP * kSumCW;          // doubles per tile
     constexpr int NL = 192;                      // loader threads
@@ -1105,6 +1111,7 @@ __global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, c
     const int t = blockIdx.x;
     const int yb = blockIdx.y;
     const int tid = threadIdx.x;
+    if (t == 0 && yb == 0 && tid == 0) part[T * NC2] = (double)mu_err[0];
     const int npos = chain_off[T];
     const int beg = chain_off[t], end = chain_off[t + 1];
     const int ntiles = (end - beg + kSumTP - 1) / kSumTP;
@@ -1366,10 +1373,12 @@ __global__ void k_ftr_gather(const TermPool OUT, const uint32_t* __restrict__ pe
 // Window of sorted position sp on the first search axis: positions r0 <= r < r1 of
 // its bucket with value in (v - ep, v + ep] (a window must hold two points,
 // ftr_binary_search, term_reduction.hpp:89-157).  wcnt = r1 - r0, or 0.
+// Also clears each position's edge count for the edge passes that follow.
 __global__ void k_ftr_win(int n, const int* __restrict__ boff, int nb, const double* __restrict__ vals,
-                          int* __restrict__ wlo, int64_t* __restrict__ wcnt) {
+                          int* __restrict__ wlo, int64_t* __restrict__ wcnt, int* __restrict__ ecnt) {
     const int sp = blockIdx.x * blockDim.x + threadIdx.x;
     if (sp >= n) return;
+    ecnt[sp] = 0;
     const int bk = bucket_of(boff, nb, sp);
     const int beg = boff[bk], end = boff[bk + 1];
     const double v = vals[sp];
@@ -2467,6 +2476,51 @@ __global__ void k_scan_total_mapped(const A* __restrict__ off, const B* __restri
     const int64_t v = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
     out[0] = v;
     *(volatile int64_t*)host_out = v;
+}
+
+// Scan of up to kSmallScan items by one workgroup: the small arrays of small estimators,
+// where a library scan costs more launches than work.  Each thread sums a contiguous run, a
+// block scan of the run sums gives each run's offset, and the run is written out.  With
+// `wide`, the input is also stored widened; with `out_total` / `host_total`, the sum of all
+// items goes to a device scalar and a device-mapped host slot.  Integer sums, so the result
+// is the same as any other summation order's.
+constexpr int kSmallScan = 65536;
+constexpr int kSmallScanThreads = 1024;
+template <class In, class Out, bool Inclusive>
+__global__ __launch_bounds__(kSmallScanThreads) void k_small_scan(const In* __restrict__ in, Out* __restrict__ out,
+                                                                  int n, Out* __restrict__ wide,
+                                                                  int64_t* __restrict__ out_total, int64_t* host_total) {
+    __shared__ Out part[kSmallScanThreads];
+    const int t = threadIdx.x;
+    const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
+    const int b = min(n, t * per), e = min(n, b + per);
+    Out sum = 0;
+    for (int i = b; i < e; i++) sum += (Out)in[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < kSmallScanThreads; o <<= 1) {
+        const Out v = t >= o ? part[t - o] : (Out)0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    Out run = part[t] - sum;
+    for (int i = b; i < e; i++) {
+        const Out x = (Out)in[i];
+        if (wide) wide[i] = x;
+        if (Inclusive) {
+            run += x;
+            out[i] = run;
+        } else {
+            out[i] = run;
+            run += x;
+        }
+    }
+    if (t == kSmallScanThreads - 1) {
+        const int64_t tot = (int64_t)part[t];
+        if (out_total) out_total[0] = tot;
+        if (host_total) *(volatile int64_t*)host_total = tot;
+    }
 }
 
 __global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
