@@ -19,7 +19,7 @@ ARCH = os.environ.get("MCE_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off keeps every multiply and add separately rounded, as in the
 # reference's x86-64 build; packed FP32 instructions are disabled for gfx950.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
@@ -68,6 +68,10 @@ def declare_pycauchy(L, sym=lambda name: name):
         getattr(L, sym("pycauchy_batch_single_step_ltiv")).argtypes = [ct.POINTER(ct.c_void_p), ct.c_int, _dpp, _ip,
                                                                        _dpp, _ip, ct.POINTER(ct.c_void_p), _ip,
                                                                        ct.c_int]
+        if hasattr(L, sym("pycauchy_batch_single_step_ltiv_merged")):
+            getattr(L, sym("pycauchy_batch_single_step_ltiv_merged")).restype = ct.c_int
+            getattr(L, sym("pycauchy_batch_single_step_ltiv_merged")).argtypes = \
+                getattr(L, sym("pycauchy_batch_single_step_ltiv")).argtypes
         getattr(L, sym("pycauchy_batch_single_step_nonlin")).restype = ct.c_int
         getattr(L, sym("pycauchy_batch_single_step_nonlin")).argtypes = [ct.POINTER(ct.c_void_p), ct.c_int, _dpp,
                                                                          _ip, _dpp, _ip, _ip,
@@ -147,6 +151,10 @@ def lib():
     L.mce_profile_enable.argtypes = [ct.c_void_p, ct.c_int]
     L.mce_profile_read.argtypes = [ct.c_void_p, _dp]
     L.mce_profile_reset.argtypes = [ct.c_void_p]
+    L.mce_batch_profile_enable.restype = None
+    L.mce_batch_profile_enable.argtypes = [ct.c_int]
+    L.mce_batch_profile_read.restype = None
+    L.mce_batch_profile_read.argtypes = [_dp]
     L.mce_last_error.restype = ct.c_char_p
     L.mce_step_record.restype = ct.c_int
     L.mce_step_record.argtypes = [ct.c_void_p, _dp, _ip, ct.c_int]

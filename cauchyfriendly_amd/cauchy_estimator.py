@@ -455,10 +455,12 @@ def eval_2d_relative_system_cpdf(Trel, secondary, primary, rsys_approx_eps, xlow
         float(rsys_approx_eps), float(xlow), float(xhigh), float(delta_x), float(ylow), float(yhigh), float(delta_y))
 
 
-def step_estimators(estimators, msmts, controls=None, full_info=False, num_threads=0):
+def step_estimators(estimators, msmts, controls=None, full_info=False, num_threads=0, merged_groups=0):
     """Steps every estimator once, as est.step(msmts[i], controls[i]) would, but concurrently:
     the estimators run on their own HIP streams from up to num_threads host threads (0: one
-    per hardware thread).  All must be initialized estimators of one mode on this library.
+    per hardware thread).  With merged_groups > 0 (LTI / LTV estimators) they are dealt to
+    that many host threads instead, and each group's updates launch every kernel once for
+    the whole group.  All must be initialized estimators of one mode on this library.
     Returns the list of est.step return values."""
     ests = list(estimators)
     if not ests:
@@ -485,6 +487,8 @@ def step_estimators(estimators, msmts, controls=None, full_info=False, num_threa
     handles = [e.py_handle for e in ests]
     if mode == "nonlin":
         outs = pc.pycauchy_batch_single_step_nonlin(handles, zs, us, [e.step_count != 0 for e in ests], num_threads)
+    elif merged_groups > 0:
+        outs = pc.pycauchy_batch_single_step_ltiv_merged(handles, zs, us, int(merged_groups))
     else:
         outs = pc.pycauchy_batch_single_step_ltiv(handles, zs, us, num_threads)
     return [e._absorb_step(o, z, bool(full_info)) for e, o, z in zip(ests, outs, zs)]

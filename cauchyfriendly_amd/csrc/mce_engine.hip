@@ -10,11 +10,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <coroutine>
 #include <functional>
 #include <initializer_list>
 #include <mutex>
+#include <optional>
 #include <cstdio>
 #include <climits>
 #include <cstdlib>
@@ -25,6 +28,7 @@
 #include <vector>
 
 #include "../../include/cauchy_mi355x.h"
+#include "mce_coro.h"
 #include "mce_host_linalg.h"
 #include "mce_kernels.h"
 #include "mce_tangent.h"
@@ -128,6 +132,103 @@ void allow_lds(const void* fn, size_t bytes) {
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsPerGroup - stat)));
     raised.emplace_back(fn, stat);
 }
+
+// ---------------------------------------------------------------- merged launches
+// What an engine stepped by a batch scheduler hands over instead of issuing it on a stream:
+// kernel launches (the body's merged-launch symbol, its grid, and the entry holding its
+// arguments), host-side stream work (library calls on large inputs, run in order at the
+// scheduler's flush), uploads (bytes staged at once, copied in order), and the device
+// reads that suspend its update until the scheduler has the data on the host.
+struct BatchReq {
+    enum Kind : uint8_t { kKernel, kHost, kUpload };
+    Kind kind = kKernel;
+    const void* fn = nullptr;  // k_many symbol
+    const char* name = "";
+    uint32_t gx = 0, gy = 0, threads = 0;
+    size_t lds = 0;
+    uint32_t esize = 0, ealign = 0;
+    size_t off = 0;            // entry (kernel) or payload (upload) in BatchSlot::bytes
+    void* dst = nullptr;       // upload destination
+    size_t nbytes = 0;         // upload length
+    std::function<void(hipStream_t)> host;
+};
+struct BatchRead {
+    const void* src;
+    size_t bytes;
+    void* dst;
+};
+struct BatchSlot {
+    std::vector<BatchReq> chain;
+    std::vector<uint8_t> bytes;
+    std::vector<BatchRead> reads;
+    std::coroutine_handle<> waiting;
+    // when the scheduler profiles: device counter of k_group_eval's algorithmic bytes
+    unsigned long long* geval_bytes = nullptr;
+    size_t append(const void* p, size_t n, size_t align) {
+        size_t off = (bytes.size() + align - 1) / align * align;
+        bytes.resize(off + n);
+        memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+    template <class E>
+    void push_kernel(const void* fn, const char* name, const dim3& g, const dim3& b, size_t lds, const E& e) {
+        BatchReq r;
+        r.kind = BatchReq::kKernel;
+        r.fn = fn;
+        r.name = name;
+        r.gx = g.x;
+        r.gy = g.y;
+        r.threads = b.x * b.y * b.z;
+        r.lds = lds;
+        r.esize = sizeof(E);
+        r.ealign = alignof(E);
+        r.off = append(&e, sizeof(E), alignof(E));
+        chain.push_back(std::move(r));
+    }
+    void push_upload(void* dst, const void* src, size_t n) {
+        if (n == 0) return;
+        BatchReq r;
+        r.kind = BatchReq::kUpload;
+        r.name = "upload";
+        r.dst = dst;
+        r.nbytes = n;
+        r.off = append(src, n, 16);
+        chain.push_back(std::move(r));
+    }
+    void push_host(std::function<void(hipStream_t)> f, const char* name) {
+        BatchReq r;
+        r.kind = BatchReq::kHost;
+        r.name = name;
+        r.host = std::move(f);
+        chain.push_back(std::move(r));
+    }
+    void clear_chain() {
+        chain.clear();
+        bytes.clear();
+    }
+};
+
+// Launches body F on grid g x block b: at once on stream s, or, when bs is set (a batch
+// scheduler steps the engine), as an entry of the scheduler's next merged launch.
+template <auto F, class... Args>
+void mce_run(BatchSlot* bs, const dim3& g, const dim3& b, size_t lds, hipStream_t s, const char* name,
+             Args&&... args) {
+    using K = Kern<F>;
+    if (!launch_ok(g, b, lds, name)) return;
+    if (g.z != 1) throw std::runtime_error(std::string("launch of ") + name + ": 3D grids are not supported");
+    if (bs) {
+        bs->push_kernel(K::many(), name, g, b, lds, K::entry(g.x, g.y, std::forward<Args>(args)...));
+        return;
+    }
+    allow_lds(K::one(), lds);
+    K::launch(g, b, lds, s, std::forward<Args>(args)...);
+    launch_done(s, name);
+}
+
+// Launch sites outside an engine have no scheduler.
+inline BatchSlot* bslot() { return nullptr; }
+
+#define MCE_RUN(K_, G_, B_, L_, S_, ...) mce_run<&K_>(bslot(), (G_), (B_), (L_), (S_), #K_, __VA_ARGS__)
 
 // Device blocks of up to kMaxBlock bytes come from a process-wide cache with one free list
 // per power-of-two size.  An estimator that outgrows its buffers, or a new estimator of a
@@ -588,11 +689,17 @@ struct Engine {
     int64_t* h_total = nullptr;      // the thread's mapped total slot, host and device views
     int64_t* h_total_dev = nullptr;
 
+    // Set while a batch scheduler steps this engine: launches, uploads and library calls
+    // become requests to it, and device reads suspend the update until it has the data.
+    BatchSlot* bs = nullptr;
+    BatchSlot* bslot() const { return bs; }
+
     // Takes the calling thread's streams and staging buffer; every entry point starts here.
+    // Under a batch scheduler everything runs in order on its one stream.
     void bind() {
         t_streams.init();
         st = t_streams.st;
-        st2 = t_streams.st2;
+        st2 = bs ? t_streams.st : t_streams.st2;
         h_stage = t_streams.stage;
         h_total = t_streams.total;
         h_total_dev = t_streams.total_dev;
@@ -757,12 +864,64 @@ struct Engine {
     }
     std::vector<double> tan_dbeta;  // raw process-scale seeds of the update, ntan x pncc
 
+    // A device-wide library call (hipcub) with its temporary storage: run now, or, under a
+    // batch scheduler, handed to it with every argument captured by value and run in order
+    // at its next flush.  The size query touches no stream.
     template <class F>
-    void cub(F f) {
+    void cub(const char* name, F f) {
         size_t bytes = 0;
-        HIPCHK(f((void*)nullptr, bytes));
+        HIPCHK(f((void*)nullptr, bytes, st));
         cub_tmp.reserve(bytes + 16);
-        HIPCHK(f((void*)cub_tmp.ptr, bytes));
+        void* tmp = cub_tmp.ptr;
+        if (bs) {
+            bs->push_host([f, tmp, bytes](hipStream_t s) mutable { HIPCHK(f(tmp, bytes, s)); }, name);
+            return;
+        }
+        HIPCHK(f(tmp, bytes, st));
+    }
+    // Stable sort of n (key, value) pairs by key bits [b0, b1): the library radix sort, or one
+    // workgroup per estimator in a merged launch for small inputs under a batch scheduler
+    // (the same order: both are stable sorts over the same bits).
+    template <class K, class V>
+    void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, int n, int b0, int b1) {
+        if (n <= 0) return;
+        if (bs && n <= kSmallSort) {
+            MCE_RUN((k_small_sort<K, V>), dim3(1), dim3(kSmallSortThreads), 0, st, kin, kout, vin, vout, n, b0, b1);
+            return;
+        }
+        cub("radix sort", [=](void* t, size_t& bytes, hipStream_t s) {
+            return hipcub::DeviceRadixSort::SortPairs(t, bytes, kin, kout, vin, vout, n, b0, b1, s);
+        });
+    }
+    // out <- in[i] where flags[i] != 0, in order; the count into *count.
+    void select_flagged(const uint32_t* in_, const uint8_t* flags, uint32_t* out_, int* count, int n) {
+        if (bs && n <= kSmallScan) {
+            MCE_RUN(k_small_select, dim3(1), dim3(kSmallScanThreads), 0, st, in_, flags, out_, count, n);
+            return;
+        }
+        cub("select", [=](void* t, size_t& bytes, hipStream_t s) {
+            return hipcub::DeviceSelect::Flagged(t, bytes, in_, flags, out_, count, n, s);
+        });
+    }
+    // *out <- max of n ints.
+    void reduce_max(const int* in_, int* out_, int n) {
+        if (bs && n <= kSmallScan) {
+            MCE_RUN(k_small_max, dim3(1), dim3(1024), 0, st, in_, out_, n);
+            return;
+        }
+        cub("reduce max", [=](void* t, size_t& bytes, hipStream_t s) {
+            return hipcub::DeviceReduce::Max(t, bytes, in_, out_, n, s);
+        });
+    }
+    // Host-to-device copy ordered on the engine's stream (staged by the scheduler in batch
+    // mode, so the source may change once this returns).
+    void upload(void* dst, const void* src, size_t bytes) {
+        if (bytes == 0) return;
+        if (bs) {
+            bs->push_upload(dst, src, bytes);
+            return;
+        }
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
     }
 
     struct FillSpec {
@@ -786,7 +945,7 @@ struct Engine {
         }
         if (n == 0) return;
         const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (most / 4 + 255) / 256));
-        MCE_LAUNCH(k_fill_multi, dim3(blocks, n), dim3(256), 0, st, f);
+        MCE_RUN(k_fill_multi, dim3(blocks, n), dim3(256), 0, st, f);
         HIPCHK(hipGetLastError());
     }
 
@@ -806,6 +965,46 @@ struct Engine {
         HIPCHK(hipStreamSynchronize(st));
         memcpy(out_, h_stage, n * sizeof(T));
     }
+    // Awaitable device read of `bytes` bytes into host memory `dst` (nothing: just wait until
+    // the engine's queued work is done).  On its own the engine copies and synchronises its
+    // stream where the read is awaited; under a batch scheduler the update suspends and the
+    // scheduler reads every waiting engine's ranges after one flush and one synchronisation.
+    struct ReadAw {
+        Engine* e;
+        const void* src;
+        size_t bytes;
+        void* dst;
+        bool await_ready() {
+            if (e->bs) return false;
+            if (bytes == 0) {
+                HIPCHK(hipStreamSynchronize(e->st));
+            } else if (bytes <= kStageBytes) {
+                HIPCHK(hipMemcpyAsync(e->h_stage, src, bytes, hipMemcpyDeviceToHost, e->st));
+                HIPCHK(hipStreamSynchronize(e->st));
+                memcpy(dst, e->h_stage, bytes);
+            } else {
+                HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->st));
+                HIPCHK(hipStreamSynchronize(e->st));
+            }
+            return true;
+        }
+        void await_suspend(std::coroutine_handle<> h) {
+            if (bytes) e->bs->reads.push_back(BatchRead{src, bytes, dst});
+            e->bs->waiting = h;
+        }
+        void await_resume() {}
+    };
+    template <class T>
+    ReadAw rd(const T* src, T* dst, int n) {
+        return ReadAw{this, src, n * sizeof(T), dst};
+    }
+    ReadAw sync_point() { return ReadAw{this, nullptr, 0, nullptr}; }
+    template <class T>
+    Co<T> rd1(const T* src) {
+        T v{};
+        co_await rd(src, &v, 1);
+        co_return v;
+    }
     // Blocking copy ordered on the engine's stream.
     void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
         if (bytes == 0) return;
@@ -815,20 +1014,31 @@ struct Engine {
     // off[n-1] + cnt[n-1] (the total of an exclusive scan) into device scalar `slot`.
     template <class A, class B>
     void enqueue_total(const A* off, const B* cnt, int n, int slot) {
-        MCE_LAUNCH((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
+        MCE_RUN((k_scan_total<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr + slot);
     }
-    // off[n-1] + cnt[n-1] read by the host: the kernel writes it into the mapped host slot (and
-    // d_scal[0]), so no copy is launched.  The slot holds -1 until the kernel's value arrives;
-    // should it not have arrived when the stream is done, d_scal[0] is copied as before.
-    template <class A, class B>
-    int64_t read_total(const A* off, const B* cnt, int n) {
-        volatile int64_t* slot = h_total;
-        *slot = -1;
-        MCE_LAUNCH((k_scan_total_mapped<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr, h_total_dev);
+    // Where a total computed on the device goes: d_scal[0], and on its own the engine also
+    // has the kernel store it into the thread's device-mapped host slot (so no copy is
+    // launched to read it).  The slot holds -1 until the kernel's value arrives.
+    int64_t* mapped_slot() {
+        if (bs) return nullptr;
+        *(volatile int64_t*)h_total = -1;
+        return h_total_dev;
+    }
+    // The total a kernel just stored through mapped_slot(): from the mapped slot once the
+    // stream is done (d_scal[0] should the value not have arrived), or from d_scal[0] by a
+    // scheduler read.
+    Co<int64_t> take_total() {
+        if (bs) co_return co_await rd1(d_scal.ptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        const int64_t v = *slot;
-        return v >= 0 ? v : read1(d_scal.ptr);
+        const int64_t v = *(volatile int64_t*)h_total;
+        co_return v >= 0 ? v : read1(d_scal.ptr);
+    }
+    // off[n-1] + cnt[n-1] read by the host.
+    template <class A, class B>
+    Co<int64_t> read_total(const A* off, const B* cnt, int n) {
+        MCE_RUN((k_scan_total_mapped<A, B>), dim3(1), dim3(1), 0, st, off, cnt, n, d_scal.ptr, mapped_slot());
+        co_return co_await take_total();
     }
 
     // Transpose, L1-normalise and pre-coalign Gamma/beta, cauchy_util.hpp:128-210.
@@ -983,59 +1193,55 @@ struct Engine {
     }
 
     // Scans.  Up to kSmallScan items take one workgroup and one launch (k_small_scan, which
-    // also widens and forms the total); larger inputs the library's device scan.  A total is
-    // read back through the thread's device-mapped host slot, as read_total does.
-    int64_t mapped_total() {
-        volatile int64_t* slot = h_total;
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-        const int64_t v = *slot;
-        return v >= 0 ? v : read1(d_scal.ptr);
-    }
+    // also widens and forms the total); larger inputs the library's device scan.
     // Inclusive scan of n ints; returns the total (the last inclusive value).
-    int scan_i32_incl_total(const int* in_, int* out_, int n) {
+    Co<int> scan_i32_incl_total(const int* in_, int* out_, int n) {
         if (n <= kSmallScan) {
-            *(volatile int64_t*)h_total = -1;
-            MCE_LAUNCH((k_small_scan<int, int, true>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n,
-                       (int*)nullptr, d_scal.ptr, h_total_dev);
-            return (int)mapped_total();
+            int64_t* host = mapped_slot();
+            MCE_RUN((k_small_scan<int, int, true>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n,
+                    (int*)nullptr, d_scal.ptr, host);
+            co_return (int)(co_await take_total());
         }
-        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::InclusiveSum(t, b, in_, out_, n, st); });
-        return n > 0 ? read1(out_ + n - 1) : 0;
+        cub("inclusive scan", [=](void* t, size_t& b, hipStream_t s) {
+            return hipcub::DeviceScan::InclusiveSum(t, b, in_, out_, n, s);
+        });
+        co_return co_await rd1(out_ + n - 1);
     }
     // Exclusive scan of n ints widened to 64 bits (the widened input kept in `wide`); returns
     // the total.
-    int64_t widen_scan_total(const int* in_, int64_t* wide, int64_t* out_, int n) {
+    Co<int64_t> widen_scan_total(const int* in_, int64_t* wide, int64_t* out_, int n) {
         if (n <= kSmallScan) {
-            *(volatile int64_t*)h_total = -1;
-            MCE_LAUNCH((k_small_scan<int, int64_t, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n,
-                       wide, d_scal.ptr, h_total_dev);
-            return mapped_total();
+            int64_t* host = mapped_slot();
+            MCE_RUN((k_small_scan<int, int64_t, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, wide,
+                    d_scal.ptr, host);
+            co_return co_await take_total();
         }
         widen(in_, wide, n);
         scan_i64_excl(wide, out_, n);
-        return read_total(out_, wide, n);
+        co_return co_await read_total(out_, wide, n);
     }
     template <class T>
     void scan_excl(const T* in_, T* out_, int n) {
         if (n <= kSmallScan) {
-            MCE_LAUNCH((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
-                       (int64_t*)nullptr, (int64_t*)nullptr);
+            MCE_RUN((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
+                    (int64_t*)nullptr, (int64_t*)nullptr);
             return;
         }
-        cub([&](void* t, size_t& b) { return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, st); });
+        cub("exclusive scan", [=](void* t, size_t& b, hipStream_t s) {
+            return hipcub::DeviceScan::ExclusiveSum(t, b, in_, out_, n, s);
+        });
     }
     // Exclusive scan with its total.
     template <class T>
-    int64_t scan_excl_total(const T* in_, T* out_, int n) {
+    Co<int64_t> scan_excl_total(const T* in_, T* out_, int n) {
         if (n <= kSmallScan) {
-            *(volatile int64_t*)h_total = -1;
-            MCE_LAUNCH((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
-                       d_scal.ptr, h_total_dev);
-            return mapped_total();
+            int64_t* host = mapped_slot();
+            MCE_RUN((k_small_scan<T, T, false>), dim3(1), dim3(kSmallScanThreads), 0, st, in_, out_, n, (T*)nullptr,
+                    d_scal.ptr, host);
+            co_return co_await take_total();
         }
         scan_excl(in_, out_, n);
-        return read_total(out_, in_, n);
+        co_return co_await read_total(out_, in_, n);
     }
     void scan_i64_excl(const int64_t* in_, int64_t* out_, int n) { scan_excl(in_, out_, n); }
     void scan_i32_excl(const int* in_, int* out_, int n) { scan_excl(in_, out_, n); }
@@ -1056,14 +1262,19 @@ struct Engine {
                               st));
     }
 
-    void collect_moments(bool first) {
-        if (!moments_pending) return;
+    Co<void> collect_moments(bool first) {
+        if (!moments_pending) co_return;
         const int NC = 1 + d + d * d;
         const int T = cur_T;
         // the partial sums, then the measurement update's error bits (k_mom_sum appends them)
         std::vector<double> part((size_t)T * NC * 2 + 1);
-        HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, st2));
-        HIPCHK(hipStreamSynchronize(st2));
+        if (bs) {
+            co_await rd(mom_part.ptr, part.data(), (int)part.size());
+        } else {
+            HIPCHK(hipMemcpyAsync(part.data(), mom_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  st2));
+            HIPCHK(hipStreamSynchronize(st2));
+        }
         const int mu_err = (int)part[(size_t)T * NC * 2];
         moments_pending = false;
         if (prof.on) {
@@ -1108,22 +1319,32 @@ struct Engine {
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
+        return run_sync(update_co(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u));
+    }
+    // The update as a task.  On its own it never suspends, keeps the allocations it outgrows
+    // until its streams are idle and gives them back at its end; under a batch scheduler it
+    // suspends at each device read, and the scheduler (which sets the graveyard while the
+    // task runs) gives them back once its stream is idle.
+    Co<int> update_co(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
+                      double gamma, const double* Bm, const double* u) {
         bind();
+        if (bs) co_return co_await update_terms(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u);
         int r;
         {
             GraveyardScope gs(&graveyard);
-            r = update_terms(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u);
+            r = co_await update_terms(msmt, Phi, Gamma, beta, Hrow, gamma, Bm, u);
         }
         bury(false);
-        return r;
+        co_return r;
     }
 
-    int update_terms(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
+    Co<int> update_terms(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                      double gamma, const double* Bm, const double* u) {
         if (errs & (1 << 9)) {
             fprintf(stderr, "[Window %d:] ERROR_FZ_NEGATIVE triggered. Cannot continue stepping until this estimator has been reset!\n", win_num);
-            return errs;
+            co_return errs;
         }
+        if (bs && ntan) throw std::runtime_error("tangent mode steps one estimator at a time");
         if (master_step == n_mu)
             throw std::runtime_error("master_step == num_estimation_steps: the estimator must be reset before stepping again");
         skip = (master_step == n_mu - 1);
@@ -1168,10 +1389,10 @@ struct Engine {
         // parents' shape bucket offsets travel with the constants (one upload)
         sc.in_off[0] = 0;
         for (int m = 0; m < sr; m++) sc.in_off[m + 1] = sc.in_off[m] + in_count[m];
-        HIPCHK(hipMemcpyAsync(dconst.ptr, &sc, sizeof(sc), hipMemcpyHostToDevice, st));
+        upload(dconst.ptr, &sc, sizeof(sc));
         if (ntan) {
             dstan.reserve(1);
-            HIPCHK(hipMemcpyAsync(dstan.ptr, &htan, sizeof(StepTan), hipMemcpyHostToDevice, st));
+            upload(dstan.ptr, &htan, sizeof(StepTan));
         }
         const int np = Nt;
         prof.begin(st);
@@ -1195,20 +1416,20 @@ struct Engine {
         if (ntan)
             MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp_tan<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, tin.view(), np,
                                          dconst.ptr, dstan.ptr));
-        MCE_DISPATCH_D(d, MCE_LAUNCH(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
+        MCE_DISPATCH_D(d, MCE_RUN(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
                                              dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
-        const int nslots = scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
+        const int nslots = co_await scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
         prof.mark(kPhTp, st);
         if (!skip && !first) {
             // B arena offsets (64-bit)
-            barena.reserve((size_t)widen_scan_total(btcap.ptr, btcap64.ptr, bt_base.ptr, np) + 1);
+            barena.reserve((size_t)(co_await widen_scan_total(btcap.ptr, btcap64.ptr, bt_base.ptr, np)) + 1);
             enum_list.reserve(np);
-            MCE_LAUNCH(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
+            MCE_RUN(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
                                gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr + 16);
             HIPCHK(hipGetLastError());
             int ec[6];  // parents to enumerate, -, -, -, their largest shape and parent hyperplane count
-            readn(counters.ptr + 16, ec, 6);
+            co_await rd(counters.ptr + 16, ec, 6);
             const int nenum = ec[0];
             if (nenum > 0) {
                 // the 2^m-bit candidate set of the largest shape, then the parent key set of
@@ -1229,8 +1450,7 @@ struct Engine {
                 if (lds + enum_static > kLdsPerGroup)
                     throw std::runtime_error("hyperplane count too large for the cell enumerator's LDS sign-vector set");
                 MCE_DISPATCH_D(d, {
-                    allow_lds((const void*)k_btab_enum<DD>, lds);
-                    MCE_LAUNCH(k_btab_enum<DD>, dim3(nenum), dim3(enum_threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
+                    MCE_RUN(k_btab_enum<DD>, dim3(nenum), dim3(enum_threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
                                dcombos.ptr, dcombo_off.ptr, dcombo_cnt.ptr, gk[gcur].ptr, barena.ptr, btcap.ptr,
                                counters.ptr + 8, words, psearch);
                 });
@@ -1253,16 +1473,16 @@ struct Engine {
         // child rows in registers when the largest shape fits a compile-time bound
         const int mu_km = MM <= 8 ? 8 : MM <= 14 ? 14 : 0;
         if (mu_km == 8) {
-            MCE_DISPATCH_D(d, MCE_LAUNCH((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
+            MCE_DISPATCH_D(d, MCE_RUN((k_mu_r<DD, 8>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN,
                                                  np, slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr,
                                                  gv[gcur].ptr, gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         } else if (mu_km == 14 && d <= 5) {
-            MCE_DISPATCH_D(d, if constexpr (DD <= 5) MCE_LAUNCH(
+            MCE_DISPATCH_D(d, if constexpr (DD <= 5) MCE_RUN(
                                   (k_mu_r<DD, 14>), dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                   slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr, gy.ptr, keys.ptr,
                                   chain_key.ptr, counters.ptr + 17));
         } else {
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
+            MCE_DISPATCH_D(d, MCE_RUN(k_mu<DD>, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, IN, np,
                                                  slot_base.ptr, OUT, nslots, dconst.ptr, gk[gcur].ptr, gv[gcur].ptr,
                                                  gy.ptr, keys.ptr, chain_key.ptr, counters.ptr + 17));
         }
@@ -1279,23 +1499,22 @@ struct Engine {
         prof.mark(kPhMu, st);
 
         // ---- moments: slots in reference-thread chain order, then the chains on st2
-        MCE_LAUNCH(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
-        cub([&](void* t_, size_t& b_) {
-            return hipcub::DeviceRadixSort::SortPairs(t_, b_, chain_key.ptr, chain_skey.ptr, iota.ptr, chain_perm.ptr,
-                                                      nslots, 0, 4, st);
-        });
-        MCE_LAUNCH(k_chain_off, dim3(1), dim3(16), 0, st, chain_skey.ptr, nslots, T, chain_off.ptr);
-        HIPCHK(hipEventRecord(ev_mu, st));
-        HIPCHK(hipStreamWaitEvent(st2, ev_mu, 0));
+        MCE_RUN(k_iota, dim3(ceil_div(nslots, 256)), dim3(256), 0, st, iota.ptr, nslots);
+        sort_pairs(chain_key.ptr, chain_skey.ptr, iota.ptr, chain_perm.ptr, nslots, 0, 4);
+        MCE_RUN(k_chain_off, dim3(1), dim3(16), 0, st, chain_skey.ptr, nslots, T, chain_off.ptr);
+        if (!bs) {
+            HIPCHK(hipEventRecord(ev_mu, st));
+            HIPCHK(hipStreamWaitEvent(st2, ev_mu, 0));
+        }
         if (prof.on) HIPCHK(hipEventRecord(ev_c0, st2));
         {
             int ncp = 0, nyb = 0;
             MCE_DISPATCH_D(d, (nyb = mom_blocks<DD>(), ncp = nyb * kSumCW / 2));
             mom_x.reserve((size_t)nyb * nslots * kSumCW);
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mom_terms<DD>, dim3(ceil_div((int64_t)nslots * ncp, 256)), dim3(256),
+            MCE_DISPATCH_D(d, MCE_RUN(k_mom_terms<DD>, dim3(ceil_div((int64_t)nslots * ncp, 256)), dim3(256),
                                                  0, st2, gy.ptr, out.flags.ptr, chain_perm.ptr, chain_off.ptr, T,
                                                  nslots, mom_x.ptr));
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
+            MCE_DISPATCH_D(d, MCE_RUN(k_mom_sum<DD>, dim3(T, nyb), dim3(256), 0, st2, mom_x.ptr,
                                                  chain_off.ptr, T, mom_part.ptr, counters.ptr + 17));
         }
         HIPCHK(hipGetLastError());
@@ -1304,14 +1523,13 @@ struct Engine {
         moments_pending = true;
         diag_mom.assign(6 + 2 * d + 2 * d * d, 0.0);
         diag_cnt.assign(4 + 2 * sr, 0);
-        if (first || skip) collect_moments(first);
+        if (first || skip) co_await collect_moments(first);
 
         if (skip) {
-            MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+            MCE_RUN(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> h(sr + 1);
-            HIPCHK(hipMemcpyAsync(h.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            co_await rd(hist.ptr, h.data(), sr + 1);
             Nt = 0;
             for (int m = 0; m < sr; m++) {
                 diag_cnt[4 + m] = h[m];
@@ -1326,15 +1544,11 @@ struct Engine {
             perm.reserve(nslots);
             skeys.reserve(nslots);
             // iota still holds 0..nslots-1: the chain sort only read it
-            cub([&](void* t, size_t& b) {
-                return hipcub::DeviceRadixSort::SortPairs(t, b, keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0,
-                                                          kGatherKeyBits, st);
-            });
-            MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
+            sort_pairs(keys.ptr, skeys.ptr, iota.ptr, perm.ptr, nslots, 0, kGatherKeyBits);
+            MCE_RUN(k_hist, dim3(std::min(ceil_div(nslots, 256), 1024)), dim3(256), 0, st, nslots, sr + 1, 0,
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> cnt(sr + 1);
-            HIPCHK(hipMemcpyAsync(cnt.data(), hist.ptr, (sr + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            co_await rd(hist.ptr, cnt.data(), sr + 1);
             prof.mark(kPhGather, st);
             int nt_mu = 0;
             for (int m = 0; m < sr; m++) {
@@ -1344,14 +1558,14 @@ struct Engine {
             diag_cnt[1] = nt_mu;
             terms_generated += nt_mu;
             if (first) {
-                first_gtables(nt_mu);
+                co_await first_gtables(nt_mu);
             } else {
-                reduce_and_flatten(cnt);
+                co_await reduce_and_flatten(cnt);
             }
             for (int m = 0; m < sr; m++) diag_cnt[4 + sr + m] = in_count[m];
         }
         {
-            const int guard = read1(counters.ptr + 8);
+            const int guard = co_await rd1(counters.ptr + 8);
             if (guard) {
                 char msg[160];
                 snprintf(msg, sizeof(msg), "device bound guard tripped (flags 0x%x) at update %d", guard, master_step);
@@ -1374,13 +1588,13 @@ struct Engine {
         diag_cnt[2] = Nt;
         diag_cnt[3] = threads_used;
         have_diag = true;
-        return errs;
+        co_return errs;
     }
 
     void widen(const int* src, int64_t* dst, int n);
 
     // g-tables of the first update's d+1 terms, which become the parents.
-    void first_gtables(int n) {
+    Co<void> first_gtables(int n) {
         const int cells = 1 << (d - 1);
         const int nx = gcur ^ 1;
         gk[nx].reserve((size_t)n * cells);
@@ -1388,7 +1602,7 @@ struct Engine {
         in.reserve(n);
         TermPool OUT = out.view();
         TermPool NEXT = in.view();
-        MCE_DISPATCH_D(d, MCE_LAUNCH(k_gtable_first<DD>, dim3(1), dim3(64), 0, st, OUT, n, perm.ptr, gscale,
+        MCE_DISPATCH_D(d, MCE_RUN(k_gtable_first<DD>, dim3(1), dim3(64), 0, st, OUT, n, perm.ptr, gscale,
                                              gk[nx].ptr, gv[nx].ptr, NEXT));
         if (ntan) {
             dgv[nx].reserve((size_t)n * cells * ntan);
@@ -1397,7 +1611,7 @@ struct Engine {
                                          tout.view(), n, perm.ptr, gscale, dgscale_dev.ptr, dgv[nx].ptr, tin.view()));
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
+        co_await sync_point();
         gcur = nx;
         in_count.assign(sr, 0);
         in_count[d] = n;
@@ -1407,7 +1621,7 @@ struct Engine {
 
     // Fast term reduction and g-tables for every shape bucket
     // (fast_term_reduction_and_create_gtables, cauchy_estimator.hpp:981-1177).
-    void reduce_and_flatten(const std::vector<int>& cnt) {
+    Co<void> reduce_and_flatten(const std::vector<int>& cnt) {
         TermPool OUT = out.view();
         std::vector<int> off(sr + 1, 0);
         for (int m = 0; m < sr; m++) off[m + 1] = off[m] + cnt[m];
@@ -1418,7 +1632,7 @@ struct Engine {
         grp_m.reserve(total);
         mem_slots.reserve(total);
         std::vector<int> gbase_of;
-        const int ng = reduce_all(OUT, off, gbase_of);
+        const int ng = co_await reduce_all(OUT, off, gbase_of);
         prof.mark(kPhFtr, st);
 
         // ---- group B-tables
@@ -1427,9 +1641,9 @@ struct Engine {
         b2n.reserve(ng);
         b2n64.reserve(ng);
         goff.reserve(ng);
-        MCE_LAUNCH(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
+        MCE_RUN(k_group_ub, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng, grp_root.ptr, grp_m.ptr, d,
                            (int*)b2n.ptr);
-        const int64_t ub_total = widen_scan_total(b2n.ptr, ub64.ptr, b2off.ptr, ng);
+        const int64_t ub_total = co_await widen_scan_total(b2n.ptr, ub64.ptr, b2off.ptr, ng);
         b2arena.reserve((size_t)ub_total + 1);
         for (int m = 1; m < sr; m++) {
             const int ngb = gbase_of[m + 1] - gbase_of[m];
@@ -1438,17 +1652,16 @@ struct Engine {
             int wpb = 4;
             while (wpb > 1 && (size_t)wpb * words * 4 > 65536) wpb--;
             const size_t lds = (size_t)wpb * words * 4;
-            allow_lds((const void*)k_group_btab, lds);
-            MCE_LAUNCH(k_group_btab, dim3(ceil_div(ngb, wpb)), dim3(64 * wpb), lds, st, OUT, gbase_of[m], ngb,
+            MCE_RUN(k_group_btab, dim3(ceil_div(ngb, wpb)), dim3(64 * wpb), lds, st, OUT, gbase_of[m], ngb,
                                m, d, grp_root.ptr, barena.ptr, b2off.ptr, ub64.ptr, b2arena.ptr, b2n.ptr, words,
                                counters.ptr + 8);
             HIPCHK(hipGetLastError());
         }
-        const int64_t cells_total = widen_scan_total(b2n.ptr, b2n64.ptr, goff.ptr, ng);
+        const int64_t cells_total = co_await widen_scan_total(b2n.ptr, b2n64.ptr, goff.ptr, ng);
         prof.mark(kPhGBtab, st);
 
         // ---- group g-tables need the update's g scale 1/(2 pi Re fz)
-        collect_moments(false);
+        co_await collect_moments(false);
         const int nx = gcur ^ 1;
         gk[nx].reserve((size_t)cells_total + 1);
         gv[nx].reserve((size_t)cells_total + 1);
@@ -1461,7 +1674,7 @@ struct Engine {
         surv.reserve(ng);
         gxor.reserve(ng);
         sort_list.reserve(ng);
-        if (prof.on) {
+        if (prof.on && !bs) {
             if (!prof.k0) {
                 HIPCHK(hipEventCreate(&prof.k0));
                 HIPCHK(hipEventCreate(&prof.k1));
@@ -1510,8 +1723,7 @@ struct Engine {
             defer_k0.reserve(std::max(ng, 1));
 #define MCE_GEVAL(M_, K_)                                                                                        \
     do {                                                                                                          \
-        allow_lds((const void*)k_group_eval<M_, K_>, lds);                                                        \
-        MCE_LAUNCH((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
+        MCE_RUN((k_group_eval<M_, K_>), dim3(ng), dim3(64), lds, st, OUT, ng, d, wmax, grp_root.ptr,    \
                            grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
                            b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
                            gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 18, counters.ptr + 19,          \
@@ -1533,11 +1745,11 @@ struct Engine {
 #undef MCE_GEVAL
         }
         HIPCHK(hipGetLastError());
-        if (prof.on) {
+        if (prof.on && !bs) {
             HIPCHK(hipEventRecord(prof.k1, st));
             bytes_acc.reserve(1);
-            HIPCHK(hipMemsetAsync(bytes_acc.ptr, 0, sizeof(unsigned long long), st));
-            MCE_LAUNCH(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
+            fill_multi({{bytes_acc.ptr, sizeof(unsigned long long), 0}});
+            MCE_RUN(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr,
                                grp_mcnt.ptr, b2n.ptr, surv.ptr, defer_k0.ptr,
                                ge_mode >= kLookupRank ? big_group_members : INT_MAX, bytes_acc.ptr);
             const unsigned long long bytes = read1(bytes_acc.ptr);
@@ -1547,9 +1759,11 @@ struct Engine {
             prof.geval_bytes += (double)bytes;
             prof.geval_launches++;
             pstat.reserve(2);
-            cub([&](void* t_, size_t& b_) { return hipcub::DeviceReduce::Max(t_, b_, grp_mcnt.ptr, pstat.ptr, ng, st); });
-            cub([&](void* t_, size_t& b_) {
-                return hipcub::DeviceReduce::Sum(t_, b_, grp_mcnt.ptr, pstat.ptr + 1, ng, st);
+            const int* mc = grp_mcnt.ptr;
+            int* ps = pstat.ptr;
+            reduce_max(mc, ps, ng);
+            cub("reduce sum", [=](void* t_, size_t& b_, hipStream_t s_) {
+                return hipcub::DeviceReduce::Sum(t_, b_, mc, ps + 1, ng, s_);
             });
             int ps_[2];
             HIPCHK(hipMemcpyAsync(ps_, pstat.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1558,9 +1772,12 @@ struct Engine {
             prof.members += ps_[1];
             prof.max_group = std::max(prof.max_group, (double)(ps_[0] + 1));
         }
-        if (ge_mode >= kLookupRank) big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
+        if (bs && bs->geval_bytes)
+            MCE_RUN(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr, grp_mcnt.ptr, b2n.ptr,
+                    surv.ptr, defer_k0.ptr, ge_mode >= kLookupRank ? big_group_members : INT_MAX, bs->geval_bytes);
+        if (ge_mode >= kLookupRank) co_await big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
         int nr_[2];
-        readn(counters.ptr + 18, nr_, 2);
+        co_await rd(counters.ptr + 18, nr_, 2);
         const int nsort = nr_[0];
         const int rare = nr_[1];
         // bits 1 and 2 note parent (old) terms among a group's members, which stage_term
@@ -1571,8 +1788,7 @@ struct Engine {
             tmpv.reserve((size_t)cells_total + 1);
             const int words = MM >= 5 ? (1 << (MM - 5)) : 1;
             const size_t lds = (size_t)words * 4;
-            allow_lds((const void*)k_group_sort, lds);
-            MCE_LAUNCH(k_group_sort, dim3(nsort), dim3(256), lds, st, nsort, sort_list.ptr, grp_m.ptr, b2n.ptr,
+            MCE_RUN(k_group_sort, dim3(nsort), dim3(256), lds, st, nsort, sort_list.ptr, grp_m.ptr, b2n.ptr,
                                goff.ptr, gk[nx].ptr, gv[nx].ptr, tmpk.ptr, tmpv.ptr);
             HIPCHK(hipGetLastError());
         }
@@ -1588,11 +1804,11 @@ struct Engine {
         // ---- survivors become the next parents (shape ascending, root index ascending)
         sflag.reserve(ng);
         spos.reserve(ng);
-        MCE_LAUNCH(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
+        MCE_RUN(k_surv_flags, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, surv.ptr, sflag.ptr);
         scan_i32_excl(sflag.ptr, spos.ptr, ng);
         in.reserve(std::max(ng, 1));  // survivors <= groups; their count comes with the histogram
         TermPool NEXT = in.view();
-        MCE_DISPATCH_D(d, MCE_LAUNCH(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
+        MCE_DISPATCH_D(d, MCE_RUN(k_compact<DD>, dim3(ceil_div(ng, 256)), dim3(256), 0, st, OUT, ng,
                                              surv.ptr, spos.ptr, grp_m.ptr, goff.ptr, b2n.ptr, NEXT));
         if (ntan) {
             tin.reserve(std::max(ng, 1));
@@ -1600,15 +1816,13 @@ struct Engine {
                                          surv.ptr, spos.ptr, grp_m.ptr, tin.view()));
         }
         HIPCHK(hipGetLastError());
-        MCE_LAUNCH(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
+        MCE_RUN(k_hist, dim3(std::min(ceil_div(ng, 256), 1024)), dim3(256), 0, st, ng, sr + 1, 1,
                            (const uint8_t*)nullptr, grp_m.ptr, surv.ptr, (const uint32_t*)nullptr, hist_mx.ptr);
         std::vector<int> h(sr + 2);
         {
             // histogram and the largest g-table (the next update's parent tables) in one read
-            cub([&](void* t_, size_t& b_) {
-                return hipcub::DeviceReduce::Max(t_, b_, b2n.ptr, hist_mx.ptr + sr + 1, ng, st);
-            });
-            readn(hist_mx.ptr, h.data(), sr + 2);
+            reduce_max(b2n.ptr, hist_mx.ptr + sr + 1, ng);
+            co_await rd(hist_mx.ptr, h.data(), sr + 2);
             max_gp_cells = h[sr + 1];
         }
         prof.mark(kPhCompact, st);
@@ -1624,15 +1838,15 @@ struct Engine {
     // Groups with more than big_group_members members: their members' contributions are
     // evaluated one wavefront per member (k_group_members) and added in member order
     // (k_group_sum), after k_group_eval has settled each group's root.
-    void big_groups(int ng, int mode, size_t lds, int wmax, double gscale, int nx) {
-        if (ng <= 0) return;
+    Co<void> big_groups(int ng, int mode, size_t lds, int wmax, double gscale, int nx) {
+        if (ng <= 0) co_return;
         big_icnt.reserve(ng);
         big_ioff.reserve(ng);
         big_hcnt.reserve(ng);
         big_hoff.reserve(ng);
         big_ccnt.reserve(ng);
         big_coff.reserve(ng);
-        MCE_LAUNCH(k_big_prep, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, big_group_members, grp_mcnt.ptr,
+        MCE_RUN(k_big_prep, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, big_group_members, grp_mcnt.ptr,
                            b2n.ptr, big_icnt.ptr, big_ccnt.ptr, big_hcnt.ptr);
         scan_i32_excl(big_icnt.ptr, big_ioff.ptr, ng);
         scan_i32_excl(big_hcnt.ptr, big_hoff.ptr, ng);
@@ -1641,9 +1855,9 @@ struct Engine {
         enqueue_total(big_hoff.ptr, big_hcnt.ptr, ng, 1);
         enqueue_total(big_coff.ptr, big_ccnt.ptr, ng, 2);
         int64_t tt[3];
-        readn(d_scal.ptr, tt, 3);
+        co_await rd(d_scal.ptr, tt, 3);
         const int ni = (int)tt[0];
-        if (ni == 0) return;
+        if (ni == 0) co_return;
         const int nh = (int)tt[1];
         const int64_t nc = tt[2];
         contrib.reserve((size_t)std::max<int64_t>(nc, 1));
@@ -1651,8 +1865,7 @@ struct Engine {
         TermPool OUT = out.view();
 #define MCE_GMEM(M_, K_)                                                                                         \
     do {                                                                                                          \
-        allow_lds((const void*)k_group_members<M_, K_>, lds);                                                     \
-        MCE_LAUNCH((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
+        MCE_RUN((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
                            ni, big_coff.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr,    \
                            b2off.ptr, b2n.ptr, gk[gcur].ptr, gv[gcur].ptr, gscale, surv.ptr, gxor.ptr,             \
                            defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19, ntan ? mem_add.ptr : nullptr);   \
@@ -1669,7 +1882,7 @@ struct Engine {
         else MCE_GMEM(kLookupRank, 32);
 #undef MCE_GMEM
         HIPCHK(hipGetLastError());
-        MCE_LAUNCH(k_group_sum, dim3(nh), dim3(64), 0, st, ng, big_hoff.ptr, nh, big_ioff.ptr, big_coff.ptr,
+        MCE_RUN(k_group_sum, dim3(nh), dim3(64), 0, st, ng, big_hoff.ptr, nh, big_ioff.ptr, big_coff.ptr,
                            grp_mcnt.ptr, b2n.ptr, goff.ptr, surv.ptr, defer_k0.ptr, contrib.ptr, cneg.ptr, gv[nx].ptr);
         HIPCHK(hipGetLastError());
     }
@@ -1677,17 +1890,17 @@ struct Engine {
     // Fast term reduction of every shape bucket at once.  off[m] is the first gathered
     // position of shape m; fills the reduction groups (shape ascending, roots ascending,
     // members ascending per root) and gbase_of[m], the first group of shape m.
-    int reduce_all(const TermPool& OUT, const std::vector<int>& off, std::vector<int>& gbase_of) {
+    Co<int> reduce_all(const TermPool& OUT, const std::vector<int>& off, std::vector<int>& gbase_of) {
         const int N = off[sr];
         gbase_of.assign(sr + 1, 0);
-        if (N == 0) return 0;
+        if (N == 0) co_return 0;
         const int a0 = g_search_order[0];
         if (a0 < 0 || a0 >= d)
             throw std::runtime_error("term-reduction search ordering names axis " + std::to_string(a0) + " of a " +
                                      std::to_string(d) + "-state estimator");
         h_boff = off;
         dboff.reserve(sr + 1);
-        HIPCHK(hipMemcpyAsync(dboff.ptr, h_boff.data(), (sr + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+        upload(dboff.ptr, h_boff.data(), (sr + 1) * sizeof(int));
         const int* boff = dboff.ptr;
         fkeys.reserve(N);
         fskeys.reserve(N);
@@ -1700,44 +1913,40 @@ struct Engine {
         eoff.reserve(N);
         const dim3 g1(ceil_div(N, 256)), b1(256);
         // value order on the first search axis, then (stable) bucket order
-        MCE_LAUNCH(k_ftr_keys, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, fkeys.ptr, fidx.ptr);
-        cub([&](void* t, size_t& b) {
-            return hipcub::DeviceRadixSort::SortPairs(t, b, fkeys.ptr, fskeys.ptr, fidx.ptr, fsidx.ptr, N, 0, 64, st);
-        });
-        MCE_LAUNCH(k_ftr_bkeys, g1, b1, 0, st, N, boff, sr, fsidx.ptr, bkey.ptr);
+        MCE_RUN(k_ftr_keys, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, fkeys.ptr, fidx.ptr);
+        sort_pairs(fkeys.ptr, fskeys.ptr, fidx.ptr, fsidx.ptr, N, 0, 64);
+        MCE_RUN(k_ftr_bkeys, g1, b1, 0, st, N, boff, sr, fsidx.ptr, bkey.ptr);
         int bbits = 1;
         while ((1 << bbits) < sr) bbits++;
-        cub([&](void* t, size_t& b) {
-            return hipcub::DeviceRadixSort::SortPairs(t, b, bkey.ptr, bskey.ptr, fsidx.ptr, fidx.ptr, N, 0, bbits, st);
-        });
+        sort_pairs(bkey.ptr, bskey.ptr, fsidx.ptr, fidx.ptr, N, 0, bbits);
         const uint32_t* sidx = fidx.ptr;
-        MCE_LAUNCH(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
+        MCE_RUN(k_ftr_vals, g1, b1, 0, st, OUT, perm.ptr, N, d, a0, sidx, fvals.ptr);
         // candidate windows on the first search axis; the predicate runs one thread per
         // (term, window slot) pair unless the pairs would not fit, then one per term
         wlo.reserve(N);
         wcnt.reserve(N);
         woff.reserve(N);
-        MCE_LAUNCH(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr, ecnt.ptr);
-        const int64_t npairs = scan_excl_total(wcnt.ptr, woff.ptr, N);
+        MCE_RUN(k_ftr_win, g1, b1, 0, st, N, boff, sr, fvals.ptr, wlo.ptr, wcnt.ptr, ecnt.ptr);
+        const int64_t npairs = co_await scan_excl_total(wcnt.ptr, woff.ptr, N);
         const bool by_pairs = npairs < ((int64_t)1 << 31);
         if (by_pairs && npairs > 0) {
             fgb.reserve((size_t)N * d);
             fgp.reserve((size_t)N * MM);
             fgA.reserve((size_t)N * MM);
             pflag.reserve((size_t)npairs);
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_gather<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, sidx,
+            MCE_DISPATCH_D(d, MCE_RUN(k_ftr_gather<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr, sidx,
                                                  fgb.ptr, fgp.ptr, fgA.ptr));
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs, N,
+            MCE_DISPATCH_D(d, MCE_RUN(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs, N,
                                                  boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr, fgp.ptr,
                                                  fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr, edges.ptr, 0));
         } else if (!by_pairs) {
-            MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+            MCE_DISPATCH_D(d, MCE_RUN(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
                                                  dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 0));
         }
         HIPCHK(hipGetLastError());
-        const int E = (int)scan_excl_total(ecnt.ptr, eoff.ptr, N);
+        const int E = (int)(co_await scan_excl_total(ecnt.ptr, eoff.ptr, N));
         Fc.reserve((size_t)8 * N);
-        MCE_LAUNCH(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
+        MCE_RUN(k_ftr_finit, g1, b1, 0, st, Fc.ptr, N, boff, sr);
         if (E > 0) {
             edges.reserve(E);
             inedges.reserve(E);
@@ -1755,18 +1964,18 @@ struct Engine {
                         {st0.ptr, (size_t)N, kUnknown},
                         {round_slots.ptr, kRoundSlots * sizeof(int), 0}});
             if (by_pairs) {
-                MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
+                MCE_DISPATCH_D(d, MCE_RUN(k_ftr_pairs<DD>, dim3(ceil_div(npairs, 256)), b1, 0, st, npairs,
                                                      N, boff, sr, dconst.ptr, woff.ptr, wlo.ptr, sidx, MM, fgb.ptr,
                                                      fgp.ptr, fgA.ptr, pflag.ptr, ecnt.ptr, eoff.ptr, cursor.ptr,
                                                      edges.ptr, 1));
             } else {
-                MCE_DISPATCH_D(d, MCE_LAUNCH(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
+                MCE_DISPATCH_D(d, MCE_RUN(k_ftr_edges<DD>, g1, b1, 0, st, OUT, perm.ptr, N, boff, sr,
                                                      dconst.ptr, fvals.ptr, sidx, ecnt.ptr, eoff.ptr, edges.ptr, 1));
             }
-            MCE_LAUNCH(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+            MCE_RUN(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
                                (const int*)nullptr, (int*)nullptr, (int*)nullptr, 0);
             scan_i32_excl(indeg.ptr, inoff.ptr, N);
-            MCE_LAUNCH(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
+            MCE_RUN(k_ftr_inedges, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, indeg.ptr,
                                inoff.ptr, cursor2.ptr, inedges.ptr, 1);
             uint8_t* sa = st0.ptr;
             uint8_t* sb = st1.ptr;
@@ -1775,18 +1984,18 @@ struct Engine {
             int batch = 2, round = 0;
             for (int done = 0; done <= N + 1;) {
                 for (int r = 0; r < batch; r++, round++) {
-                    MCE_LAUNCH(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
+                    MCE_RUN(k_ftr_round, g1, b1, 0, st, N, inoff.ptr, indeg.ptr, inedges.ptr, sa, sb,
                                        round_slots.ptr, round);
                     std::swap(sa, sb);
                 }
                 done += batch;
-                if (read1(round_slots.ptr + (round - 1) % kRoundSlots) == 0) break;
+                if ((co_await rd1(round_slots.ptr + (round - 1) % kRoundSlots)) == 0) break;
                 batch = std::min(batch * 2, 16);
             }
-            MCE_LAUNCH(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);
+            MCE_RUN(k_ftr_claim, g1, b1, 0, st, N, boff, sr, eoff.ptr, ecnt.ptr, edges.ptr, sa, Fc.ptr);
             if (N > 10000) {
                 for (int lvl = 0; lvl < 3; lvl++)
-                    MCE_LAUNCH(k_ftr_merge, dim3(ceil_div(N, 256), 8 >> (lvl + 1)), b1, 0, st, N, boff, sr,
+                    MCE_RUN(k_ftr_merge, dim3(ceil_div(N, 256), 8 >> (lvl + 1)), b1, 0, st, N, boff, sr,
                                        Fc.ptr, lvl);
             }
             HIPCHK(hipGetLastError());
@@ -1801,30 +2010,25 @@ struct Engine {
         jiota.reserve(N);
         memcnt.reserve(N + 1);
         memoff.reserve(N + 1);
-        MCE_LAUNCH(k_ftr_rootflags, g1, b1, 0, st, N, boff, sr, Fc.ptr, isroot.ptr, memkey.ptr, memval.ptr);
-        MCE_LAUNCH(k_iota, g1, b1, 0, st, jiota.ptr, N);
-        cub([&](void* t, size_t& b) {
-            return hipcub::DeviceSelect::Flagged(t, b, jiota.ptr, isroot.ptr, roots.ptr, counters.ptr + 5, N, st);
-        });
+        MCE_RUN(k_ftr_rootflags, g1, b1, 0, st, N, boff, sr, Fc.ptr, isroot.ptr, memkey.ptr, memval.ptr);
+        MCE_RUN(k_iota, g1, b1, 0, st, jiota.ptr, N);
+        select_flagged(jiota.ptr, isroot.ptr, roots.ptr, counters.ptr + 5, N);
         int kbits = 1;
         while (((int64_t)1 << kbits) <= N) kbits++;
-        cub([&](void* t, size_t& b) {
-            return hipcub::DeviceRadixSort::SortPairs(t, b, memkey.ptr, smkey.ptr, memval.ptr, smval.ptr, N, 0, kbits,
-                                                      st);
-        });
+        sort_pairs(memkey.ptr, smkey.ptr, memval.ptr, smval.ptr, N, 0, kbits);
         gbase_dev.reserve(sr + 1);
         const int* d_nroots = counters.ptr + 5;
-        MCE_LAUNCH(k_ftr_gbase, dim3(1), dim3(64), 0, st, d_nroots, roots.ptr, boff, sr, gbase_dev.ptr);
-        HIPCHK(hipMemsetAsync(memcnt.ptr, 0, (N + 1) * sizeof(int), st));
-        MCE_LAUNCH(k_ftr_memcount, g1, b1, 0, st, N, d_nroots, smkey.ptr, memcnt.ptr);
-        MCE_LAUNCH(k_ftr_memslots, g1, b1, 0, st, N, d_nroots, smval.ptr, perm.ptr, mem_slots.ptr);
+        MCE_RUN(k_ftr_gbase, dim3(1), dim3(64), 0, st, d_nroots, roots.ptr, boff, sr, gbase_dev.ptr);
+        fill_multi({{memcnt.ptr, (N + 1) * sizeof(int), 0}});
+        MCE_RUN(k_ftr_memcount, g1, b1, 0, st, N, d_nroots, smkey.ptr, memcnt.ptr);
+        MCE_RUN(k_ftr_memslots, g1, b1, 0, st, N, d_nroots, smval.ptr, perm.ptr, mem_slots.ptr);
         scan_i32_excl(memcnt.ptr, memoff.ptr, N);
-        MCE_LAUNCH(k_ftr_groups, g1, b1, 0, st, d_nroots, roots.ptr, perm.ptr, boff, sr, memcnt.ptr,
+        MCE_RUN(k_ftr_groups, g1, b1, 0, st, d_nroots, roots.ptr, perm.ptr, boff, sr, memcnt.ptr,
                            memoff.ptr, grp_root.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr);
         HIPCHK(hipGetLastError());
         // group offsets per shape; the last one is the root count
-        readn(gbase_dev.ptr, gbase_of.data(), sr + 1);
-        return gbase_of[sr];
+        co_await rd(gbase_dev.ptr, gbase_of.data(), sr + 1);
+        co_return gbase_of[sr];
     }
 
     // Deterministic transform of every parent term (cauchy_estimator.hpp:1312-1355).
@@ -1896,7 +2100,7 @@ struct Engine {
 
 void Engine::widen(const int* src, int64_t* dst, int n) {
     if (n <= 0) return;
-    MCE_LAUNCH(k_widen, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, n);
+    MCE_RUN(k_widen, dim3(ceil_div(n, 256)), dim3(256), 0, st, src, dst, n);
 }
 
 void Engine::deterministic_transform(const double* Trans, const double* bias) {
@@ -2212,10 +2416,16 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
         MCE_LAUNCH(k_colsum_part, dim3(kColSlices, 14), dim3(256), 0, st, P.rs_mom.ptr, npairs,
                            P.rs_part.ptr);
         MCE_LAUNCH(k_colsum_finish, dim3(1), dim3(64), 0, st, P.rs_part.ptr, 14, P.rs_sums.ptr);
-        P.cub([&](void* t, size_t& b) {
-            return hipcub::DeviceSelect::Flagged(t, b, hipcub::CountingInputIterator<int>(0), P.rs_keep.ptr,
-                                                 P.rs_idx.ptr, P.rs_cnt.ptr, (int)npairs, st);
-        });
+        {
+            const uint8_t* keep = P.rs_keep.ptr;
+            int* idx = P.rs_idx.ptr;
+            int* cnt = P.rs_cnt.ptr;
+            const int np2 = (int)npairs;
+            P.cub("select", [=](void* t, size_t& b, hipStream_t s_) {
+                return hipcub::DeviceSelect::Flagged(t, b, hipcub::CountingInputIterator<int>(0), keep, idx, cnt, np2,
+                                                     s_);
+            });
+        }
         HIPCHK(hipGetLastError());
     }
     double sums[14] = {0};
@@ -2293,7 +2503,7 @@ void Engine::finalize_extended_moments(double* x_bar) {
         VecArg dx;
         for (int i = 0; i < d; i++) dx.v[i] = mean[i].re;
         TermPool IN = in.view();
-        MCE_DISPATCH_D(d, MCE_LAUNCH(k_sub_b<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dx));
+        MCE_DISPATCH_D(d, MCE_RUN(k_sub_b<DD>, dim3(ceil_div(Nt, 256)), dim3(256), 0, st, IN, Nt, dx));
         HIPCHK(hipGetLastError());
     }
     for (int i = 0; i < d; i++) mean[i].re += x_bar[i];
@@ -2467,6 +2677,556 @@ T* mallocd(size_t n) {
     return p;
 }
 
+// Records a failed update: into *err for a batched estimator (its first failure), else as
+// the calling thread's last error.
+void note_failure(std::exception_ptr fail, const char* where, std::string* err) {
+    try {
+        std::rethrow_exception(fail);
+    } catch (const std::exception& ex) {
+        if (!err) {
+            set_err(where, ex);
+        } else if (err->empty()) {
+            *err = std::string(where) + ": " + ex.what();
+            fprintf(stderr, "[cauchy_mi355x] %s\n", err->c_str());
+        }
+    }
+}
+
+Co<void> step_ltiv_task(void* handle, double* msmts, int size_msmts, double* controls, int size_controls,
+                               double** out_Phi, int* size_out_Phi, double** out_Gamma, int* size_out_Gamma,
+                               double** out_B, int* size_out_B, double** out_H, int* size_out_H, double** out_beta,
+                               int* size_out_beta, double** out_gamma, int* size_out_gamma, double** out_fz,
+                               int* size_out_fz, double** out_xhat, int* size_out_xhat, double** out_Phat,
+                               int* size_out_Phat, double** out_cerr_fz, int* size_out_cerr_fz,
+                               double** out_cerr_xhat, int* size_out_cerr_xhat, double** out_cerr_Phat,
+                               int* size_out_cerr_Phat, int** out_err_code, int* size_out_err_code,
+                        std::string* err) {
+    (void)size_controls;
+    Handle* h = (Handle*)handle;
+    Engine* e = h->eng;
+    CauchyDynamicsUpdateContainer& duc = h->duc;
+    duc.u = controls;
+    const int p = duc.p, n = duc.n, cmcc = duc.cmcc, pncc = duc.pncc;
+    const int nm = size_msmts;
+    const bool is_ltv = h->cb != nullptr;
+    // dynamics outputs (pycauchy.hpp:379-419)
+    if (!is_ltv) {
+        *size_out_Phi = *size_out_Gamma = *size_out_B = *size_out_H = *size_out_beta = *size_out_gamma = 0;
+        *out_Phi = mallocd<double>(0);
+        *out_Gamma = mallocd<double>(0);
+        *out_B = mallocd<double>(0);
+        *out_H = mallocd<double>(0);
+        *out_beta = mallocd<double>(0);
+        *out_gamma = mallocd<double>(0);
+    } else {
+        *size_out_Phi = n * n;
+        *out_Phi = mallocd<double>(n * n);
+        *size_out_Gamma = n * pncc;
+        *out_Gamma = mallocd<double>(n * pncc);
+        *size_out_B = n * cmcc;
+        *out_B = mallocd<double>(n * cmcc);
+        *size_out_H = nm * n;
+        *out_H = mallocd<double>(nm * n);
+        *size_out_beta = pncc;
+        *out_beta = mallocd<double>(pncc);
+        *size_out_gamma = nm;
+        *out_gamma = mallocd<double>(nm);
+    }
+    *size_out_fz = nm;
+    *out_fz = mallocd<double>(nm);
+    *size_out_xhat = nm * n;
+    *out_xhat = mallocd<double>(nm * n);
+    *size_out_Phat = nm * n * n;
+    *out_Phat = mallocd<double>(nm * n * n);
+    *size_out_cerr_fz = nm;
+    *out_cerr_fz = mallocd<double>(nm);
+    *size_out_cerr_xhat = nm;
+    *out_cerr_xhat = mallocd<double>(nm);
+    *size_out_cerr_Phat = nm;
+    *out_cerr_Phat = mallocd<double>(nm);
+    *size_out_err_code = nm;
+    *out_err_code = mallocd<int>(nm);
+    const int start = p - nm;
+    if (is_ltv) {
+        for (int i = 0; i < n; i++) duc.x[i] = e->mean[i].re;
+        h->cb(&duc);
+        memcpy(*out_Phi, duc.Phi, n * n * sizeof(double));
+        memcpy(*out_Gamma, duc.Gamma, n * pncc * sizeof(double));
+        memcpy(*out_B, duc.B, n * cmcc * sizeof(double));
+        memcpy(*out_H, duc.H + start * n, nm * n * sizeof(double));
+        memcpy(*out_beta, duc.beta, pncc * sizeof(double));
+        memcpy(*out_gamma, duc.gamma + start, nm * sizeof(double));
+    }
+    h->step_fz.clear();
+    h->step_cnt.clear();
+    for (int i = 0; i < nm; i++) {
+        const int idx = start + i;
+        std::exception_ptr fail;
+        try {
+            co_await e->update_co(msmts[i], duc.Phi, duc.Gamma, duc.beta, duc.H + idx * n, duc.gamma[idx], duc.B,
+                                  duc.u);
+        } catch (...) {
+            fail = std::current_exception();
+        }
+        if (fail) note_failure(fail, "pycauchy_single_step_ltiv", err);
+        h->record(i, e);
+        (*out_fz)[i] = e->fz.re;
+        for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
+        for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
+        (*out_cerr_fz)[i] = e->fz.im;
+        double mx = fabs(e->mean[0].im);
+        for (int j = 1; j < n; j++) mx = std::max(mx, fabs(e->mean[j].im));
+        (*out_cerr_xhat)[i] = mx;
+        double mp = fabs(e->cov[0].im);
+        for (int j = 1; j < n * n; j++) mp = std::max(mp, fabs(e->cov[j].im));
+        (*out_cerr_Phat)[i] = mp;
+        (*out_err_code)[i] = e->errs;
+    }
+}
+
+
+// ---------------------------------------------------------------- batch scheduler
+// Steps many engines on one host thread and one stream, launching each kernel of their
+// updates once for all of them.  Every engine's step runs as a task; a task runs until it
+// needs device data (a count, a total, the moment sums), handing over its launches, uploads
+// and library calls on the way.  Once every task waits or has ended, flush() issues the
+// handed-over work: requests of one engine stay in their order, and the heads of all
+// engines' request chains that name the same kernel become one merged launch (k_many), so
+// an update's ~70 launches are made once per group of engines instead of once per engine.
+// The reads of all waiting engines are then gathered into one buffer, copied to the host
+// with one copy and delivered after one synchronisation, and the tasks resume.
+// Profile of the merged g-table launches (mce_batch_profile_*): HIP events around every
+// merged k_group_eval launch on the scheduler's stream, and its algorithmic bytes counted on
+// the device by k_geval_bytes for each estimator's part, summed over all schedulers.
+struct BatchProfile {
+    std::mutex mu;
+    std::atomic<bool> on{false};
+    double launches = 0, bytes = 0, ms = 0, flushes = 0, merged = 0, library = 0;
+};
+BatchProfile& batch_profile() {
+    static BatchProfile* p = new BatchProfile();
+    return *p;
+}
+
+class BatchRunner {
+   public:
+    // MCE_BATCH_STATS=1: each run prints its flushes, launches and where its host time went.
+    struct Stats {
+        long flushes = 0, launches = 0, host_steps = 0, resumes = 0, reads = 0;
+        double t_resume = 0, t_pack = 0, t_wait = 0;
+    };
+    Stats stats;
+    static bool stats_on() {
+        static const bool on = [] {
+            const char* v = getenv("MCE_BATCH_STATS");
+            return v && v[0] && v[0] != '0';
+        }();
+        return on;
+    }
+    struct Job {
+        Engine* e = nullptr;
+        std::optional<Co<void>> task;
+        BatchSlot slot;
+        std::string err;
+        bool prof_on = false;
+    };
+
+    void run(std::vector<Job>& jobs) {
+        t_streams.init();
+        hipStream_t s = t_streams.st;
+        profiling_ = batch_profile().on.load();
+        if (profiling_) {
+            prof_acc_.reserve(1);
+            HIPCHK(hipMemsetAsync(prof_acc_.ptr, 0, sizeof(unsigned long long), s));
+            p_launches_ = p_ms_ = p_flushes_ = p_merged_ = p_library_ = 0;
+        }
+        for (Job& jb : jobs) {
+            jb.e->bs = &jb.slot;
+            jb.prof_on = jb.e->prof.on;
+            jb.e->prof.on = false;
+            jb.slot.geval_bytes = profiling_ ? prof_acc_.ptr : nullptr;
+        }
+        try {
+            drive(jobs, s);
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (Job& jb : jobs) {
+                jb.e->bs = nullptr;
+                jb.e->prof.on = jb.prof_on;
+                jb.slot.clear_chain();
+                jb.slot.reads.clear();
+            }
+            throw;
+        }
+        for (Job& jb : jobs) {
+            jb.e->bs = nullptr;
+            jb.e->prof.on = jb.prof_on;
+            jb.e->graveyard.release(jb.e->graveyard.big_bytes > Engine::kGraveyardBytes);
+        }
+        if (profiling_) {
+            unsigned long long b = 0;
+            HIPCHK(hipMemcpy(&b, prof_acc_.ptr, sizeof(b), hipMemcpyDeviceToHost));
+            BatchProfile& bp = batch_profile();
+            std::lock_guard<std::mutex> lk(bp.mu);
+            bp.launches += p_launches_;
+            bp.ms += p_ms_;
+            bp.bytes += (double)b;
+            bp.flushes += p_flushes_;
+            bp.merged += p_merged_;
+            bp.library += p_library_;
+        }
+        if (stats_on()) {
+            fprintf(stderr,
+                    "[batch] %zu engines: %ld flushes, %ld merged launches, %ld library calls, %ld resumes, %ld reads; "
+                    "host ms: resume %.2f pack %.2f wait %.2f\n",
+                    jobs.size(), stats.flushes, stats.launches, stats.host_steps, stats.resumes, stats.reads,
+                    stats.t_resume * 1e3, stats.t_pack * 1e3, stats.t_wait * 1e3);
+            stats = Stats{};
+        }
+    }
+
+   private:
+    void drive(std::vector<Job>& jobs, hipStream_t s) {
+        for (Job& jb : jobs) resume(jb, {});
+        for (;;) {
+            bool busy = false;
+            for (Job& jb : jobs)
+                if (jb.slot.waiting || !jb.slot.chain.empty()) {
+                    busy = true;
+                    break;
+                }
+            if (!busy) break;
+            flush(jobs, s);
+            for (Job& jb : jobs)
+                if (jb.slot.waiting) {
+                    const std::coroutine_handle<> h = jb.slot.waiting;
+                    jb.slot.waiting = {};
+                    resume(jb, h);
+                }
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    static double now() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+    void resume(Job& jb, std::coroutine_handle<> h) {
+        GraveyardScope gs(&jb.e->graveyard);
+        const double t0 = stats_on() ? now() : 0;
+        if (h)
+            h.resume();
+        else
+            jb.task->start();
+        if (stats_on()) {
+            stats.t_resume += now() - t0;
+            stats.resumes++;
+        }
+        if (jb.task->done()) {
+            try {
+                jb.task->result();
+            } catch (const std::exception& ex) {
+                if (jb.err.empty()) jb.err = ex.what();
+            }
+        }
+    }
+
+    struct Item {
+        int j;
+        int r;
+    };
+    struct Step {
+        BatchReq::Kind kind;
+        const void* fn;
+        uint32_t threads;
+        std::vector<Item> items;
+    };
+
+    static size_t up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+    // Grows a pinned host buffer (the stream is idle whenever this runs).
+    static void grow_pinned(uint8_t*& p, size_t& cap, size_t need) {
+        if (need <= cap) return;
+        if (p) HIPCHK(hipHostFree(p));
+        cap = std::max(need, 2 * cap);
+        HIPCHK(hipHostMalloc((void**)&p, cap, hipHostMallocDefault));
+    }
+    // Grows a device buffer; the old block goes straight back (the stream is idle).
+    static void grow_dev(DBuf<uint8_t>& b, size_t need) {
+        if (need <= b.cap) return;
+        const size_t n = std::max(need, 2 * b.cap);
+        b.release();
+        b.reserve(n);
+    }
+
+    void plan_steps(std::vector<Job>& jobs) {
+        steps_.clear();
+        const int J = (int)jobs.size();
+        head_.assign(J, 0);
+        size_t left = 0;
+        for (Job& jb : jobs) left += jb.slot.chain.size();
+        while (left) {
+            // library calls at a chain's head run first, one by one, in job order
+            bool host = false;
+            for (int j = 0; j < J; j++) {
+                auto& ch = jobs[j].slot.chain;
+                while (head_[j] < ch.size() && ch[head_[j]].kind == BatchReq::kHost) {
+                    steps_.push_back(Step{BatchReq::kHost, nullptr, 0, {Item{j, (int)head_[j]}}});
+                    head_[j]++;
+                    left--;
+                    host = true;
+                }
+            }
+            if (host) continue;
+            // then the kernel (or the uploads) at the most chain heads, for all those chains
+            keys_.clear();
+            for (int j = 0; j < J; j++) {
+                auto& ch = jobs[j].slot.chain;
+                if (head_[j] >= ch.size()) continue;
+                const BatchReq& r = ch[head_[j]];
+                const void* fn = r.kind == BatchReq::kUpload ? nullptr : r.fn;
+                bool found = false;
+                for (auto& k : keys_)
+                    if (k.fn == fn && k.threads == r.threads && k.kind == r.kind) {
+                        k.count++;
+                        found = true;
+                        break;
+                    }
+                if (!found) keys_.push_back(Key{r.kind, fn, r.threads, 1});
+            }
+            const Key* best = &keys_[0];
+            for (const Key& k : keys_)
+                if (k.count > best->count) best = &k;
+            Step st{best->kind, best->fn, best->threads, {}};
+            for (int j = 0; j < J; j++) {
+                auto& ch = jobs[j].slot.chain;
+                if (head_[j] >= ch.size()) continue;
+                const BatchReq& r = ch[head_[j]];
+                const void* fn = r.kind == BatchReq::kUpload ? nullptr : r.fn;
+                if (r.kind == best->kind && fn == best->fn && r.threads == best->threads) {
+                    st.items.push_back(Item{j, (int)head_[j]});
+                    head_[j]++;
+                    left--;
+                }
+            }
+            steps_.push_back(std::move(st));
+        }
+    }
+
+    void flush(std::vector<Job>& jobs, hipStream_t s) {
+        const double t0 = stats_on() ? now() : 0;
+        plan_steps(jobs);
+        using Copy = Kern<&k_copy_bytes>;
+        using CopyEntry = Copy::Entry;
+        auto copy_blocks = [](size_t bytes) {
+            return (unsigned)std::min<size_t>(64, std::max<size_t>(1, (bytes / 8 + 255) / 256));
+        };
+        // layout: upload payloads, then per step its first-block array and entries, then the
+        // read gather's
+        size_t off = 0;
+        pay_off_.clear();
+        for (const Step& st : steps_)
+            if (st.kind == BatchReq::kUpload)
+                for (const Item& it : st.items) {
+                    off = up(off, 16);
+                    pay_off_.push_back(off);
+                    off += jobs[it.j].slot.chain[it.r].nbytes;
+                }
+        step_off_.clear();
+        for (const Step& st : steps_) {
+            if (st.kind == BatchReq::kHost) {
+                step_off_.push_back({0, 0});
+                continue;
+            }
+            const size_t n = st.items.size();
+            size_t esize, ealign;
+            if (st.kind == BatchReq::kUpload) {
+                esize = sizeof(CopyEntry);
+                ealign = alignof(CopyEntry);
+            } else {
+                const BatchReq& r0 = jobs[st.items[0].j].slot.chain[st.items[0].r];
+                esize = r0.esize;
+                ealign = r0.ealign;
+            }
+            const size_t f = up(off, 16);
+            const size_t e = up(f + 4 * n, std::max<size_t>(16, ealign));
+            step_off_.push_back({f, e});
+            off = e + n * esize;
+        }
+        size_t nreads = 0, rbytes = 0;
+        for (Job& jb : jobs)
+            for (const BatchRead& rd : jb.slot.reads) {
+                nreads++;
+                rbytes = up(rbytes, 16) + rd.bytes;
+            }
+        const size_t rf = up(off, 16);
+        const size_t re = up(rf + 4 * nreads, std::max<size_t>(16, alignof(CopyEntry)));
+        if (nreads) off = re + nreads * sizeof(CopyEntry);
+        const size_t total = off;
+        grow_pinned(stage_, stage_cap_, total);
+        grow_dev(arena_, total);
+        if (nreads) {
+            grow_dev(rdev_, rbytes);
+            grow_pinned(rhost_, rhost_cap_, rbytes);
+        }
+        uint8_t* dev = arena_.ptr;
+        // fill the stage
+        size_t pi = 0;
+        for (size_t si = 0; si < steps_.size(); si++) {
+            const Step& st = steps_[si];
+            if (st.kind == BatchReq::kHost) continue;
+            unsigned* first = (unsigned*)(stage_ + step_off_[si].first);
+            uint8_t* ents = stage_ + step_off_[si].second;
+            unsigned blocks = 0;
+            for (size_t k = 0; k < st.items.size(); k++) {
+                const Item& it = st.items[k];
+                const BatchReq& r = jobs[it.j].slot.chain[it.r];
+                first[k] = blocks;
+                if (st.kind == BatchReq::kUpload) {
+                    const size_t po = pay_off_[pi++];
+                    memcpy(stage_ + po, jobs[it.j].slot.bytes.data() + r.off, r.nbytes);
+                    const unsigned gx = copy_blocks(r.nbytes);
+                    const CopyEntry ce = Copy::entry(gx, 1u, (const uint8_t*)(dev + po), (uint8_t*)r.dst,
+                                                     (int64_t)r.nbytes);
+                    memcpy(ents + k * sizeof(CopyEntry), &ce, sizeof(CopyEntry));
+                    blocks += gx;
+                } else {
+                    memcpy(ents + k * r.esize, jobs[it.j].slot.bytes.data() + r.off, r.esize);
+                    blocks += r.gx * r.gy;
+                }
+            }
+            step_blocks_.resize(steps_.size());
+            step_blocks_[si] = blocks;
+        }
+        unsigned rblocks = 0;
+        if (nreads) {
+            unsigned* first = (unsigned*)(stage_ + rf);
+            uint8_t* ents = stage_ + re;
+            size_t ro = 0, k = 0;
+            for (Job& jb : jobs)
+                for (const BatchRead& rd : jb.slot.reads) {
+                    ro = up(ro, 16);
+                    first[k] = rblocks;
+                    const unsigned gx = copy_blocks(rd.bytes);
+                    const CopyEntry ce =
+                        Copy::entry(gx, 1u, (const uint8_t*)rd.src, (uint8_t*)(rdev_.ptr + ro), (int64_t)rd.bytes);
+                    memcpy(ents + k * sizeof(CopyEntry), &ce, sizeof(CopyEntry));
+                    rblocks += gx;
+                    ro += rd.bytes;
+                    k++;
+                }
+        }
+        // issue: the staged bytes, the steps in order, the read gather and its copy
+        if (total) HIPCHK(hipMemcpyAsync(dev, stage_, total, hipMemcpyHostToDevice, s));
+        for (size_t si = 0; si < steps_.size(); si++) {
+            const Step& st = steps_[si];
+            if (st.kind == BatchReq::kHost) {
+                const Item& it = st.items[0];
+                jobs[it.j].slot.chain[it.r].host(s);
+                continue;
+            }
+            size_t lds = 0;
+            for (const Item& it : st.items) lds = std::max(lds, jobs[it.j].slot.chain[it.r].lds);
+            const void* fn = st.kind == BatchReq::kUpload ? Copy::many() : st.fn;
+            const uint32_t threads = st.kind == BatchReq::kUpload ? 256u : st.threads;
+            const bool timed = profiling_ && st.kind == BatchReq::kKernel &&
+                               strstr(jobs[st.items[0].j].slot.chain[st.items[0].r].name, "k_group_eval") != nullptr;
+            if (timed) {
+                if (timed_.size() <= timed_n_) timed_.resize(timed_n_ + 1, {nullptr, nullptr});
+                auto& evp = timed_[timed_n_++];
+                if (!evp.first) {
+                    HIPCHK(hipEventCreate(&evp.first));
+                    HIPCHK(hipEventCreate(&evp.second));
+                }
+                HIPCHK(hipEventRecord(evp.first, s));
+                launch_many(fn, step_blocks_[si], threads, lds, dev + step_off_[si].first,
+                            dev + step_off_[si].second, (int)st.items.size(), s);
+                HIPCHK(hipEventRecord(evp.second, s));
+            } else {
+                launch_many(fn, step_blocks_[si], threads, lds, dev + step_off_[si].first,
+                            dev + step_off_[si].second, (int)st.items.size(), s);
+            }
+        }
+        if (nreads) {
+            launch_many(Copy::many(), rblocks, 256, 0, dev + rf, dev + re, (int)nreads, s);
+            HIPCHK(hipMemcpyAsync(rhost_, rdev_.ptr, rbytes, hipMemcpyDeviceToHost, s));
+        }
+        const double t1 = stats_on() ? now() : 0;
+        HIPCHK(hipStreamSynchronize(s));
+        if (profiling_) {
+            for (size_t k = 0; k < timed_n_; k++) {
+                float t = 0;
+                HIPCHK(hipEventElapsedTime(&t, timed_[k].first, timed_[k].second));
+                p_ms_ += t;
+                p_launches_ += 1;
+            }
+            timed_n_ = 0;
+            p_flushes_ += 1;
+            for (const Step& st : steps_) (st.kind == BatchReq::kHost ? p_library_ : p_merged_) += 1;
+        }
+        if (stats_on()) {
+            const double t2 = now();
+            stats.t_pack += t1 - t0;
+            stats.t_wait += t2 - t1;
+            stats.flushes++;
+            stats.reads += (long)nreads;
+            for (const Step& st : steps_) (st.kind == BatchReq::kHost ? stats.host_steps : stats.launches)++;
+        }
+        if (nreads) {
+            size_t ro = 0;
+            for (Job& jb : jobs)
+                for (const BatchRead& rd : jb.slot.reads) {
+                    ro = up(ro, 16);
+                    memcpy(rd.dst, rhost_ + ro, rd.bytes);
+                    ro += rd.bytes;
+                }
+        }
+        for (Job& jb : jobs) {
+            jb.slot.clear_chain();
+            jb.slot.reads.clear();
+        }
+    }
+
+    void launch_many(const void* fn, unsigned blocks, uint32_t threads, size_t lds, uint8_t* first, uint8_t* ents,
+                     int n, hipStream_t s) {
+        if (blocks == 0 || n == 0) return;
+        if (!launch_ok(dim3(blocks), dim3(threads), lds, "merged launch")) return;
+        allow_lds(fn, lds);
+        void* args[] = {(void*)&first, (void*)&ents, (void*)&n};
+        HIPCHK(hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, lds, s));
+        launch_done(s, "merged launch");
+    }
+
+    struct Key {
+        BatchReq::Kind kind;
+        const void* fn;
+        uint32_t threads;
+        int count;
+    };
+    bool profiling_ = false;
+    DBuf<unsigned long long> prof_acc_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed_;
+    size_t timed_n_ = 0;
+    double p_launches_ = 0, p_ms_ = 0, p_flushes_ = 0, p_merged_ = 0, p_library_ = 0;
+    std::vector<Step> steps_;
+    std::vector<size_t> head_;
+    std::vector<Key> keys_;
+    std::vector<size_t> pay_off_;
+    std::vector<std::pair<size_t, size_t>> step_off_;
+    std::vector<unsigned> step_blocks_;
+    DBuf<uint8_t> arena_, rdev_;
+    uint8_t* stage_ = nullptr;
+    uint8_t* rhost_ = nullptr;
+    size_t stage_cap_ = 0, rhost_cap_ = 0;
+};
+
+// The calling thread's scheduler; it keeps its buffers for the next batch call and lives as
+// long as the process (pool threads outlive main, so it is never destroyed).
+BatchRunner& batch_runner() {
+    static thread_local BatchRunner* r = new BatchRunner();
+    return *r;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -2541,84 +3301,11 @@ void pycauchy_single_step_ltiv(void* handle, double* msmts, int size_msmts, doub
                                int* size_out_Phat, double** out_cerr_fz, int* size_out_cerr_fz,
                                double** out_cerr_xhat, int* size_out_cerr_xhat, double** out_cerr_Phat,
                                int* size_out_cerr_Phat, int** out_err_code, int* size_out_err_code) {
-    (void)size_controls;
-    Handle* h = (Handle*)handle;
-    Engine* e = h->eng;
-    CauchyDynamicsUpdateContainer& duc = h->duc;
-    duc.u = controls;
-    const int p = duc.p, n = duc.n, cmcc = duc.cmcc, pncc = duc.pncc;
-    const int nm = size_msmts;
-    const bool is_ltv = h->cb != nullptr;
-    // dynamics outputs (pycauchy.hpp:379-419)
-    if (!is_ltv) {
-        *size_out_Phi = *size_out_Gamma = *size_out_B = *size_out_H = *size_out_beta = *size_out_gamma = 0;
-        *out_Phi = mallocd<double>(0);
-        *out_Gamma = mallocd<double>(0);
-        *out_B = mallocd<double>(0);
-        *out_H = mallocd<double>(0);
-        *out_beta = mallocd<double>(0);
-        *out_gamma = mallocd<double>(0);
-    } else {
-        *size_out_Phi = n * n;
-        *out_Phi = mallocd<double>(n * n);
-        *size_out_Gamma = n * pncc;
-        *out_Gamma = mallocd<double>(n * pncc);
-        *size_out_B = n * cmcc;
-        *out_B = mallocd<double>(n * cmcc);
-        *size_out_H = nm * n;
-        *out_H = mallocd<double>(nm * n);
-        *size_out_beta = pncc;
-        *out_beta = mallocd<double>(pncc);
-        *size_out_gamma = nm;
-        *out_gamma = mallocd<double>(nm);
-    }
-    *size_out_fz = nm;
-    *out_fz = mallocd<double>(nm);
-    *size_out_xhat = nm * n;
-    *out_xhat = mallocd<double>(nm * n);
-    *size_out_Phat = nm * n * n;
-    *out_Phat = mallocd<double>(nm * n * n);
-    *size_out_cerr_fz = nm;
-    *out_cerr_fz = mallocd<double>(nm);
-    *size_out_cerr_xhat = nm;
-    *out_cerr_xhat = mallocd<double>(nm);
-    *size_out_cerr_Phat = nm;
-    *out_cerr_Phat = mallocd<double>(nm);
-    *size_out_err_code = nm;
-    *out_err_code = mallocd<int>(nm);
-    const int start = p - nm;
-    if (is_ltv) {
-        for (int i = 0; i < n; i++) duc.x[i] = e->mean[i].re;
-        h->cb(&duc);
-        memcpy(*out_Phi, duc.Phi, n * n * sizeof(double));
-        memcpy(*out_Gamma, duc.Gamma, n * pncc * sizeof(double));
-        memcpy(*out_B, duc.B, n * cmcc * sizeof(double));
-        memcpy(*out_H, duc.H + start * n, nm * n * sizeof(double));
-        memcpy(*out_beta, duc.beta, pncc * sizeof(double));
-        memcpy(*out_gamma, duc.gamma + start, nm * sizeof(double));
-    }
-    h->step_fz.clear();
-    h->step_cnt.clear();
-    for (int i = 0; i < nm; i++) {
-        const int idx = start + i;
-        try {
-            e->update(msmts[i], duc.Phi, duc.Gamma, duc.beta, duc.H + idx * n, duc.gamma[idx], duc.B, duc.u);
-        } catch (const std::exception& ex) {
-            set_err("pycauchy_single_step_ltiv", ex);
-        }
-        h->record(i, e);
-        (*out_fz)[i] = e->fz.re;
-        for (int j = 0; j < n; j++) (*out_xhat)[i * n + j] = e->mean[j].re;
-        for (int j = 0; j < n * n; j++) (*out_Phat)[i * n * n + j] = e->cov[j].re;
-        (*out_cerr_fz)[i] = e->fz.im;
-        double mx = fabs(e->mean[0].im);
-        for (int j = 1; j < n; j++) mx = std::max(mx, fabs(e->mean[j].im));
-        (*out_cerr_xhat)[i] = mx;
-        double mp = fabs(e->cov[0].im);
-        for (int j = 1; j < n * n; j++) mp = std::max(mp, fabs(e->cov[j].im));
-        (*out_cerr_Phat)[i] = mp;
-        (*out_err_code)[i] = e->errs;
-    }
+    run_sync(step_ltiv_task(handle, msmts, size_msmts, controls, size_controls, out_Phi, size_out_Phi, out_Gamma,
+                            size_out_Gamma, out_B, size_out_B, out_H, size_out_H, out_beta, size_out_beta, out_gamma,
+                            size_out_gamma, out_fz, size_out_fz, out_xhat, size_out_xhat, out_Phat, size_out_Phat,
+                            out_cerr_fz, size_out_cerr_fz, out_cerr_xhat, size_out_cerr_xhat, out_cerr_Phat,
+                            size_out_cerr_Phat, out_err_code, size_out_err_code, nullptr));
 }
 
 void* pycauchy_initialize_nonlin(int num_steps, double* x0, int size_x0, double* A0, int size_A0, double* p0,
@@ -2788,6 +3475,85 @@ int pycauchy_batch_single_step_ltiv(void** handles, int count, double** msmts, c
                                   (double**)&o[7], &n[7], (double**)&o[8], &n[8], (double**)&o[9], &n[9],
                                   (double**)&o[10], &n[10], (double**)&o[11], &n[11], (int**)&o[12], &n[12]);
     });
+}
+
+int pycauchy_batch_single_step_ltiv_merged(void** handles, int count, double** msmts, const int* size_msmts,
+                                           double** controls, const int* size_controls, void** outs, int* out_sizes,
+                                           int num_groups) {
+    if (count <= 0) return 0;
+    const int G = std::max(1, std::min(num_groups, count));
+    // estimators with the most terms first, dealt round-robin so each group gets a share
+    std::vector<int> order(count);
+    std::vector<int> nt(count);
+    for (int i = 0; i < count; i++) {
+        order[i] = i;
+        nt[i] = pycauchy_single_step_get_number_of_terms(handles[i]);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nt[a] > nt[b]; });
+    std::vector<std::vector<int>> groups(G);
+    for (int k = 0; k < count; k++) groups[k % G].push_back(order[k]);
+    std::vector<std::string> errors(count);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        int g;
+        while ((g = next.fetch_add(1)) < G) {
+            const std::vector<int>& mem = groups[g];
+            std::vector<BatchRunner::Job> jobs(mem.size());
+            for (size_t k = 0; k < mem.size(); k++) {
+                const int i = mem[k];
+                jobs[k].e = ((Handle*)handles[i])->eng;
+                void** o = outs + 13 * (int64_t)i;
+                int* n = out_sizes + 13 * (int64_t)i;
+                double* u = controls ? controls[i] : nullptr;
+                const int nu = size_controls ? size_controls[i] : 0;
+                jobs[k].task.emplace(step_ltiv_task(
+                    handles[i], msmts[i], size_msmts[i], u, nu, (double**)&o[0], &n[0], (double**)&o[1], &n[1],
+                    (double**)&o[2], &n[2], (double**)&o[3], &n[3], (double**)&o[4], &n[4], (double**)&o[5], &n[5],
+                    (double**)&o[6], &n[6], (double**)&o[7], &n[7], (double**)&o[8], &n[8], (double**)&o[9], &n[9],
+                    (double**)&o[10], &n[10], (double**)&o[11], &n[11], (int**)&o[12], &n[12], &jobs[k].err));
+            }
+            try {
+                batch_runner().run(jobs);
+            } catch (const std::exception& ex) {
+                for (auto& jb : jobs)
+                    if (jb.err.empty()) jb.err = std::string("batch scheduler: ") + ex.what();
+            }
+            for (size_t k = 0; k < mem.size(); k++) errors[mem[k]] = jobs[k].err;
+        }
+    };
+    if (G > 1) {
+        const std::function<void()> job = work;
+        worker_pool().run(G - 1, job);
+    } else {
+        work();
+    }
+    int failed = 0;
+    std::string first;
+    for (int i = 0; i < count; i++)
+        if (!errors[i].empty()) {
+            if (!failed) first = "estimator " + std::to_string(i) + ": " + errors[i];
+            failed++;
+        }
+    g_last_error = first;
+    return failed;
+}
+
+void mce_batch_profile_enable(int on) {
+    BatchProfile& bp = batch_profile();
+    std::lock_guard<std::mutex> lk(bp.mu);
+    bp.on = on != 0;
+    bp.launches = bp.bytes = bp.ms = bp.flushes = bp.merged = bp.library = 0;
+}
+
+void mce_batch_profile_read(double* out) {
+    BatchProfile& bp = batch_profile();
+    std::lock_guard<std::mutex> lk(bp.mu);
+    out[0] = bp.launches;
+    out[1] = bp.bytes;
+    out[2] = bp.ms;
+    out[3] = bp.flushes;
+    out[4] = bp.merged;
+    out[5] = bp.library;
 }
 
 int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts, const int* size_msmts,

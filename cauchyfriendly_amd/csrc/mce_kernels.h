@@ -13,7 +13,12 @@
 //   k_group_sort       key order for groups whose root was replaced
 //   k_compact          survivors become the next step's parents
 #pragma once
+#include <climits>
+
+#include <hipcub/block/block_radix_sort.hpp>
+
 #include "mce_device.h"
+#include "mce_multi.h"
 #include "mce_pool.h"
 
 namespace mce {
@@ -232,9 +237,9 @@ __device__ inline bool wave_all(bool pred) { return __all(pred); }
 // cauchy_term.hpp:435-531 (time_prop, normalize_hps, tp_coalign), with the B-table
 // mode decision of cauchy_estimator.hpp:678-708 (serial) / 1614-1637 (threaded).
 template <int D>
-__global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const int* __restrict__ cen_half,
+__device__ __forceinline__ void k_tp(MG<1024, 1> mg, TermPool P, int n, const StepConst* __restrict__ sc, const int* __restrict__ cen_half,
                      int* __restrict__ slots, int* __restrict__ btcap, int* __restrict__ err) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int MM = P.MM;
     int m = P.m[i];
@@ -324,9 +329,9 @@ __global__ void k_tp(TermPool P, int n, const StepConst* __restrict__ sc, const 
 }
 
 // B-tables that need no enumeration; enumeration jobs are appended to a list.
-__global__ void k_btab_simple(TermPool P, int n, const int64_t* __restrict__ bt_base, const uint32_t* __restrict__ gkeys,
+__device__ __forceinline__ void k_btab_simple(MG<1024, 1> mg, TermPool P, int n, const int64_t* __restrict__ bt_base, const uint32_t* __restrict__ gkeys,
                               uint32_t* __restrict__ Barena, int* __restrict__ enum_list, int* __restrict__ enum_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint8_t mode = P.mode[i];
     const int64_t off = bt_base[i];
@@ -372,7 +377,7 @@ __global__ void k_btab_simple(TermPool P, int n, const int64_t* __restrict__ bt_
 // or, when `psearch` is set (both sets would not fit in LDS: 2^20 + 2^18 bits), are
 // looked up by binary search in the parent's sorted keys in global memory.
 template <int D>
-__global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __restrict__ list,
+__device__ __forceinline__ void k_btab_enum(MG<256, 1> mg, TermPool P, const int* __restrict__ list,
                                                    const double* __restrict__ bpert, const uint8_t* __restrict__ combos,
                                                    const int64_t* __restrict__ combo_off, const int* __restrict__ combo_cnt,
                                                    const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ Barena,
@@ -380,7 +385,7 @@ __global__ __launch_bounds__(256) void k_btab_enum(TermPool P, const int* __rest
                                                    int psearch) {
     extern __shared__ uint32_t bits[];
     __shared__ int scan_buf[256];
-    const int i = list[blockIdx.x];
+    const int i = list[mg.bx];
     const int MM = P.MM;
     const int m = P.m[i];
     const int phc = P.phc[i];
@@ -659,11 +664,11 @@ __device__ inline int dev_mu_coalign(double* A, double* p, double* q, uint8_t* c
 // the parent itself.  Writes the gather key (shape, reference thread, is_child)
 // that reproduces the reference's term order (cauchy_estimator.hpp:898-937, 1683-1704).
 template <int D>
-__global__ void k_mu(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT, int nslots,
+__device__ __forceinline__ void k_mu(MG<1024, 1> mg, TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT, int nslots,
                      const StepConst* __restrict__ sc, const uint32_t* __restrict__ gk_prev,
                      const cplx* __restrict__ gv_prev, double* __restrict__ gy, uint32_t* __restrict__ keys_out,
                      uint32_t* __restrict__ chain_key, int* __restrict__ err) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = mg.bx * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int lo = 0, hi = np - 1;
     while (lo < hi) {
@@ -828,12 +833,12 @@ __device__ inline double dev_child_row(const double* A, const double* p, const d
 // are kept, for the coalignment filter.  When the filter finds a possible merge (a
 // rare case) the rows are rebuilt in memory and coaligned there, as k_mu does.
 template <int D, int KM>
-__global__ __launch_bounds__(256, 4) void k_mu_r(TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
+__device__ __forceinline__ void k_mu_r(MG<256, 4> mg, TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
                                              int nslots, const StepConst* __restrict__ sc,
                                              const uint32_t* __restrict__ gk_prev, const cplx* __restrict__ gv_prev,
                                              double* __restrict__ gy, uint32_t* __restrict__ keys_out,
                                              uint32_t* __restrict__ chain_key, int* __restrict__ err) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = mg.bx * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int lo = 0, hi = np - 1;
     while (lo < hi) {
@@ -1057,13 +1062,13 @@ __host__ __device__ constexpr int mom_blocks() {
 // Products with the reference's operations: g, g*y_j and the negated (g*y_j)*y_k
 // (the reference subtracts the latter; a - b == a + (-b) exactly).
 template <int D>
-__global__ void k_mom_terms(const double* __restrict__ gy, const uint8_t* __restrict__ flags,
+__device__ __forceinline__ void k_mom_terms(MG<1024, 1> mg, const double* __restrict__ gy, const uint8_t* __restrict__ flags,
                             const uint32_t* __restrict__ chain, const int* __restrict__ chain_off, int T, int nslots,
                             double* __restrict__ X) {
     constexpr int NS = 2 + 2 * D;
     constexpr int NC = 1 + D + D * D;
     constexpr int NCP = mom_blocks<D>() * kSumCW / 2;  // padded components per position
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)mg.bx * blockDim.x + threadIdx.x;
     const int pos = (int)(i / NCP);
     const int cc = (int)(i - (int64_t)pos * NCP);
     if (pos >= nslots) return;
@@ -1100,7 +1105,7 @@ __global__ void k_mom_terms(const double* __restrict__ gy, const uint8_t* __rest
 template <int D>
 // part[T * NC2] also receives the measurement update's error bits (*mu_err), so one copy
 // brings both back.
-__global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, const int* __restrict__ chain_off,
+__device__ __forceinline__ void k_mom_sum(MG<256, 1> mg, const double* __restrict__ X, const int* __restrict__ chain_off,
                                                  int T, double* __restrict__ part, const int* __restrict__ mu_err) {
     constexpr int NC2 = 2 * (1 + D + D * D);
     constexpr int TE = kSumTP * kSumCW;          // doubles per tile
@@ -1108,8 +1113,8 @@ __global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, c
     constexpr int PER = TE / NL;                 // doubles per loader thread per tile
     static_assert(PER * NL == TE, "a tile is an exact multiple of the loader threads");
     __shared__ double ring[2][TE];
-    const int t = blockIdx.x;
-    const int yb = blockIdx.y;
+    const int t = mg.bx;
+    const int yb = mg.by;
     const int tid = threadIdx.x;
     if (t == 0 && yb == 0 && tid == 0) part[T * NC2] = (double)mu_err[0];
     const int npos = chain_off[T];
@@ -1199,12 +1204,12 @@ __global__ __launch_bounds__(256) void k_mom_sum(const double* __restrict__ X, c
 
 // Histogram of small integer values privatised in LDS (nbins <= 64).  mode 0: valid
 // slots by shape; mode 1: surviving groups by shape; mode 2: chain keys.
-__global__ void k_hist(int n, int nbins, int mode, const uint8_t* __restrict__ flags, const int* __restrict__ vals,
+__device__ __forceinline__ void k_hist(MG<1024, 1> mg, int n, int nbins, int mode, const uint8_t* __restrict__ flags, const int* __restrict__ vals,
                        const int* __restrict__ surv, const uint32_t* __restrict__ ukeys, int* __restrict__ hist) {
     __shared__ int h[64];
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) h[b] = 0;
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = mg.bx * blockDim.x + threadIdx.x; i < n; i += mg.gx * blockDim.x) {
         int v = -1;
         if (mode == 0) {
             if (flags[i] & kValid) v = vals[i];
@@ -1221,8 +1226,8 @@ __global__ void k_hist(int n, int nbins, int mode, const uint8_t* __restrict__ f
 }
 
 // ------------------------------------------------------------------ gather
-__global__ void k_iota(uint32_t* __restrict__ v, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_iota(MG<1024, 1> mg, uint32_t* __restrict__ v, int n) {
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i < n) v[i] = (uint32_t)i;
 }
 
@@ -1249,25 +1254,25 @@ __device__ inline int ftr_chunk_of(int l, int n) {
     return c > 7 ? 7 : c;
 }
 
-__global__ void k_ftr_keys(const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
+__device__ __forceinline__ void k_ftr_keys(MG<1024, 1> mg, const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
                            uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = mg.bx * blockDim.x + threadIdx.x;
     if (p >= n) return;
     keys[p] = f64_radix_key(OUT.b[(int64_t)perm[p] * d + axis]);
     idx[p] = (uint32_t)p;
 }
 
-__global__ void k_ftr_vals(const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
+__device__ __forceinline__ void k_ftr_vals(MG<1024, 1> mg, const TermPool OUT, const uint32_t* __restrict__ perm, int n, int d, int axis,
                            const uint32_t* __restrict__ sidx, double* __restrict__ vals) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = mg.bx * blockDim.x + threadIdx.x;
     if (r >= n) return;
     vals[r] = OUT.b[(int64_t)perm[sidx[r]] * d + axis];
 }
 
 // Bucket of each position of the value-sorted order (second, stable sort key).
-__global__ void k_ftr_bkeys(int n, const int* __restrict__ boff, int nb, const uint32_t* __restrict__ sidx,
+__device__ __forceinline__ void k_ftr_bkeys(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, const uint32_t* __restrict__ sidx,
                             uint32_t* __restrict__ bkey) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = mg.bx * blockDim.x + threadIdx.x;
     if (r >= n) return;
     bkey[r] = (uint32_t)bucket_of(boff, nb, (int)sidx[r]);
 }
@@ -1279,11 +1284,11 @@ __global__ void k_ftr_bkeys(int n, const int* __restrict__ boff, int nb, const u
 // per sorted position; the window is found by scanning the sorted neighbours.
 // pass 0 counts, pass 1 writes the out-edges (global positions) of term i.
 template <int D>
-__global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ perm, int n,
+__device__ __forceinline__ void k_ftr_edges(MG<1024, 1> mg, const TermPool OUT, const uint32_t* __restrict__ perm, int n,
                             const int* __restrict__ boff, int nb, const StepConst* __restrict__ sc,
                             const double* __restrict__ vals, const uint32_t* __restrict__ sidx,
                             int* __restrict__ cnt, const int* __restrict__ eoff, int* __restrict__ edges, int pass) {
-    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sp = mg.bx * blockDim.x + threadIdx.x;
     if (sp >= n) return;
     const double ep = kReductionEps;
     const int bk = bucket_of(boff, nb, sp);
@@ -1354,10 +1359,10 @@ __global__ void k_ftr_edges(const TermPool OUT, const uint32_t* __restrict__ per
 // into the value-sorted order: b (d), p (first m) and the first m entries of the
 // flattened A, row stride MM.
 template <int D>
-__global__ void k_ftr_gather(const TermPool OUT, const uint32_t* __restrict__ perm, int n,
+__device__ __forceinline__ void k_ftr_gather(MG<1024, 1> mg, const TermPool OUT, const uint32_t* __restrict__ perm, int n,
                              const int* __restrict__ boff, int nb, const uint32_t* __restrict__ sidx,
                              double* __restrict__ gb, double* __restrict__ gp, double* __restrict__ gA) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = mg.bx * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int m = bucket_of(boff, nb, r);
     const int MM = OUT.MM;
@@ -1374,9 +1379,9 @@ __global__ void k_ftr_gather(const TermPool OUT, const uint32_t* __restrict__ pe
 // its bucket with value in (v - ep, v + ep] (a window must hold two points,
 // ftr_binary_search, term_reduction.hpp:89-157).  wcnt = r1 - r0, or 0.
 // Also clears each position's edge count for the edge passes that follow.
-__global__ void k_ftr_win(int n, const int* __restrict__ boff, int nb, const double* __restrict__ vals,
+__device__ __forceinline__ void k_ftr_win(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, const double* __restrict__ vals,
                           int* __restrict__ wlo, int64_t* __restrict__ wcnt, int* __restrict__ ecnt) {
-    const int sp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sp = mg.bx * blockDim.x + threadIdx.x;
     if (sp >= n) return;
     ecnt[sp] = 0;
     const int bk = bucket_of(boff, nb, sp);
@@ -1409,14 +1414,14 @@ __global__ void k_ftr_win(int n, const int* __restrict__ boff, int nb, const dou
 // pass 1 writes the flagged pairs into term i's list (edge order within a list does
 // not matter to the resolution that follows).
 template <int D>
-__global__ void k_ftr_pairs(int64_t npairs, int n, const int* __restrict__ boff, int nb,
+__device__ __forceinline__ void k_ftr_pairs(MG<1024, 1> mg, int64_t npairs, int n, const int* __restrict__ boff, int nb,
                             const StepConst* __restrict__ sc, const int64_t* __restrict__ woff,
                             const int* __restrict__ wlo, const uint32_t* __restrict__ sidx, int MM,
                             const double* __restrict__ gb, const double* __restrict__ gp,
                             const double* __restrict__ gA, uint8_t* __restrict__ flag, int* __restrict__ ecnt,
                             const int* __restrict__ eoff, int* __restrict__ cursor, int* __restrict__ edges,
                             int pass) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = (int64_t)mg.bx * blockDim.x + threadIdx.x;
     if (q >= npairs) return;
     // owning position: largest sp with woff[sp] <= q
     int lo = 0, hi = n - 1;
@@ -1464,11 +1469,11 @@ __global__ void k_ftr_pairs(int64_t npairs, int n, const int* __restrict__ boff,
 }
 
 // In-degree over edges whose ends share a chunk (pass 0), or the in-edge lists (pass 1).
-__global__ void k_ftr_inedges(int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
+__device__ __forceinline__ void k_ftr_inedges(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
                               const int* __restrict__ ecnt, const int* __restrict__ edges, int* __restrict__ indeg,
                               const int* __restrict__ inoff, int* __restrict__ cursor, int* __restrict__ inedges,
                               int pass) {
-    const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ig = mg.bx * blockDim.x + threadIdx.x;
     if (ig >= n) return;
     const int bk = bucket_of(boff, nb, ig);
     const int beg = boff[bk], nbk = boff[bk + 1] - beg;
@@ -1497,13 +1502,13 @@ struct FillArgs {
     int64_t bytes[kFillRegions];
     uint32_t val[kFillRegions];
 };
-__global__ void k_fill_multi(FillArgs f) {
-    const int k = blockIdx.y;
+__device__ __forceinline__ void k_fill_multi(MG<1024, 1> mg, FillArgs f) {
+    const int k = mg.by;
     uint8_t* p = f.ptr[k];
     const int64_t nb = f.bytes[k];
     const uint32_t v = f.val[k] & 0xffu;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)mg.gx * blockDim.x;
+    const int64_t t = (int64_t)mg.bx * blockDim.x + threadIdx.x;
     int64_t words = 0;
     if (((uintptr_t)p & 3) == 0) {
         words = nb / 4;
@@ -1518,10 +1523,10 @@ __global__ void k_fill_multi(FillArgs f) {
 // round's slot (no other thread of round r touches it, and round r + 1 starts after round r
 // ends), so the rounds need no fill between them.
 constexpr int kRoundSlots = 64;
-__global__ void k_ftr_round(int n, const int* __restrict__ inoff, const int* __restrict__ indeg,
+__device__ __forceinline__ void k_ftr_round(MG<1024, 1> mg, int n, const int* __restrict__ inoff, const int* __restrict__ indeg,
                             const int* __restrict__ inedges, const uint8_t* __restrict__ st_in,
                             uint8_t* __restrict__ st_out, int* __restrict__ slots, int r) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = mg.bx * blockDim.x + threadIdx.x;
     if (j == 0) slots[(r + 1) % kRoundSlots] = 0;
     int* unresolved = slots + r % kRoundSlots;
     if (j >= n) return;
@@ -1545,8 +1550,8 @@ __global__ void k_ftr_round(int n, const int* __restrict__ inoff, const int* __r
 // Flag arrays F_c (c < 8) over global positions, holding bucket-local indices:
 // F_c[j] = smallest chunk-c root claiming j, else j itself.  Buckets reduced in one
 // chunk use F_0 only.
-__global__ void k_ftr_finit(int* __restrict__ F, int n, const int* __restrict__ boff, int nb) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_ftr_finit(MG<1024, 1> mg, int* __restrict__ F, int n, const int* __restrict__ boff, int nb) {
+    const int p = mg.bx * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int bk = bucket_of(boff, nb, p);
     const int l = p - boff[bk];
@@ -1554,10 +1559,10 @@ __global__ void k_ftr_finit(int* __restrict__ F, int n, const int* __restrict__ 
     for (int c = 0; c < nc; c++) F[(int64_t)c * n + p] = l;
 }
 
-__global__ void k_ftr_claim(int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
+__device__ __forceinline__ void k_ftr_claim(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, const int* __restrict__ eoff,
                             const int* __restrict__ ecnt, const int* __restrict__ edges,
                             const uint8_t* __restrict__ st, int* __restrict__ F) {
-    const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ig = mg.bx * blockDim.x + threadIdx.x;
     if (ig >= n || st[ig] != kRoot) return;
     const int bk = bucket_of(boff, nb, ig);
     const int beg = boff[bk], nbk = boff[bk + 1] - beg;
@@ -1569,9 +1574,9 @@ __global__ void k_ftr_claim(int n, const int* __restrict__ boff, int nb, const i
 // One level of the flag-array merge tree of every 8-chunk bucket,
 // term_reduction.hpp:592-649.  Every value of a flag array is a fixed point of it,
 // so a level is data parallel.
-__global__ void k_ftr_merge(int n, const int* __restrict__ boff, int nb, int* __restrict__ F, int level) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
+__device__ __forceinline__ void k_ftr_merge(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, int* __restrict__ F, int level) {
+    const int p = mg.bx * blockDim.x + threadIdx.x;
+    const int k = mg.by;
     if (p >= n) return;
     const int bk = bucket_of(boff, nb, p);
     const int beg = boff[bk], nbk = boff[bk + 1] - beg;
@@ -1589,10 +1594,10 @@ __global__ void k_ftr_merge(int n, const int* __restrict__ boff, int nb, int* __
 }
 
 // Root flags; members are keyed by their root's global position, roots by n.
-__global__ void k_ftr_rootflags(int n, const int* __restrict__ boff, int nb, const int* __restrict__ F,
+__device__ __forceinline__ void k_ftr_rootflags(MG<1024, 1> mg, int n, const int* __restrict__ boff, int nb, const int* __restrict__ F,
                                 uint8_t* __restrict__ isroot, uint32_t* __restrict__ memkey,
                                 uint32_t* __restrict__ memval) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = mg.bx * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int beg = boff[bucket_of(boff, nb, p)];
     const int f = F[p];
@@ -1603,22 +1608,22 @@ __global__ void k_ftr_rootflags(int n, const int* __restrict__ boff, int nb, con
 }
 
 // The root count is read on the device (grids are sized by the term count n).
-__global__ void k_ftr_memcount(int n, const int* __restrict__ nroots, const uint32_t* __restrict__ skey,
+__device__ __forceinline__ void k_ftr_memcount(MG<1024, 1> mg, int n, const int* __restrict__ nroots, const uint32_t* __restrict__ skey,
                                int* __restrict__ cnt) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = mg.bx * blockDim.x + threadIdx.x;
     if (k >= n - *nroots) return;
     atomicAdd(&cnt[skey[k]], 1);
 }
 
-__global__ void k_ftr_memslots(int n, const int* __restrict__ nroots, const uint32_t* __restrict__ sval,
+__device__ __forceinline__ void k_ftr_memslots(MG<1024, 1> mg, int n, const int* __restrict__ nroots, const uint32_t* __restrict__ sval,
                                const uint32_t* __restrict__ perm, int* __restrict__ mem_slots) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = mg.bx * blockDim.x + threadIdx.x;
     if (k >= n - *nroots) return;
     mem_slots[k] = (int)perm[sval[k]];
 }
 
 // First reduction group of every bucket: gbase[m] = number of roots before boff[m].
-__global__ void k_ftr_gbase(const int* __restrict__ d_nroots, const uint32_t* __restrict__ roots,
+__device__ __forceinline__ void k_ftr_gbase(MG<1024, 1> mg, const int* __restrict__ d_nroots, const uint32_t* __restrict__ roots,
                             const int* __restrict__ boff, int nb, int* __restrict__ gbase) {
     const int m = threadIdx.x;
     if (m > nb) return;
@@ -1634,12 +1639,12 @@ __global__ void k_ftr_gbase(const int* __restrict__ d_nroots, const uint32_t* __
 }
 
 // Reduction groups in the reference's order: buckets ascending, roots ascending.
-__global__ void k_ftr_groups(const int* __restrict__ nroots, const uint32_t* __restrict__ roots,
+__device__ __forceinline__ void k_ftr_groups(MG<1024, 1> mg, const int* __restrict__ nroots, const uint32_t* __restrict__ roots,
                              const uint32_t* __restrict__ perm, const int* __restrict__ boff, int nb,
                              const int* __restrict__ memcnt, const int* __restrict__ memoff,
                              int* __restrict__ grp_root, int* __restrict__ grp_moff, int* __restrict__ grp_mcnt,
                              int* __restrict__ grp_m) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = mg.bx * blockDim.x + threadIdx.x;
     if (g >= *nroots) return;
     const uint32_t r = roots[g];
     grp_root[g] = (int)perm[r];
@@ -1650,9 +1655,9 @@ __global__ void k_ftr_groups(const int* __restrict__ nroots, const uint32_t* __r
 
 // ------------------------------------------------------------------ group B-tables
 // Upper bound of each group's B-table size.
-__global__ void k_group_ub(const TermPool OUT, int ng, const int* __restrict__ grp_root, const int* __restrict__ grp_m,
+__device__ __forceinline__ void k_group_ub(MG<1024, 1> mg, const TermPool OUT, int ng, const int* __restrict__ grp_root, const int* __restrict__ grp_m,
                            int d, int* __restrict__ ub) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = mg.bx * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int s = grp_root[g];
     const int m = grp_m[g];
@@ -1666,14 +1671,14 @@ __global__ void k_group_ub(const TermPool OUT, int ng, const int* __restrict__ g
 // (make_new_child_btable, cell_enumeration.hpp:206-371); parents keep their own
 // post-update cells.  One wavefront per group, an LDS bitset of 2^m bits per
 // wavefront, sorted output.
-__global__ void k_group_btab(const TermPool OUT, int g0, int ngb, int m, int d, const int* __restrict__ grp_root,
+__device__ __forceinline__ void k_group_btab(MG<1024, 1> mg, const TermPool OUT, int g0, int ngb, int m, int d, const int* __restrict__ grp_root,
                              const uint32_t* __restrict__ Barena, const int64_t* __restrict__ b2_off,
                              const int64_t* __restrict__ ub, uint32_t* __restrict__ B2, int* __restrict__ b2_n,
                              int words_per_wave, int* __restrict__ err) {
     extern __shared__ uint32_t sh[];
     const int wv = threadIdx.x / kWave;
     const int lane = wave_lane();
-    const int g = g0 + blockIdx.x * (blockDim.x / kWave) + wv;
+    const int g = g0 + mg.bx * (blockDim.x / kWave) + wv;
     if (g >= g0 + ngb) return;
     uint32_t* bits = sh + wv * words_per_wave;
     const int s = grp_root[g];
@@ -1986,7 +1991,7 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
 // the root's in member order (add_gtables, flattening.hpp:259-315).
 // One wavefront per group; lanes own cells.
 template <int MODE, int KM>
-__global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, int d, int wmax,
+__device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, int ng, int d, int wmax,
                                                   const int* __restrict__ grp_root,
                                                   const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
                                                   const int* __restrict__ grp_m, const int* __restrict__ mem_slots,
@@ -2003,7 +2008,7 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
     double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
     __shared__ int8_t cs_s[kMaxM];
-    const int g = blockIdx.x;
+    const int g = mg.bx;
     if (g >= ng) return;
     const int lane = threadIdx.x;
     const int m = grp_m[g];
@@ -2186,9 +2191,9 @@ __global__ __launch_bounds__(64) void k_group_eval(const TermPool OUT, int ng, i
 
 // Work lists of the groups whose members are added in parallel: per group the
 // member count, the contribution entries (members x cells) and the 64-cell chunks.
-__global__ void k_big_prep(int ng, int big_members, const int* __restrict__ grp_mcnt, const int* __restrict__ b2_n,
+__device__ __forceinline__ void k_big_prep(MG<1024, 1> mg, int ng, int big_members, const int* __restrict__ grp_mcnt, const int* __restrict__ b2_n,
                            int* __restrict__ icnt, int64_t* __restrict__ ccnt, int* __restrict__ hcnt) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = mg.bx * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int nm = grp_mcnt[g];
     const bool big = nm > big_members;
@@ -2214,7 +2219,7 @@ __device__ inline int owner_of(const int* __restrict__ off, int n, int item) {
 // the member's table contributes (-0.0, -0.0), which leaves any sum unchanged; a
 // negligible member is flagged in cneg and contributes nothing.
 template <int MODE, int KM>
-__global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng, int d, int wmax,
+__device__ __forceinline__ void k_group_members(MG<64, 1> mg, const TermPool OUT, int ng, int d, int wmax,
                                                      const int* __restrict__ ioff, int nitems,
                                                      const int64_t* __restrict__ coff,
                                                      const int* __restrict__ grp_moff, const int* __restrict__ grp_mcnt,
@@ -2231,7 +2236,7 @@ __global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng
     double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
     __shared__ int8_t cs_s[kMaxM];
-    const int item = blockIdx.x;
+    const int item = mg.bx;
     if (item >= nitems) return;
     const int g = owner_of(ioff, ng, item);
     const int k = item - ioff[g];
@@ -2292,13 +2297,13 @@ __global__ __launch_bounds__(64) void k_group_members(const TermPool OUT, int ng
 
 // Adds the member contributions of each large group into its root's table, one
 // lane per cell, members in order (the sum order of add_gtables).
-__global__ __launch_bounds__(64) void k_group_sum(int ng, const int* __restrict__ hoff, int nchunks,
+__device__ __forceinline__ void k_group_sum(MG<64, 1> mg, int ng, const int* __restrict__ hoff, int nchunks,
                                                  const int* __restrict__ ioff, const int64_t* __restrict__ coff,
                                                  const int* __restrict__ grp_mcnt, const int* __restrict__ b2_n,
                                                  const int64_t* __restrict__ g_off, const int* __restrict__ surv,
                                                  const int* __restrict__ defer_k0, const cplx* __restrict__ contrib,
                                                  const uint8_t* __restrict__ cneg, cplx* __restrict__ gv_out) {
-    const int item = blockIdx.x;
+    const int item = mg.bx;
     if (item >= nchunks) return;
     const int g = owner_of(hoff, ng, item);
     if (surv[g] < 0) return;
@@ -2319,13 +2324,13 @@ __global__ __launch_bounds__(64) void k_group_sum(int ng, const int* __restrict_
 
 // Restores ascending key order of a group table whose keys were re-oriented by
 // a root replacement.  One block per listed group; rank from an LDS bitset.
-__global__ void k_group_sort(int nlist, const int* __restrict__ list, const int* __restrict__ grp_m,
+__device__ __forceinline__ void k_group_sort(MG<1024, 1> mg, int nlist, const int* __restrict__ list, const int* __restrict__ grp_m,
                              const int* __restrict__ b2_n, const int64_t* __restrict__ g_off,
                              uint32_t* __restrict__ gk, cplx* __restrict__ gv, uint32_t* __restrict__ tmpk,
                              cplx* __restrict__ tmpv) {
     extern __shared__ uint32_t bits[];
     __shared__ int wbase[1024];
-    const int g = list[blockIdx.x];
+    const int g = list[mg.bx];
     const int m = grp_m[g];
     const int n = b2_n[g];
     const int W = m >= 5 ? (1 << (m - 5)) : 1;
@@ -2371,10 +2376,10 @@ __global__ void k_group_sort(int nlist, const int* __restrict__ list, const int*
 
 // ------------------------------------------------------------------ compaction
 template <int D>
-__global__ void k_compact(const TermPool OUT, int ng, const int* __restrict__ surv, const int* __restrict__ pos,
+__device__ __forceinline__ void k_compact(MG<1024, 1> mg, const TermPool OUT, int ng, const int* __restrict__ surv, const int* __restrict__ pos,
                           const int* __restrict__ grp_m, const int64_t* __restrict__ g_off,
                           const int* __restrict__ b2_n, TermPool NEXT) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = mg.bx * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const int r = surv[g];
     if (r < 0) return;
@@ -2397,9 +2402,9 @@ __global__ void k_compact(const TermPool OUT, int ng, const int* __restrict__ su
 
 // g-tables of the first update's terms (make_gtable_first, flattening.hpp:14-67).
 template <int D>
-__global__ void k_gtable_first(const TermPool OUT, int n, const uint32_t* __restrict__ perm, double gscale,
+__device__ __forceinline__ void k_gtable_first(MG<1024, 1> mg, const TermPool OUT, int n, const uint32_t* __restrict__ perm, double gscale,
                                uint32_t* __restrict__ gk, cplx* __restrict__ gv, TermPool NEXT) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = mg.bx * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t s = perm[r];
     const int cells = 1 << (D - 1);
@@ -2433,10 +2438,10 @@ __global__ void k_gtable_first(const TermPool OUT, int n, const uint32_t* __rest
 // output table (key + complex value, 20 B/cell); per evaluated term (root and every
 // member), its record (q, p, A rows, scalars, coalignment maps) and, per cell, the two
 // parent g-table entries its numerators need (2 x 20 B).
-__global__ void k_geval_bytes(int ng, int d, const int* __restrict__ grp_m, const int* __restrict__ grp_mcnt,
+__device__ __forceinline__ void k_geval_bytes(MG<1024, 1> mg, int ng, int d, const int* __restrict__ grp_m, const int* __restrict__ grp_mcnt,
                               const int* __restrict__ b2_n, const int* __restrict__ surv,
                               const int* __restrict__ defer_k0, int big_members, unsigned long long* __restrict__ acc) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = mg.bx * blockDim.x + threadIdx.x;
     if (g >= ng) return;
     const unsigned long long n = (unsigned long long)b2_n[g];
     const unsigned long long m = (unsigned long long)grp_m[g];
@@ -2450,7 +2455,7 @@ __global__ void k_geval_bytes(int ng, int d, const int* __restrict__ grp_m, cons
 }
 
 // Start of each reference thread's chain in the sorted chain keys (T+1 entries).
-__global__ void k_chain_off(const uint32_t* __restrict__ skeys, int n, int T, int* __restrict__ off) {
+__device__ __forceinline__ void k_chain_off(MG<1024, 1> mg, const uint32_t* __restrict__ skeys, int n, int T, int* __restrict__ off) {
     const int t = threadIdx.x;
     if (t > T) return;
     int lo = 0, hi = n;
@@ -2465,17 +2470,18 @@ __global__ void k_chain_off(const uint32_t* __restrict__ skeys, int n, int T, in
 // ------------------------------------------------------------------ small helpers
 // Total of an exclusive scan: off[n-1] + cnt[n-1].
 template <class A, class B>
-__global__ void k_scan_total(const A* __restrict__ off, const B* __restrict__ cnt, int n, int64_t* __restrict__ out) {
+__device__ __forceinline__ void k_scan_total(MG<1024, 1> mg, const A* __restrict__ off, const B* __restrict__ cnt, int n, int64_t* __restrict__ out) {
     out[0] = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
 }
 
-// The same total, also stored into a device-mapped host slot.
+// The same total, also stored into a device-mapped host slot when one is given.
 template <class A, class B>
-__global__ void k_scan_total_mapped(const A* __restrict__ off, const B* __restrict__ cnt, int n,
-                                    int64_t* __restrict__ out, int64_t* host_out) {
+__device__ __forceinline__ void k_scan_total_mapped(MG<1024, 1> mg, const A* __restrict__ off,
+                                                    const B* __restrict__ cnt, int n, int64_t* __restrict__ out,
+                                                    int64_t* host_out) {
     const int64_t v = n > 0 ? (int64_t)off[n - 1] + (int64_t)cnt[n - 1] : 0;
     out[0] = v;
-    *(volatile int64_t*)host_out = v;
+    if (host_out) *(volatile int64_t*)host_out = v;
 }
 
 // Scan of up to kSmallScan items by one workgroup: the small arrays of small estimators,
@@ -2487,7 +2493,7 @@ __global__ void k_scan_total_mapped(const A* __restrict__ off, const B* __restri
 constexpr int kSmallScan = 65536;
 constexpr int kSmallScanThreads = 1024;
 template <class In, class Out, bool Inclusive>
-__global__ __launch_bounds__(kSmallScanThreads) void k_small_scan(const In* __restrict__ in, Out* __restrict__ out,
+__device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const In* __restrict__ in, Out* __restrict__ out,
                                                                   int n, Out* __restrict__ wide,
                                                                   int64_t* __restrict__ out_total, int64_t* host_total) {
     __shared__ Out part[kSmallScanThreads];
@@ -2523,12 +2529,12 @@ __global__ __launch_bounds__(kSmallScanThreads) void k_small_scan(const In* __re
     }
 }
 
-__global__ void k_widen(const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_widen(MG<1024, 1> mg, const int* __restrict__ src, int64_t* __restrict__ dst, int n) {
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (int64_t)src[i];
 }
-__global__ void k_surv_flags(int n, const int* __restrict__ surv, int* __restrict__ f) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_surv_flags(MG<1024, 1> mg, int n, const int* __restrict__ surv, int* __restrict__ f) {
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i < n) f[i] = surv[i] >= 0 ? 1 : 0;
 }
 template <int D>
@@ -2705,8 +2711,8 @@ __global__ void k_cpdf1d_first(const double* __restrict__ direct_sum, double nor
 
 // b <- b - dx for every term (cauchy_estimator.hpp:1371-1380, sub_vecs).
 template <int D>
-__global__ void k_sub_b(TermPool P, int n, VecArg dx) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_sub_b(MG<1024, 1> mg, TermPool P, int n, VecArg dx) {
+    const int i = mg.bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double* b = P.b + (int64_t)i * D;
 #pragma unroll
@@ -3287,6 +3293,99 @@ __global__ void k_colsum_finish(const double* __restrict__ part, int cols, doubl
     double s = 0.0;
     for (int b = 0; b < kColSlices; b++) s += part[c * kColSlices + b];
     out[c] = s;
+}
+
+// ------------------------------------------------------------------ merged-launch plumbing
+// Byte copy of one staged payload or readback (8-byte words when both ends and the length
+// allow).
+__device__ __forceinline__ void k_copy_bytes(MG<256, 1> mg, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                             int64_t bytes) {
+    const int64_t t = (int64_t)mg.bx * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)mg.gx * blockDim.x;
+    if ((((uintptr_t)src | (uintptr_t)dst | (uintptr_t)bytes) & 7) == 0) {
+        const uint64_t* s = (const uint64_t*)src;
+        uint64_t* d = (uint64_t*)dst;
+        for (int64_t i = t; i < bytes / 8; i += stride) d[i] = s[i];
+    } else {
+        for (int64_t i = t; i < bytes; i += stride) dst[i] = src[i];
+    }
+}
+
+// Stable sort of up to kSmallSort (key, value) pairs by key bits [begin_bit, end_bit) in one
+// workgroup: the order a stable device radix sort over the same bits gives.  Slots past n
+// hold all-ones keys, which sort after every real key of the same digits (stability keeps
+// the real ones first), and are not written back.
+constexpr int kSmallSortThreads = 256;
+constexpr int kSmallSortItems = 16;
+constexpr int kSmallSort = kSmallSortThreads * kSmallSortItems;
+template <class K, class V>
+__device__ __forceinline__ void k_small_sort(MG<kSmallSortThreads, 1> mg, const K* __restrict__ kin,
+                                             K* __restrict__ kout, const V* __restrict__ vin, V* __restrict__ vout,
+                                             int n, int begin_bit, int end_bit) {
+    (void)mg;
+    using Sort = hipcub::BlockRadixSort<K, kSmallSortThreads, kSmallSortItems, V>;
+    __shared__ typename Sort::TempStorage tmp;
+    K k[kSmallSortItems];
+    V v[kSmallSortItems];
+    const int base = threadIdx.x * kSmallSortItems;
+#pragma unroll
+    for (int j = 0; j < kSmallSortItems; j++) {
+        const int i = base + j;
+        k[j] = i < n ? kin[i] : (K)~(K)0;
+        v[j] = i < n ? vin[i] : (V)0;
+    }
+    Sort(tmp).Sort(k, v, begin_bit, end_bit);
+#pragma unroll
+    for (int j = 0; j < kSmallSortItems; j++) {
+        const int i = base + j;
+        if (i < n) {
+            kout[i] = k[j];
+            vout[i] = v[j];
+        }
+    }
+}
+
+// Stable compaction of in[i] where flags[i] != 0 for up to kSmallScan items in one
+// workgroup, with the selected count in *count (a device select's result).
+__device__ __forceinline__ void k_small_select(MG<kSmallScanThreads, 1> mg, const uint32_t* __restrict__ in,
+                                               const uint8_t* __restrict__ flags, uint32_t* __restrict__ out,
+                                               int* __restrict__ count, int n) {
+    (void)mg;
+    __shared__ int part[kSmallScanThreads];
+    const int t = threadIdx.x;
+    const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
+    const int b = min(n, t * per), e = min(n, b + per);
+    int c = 0;
+    for (int i = b; i < e; i++) c += flags[i] != 0;
+    part[t] = c;
+    __syncthreads();
+    for (int o = 1; o < kSmallScanThreads; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int w = part[t] - c;
+    for (int i = b; i < e; i++)
+        if (flags[i]) out[w++] = in[i];
+    if (t == kSmallScanThreads - 1) count[0] = part[t];
+}
+
+// Largest of n ints in one workgroup (INT_MIN when n = 0, as a device max
+// reduction gives).
+__device__ __forceinline__ void k_small_max(MG<1024, 1> mg, const int* __restrict__ in, int* __restrict__ out,
+                                            int n) {
+    (void)mg;
+    __shared__ int part[1024];
+    int m = INT_MIN;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = max(m, in[i]);
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] = max(part[threadIdx.x], part[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
 }
 
 }  // namespace mce

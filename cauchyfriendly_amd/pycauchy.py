@@ -183,6 +183,11 @@ class _Bindings:
         """pycauchy_single_step_ltiv of every handle, concurrently; a list of its 13-tuples."""
         return self._batch("pycauchy_batch_single_step_ltiv", 13, handles, msmts, controls, (), num_threads)
 
+    def pycauchy_batch_single_step_ltiv_merged(self, handles, msmts, controls, num_groups=4):
+        """pycauchy_single_step_ltiv of every handle, with each kernel of the updates launched
+        once per group of estimators (num_groups host threads); a list of its 13-tuples."""
+        return self._batch("pycauchy_batch_single_step_ltiv_merged", 13, handles, msmts, controls, (), num_groups)
+
     def pycauchy_batch_single_step_nonlin(self, handles, msmts, controls, with_propagate, num_threads=0):
         """pycauchy_single_step_nonlin of every handle, concurrently; a list of its 15-tuples."""
         wp = (ct.c_int * len(handles))(*[1 if w else 0 for w in with_propagate])

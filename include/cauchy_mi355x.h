@@ -117,6 +117,18 @@ int pycauchy_batch_single_step_ltiv(void** handles, int count, double** msmts, c
                                     double** controls, const int* size_controls, void** outs, int* out_sizes,
                                     int num_threads);
 
+/* Batched stepping with merged launches (no counterpart in the reference).  Same arguments,
+ * outputs and results as pycauchy_batch_single_step_ltiv, but the estimators are dealt to
+ * num_groups host threads, each with one HIP stream, and the estimators of a group step
+ * together: every kernel of their updates is launched once for the whole group, each
+ * estimator's part a block range of that launch with its own arguments.  The kernels,
+ * their order per estimator and each estimator's arithmetic are those of the single step.
+ * Not for estimators in tangent mode.  Returns the number of estimators that reported an
+ * error (message in mce_last_error()). */
+int pycauchy_batch_single_step_ltiv_merged(void** handles, int count, double** msmts, const int* size_msmts,
+                                           double** controls, const int* size_controls, void** outs, int* out_sizes,
+                                           int num_groups);
+
 /* Batched pycauchy_single_step_nonlin, as above with 15 outputs per estimator (j = 14, the
  * error codes, is an int*) and with_propagate[i] != 0 for the step's propagation. */
 int pycauchy_batch_single_step_nonlin(void** handles, int count, double** msmts, const int* size_msmts,
@@ -367,6 +379,14 @@ void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax
 void mce_profile_enable(void* handle, int on);
 void mce_profile_read(void* handle, double* out_ms12);
 void mce_profile_reset(void* handle);
+
+/* Profile of merged batch stepping (pycauchy_batch_single_step_ltiv_merged), summed over
+ * its schedulers since mce_batch_profile_enable(1) (which also resets it): out[0] merged
+ * group g-table launches, out[1] their algorithmic bytes, out[2] their kernel time (ms,
+ * HIP events on the scheduler's stream), out[3] flushes, out[4] merged launches, out[5]
+ * library calls run one estimator at a time.  out must hold 6 doubles. */
+void mce_batch_profile_enable(int on);
+void mce_batch_profile_read(double* out);
 
 /* Last error message ("" when none). */
 const char* mce_last_error(void);

@@ -79,6 +79,7 @@ import_array();
 /* This library's additions that take arrays of pointers or unsized buffers stay C-only
  * (Python reaches them through cauchyfriendly_amd); the module is the reference's surface. */
 %ignore pycauchy_batch_single_step_ltiv;
+%ignore pycauchy_batch_single_step_ltiv_merged;
 %ignore pycauchy_batch_single_step_nonlin;
 %ignore mce_estimator_create;
 %ignore mce_estimator_step;

@@ -40,8 +40,9 @@ def test_batch_recipe_is_ragged():
     assert len({s.n for s in ss}) >= 3 and len({s.num_steps for s in ss}) >= 3
 
 
-@pytest.mark.gpu
-def test_batch_step_matches_single_steps_and_oracle():
+def _check_batch(**step_kw):
+    """Steps the recipe's estimators alone and together (step_estimators(**step_kw)); the
+    batched records must equal the single ones, and both the oracle's."""
     from cauchyfriendly_amd.cauchy_estimator import step_estimators
     ss = _systems()
     single = []
@@ -57,7 +58,7 @@ def test_batch_step_matches_single_steps_and_oracle():
     batched = [[] for _ in ss]
     for k in range(max(s.num_steps for s in ss)):
         live = [i for i, s in enumerate(ss) if k < s.num_steps]
-        step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], num_threads=4)
+        step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], **step_kw)
         for i in live:
             batched[i].append(_record(ests[i]))
     for e in ests:
@@ -70,13 +71,60 @@ def test_batch_step_matches_single_steps_and_oracle():
             fz1, x1, P1, err1, nt1 = single[i][k]
             assert nt == nt1 and list(err) == list(err1), f"{where}: batched counts/errors differ from single"
             for a, b, what in ((fz, fz1, "fz"), (x, x1, "xhat"), (P, P1, "Phat")):
-                assert rel_err(a, b) <= 1e-12, f"{where}: batched {what} differs from single"
+                # the same kernels with the same arguments: the same bits
+                assert np.array_equal(a, b), f"{where}: batched {what} differs from single"
             o = oracle_recs[k]
             assert nt == o["nt_ftr"], f"{where}: terms {nt} != oracle {o['nt_ftr']}"
             assert int(err[-1]) == o["err"], f"{where}: error bits"
             assert rel_err(x, o["mean"].real) <= TOL, f"{where}: xhat vs oracle"
             assert rel_err(P, o["cov"].real.reshape(-1)) <= TOL, f"{where}: Phat vs oracle"
             assert rel_err(np.array([fz[-1]]), np.array([o["fz_public"].real])) <= TOL, f"{where}: fz vs oracle"
+
+
+@pytest.mark.gpu
+def test_batch_step_matches_single_steps_and_oracle():
+    _check_batch(num_threads=4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("groups", [1, 3])
+def test_merged_batch_matches_single_steps_and_oracle(groups):
+    """Each kernel launched once per group of estimators (merged launches, one-workgroup
+    sorts and selects for small inputs) gives each estimator its single-step results."""
+    _check_batch(merged_groups=groups)
+
+
+def _run_recipe(count, **step_kw):
+    from cauchyfriendly_amd.cauchy_estimator import step_estimators
+    ss = W.batch_systems(count=count)
+    ests = [_make(s) for s in ss]
+    recs = [[] for _ in ss]
+    for k in range(max(s.num_steps for s in ss)):
+        live = [i for i, s in enumerate(ss) if k < s.num_steps]
+        step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], **step_kw)
+        for i in live:
+            recs[i].append(_record(ests[i]))
+    for e in ests:
+        e.shutdown()
+    return ss, recs
+
+
+@pytest.mark.gpu
+def test_merged_batch_of_512_matches_threaded_batch():
+    """The full 512-estimator ragged batch, merged (4 groups) against the threaded batch,
+    every estimator and update: the same counts, error bits and moment bits."""
+    ss, merged = _run_recipe(512, merged_groups=4)
+    _, threaded = _run_recipe(512, num_threads=16)
+    updates = 0
+    for i, s in enumerate(ss):
+        for k in range(s.num_steps):
+            fz, x, P, err, nt = merged[i][k]
+            fz1, x1, P1, err1, nt1 = threaded[i][k]
+            where = f"{s.name} step {k}"
+            assert nt == nt1 and list(err) == list(err1), where
+            assert np.array_equal(fz, fz1) and np.array_equal(x, x1) and np.array_equal(P, P1), where
+            updates += 1
+    assert updates == sum(s.num_steps for s in ss)
 
 
 @pytest.mark.gpu
