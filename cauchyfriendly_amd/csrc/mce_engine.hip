@@ -25,6 +25,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cauchy_mi355x.h"
@@ -889,9 +890,47 @@ struct Engine {
             MCE_RUN((k_small_sort<K, V>), dim3(1), dim3(kSmallSortThreads), 0, st, kin, kout, vin, vout, n, b0, b1);
             return;
         }
+        if constexpr (std::is_same_v<V, uint32_t>) {
+            if (bs && n <= kRadixMax && b0 == 0) {
+                radix_sort(kin, kout, vin, vout, n, b1);
+                return;
+            }
+        }
         cub("radix sort", [=](void* t, size_t& bytes, hipStream_t s) {
             return hipcub::DeviceRadixSort::SortPairs(t, bytes, kin, kout, vin, vout, n, b0, b1, s);
         });
+    }
+    // The multi-block radix sort (k_radix_hist, k_small_scan, k_radix_scatter per 8-bit
+    // digit) over bits [0, nbits), ping-ponging through rs_keys / rs_vals so that the last
+    // pass writes kout / vout.  Its launches merge with other estimators' under a scheduler.
+    DBuf<uint8_t> rs_keys;
+    DBuf<uint32_t> rs_vals;
+    DBuf<int> rs_hist, rs_offs;
+    template <class K>
+    void radix_sort(const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, int n, int nbits) {
+        const int nc = ceil_div(n, kRadixChunk);
+        const int passes = (nbits + 7) / 8;
+        rs_keys.reserve((size_t)n * sizeof(K));
+        rs_vals.reserve(n);
+        rs_hist.reserve((size_t)kRadixBins * nc);
+        rs_offs.reserve((size_t)kRadixBins * nc);
+        K* tk = (K*)rs_keys.ptr;
+        uint32_t* tv = rs_vals.ptr;
+        const K* sk = kin;
+        const uint32_t* sv = vin;
+        for (int p = 0; p < passes; p++) {
+            const bool to_out = ((passes - 1 - p) % 2) == 0;
+            K* dk = to_out ? kout : tk;
+            uint32_t* dv = to_out ? vout : tv;
+            const int shift = 8 * p, bits = std::min(8, nbits - 8 * p);
+            MCE_RUN((k_radix_hist<K>), dim3(nc), dim3(kRadixThreads), 0, st, sk, n, shift, bits, rs_hist.ptr, nc);
+            MCE_RUN((k_small_scan<int, int, false>), dim3(1), dim3(kSmallScanThreads), 0, st, (const int*)rs_hist.ptr,
+                    rs_offs.ptr, kRadixBins * nc, (int*)nullptr, (int64_t*)nullptr, (int64_t*)nullptr);
+            MCE_RUN((k_radix_scatter<K>), dim3(nc), dim3(kRadixThreads), 0, st, sk, sv, dk, dv, n, shift, bits,
+                    (const int*)rs_offs.ptr, nc);
+            sk = dk;
+            sv = dv;
+        }
     }
     // out <- in[i] where flags[i] != 0, in order; the count into *count.
     void select_flagged(const uint32_t* in_, const uint8_t* flags, uint32_t* out_, int* count, int n) {

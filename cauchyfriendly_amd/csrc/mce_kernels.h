@@ -2442,16 +2442,21 @@ __device__ __forceinline__ void k_geval_bytes(MG<1024, 1> mg, int ng, int d, con
                               const int* __restrict__ b2_n, const int* __restrict__ surv,
                               const int* __restrict__ defer_k0, int big_members, unsigned long long* __restrict__ acc) {
     const int g = mg.bx * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    const unsigned long long n = (unsigned long long)b2_n[g];
-    const unsigned long long m = (unsigned long long)grp_m[g];
-    const unsigned long long rec = m * (16ull + 8ull * d) + 2ull * m + 52ull;
-    // terms evaluated by k_group_eval: the root and the members it adds itself (a
-    // large group's later members are evaluated by k_group_members)
-    const int nm = grp_mcnt[g];
-    const int own = (nm > big_members && surv[g] >= 0) ? defer_k0[g] : nm;
-    const unsigned long long evals = 1ull + (unsigned long long)own;
-    atomicAdd(acc, 24ull * n + evals * (40ull * n + rec));
+    unsigned long long v = 0;
+    if (g < ng) {
+        const unsigned long long n = (unsigned long long)b2_n[g];
+        const unsigned long long m = (unsigned long long)grp_m[g];
+        const unsigned long long rec = m * (16ull + 8ull * d) + 2ull * m + 52ull;
+        // terms evaluated by k_group_eval: the root and the members it adds itself (a
+        // large group's later members are evaluated by k_group_members)
+        const int nm = grp_mcnt[g];
+        const int own = (nm > big_members && surv[g] >= 0) ? defer_k0[g] : nm;
+        const unsigned long long evals = 1ull + (unsigned long long)own;
+        v = 24ull * n + evals * (40ull * n + rec);
+    }
+    // one atomic per wavefront: the counter is shared by every group of the launch
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd(acc, v);
 }
 
 // Start of each reference thread's chain in the sorted chain keys (T+1 entries).
@@ -3386,6 +3391,82 @@ __device__ __forceinline__ void k_small_max(MG<1024, 1> mg, const int* __restric
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// Stable LSD radix sort of (key, value) pairs for inputs of kSmallSort .. kRadixMax items,
+// one digit of up to 8 bits per pass, in chunks of kRadixChunk items per block, for many
+// estimators at once: (1) each chunk's digit histogram, stored digit-major; (2) an exclusive
+// scan over it (k_small_scan) gives every (digit, chunk) its first output slot; (3) each
+// chunk sorts itself stably by the digit and writes every item to its digit's slot plus its
+// rank among the chunk's items of that digit.  Equal digits keep chunk order and in-chunk
+// order, so each pass is stable and the passes over bits [0, end) give the order of a stable
+// sort over those bits.
+constexpr int kRadixThreads = 256;
+constexpr int kRadixItems = 16;
+constexpr int kRadixChunk = kRadixThreads * kRadixItems;
+constexpr int kRadixBins = 256;
+constexpr int kRadixMax = kSmallScan / kRadixBins * kRadixChunk;  // the digit-major scan fits k_small_scan
+template <class K>
+__device__ __forceinline__ void k_radix_hist(MG<kRadixThreads, 1> mg, const K* __restrict__ keys, int n, int shift,
+                                             int bits, int* __restrict__ hist, int nchunks) {
+    __shared__ int h[kRadixBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = mg.bx * kRadixChunk;
+    const K mask = (K)((1u << bits) - 1u);
+    for (int j = threadIdx.x; j < kRadixChunk; j += kRadixThreads) {
+        const int i = base + j;
+        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & mask)], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nchunks + mg.bx] = h[threadIdx.x];
+}
+template <class K>
+__device__ __forceinline__ void k_radix_scatter(MG<kRadixThreads, 1> mg, const K* __restrict__ kin,
+                                                const uint32_t* __restrict__ vin, K* __restrict__ kout,
+                                                uint32_t* __restrict__ vout, int n, int shift, int bits,
+                                                const int* __restrict__ offs, int nchunks) {
+    using Sort = hipcub::BlockRadixSort<K, kRadixThreads, kRadixItems, uint32_t>;
+    __shared__ typename Sort::TempStorage tmp;
+    __shared__ int first[kRadixBins];
+    const int base = mg.bx * kRadixChunk;
+    const int nreal = min(n - base, kRadixChunk);
+    const K mask = (K)((1u << bits) - 1u);
+    K k[kRadixItems];
+    uint32_t v[kRadixItems];
+    first[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRadixItems; j++) {
+        const int p = threadIdx.x * kRadixItems + j;
+        const bool real = p < nreal;
+        k[j] = real ? kin[base + p] : (K)~(K)0;
+        v[j] = real ? vin[base + p] : 0u;
+        if (real) atomicAdd(&first[(int)((k[j] >> shift) & mask)], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the chunk's digit counts: first[d] = items of smaller digits
+    const int c = first[threadIdx.x];
+    for (int o = 1; o < kRadixBins; o <<= 1) {
+        const int add = (int)threadIdx.x >= o ? first[threadIdx.x - o] : 0;
+        __syncthreads();
+        first[threadIdx.x] += add;
+        __syncthreads();
+    }
+    first[threadIdx.x] -= c;
+    // padding keys are all ones: they sort after every real item of the last digit
+    Sort(tmp).Sort(k, v, shift, shift + bits);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRadixItems; j++) {
+        const int p = threadIdx.x * kRadixItems + j;
+        if (p < nreal) {
+            const int dg = (int)((k[j] >> shift) & mask);
+            const int dst = offs[dg * nchunks + mg.bx] + (p - first[dg]);
+            kout[dst] = k[j];
+            vout[dst] = v[j];
+        }
+    }
 }
 
 }  // namespace mce

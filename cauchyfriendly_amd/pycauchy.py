@@ -24,6 +24,10 @@ _dp = ct.POINTER(ct.c_double)
 _ip = ct.POINTER(ct.c_int)
 
 
+_EMPTY_D = np.zeros(0)
+_EMPTY_D.flags.writeable = False
+
+
 def _arr(x):
     a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
     return a, a.ctypes.data_as(_dp), int(a.size)
@@ -154,26 +158,20 @@ class _Bindings:
         outs = (ct.c_void_p * (nout * count))()
         sizes = (ct.c_int * (nout * count))()
         failed = self._f(fname)(hs, count, mp, mn, cp, cn, *extra, outs, sizes, int(num_threads))
-        # every output copied (and freed) in one native call, then handed out as views
+        # every output copied (and freed) in one native call, then handed out as views (empty
+        # outputs share one empty array)
         sz = np.frombuffer(sizes, dtype=np.int32).reshape(count, nout).clip(min=0)
-        dbuf = np.empty(int(sz[:, :-1].sum()))
+        dsz = sz[:, :-1].reshape(-1)
+        dbuf = np.empty(int(dsz.sum()))
         ibuf = np.empty(int(sz[:, -1].sum()), dtype=np.int32)
         _native.lib().mce_collect_outputs(outs, sizes, count, nout, dbuf.ctypes.data_as(_dp),
                                           ibuf.ctypes.data_as(_ip))
-        dends = np.cumsum(sz[:, :-1].reshape(-1)).tolist()
+        ends = np.cumsum(dsz)
+        dviews = [dbuf[a:b] if b > a else _EMPTY_D for a, b in zip((ends - dsz).tolist(), ends.tolist())]
         iends = np.cumsum(sz[:, -1]).tolist()
-        res = []
-        d0 = i0 = k = 0
-        for i in range(count):
-            r = []
-            for _ in range(nout - 1):
-                e = dends[k]
-                r.append(dbuf[d0:e])
-                d0 = e
-                k += 1
-            r.append(ibuf[i0:iends[i]])
-            i0 = iends[i]
-            res.append(tuple(r))
+        istarts = [0] + iends[:-1]
+        m = nout - 1
+        res = [tuple(dviews[i * m:(i + 1) * m]) + (ibuf[istarts[i]:iends[i]],) for i in range(count)]
         if failed:
             msg = self._last_error() if self._last_error else ""
             raise RuntimeError(f"{fname}: {failed} estimator(s) failed: {msg}")

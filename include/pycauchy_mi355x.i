@@ -88,6 +88,8 @@ import_array();
 %ignore mce_estimator_destroy;
 %ignore mce_estimator_finalize_extended_moments;
 %ignore mce_cpdf1d_noalloc;
+%ignore mce_batch_profile_enable;
+%ignore mce_batch_profile_read;
 %ignore mce_estimator_enable_tangents;
 %ignore mce_estimator_step_tangent;
 %ignore mce_last_update;
