@@ -3993,6 +3993,27 @@ void mce_profile_read(void* handle, double* out) {
     out[kNumPhases + 6] = p.max_group;
 }
 
+int mce_estimator_set_shapes(void* handle, int enum_threads, int big_group_members, int geval_mode) {
+    try {
+        Engine* e = ((Handle*)handle)->eng;
+        if (enum_threads != 0) {
+            if (enum_threads != 64 && enum_threads != 128 && enum_threads != 256)
+                throw std::runtime_error("enumerator lanes per parent must be 64, 128 or 256");
+            e->enum_threads = enum_threads;
+        }
+        if (big_group_members > 0) e->big_group_members = big_group_members;
+        if (geval_mode != 0) {
+            if (geval_mode < -1 || geval_mode > kLookupRankVals + 1)
+                throw std::runtime_error("g-table lookup mode must be -1 (automatic) or 1..4 (mode + 1)");
+            e->geval_mode_override = geval_mode < 0 ? -1 : geval_mode - 1;
+        }
+        return 0;
+    } catch (const std::exception& ex) {
+        set_err("mce_estimator_set_shapes", ex);
+        return -1;
+    }
+}
+
 void mce_profile_reset(void* handle) {
     Profiler& p = ((Handle*)handle)->eng->prof;
     for (int i = 0; i < kNumPhases; i++) p.ms[i] = 0;

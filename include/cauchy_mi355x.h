@@ -380,6 +380,16 @@ void mce_profile_enable(void* handle, int on);
 void mce_profile_read(void* handle, double* out_ms12);
 void mce_profile_reset(void* handle);
 
+/* Kernel shapes of one estimator (no counterpart in the reference), set by the autotuner
+ * (cauchyfriendly_amd/autotune.py); results never depend on them.  enum_threads: lanes per
+ * parent in the time-propagation cell enumerator (64, 128 or 256; 0 keeps it);
+ * big_group_members: reduction groups with more members evaluate them one wavefront per
+ * member (> 0; 0 keeps it); geval_mode: parent g-table lookup of the g-table kernel, 1..4
+ * for binary search in global memory, binary search in LDS, LDS rank structure, rank
+ * structure with LDS values (a mode the estimator cannot use is ignored), -1 automatic,
+ * 0 keeps it.  Returns 0, or -1 with mce_last_error() set. */
+int mce_estimator_set_shapes(void* handle, int enum_threads, int big_group_members, int geval_mode);
+
 /* Profile of merged batch stepping (pycauchy_batch_single_step_ltiv_merged), summed over
  * its schedulers since mce_batch_profile_enable(1) (which also resets it): out[0] merged
  * group g-table launches, out[1] their algorithmic bytes, out[2] their kernel time (ms,

@@ -101,6 +101,7 @@ import_array();
 %ignore mce_profile_enable;
 %ignore mce_profile_read;
 %ignore mce_profile_reset;
+%ignore mce_estimator_set_shapes;
 %ignore mce_step_record;
 %ignore mce_collect_outputs;
 

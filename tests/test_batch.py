@@ -94,6 +94,37 @@ def test_merged_batch_matches_single_steps_and_oracle(groups):
     _check_batch(merged_groups=groups)
 
 
+@pytest.mark.gpu
+def test_merged_batch_with_tuned_shapes_matches_single_steps():
+    """The autotuner's other launch shapes (enumerator lanes, wavefront-per-member
+    threshold, LDS parent values) change no result."""
+    from cauchyfriendly_amd import autotune
+    from cauchyfriendly_amd.cauchy_estimator import step_estimators
+    ss = _systems()
+    ref = []
+    for s in ss:
+        e = _make(s)
+        recs = []
+        for k in range(s.num_steps):
+            e.step(s.zs[k], None)
+            recs.append(_record(e))
+        e.shutdown()
+        ref.append(recs)
+    ests = [_make(s) for s in ss]
+    autotune.set_shapes(ests, enum_threads=256, big_group_members=8, geval_mode=4)
+    for k in range(max(s.num_steps for s in ss)):
+        live = [i for i, s in enumerate(ss) if k < s.num_steps]
+        step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], merged_groups=2)
+        for i in live:
+            fz, x, P, err, nt = _record(ests[i])
+            fz1, x1, P1, err1, nt1 = ref[i][k]
+            where = f"{ss[i].name} step {k}"
+            assert nt == nt1 and list(err) == list(err1), where
+            assert np.array_equal(fz, fz1) and np.array_equal(x, x1) and np.array_equal(P, P1), where
+    for e in ests:
+        e.shutdown()
+
+
 def _run_recipe(count, **step_kw):
     from cauchyfriendly_amd.cauchy_estimator import step_estimators
     ss = W.batch_systems(count=count)
