@@ -415,7 +415,7 @@ struct PoolStore {
     DBuf<uint32_t> hflag;
     DBuf<uint8_t> flags, cmap, mode;
     DBuf<int8_t> csmap;
-    DBuf<int64_t> gp_off, bt_off;
+    DBuf<int64_t> gp_off, bt_off, rk_off;
     void reserve(int64_t n) {
         if (n <= cap) return;
         int64_t nn = std::max<int64_t>(n, cap + cap / 2);
@@ -441,6 +441,7 @@ struct PoolStore {
         csmap.reserve((size_t)nn * MM);
         gp_off.reserve(nn);
         bt_off.reserve(nn);
+        rk_off.reserve(nn);
         cap = nn;
     }
     TermPool view() const {
@@ -464,6 +465,7 @@ struct PoolStore {
         t.cmap = cmap.ptr;
         t.csmap = csmap.ptr;
         t.gp_off = gp_off.ptr;
+        t.rk_off = rk_off.ptr;
         t.gp_n = gp_n.ptr;
         t.bt_off = bt_off.ptr;
         t.bt_n = bt_n.ptr;
@@ -1355,6 +1357,45 @@ struct Engine {
         }
     }
 
+    // Parent lookup of this update's g-table kernel: the rank structure over the 2^(MM-1)
+    // half-storage keys when it fits in LDS (with the parent values too when those fit), else
+    // binary search; with its dynamic LDS and the rank structure's word capacity.  It depends
+    // only on the largest shape, the parents' largest g-table and the override, all fixed
+    // for the update.
+    void pick_lookup(int& mode, size_t& lds, int& wmax) const {
+        wmax = (std::max(4, (((1 << (MM - 1)) + 31) >> 5)) + 3) & ~3;  // multiple of 4 words
+        const size_t rank_b = (size_t)wmax * 12;  // parent bitset, prefix counts, group bitset
+        const size_t vals_b = (size_t)std::max(max_gp_cells, 1) * sizeof(cplx);
+        const size_t keys_b = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
+        const size_t budget = 48 * 1024;
+        // the rank structure with values read from global memory is the default: the
+        // parent values are L2-resident and the smaller LDS footprint keeps more wavefronts
+        // resident than staging the values (measured on MI355X)
+        if (MM <= 24 && rank_b <= budget) {
+            mode = kLookupRank;
+            lds = rank_b;
+        } else if (keys_b <= budget) {
+            mode = kLookupSearchLds;
+            lds = keys_b;
+        } else {
+            mode = kLookupSearch;
+            lds = 0;
+        }
+        if (geval_mode_override >= 0 && geval_mode_override <= kLookupRankVals &&
+            (geval_mode_override < kLookupRank || mode == kLookupRank) &&
+            (geval_mode_override != kLookupRankVals || rank_b + vals_b <= budget)) {
+            mode = geval_mode_override;
+            lds = mode == kLookupRankVals ? rank_b + vals_b
+                  : mode == kLookupRank   ? rank_b
+                  : mode == kLookupSearchLds ? keys_b
+                                             : 0;
+        }
+    }
+    // Words of one parent's rank structure: its keys lie below 2^(phc-1).
+    static int rank_words(int phc) { return phc >= 1 ? (int)((((uint32_t)1 << (phc - 1)) + 31u) >> 5) : 0; }
+    DBuf<uint2> rank_arena;  // rank structures of the current parents' g-tables (k_rank_build)
+    int upd_mode = -1;       // this update's parent lookup (pick_lookup at the update's start)
+
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
@@ -1425,9 +1466,18 @@ struct Engine {
                 throw std::runtime_error("term-reduction search ordering names axis " + std::to_string(sc.order[i]) +
                                          " of a " + std::to_string(d) + "-state estimator");
         }
-        // parents' shape bucket offsets travel with the constants (one upload)
+        // parents' shape bucket offsets travel with the constants (one upload), and with a
+        // rank-structure lookup this update, each bucket's first word in the rank arena
         sc.in_off[0] = 0;
         for (int m = 0; m < sr; m++) sc.in_off[m + 1] = sc.in_off[m] + in_count[m];
+        {
+            size_t lds_;
+            int wmax_;
+            pick_lookup(upd_mode, lds_, wmax_);
+        }
+        const bool build_rank = !first && !skip && upd_mode >= kLookupRank;
+        sc.rk_base[0] = 0;
+        for (int m = 0; m < sr; m++) sc.rk_base[m + 1] = sc.rk_base[m] + (int64_t)in_count[m] * rank_words(m);
         upload(dconst.ptr, &sc, sizeof(sc));
         if (ntan) {
             dstan.reserve(1);
@@ -1459,6 +1509,16 @@ struct Engine {
                                              dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
         const int nslots = co_await scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
+        if (build_rank) {
+            // each parent's rank structure once, before its children copy its offset
+            int wmax_;
+            size_t lds_;
+            int mode_;
+            pick_lookup(mode_, lds_, wmax_);
+            rank_arena.reserve((size_t)std::max<int64_t>(sc.rk_base[sr], 1));
+            MCE_RUN(k_rank_build, dim3(np), dim3(64), (size_t)wmax_ * 8, st, IN, np, dconst.ptr, gk[gcur].ptr,
+                    rank_arena.ptr, wmax_);
+        }
         prof.mark(kPhTp, st);
         if (!skip && !first) {
             // B arena offsets (64-bit)
@@ -1723,37 +1783,10 @@ struct Engine {
         int ge_mode = 0, ge_wmax = 0;
         size_t ge_lds = 0;
         {
-            // parent lookups: rank structure over the 2^(MM-1) half-storage keys when it
-            // fits in LDS (with the parent values too when those fit), else binary search
-            const int wmax = (std::max(4, (((1 << (MM - 1)) + 31) >> 5)) + 3) & ~3;  // multiple of 4 words
-            const size_t rank_b = (size_t)wmax * 12;  // parent bitset, prefix counts, group bitset
-            const size_t vals_b = (size_t)std::max(max_gp_cells, 1) * sizeof(cplx);
-            const size_t keys_b = (size_t)std::max(max_gp_cells, 1) * sizeof(uint32_t);
-            const size_t budget = 48 * 1024;
-            int mode;
+            int mode, wmax;
             size_t lds;
-            // the rank structure with values read from global memory is the default:
-            // the parent values are L2-resident and the smaller LDS footprint keeps more
-            // wavefronts resident than staging the values (measured on MI355X)
-            if (MM <= 24 && rank_b <= budget) {
-                mode = kLookupRank;
-                lds = rank_b;
-            } else if (keys_b <= budget) {
-                mode = kLookupSearchLds;
-                lds = keys_b;
-            } else {
-                mode = kLookupSearch;
-                lds = 0;
-            }
-            if (geval_mode_override >= 0 && geval_mode_override <= kLookupRankVals &&
-                (geval_mode_override < kLookupRank || mode == kLookupRank) &&
-                (geval_mode_override != kLookupRankVals || rank_b + vals_b <= budget)) {
-                mode = geval_mode_override;
-                lds = mode == kLookupRankVals ? rank_b + vals_b
-                      : mode == kLookupRank   ? rank_b
-                      : mode == kLookupSearchLds ? keys_b
-                                                 : 0;
-            }
+            pick_lookup(mode, lds, wmax);
+            if (mode != upd_mode) throw std::logic_error("g-table lookup mode changed within an update");
             geval_mode = mode;
             ge_mode = mode;
             ge_lds = lds;
@@ -1766,7 +1799,7 @@ struct Engine {
                            grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr, b2off.ptr,           \
                            b2n.ptr, goff.ptr, gk[gcur].ptr, gv[gcur].ptr, gk[nx].ptr, gv[nx].ptr, scratch.ptr,     \
                            gscale, surv.ptr, gxor.ptr, sort_list.ptr, counters.ptr + 18, counters.ptr + 19,          \
-                           big_members, defer_k0.ptr, ntan ? mem_add.ptr : nullptr);                             \
+                           big_members, defer_k0.ptr, ntan ? mem_add.ptr : nullptr, (const uint2*)rank_arena.ptr);  \
     } while (0)
             // rows held in registers: the smallest compile-time bound >= the largest shape
             if (mode == kLookupRankVals && MM <= 8) MCE_GEVAL(kLookupRankVals, 8);
@@ -1907,7 +1940,8 @@ struct Engine {
         MCE_RUN((k_group_members<M_, K_>), dim3(ni), dim3(64), lds, st, OUT, ng, d, wmax, big_ioff.ptr, \
                            ni, big_coff.ptr, grp_moff.ptr, grp_mcnt.ptr, grp_m.ptr, mem_slots.ptr, b2arena.ptr,    \
                            b2off.ptr, b2n.ptr, gk[gcur].ptr, gv[gcur].ptr, gscale, surv.ptr, gxor.ptr,             \
-                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19, ntan ? mem_add.ptr : nullptr);   \
+                           defer_k0.ptr, contrib.ptr, cneg.ptr, counters.ptr + 19, ntan ? mem_add.ptr : nullptr,    \
+                           (const uint2*)rank_arena.ptr);                                                          \
     } while (0)
         if (mode == kLookupRankVals && MM <= 8) MCE_GMEM(kLookupRankVals, 8);
         else if (mode == kLookupRankVals && MM <= 11) MCE_GMEM(kLookupRankVals, 11);

@@ -45,6 +45,7 @@ struct StepConst {
     int d, ng, cmcc, with_tp, skip, first, T, threaded;
     int order[kMaxD];          // fast-term-reduction axis order
     int in_off[kMaxM + 2];     // parents' shape bucket offsets (bucket m: [in_off[m], in_off[m+1]))
+    int64_t rk_base[kMaxM + 2];  // first rank-arena word of each shape bucket's rank structures
 };
 
 // ------------------------------------------------------------------ small dense linear algebra
@@ -751,6 +752,7 @@ __device__ __forceinline__ void k_mu(MG<1024, 1> mg, TermPool IN, int np, const 
     OUT.pbc[s] = m;
     OUT.z[s] = t;
     OUT.gp_off[s] = IN.gp_off[P];
+    OUT.rk_off[s] = IN.rk_off[P];
     OUT.gp_n[s] = IN.gp_n[P];
     const uint32_t enc = negs;
     uint32_t lhp = enc;
@@ -900,6 +902,7 @@ __device__ __forceinline__ void k_mu_r(MG<256, 4> mg, TermPool IN, int np, const
     OUT.pbc[s] = m;
     OUT.z[s] = t;
     OUT.gp_off[s] = IN.gp_off[P];
+    OUT.rk_off[s] = IN.rk_off[P];
     OUT.gp_n[s] = IN.gp_n[P];
     const uint32_t enc = negs;
     uint32_t lhp = enc;
@@ -1894,8 +1897,8 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
 // own key bitset (built by the kernel), then the parent values.
 template <int MODE, int KM>
 __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const uint32_t* gk_prev,
-                                      const cplx* gv_prev, double gscale, uint32_t* lds, int wmax, double (&qr)[KM],
-                                      uint8_t* cm_s, int8_t* cs_s, int* rare) {
+                                      const cplx* gv_prev, const uint2* rank, double gscale, uint32_t* lds, int wmax,
+                                      double (&qr)[KM], uint8_t* cm_s, int8_t* cs_s, int* rare) {
     EvalTerm T;
     const int64_t go = O.gp_off[s];
     T.pn = O.gp_n[s];
@@ -1922,51 +1925,22 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         for (int i = lane; i < T.pn; i += kWave) lds[i] = src[i];
         T.pkeys = lds;
     } else if (MODE >= kLookupRank) {
-        // keys are sorted ascending; W words cover those below klim.  A key at or above
-        // klim (a coaligned child's cell whose bit m-1 was left set, kept as a parent key)
-        // sorts after every key below it and is never looked up (lookups reverse their
-        // key below klim, eval_gs.hpp:116-151), so it only ends the prefix counts.
-        const uint32_t* src = gk_prev + go;
+        // the parent's rank structure, built once per update by k_rank_build
         const int W = (int)((T.klim + 31u) >> 5);
         uint32_t* bits = lds;
         uint32_t* pre = lds + wmax;
-        for (int w = lane; w < W; w += kWave) bits[w] = 0;
-        __syncthreads();
-        // eight keys (and, with LDS values, eight values) per lane per round; all
-        // loads of a round are issued before the LDS updates
-        constexpr int U = 8;
-        cplx* dst = reinterpret_cast<cplx*>(lds + 3 * wmax);
-        const cplx* vs = gv_prev + go;
-        for (int i0 = 0; i0 < T.pn; i0 += U * kWave) {
-            uint32_t kc[U], kp[U];
-            cplx vv[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = min(i0 + lane + u * kWave, T.pn - 1);
-                kc[u] = src[i];
-                kp[u] = src[i > 0 ? i - 1 : 0];
-                if (MODE == kLookupRankVals) vv[u] = vs[i];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int i = i0 + lane + u * kWave;
-                if (i >= T.pn) continue;
-                if (MODE == kLookupRankVals) dst[i] = vv[u];
-                const uint32_t key = kc[u];
-                if (key < T.klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
-                // pre[x] = index of the first key in word >= x, filled for the words
-                // between the previous key's word and this key's word
-                const int w = min((int)(key >> 5), W - 1);
-                const int wp = i ? min((int)(kp[u] >> 5), W - 1) : -1;
-                for (int x = wp + 1; x <= w; x++) pre[x] = i;
-            }
+        const uint2* R = rank + O.rk_off[s];
+        for (int w = lane; w < W; w += kWave) {
+            const uint2 v = R[w];
+            bits[w] = v.x;
+            pre[w] = v.y;
         }
-        const int lastw = T.pn ? min((int)(src[T.pn - 1] >> 5), W - 1) : -1;
-        for (int w = lane; w < W; w += kWave)
-            if (w > lastw) pre[w] = T.pn;
         T.rbits = bits;
         T.rpre = pre;
         if (MODE == kLookupRankVals) {
+            cplx* dst = reinterpret_cast<cplx*>(lds + 3 * wmax);
+            const cplx* vs = gv_prev + go;
+            for (int i = lane; i < T.pn; i += kWave) dst[i] = vs[i];
             if (T.pn == 0 && lane == 0) dst[0] = cmake(0, 0);
             T.pvals = dst;
         }
@@ -1982,6 +1956,63 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
     }
     __syncthreads();
     return T;
+}
+
+// Rank structure of each parent's g-table, once per update: a bitset of its keys below
+// 2^(phc-1) and, per 32-bit word, the index of its first key at or after the word's first
+// key value.  Every child of the parent evaluated by the g-table kernel reads it from here
+// (stage_term) instead of rebuilding it from the keys.  One wavefront per parent; the
+// structure is built in LDS (wmax words of bits, then wmax of prefix counts) and written
+// out as (bits, prefix) pairs at sc->rk_base[phc] + (index in its shape bucket) * W.
+__device__ __forceinline__ void k_rank_build(MG<64, 1> mg, const TermPool P, int n, const StepConst* __restrict__ sc,
+                                             const uint32_t* __restrict__ gkeys, uint2* __restrict__ rank, int wmax) {
+    extern __shared__ __align__(16) uint32_t lds_dyn[];
+    const int t = mg.bx;
+    if (t >= n) return;
+    const int lane = threadIdx.x;
+    const int phc = P.phc[t];
+    const uint32_t klim = 1u << (phc - 1);
+    const int W = (int)((klim + 31u) >> 5);
+    const int64_t off = sc->rk_base[phc] + (int64_t)(t - sc->in_off[phc]) * W;
+    if (lane == 0) P.rk_off[t] = off;
+    const int pn = P.gp_n[t];
+    const uint32_t* src = gkeys + P.gp_off[t];
+    // keys are sorted ascending; W words cover those below klim.  A key at or above klim (a
+    // coaligned child's cell whose bit m-1 was left set, kept as a parent key) sorts after
+    // every key below it and is never looked up (lookups reverse their key below klim,
+    // eval_gs.hpp:116-151), so it only ends the prefix counts.
+    uint32_t* bits = lds_dyn;
+    uint32_t* pre = lds_dyn + wmax;
+    for (int w = lane; w < W; w += kWave) bits[w] = 0;
+    __syncthreads();
+    // eight keys per lane per round; all loads of a round are issued before the LDS updates
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < pn; i0 += U * kWave) {
+        uint32_t kc[U], kp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = min(i0 + lane + u * kWave, pn - 1);
+            kc[u] = src[i];
+            kp[u] = src[i > 0 ? i - 1 : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + lane + u * kWave;
+            if (i >= pn) continue;
+            const uint32_t key = kc[u];
+            if (key < klim) atomicOr(&bits[key >> 5], 1u << (key & 31));
+            // pre[x] = index of the first key in word >= x, filled for the words between the
+            // previous key's word and this key's word
+            const int w = min((int)(key >> 5), W - 1);
+            const int wp = i ? min((int)(kp[u] >> 5), W - 1) : -1;
+            for (int x = wp + 1; x <= w; x++) pre[x] = i;
+        }
+    }
+    const int lastw = pn ? min((int)(src[pn - 1] >> 5), W - 1) : -1;
+    for (int w = lane; w < W; w += kWave)
+        if (w > lastw) pre[w] = pn;
+    __syncthreads();
+    for (int w = lane; w < W; w += kWave) rank[off + w] = make_uint2(bits[w], pre[w]);
 }
 
 // Flattening of one reduction group, make_gtables (flattening.hpp:351-564):
@@ -2003,7 +2034,7 @@ __device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, i
                                                   int* __restrict__ gxor, int* __restrict__ sort_list,
                                                   int* __restrict__ sort_count, int* __restrict__ rare,
                                                   int big_members, int* __restrict__ defer_k0,
-                                                  uint8_t* __restrict__ mem_add) {
+                                                  uint8_t* __restrict__ mem_add, const uint2* __restrict__ rank) {
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     double qr[KM];
     __shared__ uint8_t cm_s[kMaxM];
@@ -2040,7 +2071,7 @@ __device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, i
         // made visible by the barriers of stage_term
     }
     {
-        EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
+        EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, rank, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
         const double ps = dev_psum2(OUT, r, m);
         // negligible iff every cell is small; the test stops at the first large cell
         bool negl = true;
@@ -2067,7 +2098,7 @@ __device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, i
                 int sig = orientation_mask(OUT.A + (int64_t)lfr * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
                 if ((uint32_t)sig & top) sig ^= (int)rev;
                 X ^= (uint32_t)sig;
-                EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
+                EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, rank, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
                 const double psk = dev_psum2(OUT, sk, m);
                 bool sm = true;
                 uint32_t bnext = n > 0 ? B[min(lane, n - 1)] : 0u;
@@ -2105,7 +2136,7 @@ __device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, i
             if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
             const int sig = orientation_mask(Ar, OUT.A + (int64_t)sk * MM * d, m, d);
             const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
-            EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
+            EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, rank, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
             const double psk = dev_psum2(OUT, sk, m);
             if constexpr (MODE >= kLookupRank) {
                 // negligibility over the member's own cells B ^ X ^ sh, stopping at the
@@ -2230,7 +2261,7 @@ __device__ __forceinline__ void k_group_members(MG<64, 1> mg, const TermPool OUT
                                                      const int* __restrict__ surv, const int* __restrict__ gxor,
                                                      const int* __restrict__ defer_k0, cplx* __restrict__ contrib,
                                                      uint8_t* __restrict__ cneg, int* __restrict__ rare,
-                                                     uint8_t* __restrict__ mem_add) {
+                                                     uint8_t* __restrict__ mem_add, const uint2* __restrict__ rank) {
     static_assert(MODE >= kLookupRank, "parallel member sums use the rank lookup");
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     double qr[KM];
@@ -2264,7 +2295,7 @@ __device__ __forceinline__ void k_group_members(MG<64, 1> mg, const TermPool OUT
     if (!(OUT.flags[sk] & kIsNew) && lane == 0) atomicOr(rare, 2);
     const int sig = orientation_mask(OUT.A + (int64_t)r * MM * d, OUT.A + (int64_t)sk * MM * d, m, d);
     const uint32_t sh = ((uint32_t)sig & top) ? ((uint32_t)sig ^ rev) : (uint32_t)sig;
-    EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
+    EvalTerm Tk = stage_term<MODE, KM>(OUT, sk, m, gk_prev, gv_prev, rank, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
     const double psk = dev_psum2(OUT, sk, m);
     bool sm = true;
     for (int i0 = 0; i0 < n && sm; i0 += kWave) {
@@ -2497,25 +2528,51 @@ __device__ __forceinline__ void k_scan_total_mapped(MG<1024, 1> mg, const A* __r
 // is the same as any other summation order's.
 constexpr int kSmallScan = 65536;
 constexpr int kSmallScanThreads = 1024;
+// Inclusive scan of one value per lane across a wavefront (shuffles, no barrier).
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const T y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive prefix of each thread's `v` over a block of kSmallScanThreads threads, and the
+// block total: wavefront scans, then one wavefront scans the wavefront totals (two
+// barriers).  Every thread of the block must call it.
+template <class T>
+__device__ __forceinline__ T block_excl_scan(T v, T* total) {
+    constexpr int kWaves = kSmallScanThreads / kWave;
+    __shared__ T wsum[kWaves];
+    const int lane = (int)(threadIdx.x & (kWave - 1)), w = (int)(threadIdx.x / kWave);
+    const T x = wave_incl_scan(v);
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        T y = lane < kWaves ? wsum[lane] : (T)0;
+        y = wave_incl_scan(y);
+        if (lane < kWaves) wsum[lane] = y;
+    }
+    __syncthreads();
+    *total = wsum[kWaves - 1];
+    return (w > 0 ? wsum[w - 1] : (T)0) + x - v;
+}
+
 template <class In, class Out, bool Inclusive>
-__device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const In* __restrict__ in, Out* __restrict__ out,
-                                                                  int n, Out* __restrict__ wide,
-                                                                  int64_t* __restrict__ out_total, int64_t* host_total) {
-    __shared__ Out part[kSmallScanThreads];
+__device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const In* __restrict__ in,
+                                             Out* __restrict__ out, int n, Out* __restrict__ wide,
+                                             int64_t* __restrict__ out_total, int64_t* host_total) {
+    (void)mg;
     const int t = threadIdx.x;
     const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
     const int b = min(n, t * per), e = min(n, b + per);
     Out sum = 0;
     for (int i = b; i < e; i++) sum += (Out)in[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < kSmallScanThreads; o <<= 1) {
-        const Out v = t >= o ? part[t - o] : (Out)0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    Out run = part[t] - sum;
+    Out tot;
+    Out run = block_excl_scan(sum, &tot);
     for (int i = b; i < e; i++) {
         const Out x = (Out)in[i];
         if (wide) wide[i] = x;
@@ -2528,9 +2585,8 @@ __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const 
         }
     }
     if (t == kSmallScanThreads - 1) {
-        const int64_t tot = (int64_t)part[t];
-        if (out_total) out_total[0] = tot;
-        if (host_total) *(volatile int64_t*)host_total = tot;
+        if (out_total) out_total[0] = (int64_t)tot;
+        if (host_total) *(volatile int64_t*)host_total = (int64_t)tot;
     }
 }
 
@@ -3356,24 +3412,16 @@ __device__ __forceinline__ void k_small_select(MG<kSmallScanThreads, 1> mg, cons
                                                const uint8_t* __restrict__ flags, uint32_t* __restrict__ out,
                                                int* __restrict__ count, int n) {
     (void)mg;
-    __shared__ int part[kSmallScanThreads];
     const int t = threadIdx.x;
     const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
     const int b = min(n, t * per), e = min(n, b + per);
     int c = 0;
     for (int i = b; i < e; i++) c += flags[i] != 0;
-    part[t] = c;
-    __syncthreads();
-    for (int o = 1; o < kSmallScanThreads; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int w = part[t] - c;
+    int tot;
+    int w = block_excl_scan(c, &tot);
     for (int i = b; i < e; i++)
         if (flags[i]) out[w++] = in[i];
-    if (t == kSmallScanThreads - 1) count[0] = part[t];
+    if (t == kSmallScanThreads - 1) count[0] = tot;
 }
 
 // Largest of n ints in one workgroup (INT_MIN when n = 0, as a device max

@@ -40,6 +40,7 @@ struct TermPool {
     uint8_t* cmap = nullptr;    // [cap][MM]
     int8_t* csmap = nullptr;    // [cap][MM]
     int64_t* gp_off = nullptr;  // [cap] parent g-table offset in the previous table arena
+    int64_t* rk_off = nullptr;  // [cap] offset of the parent g-table's rank structure (rank arena words)
     int* gp_n = nullptr;        // [cap] parent g-table size (cells_gtable_p)
     int64_t* bt_off = nullptr;  // [cap] B-table (sorted) offset in the B arena
     int* bt_n = nullptr;        // [cap] B-table size (cells_gtable)
