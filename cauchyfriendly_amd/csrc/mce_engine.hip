@@ -610,6 +610,7 @@ struct Engine {
     bool skip = false;
     std::vector<double> A0, p0, b0, root, bpert;
     cplx fz, last_fz;
+    cplx fz_raw = {0, 0};  // the update's fz before the numeric checks may replace it by last_fz
     std::vector<cplx> mean, cov, last_mean, last_cov;
     double gscale = 0;
     std::vector<int> in_count;  // per-shape term counts of the parent pool
@@ -1297,7 +1298,8 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(dfz_tan.data(), dfz_dev.ptr, ntan * sizeof(cplx), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         dgscale_host.assign(ntan, 0.0);
-        for (int k = 0; k < ntan; k++) dgscale_host[k] = -gscale * dfz_tan[k].re / fz.re;
+        // gscale = 1 / (2 pi Re fz) of the update's own fz, before the numeric checks
+        for (int k = 0; k < ntan; k++) dgscale_host[k] = -gscale * dfz_tan[k].re / fz_raw.re;
         dgscale_dev.reserve(ntan);
         HIPCHK(hipMemcpyAsync(dgscale_dev.ptr, dgscale_host.data(), ntan * sizeof(double), hipMemcpyHostToDevice,
                               st));
@@ -1332,6 +1334,7 @@ struct Engine {
                 mo[2 * c + 1] += part[((size_t)t * NC + c) * 2 + 1];
             }
         fz = cmake(mo[0], mo[1]);
+        fz_raw = fz;
         for (int j = 0; j < d; j++) mean[j] = cmake(mo[2 + 2 * j], mo[3 + 2 * j]);
         for (int j = 0; j < d * d; j++) cov[j] = cmake(mo[2 + 2 * d + 2 * j], mo[3 + 2 * d + 2 * j]);
         if (first) {
@@ -2604,7 +2607,7 @@ struct Handle {
             step_fz.resize(i + 1, 0.0);
             step_cnt.resize(2 * (i + 1), 0);
         }
-        step_fz[i] = e->diag_mom[2];
+        step_fz[i] = e->fz_raw.re;
         step_cnt[2 * i] = e->diag_cnt[1];
         step_cnt[2 * i + 1] = e->diag_cnt[2];
     }
@@ -4025,6 +4028,12 @@ void mce_profile_read(void* handle, double* out) {
     out[kNumPhases + 4] = p.groups;
     out[kNumPhases + 5] = p.members;
     out[kNumPhases + 6] = p.max_group;
+}
+
+void mce_last_fz_raw(void* handle, double* out2) {
+    const Engine* e = ((Handle*)handle)->eng;
+    out2[0] = e->fz_raw.re;
+    out2[1] = e->fz_raw.im;
 }
 
 int mce_estimator_set_shapes(void* handle, int enum_threads, int big_group_members, int geval_mode) {

@@ -63,7 +63,9 @@ class GpuEstimator:
     def step(self, z, Phi, Gamma, beta, H_row, gamma, B=None, u=None, tangents=None):
         """One scalar update.  In tangent mode, tangents = (dPhi (P, d, d), dbeta (P, pncc),
         dH (P, d), dgamma (P,)), the update's seed directions, and the record gains "dfz":
-        d fz / d theta (complex, P) of the pre-reduction fz."""
+        d fz / d theta (complex, P) of the pre-reduction fz.  "fz" is the update's fz after the
+        numeric checks (the previous one when both its mean and covariance fail them), "fz_raw"
+        the update's own."""
         L = _native.lib()
         Phi_, Gam_, beta_, H_ = _d(Phi), _d(Gamma), _d(beta), _d(H_row)
         B_ = _d(B) if B is not None else (None, None)
@@ -86,10 +88,13 @@ class GpuEstimator:
         c = lambda off: complex(mom[off], mom[off + 1])
         mean = mom[6:6 + 2 * d].reshape(d, 2)
         cov = mom[6 + 2 * d:].reshape(d * d, 2)
+        raw = np.zeros(2)
+        L.mce_last_fz_raw(self.h, raw.ctypes.data_as(_dp))
         return {
             "rc": int(rc),
             "fz_public": c(0),
             "fz": c(2),
+            "fz_raw": complex(raw[0], raw[1]),
             "last_fz": c(4),
             "mean": mean[:, 0] + 1j * mean[:, 1],
             "cov": (cov[:, 0] + 1j * cov[:, 1]).reshape(d, d),

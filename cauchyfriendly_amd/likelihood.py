@@ -1,18 +1,25 @@
 """Window log-likelihood L = sum_k log fz_k of an estimator and its gradient.
 
 fz_k is the measurement likelihood that scalar update k computes before term reduction
-(the normalisation of the characteristic function, include/cauchy_estimator.hpp:340-461);
-the sum runs over every scalar update of the window (p per time step).  Gradients are
-taken by central differences of this library's forward path, every perturbed estimator of
-every sequence stepped together (step_estimators): 1 + 2 P estimators per sequence for P
-parameters.
-* loglik_gradient: single-noise LTI systems, over (beta, gamma, Phi, H);
-* scale_gradient: LTI or LTV systems with any number of noises and measurements, over the
-  Cauchy scales (beta_1 .. beta_pncc, gamma_1 .. gamma_p).
-A component is reported as valid only when the term tree's discrete structure (term counts
-after MU/MUC and after reduction at every update) is identical at theta + h e_i and
-theta - h e_i, the condition under which the likelihood is a smooth function of theta.
-This is a finite-difference gradient, not an adjoint: it costs 2 P forward passes.
+(the normalisation of the characteristic function, include/cauchy_estimator.hpp:340-461),
+taken before the numeric checks (which would replace it by the previous update's when both
+the mean and the covariance fail them); the sum runs over every scalar update of the window
+(p per time step).
+
+Gradients:
+* window_loglik_tangent / tangent_gradients / window_loglikelihood: the tangent kernels.
+  One forward pass carries P directions; every discrete decision of the term tree (the
+  coalignments, the reduction groups, the negligible terms) is the nominal forward's, so
+  this is the derivative of L with the structure held fixed.  window_loglikelihood wraps it
+  as a torch.autograd.Function.
+* loglik_gradient / scale_gradient: central differences of this library's forward path,
+  every perturbed estimator of every sequence stepped together (step_estimators): 1 + 2 P
+  estimators per sequence for P parameters; loglik_gradient over (beta, gamma, Phi, H) of
+  single-noise LTI systems, scale_gradient over the Cauchy scales (beta_1 .. beta_pncc,
+  gamma_1 .. gamma_p) of LTI or LTV systems.  A component is reported as valid only when the
+  term tree's discrete structure (term counts after MU/MUC and after reduction at every
+  update) is identical at theta + h e_i and theta - h e_i.  They are the check of the
+  tangent gradient.
 """
 from __future__ import annotations
 
@@ -272,7 +279,7 @@ def window_loglik_tangent(s, dirs, beta=None, gamma=None, Phi=None, H=None):
             Phik = s.Phi(k) if Phi is None else np.asarray(Phi)
             for i in range(s.p):
                 r = est.step(s.zs[k, i], Phik.reshape(-1), Gt, beta, H[i], gamma[i], tangents=seeds[i])
-                fz = r["fz"].real
+                fz = r["fz_raw"].real
                 L += float(np.log(fz))
                 dL += r["dfz"].real / fz
                 structure.append((r["nt_mu"], r["nt_ftr"]))

@@ -380,6 +380,12 @@ void mce_profile_enable(void* handle, int on);
 void mce_profile_read(void* handle, double* out_ms12);
 void mce_profile_reset(void* handle);
 
+/* The last scalar update's fz (re, im) before the numeric checks, which replace fz by the
+ * previous update's when both the mean and the covariance fail them (no counterpart in the
+ * reference, whose public fz is the checked one).  The window log-likelihood sum_k log fz_k
+ * and its tangent use this value. */
+void mce_last_fz_raw(void* handle, double* out2);
+
 /* Kernel shapes of one estimator (no counterpart in the reference), set by the autotuner
  * (cauchyfriendly_amd/autotune.py); results never depend on them.  enum_threads: lanes per
  * parent in the time-propagation cell enumerator (64, 128 or 256; 0 keeps it);

@@ -102,6 +102,7 @@ import_array();
 %ignore mce_profile_read;
 %ignore mce_profile_reset;
 %ignore mce_estimator_set_shapes;
+%ignore mce_last_fz_raw;
 %ignore mce_step_record;
 %ignore mce_collect_outputs;
 

@@ -67,6 +67,11 @@ def lib(kind: str = "ref"):
         if kind == "oracle":
             C.oracle_set_random_draws.argtypes = [ct.c_void_p, _dp, _dp]
             C.oracle_path_counters.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong)]
+            C.oracle_fz_raw.argtypes = [ct.c_void_p, _dp]
+            C.oracle_tape_mode.argtypes = [ct.c_void_p, ct.c_int]
+            C.oracle_tape_copy.argtypes = [ct.c_void_p, ct.c_void_p]
+            C.oracle_tape_status.restype = ct.c_int
+            C.oracle_tape_status.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong)]
         _libs[kind] = L
     return _libs[kind]
 
@@ -127,7 +132,12 @@ class RefEstimator:
         c = lambda off: complex(mom[off], mom[off + 1])
         mean = mom[6:6 + 2 * d].reshape(d, 2)
         cov = mom[6 + 2 * d:].reshape(d * d, 2)
-        return {
+        extra = {}
+        if self.kind == "oracle":
+            raw = np.zeros(2)
+            lib("oracle")._cdll.oracle_fz_raw(self.h, raw.ctypes.data_as(_dp))
+            extra["fz_raw"] = complex(raw[0], raw[1])
+        return {**extra,
             "rc": int(rc),
             "fz_public": c(0),
             "fz": c(2),
@@ -141,6 +151,20 @@ class RefEstimator:
             "shape_mu": cnt[4:4 + sr].copy(),
             "shape_ftr": cnt[4 + sr:4 + 2 * sr].copy(),
         }
+
+    def tape_record(self):
+        """Restatement only: record the reduction and negligibility decisions from now on."""
+        lib("oracle")._cdll.oracle_tape_mode(self.h, 1)
+
+    def tape_replay(self, recorded):
+        """Restatement only: replay `recorded`'s decisions instead of deciding."""
+        lib("oracle")._cdll.oracle_tape_copy(self.h, recorded.h)
+
+    def tape_status(self):
+        """(diverged, reduction decisions used, negligibility decisions used)."""
+        used = (ct.c_longlong * 2)()
+        div = lib("oracle")._cdll.oracle_tape_status(self.h, used)
+        return bool(div), int(used[0]), int(used[1])
 
     def finalize_extended(self, x_bar):
         """Extended-mode finish of the last update; returns the new x_bar."""
@@ -187,11 +211,16 @@ class RefEstimator:
         return out
 
 
-def run_system(sysobj, num_steps=None, seed=None, collect_terms=False, kind="ref"):
-    """Runs one estimator over a System's measurement record (num_steps time steps)."""
+def run_system(sysobj, num_steps=None, seed=None, collect_terms=False, kind="ref", tape=None):
+    """Runs one estimator over a System's measurement record (num_steps time steps).
+    tape (restatement only): "record", or an estimator whose recorded decisions to replay."""
     steps = sysobj.num_steps if num_steps is None else num_steps
     est = RefEstimator(steps, sysobj.n, sysobj.pncc, sysobj.p, sysobj.A0, sysobj.p0, sysobj.b0,
                        cmcc=sysobj.cmcc, seed=sysobj.srand_seed if seed is None else seed, kind=kind)
+    if tape == "record":
+        est.tape_record()
+    elif tape is not None:
+        est.tape_replay(tape)
     recs = []
     Gt = np.ascontiguousarray(sysobj.Gamma.reshape(-1))
     B = None if sysobj.B is None else np.ascontiguousarray(sysobj.B.reshape(-1))

@@ -302,11 +302,24 @@ struct Est {
     std::vector<std::vector<int>> combos;    // per m: binom(m,d) x d
     std::vector<int> cen, gen;
     cx fz, last_fz;
+    cx fz_raw;  // the update's fz before the numeric checks may replace it by last_fz
     std::vector<cx> mean, cov, last_mean, last_cov;
     double gscale = 0;
     int threads_used = 1;
     long long counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> skip_counts_cache;
+    // Decision tape (test infrastructure for frozen-structure differences): with mode 1
+    // the reduction's flag arrays and every g-table's negligibility outcome are recorded,
+    // with mode 2 they are replayed instead of decided, so a perturbed run keeps the
+    // recorded run's term structure.  A replay that meets a bucket of another size, or
+    // more decisions than recorded, sets `diverged` and decides for itself from there.
+    struct Tape {
+        int mode = 0;
+        std::vector<std::vector<int>> F;
+        std::vector<uint8_t> negl;
+        size_t fpos = 0, npos = 0;
+        bool diverged = false;
+    } tape;
 };
 
 double rand_u() { return ((double)rand() + 1.0) / ((double)RAND_MAX + 1.0); }  // random_variables.hpp:22-26
@@ -1037,6 +1050,7 @@ std::vector<int> fast_term_reduction(const std::vector<Term>& T, int d, int m) {
 // ---------------------------------------------------------------- step phases
 void finalize_and_check(Est& E) {
     const int d = E.d;
+    E.fz_raw = E.fz;
     E.gscale = kInvTwoPi / cre(E.fz);
     cx ifz = mk(0, cre(E.fz));
     for (int i = 0; i < d; i++) E.mean[i] /= ifz;
@@ -1138,6 +1152,7 @@ void step_first(Est& E, double msmt, const double* H, double gamma) {
     std::fill(E.mean.begin(), E.mean.end(), mk(0, 0));
     std::fill(E.cov.begin(), E.cov.end(), mk(0, 0));
     for (auto& t : E.bucket[d]) accumulate(E, t, true, E.fz, E.mean.data(), E.cov.data());
+    E.fz_raw = E.fz;
     E.gscale = kInvTwoPi / cre(E.fz);
     cx ifz = mk(0, cre(E.fz));
     for (int i = 0; i < d; i++) E.mean[i] /= ifz;
@@ -1264,7 +1279,28 @@ void ftr_and_gtables(Est& E) {
         std::vector<Term>& T = E.bucket[m];
         const int N = (int)T.size();
         if (N == 0) continue;
-        std::vector<int> F = fast_term_reduction(T, d, m);
+        std::vector<int> F;
+        Est::Tape& tp = E.tape;
+        if (tp.mode == 2 && !tp.diverged && tp.fpos < tp.F.size() && (int)tp.F[tp.fpos].size() == N) {
+            F = tp.F[tp.fpos++];
+        } else {
+            if (tp.mode == 2) tp.diverged = true;
+            F = fast_term_reduction(T, d, m);
+            if (tp.mode == 1) tp.F.push_back(F);
+        }
+        // make_gtable's negligibility outcome, recorded or replayed
+        auto negligible = [&](Term& t) {
+            bool r = make_gtable(t, E.gscale);
+            if (tp.mode == 1) {
+                tp.negl.push_back(r ? 1 : 0);
+            } else if (tp.mode == 2 && !tp.diverged) {
+                if (tp.npos < tp.negl.size())
+                    r = tp.negl[tp.npos++] != 0;
+                else
+                    tp.diverged = true;
+            }
+            return r;
+        };
         // ForwardFlagArray, cauchy_util.hpp:349-408
         std::vector<std::vector<int>> members(N);
         for (int j = 0; j < N; j++)
@@ -1286,7 +1322,7 @@ void ftr_and_gtables(Est& E) {
             }
             E.counters[6] += members[j].size();
             if (members[j].size() > (size_t)E.counters[7]) E.counters[7] = members[j].size();
-            if (make_gtable(*cj, E.gscale)) rt = -1;
+            if (negligible(*cj)) rt = -1;
             const std::vector<int>& mem = members[j];
             size_t k = 0;
             if (rt == -1 && !mem.empty()) E.counters[3]++;
@@ -1310,7 +1346,7 @@ void ftr_and_gtables(Est& E) {
                             ncg = ck->cells;
                         }
                     }
-                    if (!make_gtable(*ck, E.gscale)) {
+                    if (!negligible(*ck)) {
                         E.counters[4]++;
                         rt = cp;
                         cj = ck;
@@ -1343,7 +1379,7 @@ void ftr_and_gtables(Est& E) {
                         }
                     }
                 }
-                if (!make_gtable(*ck, E.gscale)) add_gtables(*cj, *ck, d);
+                if (!negligible(*ck)) add_gtables(*cj, *ck, d);
             }
             if (rt != -1) {
                 // become_parent, cauchy_term.hpp:748-755
@@ -1535,6 +1571,42 @@ int oracle_set_tr_search_idxs_ordering(const int* ordering, int n) {
     if (n > 12) return 1;
     for (int i = 0; i < n; i++) g_search_order[i] = ordering[i];
     return 0;
+}
+
+// The last update's fz before the numeric checks (which replace fz by the previous one when
+// both the mean and the covariance fail them).
+void oracle_fz_raw(void* h, double* out2) {
+    const Est& E = *(const Est*)h;
+    out2[0] = cre(E.fz_raw);
+    out2[1] = cim(E.fz_raw);
+}
+
+// Decision tape: mode 0 off, 1 record, 2 replay (from the start of the tape).
+void oracle_tape_mode(void* h, int mode) {
+    Est::Tape& t = ((Est*)h)->tape;
+    t.mode = mode;
+    t.fpos = t.npos = 0;
+    t.diverged = false;
+}
+// Copies src's recorded decisions into dst and sets dst to replay them.
+void oracle_tape_copy(void* dst, const void* src) {
+    Est::Tape& d = ((Est*)dst)->tape;
+    const Est::Tape& s = ((const Est*)src)->tape;
+    d.F = s.F;
+    d.negl = s.negl;
+    d.mode = 2;
+    d.fpos = d.npos = 0;
+    d.diverged = false;
+}
+// 1 when a replay met another structure than the recorded one; with the decisions used
+// (replaying) or recorded (recording).
+int oracle_tape_status(void* h, long long* used2) {
+    const Est::Tape& t = ((Est*)h)->tape;
+    if (used2) {
+        used2[0] = (long long)(t.mode == 1 ? t.F.size() : t.fpos);
+        used2[1] = (long long)(t.mode == 1 ? t.negl.size() : t.npos);
+    }
+    return t.diverged ? 1 : 0;
 }
 
 void oracle_path_counters(void* h, long long* out3) {

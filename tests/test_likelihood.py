@@ -28,7 +28,7 @@ def _oracle_L(s, theta, steps=None):
     structure = []
     for k in range(steps):
         r = est.step(s.zs[k, 0], Phi.reshape(-1), s.Gamma.reshape(-1), beta, H.reshape(-1), gamma[0])
-        L += np.log(r["fz"].real)
+        L += np.log(r["fz_raw"].real)
         structure.append((r["nt_mu"], r["nt_ftr"]))
     est.close()
     return L, structure
@@ -117,7 +117,7 @@ def _oracle_L_scales(s, beta, gamma):
     for k in range(s.num_steps):
         for i in range(s.p):
             r = est.step(s.zs[k, i], s.Phi(k).reshape(-1), Gt, np.asarray(beta), s.H[i], gamma[i])
-            L += np.log(r["fz"].real)
+            L += np.log(r["fz_raw"].real)
             structure.append((r["nt_mu"], r["nt_ftr"]))
     est.close()
     return L, structure
