@@ -1399,6 +1399,61 @@ struct Engine {
     DBuf<uint2> rank_arena;  // rank structures of the current parents' g-tables (k_rank_build)
     int upd_mode = -1;       // this update's parent lookup (pick_lookup at the update's start)
 
+    // Debug mode (the reference's print_basic_info / debug_print): after term reduction the
+    // public fz, mean and covariance are the moments of the reduced terms, each evaluated in
+    // its own g-table at the root point's sign vector (compute_moments(false),
+    // cauchy_estimator.hpp:524-602 with eval_g_yei_after_ftr, cauchy_term.hpp:403-433), summed
+    // in term order, and printed.  Without it fz is 1 after reduction.
+    bool debug_after_ftr = false;
+    DBuf<double> dbg_gy, dbg_x, dbg_part;
+    DBuf<uint8_t> dbg_flags;
+    DBuf<uint32_t> dbg_chain;
+    DBuf<int> dbg_off;
+    Co<void> moments_after_ftr() {
+        const int n = Nt;
+        const int NC = 1 + d + d * d;
+        TermPool P = in.view();
+        dbg_gy.reserve((size_t)std::max(n, 1) * (2 + 2 * d));
+        dbg_flags.reserve(std::max(n, 1));
+        dbg_chain.reserve(std::max(n, 1));
+        dbg_off.reserve(2);
+        dbg_part.reserve((size_t)NC * 2 + 1);
+        std::vector<double> part((size_t)NC * 2 + 1, 0.0);
+        if (n > 0) {
+            MCE_DISPATCH_D(d, MCE_RUN(k_after_ftr_gy<DD>, dim3(ceil_div(n, 256)), dim3(256), 0, st, P, n, dconst.ptr,
+                                      gk[gcur].ptr, gv[gcur].ptr, dbg_gy.ptr));
+            fill_multi({{dbg_flags.ptr, (size_t)n, kValid}});
+            MCE_RUN(k_iota, dim3(ceil_div(n, 256)), dim3(256), 0, st, dbg_chain.ptr, n);
+            const int off2[2] = {0, n};
+            upload(dbg_off.ptr, off2, sizeof(off2));
+            int ncp = 0, nyb = 0;
+            MCE_DISPATCH_D(d, (nyb = mom_blocks<DD>(), ncp = nyb * kSumCW / 2));
+            dbg_x.reserve((size_t)nyb * n * kSumCW);
+            // one chain over the terms in order: the reference's sequential sums
+            MCE_DISPATCH_D(d, MCE_RUN(k_mom_terms<DD>, dim3(ceil_div((int64_t)n * ncp, 256)), dim3(256), 0, st,
+                                      dbg_gy.ptr, dbg_flags.ptr, dbg_chain.ptr, dbg_off.ptr, 1, n, dbg_x.ptr));
+            MCE_DISPATCH_D(d, MCE_RUN(k_mom_sum<DD>, dim3(1, nyb), dim3(256), 0, st, dbg_x.ptr, dbg_off.ptr, 1,
+                                      dbg_part.ptr, counters.ptr + 17));
+            co_await rd(dbg_part.ptr, part.data(), (int)part.size());
+        }
+        fz = cmake(part[0], part[1]);
+        for (int j = 0; j < d; j++) mean[j] = cmake(part[2 + 2 * j], part[3 + 2 * j]);
+        for (int j = 0; j < d * d; j++) cov[j] = cmake(part[2 + 2 * d + 2 * j], part[3 + 2 * d + 2 * j]);
+        if (!(fz.re > 0)) throw std::runtime_error("debug moments after term reduction: Re fz <= 0");
+        const cplx ifz = cmake(0, fz.re);
+        for (int i = 0; i < d; i++) mean[i] = cdiv(mean[i], ifz);
+        for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++) cov[i * d + j] = csub(cdiv(cov[i * d + j], fz), cmul(mean[i], mean[j]));
+        printf("Moment Information (after FTR) at step %d, MU %d/%d\n", (master_step + 1) / p, (master_step % p) + 1, p);
+        printf("fz: %.16lf + %.16lfj\n", fz.re, fz.im);
+        printf("Conditional Mean:\n");
+        for (int i = 0; i < d; i++) printf("%.16lf + %.16lfj%s", mean[i].re, mean[i].im, i + 1 < d ? ", " : "\n");
+        printf("Conditional Variance:\n");
+        for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++)
+                printf("%.16lf + %.16lfj%s", cov[i * d + j].re, cov[i * d + j].im, j + 1 < d ? ", " : "\n");
+    }
+
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
@@ -1674,7 +1729,12 @@ struct Engine {
                 throw std::runtime_error(msg);
             }
         }
-        if (!skip) fz = cmake(1, 0);
+        if (!skip) {
+            if (debug_after_ftr)
+                co_await moments_after_ftr();
+            else
+                fz = cmake(1, 0);
+        }
         if (first) {
             last_mean = mean;
             last_cov = cov;
@@ -3313,7 +3373,6 @@ void* pycauchy_initialize_lti(int num_steps, double* A0, int size_A0, double* p0
                               int size_gamma, double dt, int init_step, bool debug_print) {
     (void)size_A0;
     (void)size_p0;
-    (void)debug_print;
     try {
         const int n = size_b0;
         const int pncc = size_beta;
@@ -3350,6 +3409,7 @@ void* pycauchy_initialize_lti(int num_steps, double* A0, int size_A0, double* p0
         duc.beta = h->beta.data();
         duc.gamma = h->gamma.data();
         h->eng = new Engine(num_steps, n, cmcc, pncc, p, A0, p0, b0);
+        h->eng->debug_after_ftr = debug_print;
         return h;
     } catch (const std::exception& e) {
         set_err("pycauchy_initialize_lti", e);
@@ -3392,7 +3452,6 @@ void* pycauchy_initialize_nonlin(int num_steps, double* x0, int size_x0, double*
                                  double dt, int init_step, bool debug_print) {
     (void)size_A0;
     (void)size_p0;
-    (void)debug_print;
     try {
         const int n = size_b0;
         const int pncc = size_beta;
@@ -3431,6 +3490,7 @@ void* pycauchy_initialize_nonlin(int num_steps, double* x0, int size_x0, double*
         h->ext_update = f_extended_msmt_update_callback;
         // the estimator propagates the state deviation, which has no control term
         h->eng = new Engine(num_steps, n, 0, pncc, p, A0, p0, b0);
+        h->eng->debug_after_ftr = debug_print;
         return h;
     } catch (const std::exception& e) {
         set_err("pycauchy_initialize_nonlin", e);
@@ -3869,7 +3929,6 @@ void pycauchy_get_reinitialization_statistics(void* handle, double z, double* xh
 
 void* mce_estimator_create(double* A0, double* p0, double* b0, int steps, int d, int cmcc, int pncc, int p,
                            bool print_basic_info) {
-    (void)print_basic_info;
     try {
         Handle* h = new Handle;
         h->x.assign(d, 0.0);
@@ -3880,6 +3939,7 @@ void* mce_estimator_create(double* A0, double* p0, double* b0, int steps, int d,
         h->duc.pncc = pncc;
         h->duc.p = p;
         h->eng = new Engine(steps, d, cmcc, pncc, p, A0, p0, b0);
+        h->eng->debug_after_ftr = print_basic_info;
         return h;
     } catch (const std::exception& ex) {
         set_err("mce_estimator_create", ex);

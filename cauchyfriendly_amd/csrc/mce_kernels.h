@@ -1062,6 +1062,49 @@ __host__ __device__ constexpr int mom_blocks() {
     return (2 * (1 + D + D * D) + kSumCW - 1) / kSumCW;
 }
 
+// Moments after term reduction (debug mode), cauchy_term.hpp:403-433 (eval_g_yei_after_ftr)
+// over every parent term: the sign vector of its hyperplanes at the root point (bit l set
+// where A_l . root <= 0), y_j = -(sum_l (p_l s_l) A_lj) + i b_j, and g its own g-table's value
+// at that sign vector.  Writes (g, y) per term in the layout k_mom_terms reads.
+template <int D>
+__device__ __forceinline__ void k_after_ftr_gy(MG<1024, 1> mg, const TermPool P, int n,
+                                               const StepConst* __restrict__ sc, const uint32_t* __restrict__ gk,
+                                               const cplx* __restrict__ gv, double* __restrict__ gy) {
+    const int t = mg.bx * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int m = P.m[t];
+    const int MM = P.MM;
+    const double* A = P.A + (int64_t)t * MM * D;
+    const double* p = P.p + (int64_t)t * MM;
+    const double* b = P.b + (int64_t)t * D;
+    double tmp[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) tmp[j] = 0.0;
+    int enc = 0;
+    for (int l = 0; l < m; l++) {
+        double dp = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) dp += A[l * D + k] * sc->root[k];
+        const double s = dp > 0 ? 1.0 : -1.0;
+        const double w = p[l] * s;
+#pragma unroll
+        for (int j = 0; j < D; j++) tmp[j] += w * A[l * D + j];
+        if (s < 0) enc |= 1 << l;
+    }
+    const int top = 1 << (m - 1);
+    const int rev = (m >= 32) ? -1 : ((1 << m) - 1);
+    const int64_t go = P.gp_off[t];
+    const cplx g = lookup_num(enc, top, rev, gk + go, gv + go, P.gp_n[t]);
+    double* rec = gy + (int64_t)t * (2 + 2 * D);
+    rec[0] = g.re;
+    rec[1] = g.im;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        rec[2 + 2 * j] = -tmp[j];
+        rec[3 + 2 * j] = b[j];
+    }
+}
+
 // Products with the reference's operations: g, g*y_j and the negated (g*y_j)*y_k
 // (the reference subtracts the latter; a - b == a + (-b) exactly).
 template <int D>

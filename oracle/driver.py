@@ -68,6 +68,7 @@ def lib(kind: str = "ref"):
             C.oracle_set_random_draws.argtypes = [ct.c_void_p, _dp, _dp]
             C.oracle_path_counters.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong)]
             C.oracle_fz_raw.argtypes = [ct.c_void_p, _dp]
+            C.oracle_set_debug.argtypes = [ct.c_void_p, ct.c_int]
             C.oracle_tape_mode.argtypes = [ct.c_void_p, ct.c_int]
             C.oracle_tape_copy.argtypes = [ct.c_void_p, ct.c_void_p]
             C.oracle_tape_status.restype = ct.c_int
@@ -151,6 +152,10 @@ class RefEstimator:
             "shape_mu": cnt[4:4 + sr].copy(),
             "shape_ftr": cnt[4 + sr:4 + 2 * sr].copy(),
         }
+
+    def set_debug(self, on=True):
+        """Restatement only: the reference's debug mode (public moments after reduction)."""
+        lib("oracle")._cdll.oracle_set_debug(self.h, 1 if on else 0)
 
     def tape_record(self):
         """Restatement only: record the reduction and negligibility decisions from now on."""

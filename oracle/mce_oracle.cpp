@@ -308,6 +308,7 @@ struct Est {
     int threads_used = 1;
     long long counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> skip_counts_cache;
+    bool debug = false;  // print_basic_info: the public moments after reduction are recomputed
     // Decision tape (test infrastructure for frozen-structure differences): with mode 1
     // the reduction's flag arrays and every g-table's negligibility outcome are recorded,
     // with mode 2 they are replayed instead of decided, so a perturbed run keeps the
@@ -1171,6 +1172,43 @@ void step_first(Est& E, double msmt, const double* H, double gamma) {
     }
 }
 
+// compute_moments(false), cauchy_estimator.hpp:524-602, with eval_g_yei_after_ftr,
+// cauchy_term.hpp:403-433: the moments of the reduced terms, each term's g looked up in its
+// own g-table at the sign vector of its hyperplanes at the root point, summed in term order
+// (shape ascending).  Debug mode only; replaces fz, mean and cov.
+void moments_after_ftr(Est& E) {
+    const int d = E.d;
+    cx fz = mk(0, 0);
+    std::vector<cx> mean(d, mk(0, 0)), cov(d * d, mk(0, 0)), yei(d);
+    std::vector<double> tmp(d);
+    for (int m = 1; m < E.shape_range; m++)
+        for (const Term& t : E.bucket[m]) {
+            std::fill(tmp.begin(), tmp.end(), 0.0);
+            int enc = 0;
+            for (int l = 0; l < t.m; l++) {
+                const double s = dotp(t.A.data() + l * d, E.root.data(), d) > 0 ? 1 : -1;
+                const double w = t.p[l] * s;
+                for (int j = 0; j < d; j++) tmp[j] += w * t.A[l * d + j];
+                if (s < 0) enc |= 1 << l;
+            }
+            const cx g = lookup_num(enc, 1 << (t.m - 1), (1 << t.m) - 1, t.gp.get(), t.cells_p);
+            for (int j = 0; j < d; j++) yei[j] = mk(-tmp[j], t.b[j]);
+            fz += g;
+            for (int j = 0; j < d; j++) {
+                const cx y = yei[j];
+                mean[j] += g * y;
+                for (int k = 0; k < d; k++) cov[j * d + k] -= g * y * yei[k];
+            }
+        }
+    const cx ifz = mk(0, cre(fz));
+    for (int i = 0; i < d; i++) mean[i] /= ifz;
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) cov[i * d + j] = (cov[i * d + j] / fz) - mean[i] * mean[j];
+    E.fz = fz;
+    E.mean = mean;
+    E.cov = cov;
+}
+
 struct StepRecord {
     cx fz_pre;
     std::vector<cx> mean, cov;
@@ -1490,6 +1528,7 @@ int oracle_step(void* h, double msmt, const double* Phi, const double* Gamma, co
         out_cnt[1] = E.Nt;
         shape_mu[d] = E.Nt;
         E.fz = mk(1, 0);
+        if (E.debug) moments_after_ftr(E);
         E.last_mean = E.mean;
         E.last_cov = E.cov;
         E.last_fz = E.fz;
@@ -1504,6 +1543,7 @@ int oracle_step(void* h, double msmt, const double* Phi, const double* Gamma, co
         } else {
             for (int m = 0; m < sr; m++) shape_mu[m] = (int)E.bucket[m].size();
             ftr_and_gtables(E);
+            if (E.debug) moments_after_ftr(E);
         }
     }
     E.master_step++;
@@ -1572,6 +1612,9 @@ int oracle_set_tr_search_idxs_ordering(const int* ordering, int n) {
     for (int i = 0; i < n; i++) g_search_order[i] = ordering[i];
     return 0;
 }
+
+// Debug mode (the reference's print_basic_info): public moments after reduction.
+void oracle_set_debug(void* h, int on) { ((Est*)h)->debug = on != 0; }
 
 // The last update's fz before the numeric checks (which replace fz by the previous one when
 // both the mean and the covariance fail them).

@@ -19,7 +19,7 @@ def pytest_configure(config):
 
 # fixtures of other record types (window bank, window re-initialisation, extended mode, densities,
 # relative systems, deterministic transforms)
-NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_", "rsys_", "transform_", "wm_")
+NON_ESTIMATOR_FIXTURES = ("speyer_", "window_", "nonlin_", "cpdf_", "rsys_", "transform_", "wm_", "debug_")
 
 
 def golden_names():
