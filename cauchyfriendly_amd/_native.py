@@ -153,6 +153,8 @@ def lib():
     L.mce_profile_reset.argtypes = [ct.c_void_p]
     L.mce_last_fz_raw.restype = None
     L.mce_last_fz_raw.argtypes = [ct.c_void_p, _dp]
+    L.mce_window_bank_set_mode.restype = ct.c_int
+    L.mce_window_bank_set_mode.argtypes = [ct.c_int]
     L.mce_estimator_set_shapes.restype = ct.c_int
     L.mce_estimator_set_shapes.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int]
     L.mce_batch_profile_enable.restype = None

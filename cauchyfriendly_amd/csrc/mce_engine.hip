@@ -2203,7 +2203,8 @@ struct Engine {
     // Extended-mode finish of a measurement update (cauchy_estimator.hpp:1358-1394): the
     // terms' b-vectors move by -Re(mean) (skipped on a window's last update), the mean
     // becomes x_bar + mean, and x_bar takes its real part.
-    void finalize_extended_moments(double* x_bar);
+    void finalize_extended_moments(double* x_bar) { run_sync(finalize_extended_co(x_bar)); }
+    Co<void> finalize_extended_co(double* x_bar);
 
     // Host copy of parent term `idx` of bucket `shape`.
     int term(int shape, int idx, int* info, double* A, double* p_, double* b, unsigned int* keys_, double* g) {
@@ -2632,7 +2633,7 @@ static int64_t rsys_2d(Engine& P, Engine& S, const double* Trel, double eps, dou
     return nkeep;
 }
 
-void Engine::finalize_extended_moments(double* x_bar) {
+Co<void> Engine::finalize_extended_co(double* x_bar) {
     if (ntan) throw std::runtime_error("the extended-mode finish carries no tangents");
     bind();
     if (!skip) {
@@ -2644,7 +2645,7 @@ void Engine::finalize_extended_moments(double* x_bar) {
     }
     for (int i = 0; i < d; i++) mean[i].re += x_bar[i];
     for (int i = 0; i < d; i++) x_bar[i] = mean[i].re;
-    if (!skip) HIPCHK(hipStreamSynchronize(st));
+    if (!skip) co_await sync_point();
 }
 
 // ======================================================================== C ABI

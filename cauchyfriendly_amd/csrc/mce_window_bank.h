@@ -50,6 +50,14 @@ struct BankWindow {
     int err = 0;                // numeric error bits of its last update
 };
 
+// How the windows of one measurement step: one after another; together as merged launches
+// on the calling thread; or one host thread per window, each on its window's own streams
+// (extended mode steps merged instead, so that its callbacks stay on the calling thread).
+// A bank's few windows are large estimators whose updates are chains of dependent launches:
+// on threads the chains of different windows overlap on the device, while merged launches
+// on one stream still run each chain's steps one flush after another.
+enum { kBankSequential = 0, kBankMerged = 1, kBankThreaded = 2 };
+
 struct BankInit {  // WindowInitializationMessage
     std::vector<double> A0, p0, b0, x;
 };
@@ -65,6 +73,7 @@ struct WindowBank {
     std::vector<double> boost;
     bool has_boost = false;
     int initializer = 0, initializee = 0, msmt_count = 0;
+    int mode = kBankThreaded;  // how the windows of a measurement step (mce_window_bank_set_mode)
     std::vector<char> active;
     std::vector<int> active_counts, active_errs, step_counts;
     // per window and per window step, as save_window_data keeps them
@@ -184,7 +193,9 @@ struct WindowBank {
     }
 
     // The window's updates of one measurement: every row, or only the last after a restart.
-    void run_rows(BankWindow& w, std::vector<double>& msm, bool restarted, bool keep_xbar) {
+    // A task, so that a batch scheduler can step several windows' updates together; run on
+    // its own it never suspends.
+    Co<void> rows_co(BankWindow& w, std::vector<double> msm, bool restarted, bool keep_xbar) {
         Engine* e = w.h->eng;
         CauchyDynamicsUpdateContainer& duc = w.h->duc;
         for (int i = restarted ? p - 1 : 0; i < p; i++) {
@@ -193,10 +204,12 @@ struct WindowBank {
                 f_msmt(&duc, w.h->zbar.data());
                 msm[i] -= w.h->zbar[i];
                 f_ext(&duc);
-                w.err = e->update(msm[i], duc.Phi, duc.Gamma, duc.beta, duc.H + i * n, duc.gamma[i], nullptr, nullptr);
-                e->finalize_extended_moments(duc.x);
+                w.err = co_await e->update_co(msm[i], duc.Phi, duc.Gamma, duc.beta, duc.H + i * n, duc.gamma[i],
+                                              nullptr, nullptr);
+                co_await e->finalize_extended_co(duc.x);
             } else {
-                w.err = e->update(msm[i], duc.Phi, duc.Gamma, duc.beta, duc.H + i * n, duc.gamma[i], duc.B, duc.u);
+                w.err = co_await e->update_co(msm[i], duc.Phi, duc.Gamma, duc.beta, duc.H + i * n, duc.gamma[i],
+                                              duc.B, duc.u);
             }
         }
         if (restarted) e->master_step = p;
@@ -213,50 +226,111 @@ struct WindowBank {
             throw std::runtime_error("extended mode: the dynamics callback must set is_xbar_set_for_ece");
     }
 
-    // One measurement of window j (child_window_loop's loop body, cauchy_windows.hpp:448-702).
-    // Returns whether the window reported an estimate.
-    bool window_msmt(int j, const double* msmts, const double* controls, const BankInit* init) {
+    // What window j does with one measurement, fixed before its updates run.
+    struct Plan {
+        bool run = false;         // it updates (and reports an estimate)
+        bool restart = false;     // it restarts first and takes only the last row
+        bool keep_xbar = false;   // extended mode: it keeps x_bar before its last row
+        bool reset_after = false; // it has taken num_windows measurements and resets
+    };
+
+    // One measurement of window j (child_window_loop's loop body, cauchy_windows.hpp:448-702),
+    // in three parts: begin (counters, the dynamics callback, a restart's initialisation),
+    // the updates (rows_co) and end (the reset after a full window).  Only the updates touch
+    // the device; every window's begin runs before any update of the measurement, and its
+    // end after them.
+    Plan window_begin(int j, const double* controls, const BankInit* init) {
         BankWindow& w = win[j];
-        CauchyDynamicsUpdateContainer& duc = w.h->duc;
-        duc.u = cmcc > 0 ? const_cast<double*>(controls) : nullptr;
-        std::vector<double> msm(msmts, msmts + p);
-        bool reported = false;
+        w.h->duc.u = cmcc > 0 ? const_cast<double*>(controls) : nullptr;
+        Plan pl;
         if (w.first_loop) {
             w.msmt_count += 1;
             if (w.msmt_count > j) {
                 w.win_msmt_count += 1;
                 if (w.win_msmt_count != 1) dynamics(w, !ext);
-                const bool restart = (j != 0) && (w.win_msmt_count == 1);
-                if (restart) {
-                    if (!init) throw std::runtime_error("window bank: window " + std::to_string(j) + " needs a restart");
-                    apply_init(w, *init);
-                }
-                run_rows(w, msm, restart, true);
-                reported = true;
-                if (w.win_msmt_count == W) {
-                    w.win_msmt_count = 0;
-                    w.h->eng->reset_terms();
-                }
+                pl.run = true;
+                pl.restart = (j != 0) && (w.win_msmt_count == 1);
+                pl.keep_xbar = true;
             }
             if (w.msmt_count == W) w.first_loop = false;
         } else {
             w.msmt_count += 1;
             w.win_msmt_count += 1;
             if (w.win_msmt_count != 1) dynamics(w, true);
-            const bool restart = w.win_msmt_count == 1;
-            if (restart) {
-                if (!init) throw std::runtime_error("window bank: window " + std::to_string(j) + " needs a restart");
-                apply_init(w, *init);
-            }
-            run_rows(w, msm, restart, w.win_msmt_count == W);
-            if (w.win_msmt_count == W) {
-                w.win_msmt_count = 0;
-                w.h->eng->reset_terms();
-            }
-            reported = true;
+            pl.run = true;
+            pl.restart = w.win_msmt_count == 1;
+            pl.keep_xbar = w.win_msmt_count == W;
+        }
+        pl.reset_after = pl.run && w.win_msmt_count == W;
+        if (pl.restart) {
+            if (!init) throw std::runtime_error("window bank: window " + std::to_string(j) + " needs a restart");
+            apply_init(w, *init);
+        }
+        return pl;
+    }
+
+    void window_end(int j, const Plan& pl) {
+        BankWindow& w = win[j];
+        if (pl.reset_after) {
+            w.win_msmt_count = 0;
+            w.h->eng->reset_terms();
         }
         w.step_count++;
-        return reported;
+    }
+
+    // The updates of windows js (their plans in pl) with measurement msmts, as `mode` says.
+    // Merged, the windows step together through the batch scheduler, which makes each kernel
+    // of their updates one merged launch.  Each window's results are those of its own updates
+    // run alone, in every mode.
+    void run_windows(const std::vector<int>& js, const std::vector<Plan>& pl, const double* msmts) {
+        std::vector<int> run;
+        for (int j : js)
+            if (pl[j].run) run.push_back(j);
+        auto alone = [&](int j) {
+            run_sync(rows_co(win[j], std::vector<double>(msmts, msmts + p), pl[j].restart, pl[j].keep_xbar));
+        };
+        if (run.size() < 2 || mode == kBankSequential) {
+            for (int j : run) alone(j);
+            return;
+        }
+        if (mode == kBankThreaded && !ext) {
+            std::vector<std::string> errs(run.size());
+            std::atomic<int> next{0};
+            const std::function<void()> job = [&]() {
+                int k;
+                while ((k = next.fetch_add(1)) < (int)run.size()) {
+                    try {
+                        alone(run[k]);
+                    } catch (const std::exception& ex) {
+                        errs[k] = ex.what();
+                    }
+                }
+            };
+            worker_pool().run((int)run.size() - 1, job);
+            for (size_t k = 0; k < run.size(); k++)
+                if (!errs[k].empty()) throw std::runtime_error("window " + std::to_string(run[k]) + ": " + errs[k]);
+            return;
+        }
+        std::vector<BatchRunner::Job> jobs(run.size());
+        for (size_t k = 0; k < run.size(); k++) {
+            BankWindow& w = win[run[k]];
+            jobs[k].e = w.h->eng;
+            jobs[k].task.emplace(rows_co(w, std::vector<double>(msmts, msmts + p), pl[run[k]].restart,
+                                         pl[run[k]].keep_xbar));
+        }
+        batch_runner().run(jobs);
+        for (size_t k = 0; k < run.size(); k++)
+            if (!jobs[k].err.empty())
+                throw std::runtime_error("window " + std::to_string(run[k]) + ": " + jobs[k].err);
+    }
+
+    // One measurement of window j on its own.
+    bool window_msmt(int j, const double* msmts, const double* controls, const BankInit* init) {
+        std::vector<Plan> pl(W);
+        pl[j] = window_begin(j, controls, init);
+        run_windows({j}, pl, msmts);
+        window_end(j, pl[j]);
+        return pl[j].run;
     }
 
     // save_window_data, cauchy_windows.hpp:1478-1505.
@@ -332,9 +406,15 @@ struct WindowBank {
             }
         }
         std::fill(active.begin(), active.end(), 0);
-        for (int j = 0; j < W; j++) {
-            if (j == initializee && msmt_count > 0) continue;
-            window_msmt(j, msmts, C.data(), nullptr);
+        // every window but the one that restarts, together
+        std::vector<int> js;
+        std::vector<Plan> pl(W);
+        for (int j = 0; j < W; j++)
+            if (!(j == initializee && msmt_count > 0)) js.push_back(j);
+        for (int j : js) pl[j] = window_begin(j, C.data(), nullptr);
+        run_windows(js, pl, msmts);
+        for (int j : js) {
+            window_end(j, pl[j]);
             if (j <= msmt_count) {
                 active[j] = 1;
                 active_counts[j] += 1;
@@ -545,6 +625,19 @@ void pycauchy_step(double* msmts, int size_msmts, double* controls, int size_con
 void pycauchy_shutdown() {
     delete g_bank;
     g_bank = nullptr;
+}
+
+int mce_window_bank_set_mode(int mode) {
+    if (!g_bank) {
+        set_err("mce_window_bank_set_mode", std::runtime_error("no window bank: initialize one first"));
+        return -1;
+    }
+    if (mode < kBankSequential || mode > kBankThreaded) {
+        set_err("mce_window_bank_set_mode", std::runtime_error("mode must be 0, 1 or 2"));
+        return -1;
+    }
+    g_bank->mode = mode;
+    return 0;
 }
 
 }  // extern "C"

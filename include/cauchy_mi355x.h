@@ -278,6 +278,14 @@ void pycauchy_step(double* msmts, int size_msmts, double* controls, int size_con
 /* Replaces pycauchy_shutdown, scripts/swig/cauchy/pycauchy.hpp:216-221: ends the bank. */
 void pycauchy_shutdown(void);
 
+/* How the running bank steps its windows (no counterpart in the reference, whose windows are
+ * forked processes): 0 one window after another; 1 the windows that take a measurement
+ * without restarting step together, each kernel of their updates one merged launch; 2 (the
+ * default) those windows on one host thread each, each on its own streams (extended mode
+ * steps merged instead, so that its callbacks run on the calling thread).  Results are the
+ * same in every mode.  Returns 0, or -1 with no bank running or another mode. */
+int mce_window_bank_set_mode(int mode);
+
 /* Replaces pycauchy_speyers_window_init, scripts/swig/cauchy/pycauchy.hpp:973-993: the
  * initial terms (A0 n x n rows, p0, b0; malloc'd) of a window restarted about the estimate
  * (xhat, Phat) after scalar measurement z = H x + v, v ~ Cauchy(0, gamma)

@@ -103,6 +103,7 @@ import_array();
 %ignore mce_profile_reset;
 %ignore mce_estimator_set_shapes;
 %ignore mce_last_fz_raw;
+%ignore mce_window_bank_set_mode;
 %ignore mce_step_record;
 %ignore mce_collect_outputs;
 

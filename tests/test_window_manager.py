@@ -6,7 +6,9 @@ the reporting window and its error bits are identical, and fz, x_hat, P_hat and 
 imaginary-part bounds agree within 1e-9 (relative to the largest magnitude of each output).
 Cases: a 3-state LTI bank of 4 windows over 16 measurements; a 3-state bank of 3 windows
 with a window-variance boost; a 4-state LTV bank (two process noises, two measurements per
-step, Phi_k through the dynamics callback) of 3 windows over 10 steps."""
+step, Phi_k through the dynamics callback) of 3 windows over 10 steps.  The bank steps the
+windows of a measurement together (merged launches) by default; stepped one window at a time,
+or one host thread per window, it must give the same bits."""
 import ctypes as ct
 
 import numpy as np
@@ -32,7 +34,7 @@ def _dynamics(s):
     return ct.CFUNCTYPE(None, ct.POINTER(C_CauchyDynamicsUpdateContainer))(dyn)
 
 
-def _run(rec):
+def _run(rec, mode=1):
     s = W.System.from_dict(rec["system"])
     ct.CDLL("libc.so.6").srand(ct.c_uint(rec["srand"]))
     args = (s.A0, s.p0, s.b0, s.Phi(0), s.Gamma.reshape(-1), np.zeros(0), np.ravel(s.beta), s.H.reshape(-1),
@@ -46,6 +48,8 @@ def _run(rec):
         cb = _dynamics(s)
         pycauchy.pycauchy_initialize_ltv_window_manager(nw, steps, *args, cb, False, False, False, None, 0.0, 0,
                                                          boost)
+    from cauchyfriendly_amd import _native
+    assert _native.lib().mce_window_bank_set_mode(mode) == 0
     out = []
     try:
         for k in range(steps):
@@ -77,6 +81,30 @@ def test_window_manager_matches_reference(name):
         for a, b in ((cfz, e["cerr_fz"]), (cx, e["cerr_xhat"]), (cP, e["cerr_Phat"])):
             assert abs(a - b) <= TOL * max(1.0, abs(b)) + 1e-12, f"{name} step {k}: imaginary bound {a} vs {b}"
     print(f"  worst relative error {worst:.2e}", flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 2], ids=["merged", "threaded"])
+@pytest.mark.parametrize("name", ["wm_lti3", "wm_lti3_boost", "wm_ltv4_p2"])
+def test_window_manager_modes_equal_one_window_at_a_time(name, mode):
+    """The windows' updates as merged launches, or on a host thread each, give each window
+    the bits of its own updates run one window after another."""
+    rec = load_golden(name)
+    print(f"case {name}: mode {mode} vs one window at a time", flush=True)
+    got, alone = _run(rec, mode=mode), _run(rec, mode=0)
+    for k, (g, e) in enumerate(zip(got, alone)):
+        fz, x, P, cfz, cx, cP, win, err = g
+        fz1, x1, P1, cfz1, cx1, cP1, win1, err1 = e
+        assert (win, err) == (win1, err1), f"{name} step {k}: window/err"
+        for a, b in ((fz, fz1), (x, x1), (P, P1), (cfz, cfz1), (cx, cx1), (cP, cP1)):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), f"{name} step {k}: mode {mode} differs"
+
+
+def test_window_bank_mode_switch_needs_a_bank():
+    from cauchyfriendly_amd import _native
+    L = _native.lib()
+    assert L.mce_window_bank_set_mode(1) == -1
+    assert "no window bank" in L.mce_last_error().decode()
 
 
 @pytest.mark.gpu
