@@ -1595,19 +1595,27 @@ struct Engine {
                 const int mt = ec[4], ph = ec[5];
                 if (mt < 1 || mt > MM || ph < 1 || ph >= mt)
                     throw std::runtime_error("cell enumerator: parent shapes outside the estimator's range");
+                // queues of 2T kept vertices (combination, solution) and 2T undecided
+                // combinations come first; with the largest sets, fewer lanes per parent
                 const int words = mt >= 5 ? (1 << (mt - 5)) : 1;
                 const int pwords = ph - 1 >= 5 ? (1 << (ph - 6)) : 1;
-                const size_t enum_static = 256 * sizeof(int);
+                const size_t enum_static = (256 + 4) * sizeof(int);
+                int threads = enum_threads;
+                auto queues = [&](int t) { return (size_t)2 * t * (8 * (size_t)d + 8); };
                 int psearch = 0;
-                size_t lds = (size_t)(words + pwords) * 4;
+                size_t lds = queues(threads) + (size_t)(words + pwords) * 4;
                 if (lds + enum_static > kLdsPerGroup) {
                     psearch = 1;
-                    lds = (size_t)words * 4;
+                    lds = queues(threads) + (size_t)words * 4;
+                }
+                while (lds + enum_static > kLdsPerGroup && threads > 64) {
+                    threads /= 2;
+                    lds = queues(threads) + (size_t)words * 4;
                 }
                 if (lds + enum_static > kLdsPerGroup)
                     throw std::runtime_error("hyperplane count too large for the cell enumerator's LDS sign-vector set");
                 MCE_DISPATCH_D(d, {
-                    MCE_RUN(k_btab_enum<DD>, dim3(nenum), dim3(enum_threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
+                    MCE_RUN(k_btab_enum<DD>, dim3(nenum), dim3(threads), lds, st, IN, enum_list.ptr, dbpert.ptr,
                                dcombos.ptr, dcombo_off.ptr, dcombo_cnt.ptr, gk[gcur].ptr, barena.ptr, btcap.ptr,
                                counters.ptr + 8, words, psearch);
                 });

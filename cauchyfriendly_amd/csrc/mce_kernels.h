@@ -234,6 +234,37 @@ __device__ inline int wave_excl_scan(int v, int* total) {
 
 __device__ inline bool wave_all(bool pred) { return __all(pred); }
 
+// Inclusive scan of one value per lane across a wavefront (shuffles, no barrier).
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const T y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Exclusive prefix of each thread's v over a block of whole wavefronts (at most 4) and the
+// block total, through wsum[4] in LDS (two barriers).  Every thread of the block calls it.
+__device__ __forceinline__ int block4_excl_scan(int v, int* total, int* wsum) {
+    const int lane = (int)(threadIdx.x & (kWave - 1)), w = (int)(threadIdx.x / kWave);
+    const int nw = (int)(blockDim.x / kWave);
+    const int x = wave_incl_scan(v);
+    __syncthreads();  // wsum of an earlier scan has been read
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int k = 0; k < nw; k++) {
+        const int sk = wsum[k];
+        before += k < w ? sk : 0;
+        tot += sk;
+    }
+    *total = tot;
+    return before + x - v;
+}
+
 // ------------------------------------------------------------------ time propagation
 // cauchy_term.hpp:435-531 (time_prop, normalize_hps, tp_coalign), with the B-table
 // mode decision of cauchy_estimator.hpp:678-708 (serial) / 1614-1637 (threaded).
@@ -371,12 +402,57 @@ __device__ __forceinline__ void k_btab_simple(MG<1024, 1> mg, TermPool P, int n,
     }
 }
 
+// The sign vectors of the cells around one vertex v of the arrangement: every hyperplane
+// not among the vertex's D (cmask) takes the side of v; the D of the vertex take both.
+template <int D>
+__device__ __forceinline__ void enum_vertex_cells(const double* __restrict__ A, const double* __restrict__ bp, int m,
+                                                  const uint8_t* __restrict__ cb, const double* v, uint32_t* bits) {
+    uint32_t cbit[D], cmask = 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        cbit[k] = 1u << cb[k];
+        cmask |= cbit[k];
+    }
+    // every hyperplane's side is computed (m is uniform, so no lane waits on another);
+    // those of the vertex are dropped
+    uint32_t niv = 0;
+    for (int ac = 0; ac < m; ac++)
+        if ((dev_dot<D>(A + ac * D, v) - bp[ac]) < 0) niv |= (1u << ac);
+    niv &= ~cmask;
+    for (int s = 0; s < (1 << D); s++) {
+        uint32_t sv = niv;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            if ((s >> k) & 1) sv |= cbit[k];
+        atomicOr(&bits[sv >> 5], 1u << (sv & 31));
+    }
+}
+
 // Cells of the time-propagated arrangement by vertex enumeration,
 // cell_enumeration.hpp:676-855.  One block per parent; the sign vectors seen are
 // an LDS bitset of 2^m bits, so the output comes out sorted and de-duplicated.
 // The parent's g-table keys are a second LDS bitset, `pofs` words after the first,
 // or, when `psearch` is set (both sets would not fit in LDS: 2^20 + 2^18 bits), are
 // looked up by binary search in the parent's sorted keys in global memory.
+//
+// A vertex is kept when the one-norm condition number of its D hyperplanes is at most
+// kCondEps.  The block decides that for T = blockDim.x combinations at a time, cheapest
+// test first, each test exact against the reference's value:
+//   * the LU factorisation fails (a pivot under kPluEps): the reference's value is the
+//     largest double, rejected;
+//   * the upper bound of dev_cond1 is under kCondScreen: kept;
+//   * ||A||_1 / |u_DD| > kCondEps, u_DD the last pivot: rejected.  The reference's inverse
+//     has the entry fl(1 / u_DD) (the solve of the unit vector that the pivots move to the
+//     last row: forward substitution leaves it a unit vector, back substitution divides
+//     it by u_DD), its norm is a sum of absolute values no smaller than that entry, and
+//     rounding is monotone, so the reference's product ||A||_1 ||A^-1||_1 is at least this
+//     product;
+//   * otherwise the reference's value is computed (dev_cond1).
+// On the window workloads about 28% of the combinations fail to factor, 63% pass the
+// bound, 8% are rejected by the last pivot and 2% need the full value.  The kept vertices
+// (with their solution) and the undecided combinations go to two LDS queues, which are
+// worked off T at a time, so that the vertex work and the full condition number run on
+// full wavefronts rather than on the lanes that one combination in each wavefront keeps.
 template <int D>
 __device__ __forceinline__ void k_btab_enum(MG<256, 1> mg, TermPool P, const int* __restrict__ list,
                                                    const double* __restrict__ bpert, const uint8_t* __restrict__ combos,
@@ -384,8 +460,15 @@ __device__ __forceinline__ void k_btab_enum(MG<256, 1> mg, TermPool P, const int
                                                    const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ Barena,
                                                    const int* __restrict__ btcap, int* __restrict__ err, int pofs,
                                                    int psearch) {
-    extern __shared__ uint32_t bits[];
+    extern __shared__ double enum_lds[];
     __shared__ int scan_buf[256];
+    __shared__ int wsum[4];
+    const int T = (int)blockDim.x, Q = 2 * T;
+    // queues: kept vertices (combination, solution) and undecided combinations, 2T each
+    double* qv = enum_lds;                    // [D][Q]
+    int* qa = (int*)(qv + (size_t)D * Q);     // [Q]
+    int* qs = qa + Q;                         // [Q]
+    uint32_t* bits = (uint32_t*)(qs + Q);
     const int i = list[mg.bx];
     const int MM = P.MM;
     const int m = P.m[i];
@@ -412,38 +495,103 @@ __device__ __forceinline__ void k_btab_enum(MG<256, 1> mg, TermPool P, const int
     const double* bp = bpert + (int64_t)P.tid[i] * MM;
     const uint8_t* cmb = combos + combo_off[m];
     const int nc = combo_cnt[m];
-    for (int ci = threadIdx.x; ci < nc; ci += blockDim.x) {
+    auto load = [&](int ci, double* Ac, double* bc) {
         const uint8_t* cb = cmb + ci * D;
-        double Ac[D * D], bc[D], v[D];
-        int piv[D];
 #pragma unroll
         for (int j = 0; j < D; j++) {
+            const int r = cb[j];
 #pragma unroll
-            for (int k = 0; k < D; k++) Ac[j * D + k] = A[cb[j] * D + k];
-            bc[j] = bp[cb[j]];
+            for (int k = 0; k < D; k++) Ac[j * D + k] = A[r * D + k];
+            bc[j] = bp[r];
         }
-        if (dev_cond1<D>(Ac, piv) > kCondEps) continue;
-        dev_lu_solve<D>(Ac, piv, bc, v);
-        uint32_t niv = 0;
-        int cj = 0;
-        for (int ac = 0; ac < m; ac++) {
-            if (cj < D && cb[cj] == ac) {
-                cj++;
-                continue;
+    };
+    // appends this thread's kept vertex and / or undecided combination (block-wide call)
+    int na = 0, ns = 0;
+    auto push = [&](bool keep, bool slow, int ci, const double* v) {
+        int tot;
+        const int pre = block4_excl_scan((keep ? 1 : 0) | (slow ? 1 << 16 : 0), &tot, wsum);
+        if (keep) {
+            const int q = na + (pre & 0xffff);
+            qa[q] = ci;
+#pragma unroll
+            for (int k = 0; k < D; k++) qv[k * Q + q] = v[k];
+        }
+        if (slow) qs[ns + (pre >> 16)] = ci;
+        na += tot & 0xffff;
+        ns += tot >> 16;
+        __syncthreads();
+    };
+    // the cells around the last `cnt` kept vertices (block-wide call)
+    auto drain = [&](int cnt) {
+        const int q = na - cnt + (int)threadIdx.x;
+        if ((int)threadIdx.x < cnt) {
+            double v[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) v[k] = qv[k * Q + q];
+            enum_vertex_cells<D>(A, bp, m, cmb + qa[q] * D, v, bits);
+        }
+        na -= cnt;
+        __syncthreads();
+    };
+    // the reference's condition number of the last `cnt` undecided combinations
+    auto settle = [&](int cnt) {
+        const int q = ns - cnt + (int)threadIdx.x;
+        bool keep = false;
+        int ci = 0;
+        double v[D];
+        if ((int)threadIdx.x < cnt) {
+            ci = qs[q];
+            double Ac[D * D], bc[D];
+            int piv[D];
+            load(ci, Ac, bc);
+            if (dev_cond1<D>(Ac, piv) <= kCondEps) {
+                dev_lu_solve<D>(Ac, piv, bc, v);
+                keep = true;
             }
-            if ((dev_dot<D>(A + ac * D, v) - bp[ac]) < 0) niv |= (1u << ac);
         }
-        uint32_t cbit[D];
+        ns -= cnt;
+        push(keep, false, ci, v);
+    };
+    for (int base = 0; base < nc; base += T) {
+        const int ci = base + (int)threadIdx.x;
+        bool keep = false, slow = false;
+        double v[D];
+        if (ci < nc) {
+            double Ac[D * D], bc[D];
+            int piv[D];
+            load(ci, Ac, bc);
+            const double na1 = dev_one_norm<D>(Ac);
+            double rn = 0;
 #pragma unroll
-        for (int k = 0; k < D; k++) cbit[k] = 1u << cb[k];
-        for (int s = 0; s < (1 << D); s++) {
-            uint32_t sv = niv;
+            for (int r = 0; r < D; r++) {
+                double sr = 0;
 #pragma unroll
-            for (int k = 0; k < D; k++)
-                if ((s >> k) & 1) sv |= cbit[k];
-            atomicOr(&bits[sv >> 5], 1u << (sv & 31));
+                for (int c = 0; c < D; c++) sr += fabs(Ac[r * D + c]);
+                rn = fmax(rn, sr);
+            }
+            if (dev_lu<D>(Ac, piv)) {
+                double det = 1.0, bnd = na1 * (double)D;
+#pragma unroll
+                for (int k = 0; k < D; k++) det *= fabs(Ac[k * D + k]);
+#pragma unroll
+                for (int k = 0; k < D - 1; k++) bnd *= rn;
+                if (bnd <= kCondScreen * det) {
+                    dev_lu_solve<D>(Ac, piv, bc, v);
+                    keep = true;
+                } else if (!(na1 * (1.0 / fabs(Ac[D * D - 1])) > kCondEps)) {
+                    slow = true;
+                }
+            }
+        }
+        push(keep, slow, ci, v);
+        if (na >= T) drain(T);
+        if (ns >= T) {
+            settle(T);
+            if (na >= T) drain(T);
         }
     }
+    if (ns > 0) settle(ns);
+    while (na > 0) drain(min(na, T));
     __syncthreads();
     // keep sign vectors whose parent part is a cell of the parent arrangement
     const uint32_t top_phc = 1u << (phc - 1), rev_phc = (phc >= 32) ? 0xffffffffu : ((1u << phc) - 1u);
@@ -2571,17 +2719,6 @@ __device__ __forceinline__ void k_scan_total_mapped(MG<1024, 1> mg, const A* __r
 // is the same as any other summation order's.
 constexpr int kSmallScan = 65536;
 constexpr int kSmallScanThreads = 1024;
-// Inclusive scan of one value per lane across a wavefront (shuffles, no barrier).
-template <class T>
-__device__ __forceinline__ T wave_incl_scan(T x) {
-    const int lane = (int)(threadIdx.x & (kWave - 1));
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const T y = __shfl_up(x, o, kWave);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
 
 // Exclusive prefix of each thread's `v` over a block of kSmallScanThreads threads, and the
 // block total: wavefront scans, then one wavefront scans the wavefront totals (two
