@@ -2,7 +2,7 @@
 
 Run in the container that holds the reference tree (after `make -C oracle ref`):
 
-    python tests/golden/make_window_manager_golden.py
+    python tests/golden/make_window_manager_golden.py [fixture names]
 
 The reference's C++ window manager (SlidingWindowManager, include/cauchy_windows.hpp,
 behind pycauchy_initialize_{lti,ltv}_window_manager / pycauchy_step / pycauchy_shutdown,
@@ -80,6 +80,8 @@ CASES = [
      [0.01, 0.02, 0.03]),
     ("wm_ltv4_p2", "ltv", lambda: W.ltv_system(d=4, steps=10, seed_offset=31, p=2, pncc=2, name="wm_ltv4"), 3, 10,
      4321, None),
+    # the 5-state LTV system (two process noises, two measurements per step) with 5 windows
+    ("wm_ltv5_p2", "ltv", lambda: W.config1(steps=10), 5, 10, W.config1(steps=10).srand_seed, None),
 ]
 
 
@@ -87,7 +89,10 @@ def main():
     if not os.path.exists(REF_LIB):
         raise SystemExit("build the reference first: make -C oracle ref")
     ref = pycauchy.bind(ct.CDLL(REF_LIB), sym=lambda name: "refpy_" + name[len("pycauchy_"):], name="pycauchy_ref")
+    only = set(sys.argv[1:])  # fixture names to write (all when none are given)
     for fname, mode, make, nw, steps, seed, boost in CASES:
+        if only and fname not in only:
+            continue
         s = make()
         out = run_manager(ref, mode, s, nw, steps, seed, boost)
         rec = {"system": s.to_dict(), "mode": mode, "num_windows": nw, "steps": steps, "srand": seed,

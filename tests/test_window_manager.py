@@ -6,7 +6,8 @@ the reporting window and its error bits are identical, and fz, x_hat, P_hat and 
 imaginary-part bounds agree within 1e-9 (relative to the largest magnitude of each output).
 Cases: a 3-state LTI bank of 4 windows over 16 measurements; a 3-state bank of 3 windows
 with a window-variance boost; a 4-state LTV bank (two process noises, two measurements per
-step, Phi_k through the dynamics callback) of 3 windows over 10 steps.  The bank steps the
+step, Phi_k through the dynamics callback) of 3 windows over 10 steps; the 5-state LTV system
+(two process noises, two measurements per step) in a bank of 5 windows over 10 steps.  The bank steps the
 windows of a measurement together (merged launches) by default; stepped one window at a time,
 or one host thread per window, it must give the same bits."""
 import ctypes as ct
@@ -65,7 +66,7 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["wm_lti3", "wm_lti3_boost", "wm_ltv4_p2"])
+@pytest.mark.parametrize("name", ["wm_lti3", "wm_lti3_boost", "wm_ltv4_p2", "wm_ltv5_p2"])
 def test_window_manager_matches_reference(name):
     rec = load_golden(name)
     print(f"case {name}: {rec['num_windows']} windows, {rec['steps']} measurements", flush=True)
@@ -85,7 +86,7 @@ def test_window_manager_matches_reference(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", [1, 2], ids=["merged", "threaded"])
-@pytest.mark.parametrize("name", ["wm_lti3", "wm_lti3_boost", "wm_ltv4_p2"])
+@pytest.mark.parametrize("name", ["wm_lti3", "wm_lti3_boost", "wm_ltv4_p2", "wm_ltv5_p2"])
 def test_window_manager_modes_equal_one_window_at_a_time(name, mode):
     """The windows' updates as merged launches, or on a host thread each, give each window
     the bits of its own updates run one window after another."""
