@@ -1566,7 +1566,24 @@ struct Engine {
         MCE_DISPATCH_D(d, MCE_RUN(k_tp<DD>, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, dconst.ptr,
                                              dcen_half.ptr, slots.ptr, btcap.ptr, counters.ptr + 8));
         HIPCHK(hipGetLastError());
-        const int nslots = co_await scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
+        // under a batch scheduler the slot and B-table scans (both of k_tp's counts) go out
+        // together and their totals come back in one read
+        const bool need_bt = !skip && !first;
+        const bool paired = bs && np <= kSmallScan && need_bt;
+        int nslots;
+        int64_t bt_total = 0;
+        if (paired) {
+            MCE_RUN((k_small_scan<int, int, true>), dim3(1), dim3(kSmallScanThreads), 0, st, slots.ptr,
+                    slot_base.ptr + 1, np, (int*)nullptr, d_scal.ptr + 1, (int64_t*)nullptr);
+            MCE_RUN((k_small_scan<int, int64_t, false>), dim3(1), dim3(kSmallScanThreads), 0, st, btcap.ptr,
+                    bt_base.ptr, np, btcap64.ptr, d_scal.ptr + 2, (int64_t*)nullptr);
+            int64_t tt[2];
+            co_await rd(d_scal.ptr + 1, tt, 2);
+            nslots = (int)tt[0];
+            bt_total = tt[1];
+        } else {
+            nslots = co_await scan_i32_incl_total(slots.ptr, slot_base.ptr + 1, np);
+        }
         if (build_rank) {
             // each parent's rank structure once, before its children copy its offset
             int wmax_;
@@ -1578,9 +1595,10 @@ struct Engine {
                     rank_arena.ptr, wmax_);
         }
         prof.mark(kPhTp, st);
-        if (!skip && !first) {
+        if (need_bt) {
             // B arena offsets (64-bit)
-            barena.reserve((size_t)(co_await widen_scan_total(btcap.ptr, btcap64.ptr, bt_base.ptr, np)) + 1);
+            if (!paired) bt_total = co_await widen_scan_total(btcap.ptr, btcap64.ptr, bt_base.ptr, np);
+            barena.reserve((size_t)bt_total + 1);
             enum_list.reserve(np);
             MCE_RUN(k_btab_simple, dim3(ceil_div(np, 256)), dim3(256), 0, st, IN, np, bt_base.ptr,
                                gk[gcur].ptr, barena.ptr, enum_list.ptr, counters.ptr + 16);

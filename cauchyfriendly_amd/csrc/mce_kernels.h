@@ -2741,32 +2741,59 @@ __device__ __forceinline__ T block_excl_scan(T v, T* total) {
     return (w > 0 ? wsum[w - 1] : (T)0) + x - v;
 }
 
+// Items per thread of one tile of the small scan: tiles of kSmallScanThreads x kScanItems
+// items are loaded coalesced (consecutive threads, consecutive items) into LDS, each thread
+// scans its kScanItems consecutive items there, and the results go back out coalesced.
+constexpr int kScanItems = 4;
+
 template <class In, class Out, bool Inclusive>
 __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const In* __restrict__ in,
                                              Out* __restrict__ out, int n, Out* __restrict__ wide,
                                              int64_t* __restrict__ out_total, int64_t* host_total) {
     (void)mg;
+    constexpr int kTile = kSmallScanThreads * kScanItems;
+    __shared__ Out tile[kTile];
     const int t = threadIdx.x;
-    const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
-    const int b = min(n, t * per), e = min(n, b + per);
-    Out sum = 0;
-    for (int i = b; i < e; i++) sum += (Out)in[i];
-    Out tot;
-    Out run = block_excl_scan(sum, &tot);
-    for (int i = b; i < e; i++) {
-        const Out x = (Out)in[i];
-        if (wide) wide[i] = x;
-        if (Inclusive) {
-            run += x;
-            out[i] = run;
-        } else {
-            out[i] = run;
-            run += x;
+    Out carry = 0;
+    for (int base = 0; base < n; base += kTile) {
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) {
+            const int i = base + j * kSmallScanThreads + t;
+            const Out x = i < n ? (Out)in[i] : (Out)0;
+            if (wide && i < n) wide[i] = x;
+            tile[j * kSmallScanThreads + t] = x;
         }
+        __syncthreads();
+        Out v[kScanItems], sum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) {
+            v[j] = tile[t * kScanItems + j];
+            sum += v[j];
+        }
+        Out tot;
+        Out run = block_excl_scan(sum, &tot) + carry;
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) {
+            if (Inclusive) {
+                run += v[j];
+                tile[t * kScanItems + j] = run;
+            } else {
+                tile[t * kScanItems + j] = run;
+                run += v[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) {
+            const int i = base + j * kSmallScanThreads + t;
+            if (i < n) out[i] = tile[j * kSmallScanThreads + t];
+        }
+        carry += tot;
+        __syncthreads();  // the tile is reloaded
     }
     if (t == kSmallScanThreads - 1) {
-        if (out_total) out_total[0] = (int64_t)tot;
-        if (host_total) *(volatile int64_t*)host_total = (int64_t)tot;
+        if (out_total) out_total[0] = (int64_t)carry;
+        if (host_total) *(volatile int64_t*)host_total = (int64_t)carry;
     }
 }
 
