@@ -274,10 +274,12 @@ class PyCauchyEstimator:
         else:
             xs = self._x[-n:].copy()
             Ps = self._P[-n * n:].reshape(n, n).copy()
+        # the history holds views of this step's outputs: every step's outputs are new
+        # arrays, and nothing writes into them
         mi = self.moment_info
         mi["fz"].append(self._fz[-1])
-        mi["x"].append(self._x[-n:].copy())
-        mi["P"].append(self._P[-n * n:].reshape(n, n).copy())
+        mi["x"].append(self._x[-n:])
+        mi["P"].append(self._P[-n * n:].reshape(n, n))
         mi["cerr_x"].append(self._cerr_x[-1])
         mi["cerr_P"].append(self._cerr_P[-1])
         mi["cerr_fz"].append(self._cerr_fz[-1])
@@ -479,9 +481,9 @@ def step_estimators(estimators, msmts, controls=None, full_info=False, num_threa
             raise RuntimeError("estimator is not initialized")
         if e.step_count == e.num_steps:
             raise RuntimeError(f"an estimator has already processed its {e.num_steps} steps; reset it first")
-        z_, u_ = e._checked_inputs(z, u)
-        e._msmts = z_.copy()
-        e._controls = u_.copy()
+        z_, u_ = e._checked_inputs(z, u)  # new arrays, read only by the step
+        e._msmts = z_
+        e._controls = u_
         zs.append(z_)
         us.append(u_)
     handles = [e.py_handle for e in ests]
