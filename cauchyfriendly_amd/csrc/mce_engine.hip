@@ -539,6 +539,8 @@ struct Profiler {
     long long geval_launches = 0;
     double geval_bytes = 0;
     double geval_ms = 0;
+    double alg_bytes = 0;   // algorithmic bytes of the profiled updates (whole update)
+    double alg_updates = 0; // updates profiled
     double groups = 0;      // reduction groups evaluated
     double members = 0;     // members in those groups (terms evaluated = groups + members)
     double max_group = 0;   // largest group (root + members)
@@ -1454,6 +1456,21 @@ struct Engine {
                 printf("%.16lf + %.16lfj%s", cov[i * d + j].re, cov[i * d + j].im, j + 1 < d ? ", " : "\n");
     }
 
+    // ------------------------------------------------------------ algorithmic bytes
+    // Bytes an update must move at least, counted from its parity-pinned term and cell
+    // counts (profiling only): every parent's record (A, p, q, b, c, d) and g-table read
+    // once; every child written by MU/MUC and read by the reduction; every surviving
+    // g-table written (20 B per cell) with its B-table (4 B); every merged member's table
+    // read and added (24 B per cell).
+    double term_record_bytes(int m) const { return 8.0 * (m * (d + 2) + d + 2); }
+    double prof_device_sum(int n, const int* a, const int* b, const int* keep) {
+        if (n <= 0) return 0;
+        bytes_acc.reserve(1);
+        fill_multi({{bytes_acc.ptr, sizeof(unsigned long long), 0}});
+        MCE_RUN(k_prof_sum, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, a, b, keep, bytes_acc.ptr);
+        return (double)read1(bytes_acc.ptr);
+    }
+
     // ------------------------------------------------------------ one scalar measurement update
     int update(double msmt, const double* Phi, const double* Gamma, const double* beta, const double* Hrow,
                double gamma, const double* Bm, const double* u) {
@@ -1543,6 +1560,14 @@ struct Engine {
         }
         const int np = Nt;
         prof.begin(st);
+        if (prof.on && !bs) {
+            // the parents' records and g-tables
+            double b = 0;
+            for (int m = 0; m < sr; m++) b += in_count[m] * term_record_bytes(m);
+            b += 24.0 * prof_device_sum(np, in.view().gp_n, nullptr, nullptr);
+            prof.alg_bytes += b;
+            prof.alg_updates += 1;
+        }
 
         // ---- time propagation and B-tables of the parents
         slots.reserve(np);
@@ -1713,6 +1738,8 @@ struct Engine {
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> h(sr + 1);
             co_await rd(hist.ptr, h.data(), sr + 1);
+            if (prof.on && !bs)
+                for (int m = 0; m < sr; m++) prof.alg_bytes += h[m] * (2 * term_record_bytes(m) + 2.0 * m);
             Nt = 0;
             for (int m = 0; m < sr; m++) {
                 diag_cnt[4 + m] = h[m];
@@ -1732,6 +1759,8 @@ struct Engine {
                                out.flags.ptr, out.m.ptr, (const int*)nullptr, (const uint32_t*)nullptr, hist.ptr);
             std::vector<int> cnt(sr + 1);
             co_await rd(hist.ptr, cnt.data(), sr + 1);
+            if (prof.on && !bs)
+                for (int m = 0; m < sr; m++) prof.alg_bytes += cnt[m] * (2 * term_record_bytes(m) + 2.0 * m);
             prof.mark(kPhGather, st);
             int nt_mu = 0;
             for (int m = 0; m < sr; m++) {
@@ -1784,6 +1813,7 @@ struct Engine {
     // g-tables of the first update's d+1 terms, which become the parents.
     Co<void> first_gtables(int n) {
         const int cells = 1 << (d - 1);
+        if (prof.on && !bs) prof.alg_bytes += 24.0 * n * cells;
         const int nx = gcur ^ 1;
         gk[nx].reserve((size_t)n * cells);
         gv[nx].reserve((size_t)n * cells);
@@ -1846,6 +1876,7 @@ struct Engine {
             HIPCHK(hipGetLastError());
         }
         const int64_t cells_total = co_await widen_scan_total(b2n.ptr, b2n64.ptr, goff.ptr, ng);
+        if (prof.on && !bs) prof.alg_bytes += 24.0 * (double)cells_total;
         prof.mark(kPhGBtab, st);
 
         // ---- group g-tables need the update's g scale 1/(2 pi Re fz)
@@ -1937,6 +1968,7 @@ struct Engine {
             MCE_RUN(k_geval_bytes, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, d, grp_m.ptr, grp_mcnt.ptr, b2n.ptr,
                     surv.ptr, defer_k0.ptr, ge_mode >= kLookupRank ? big_group_members : INT_MAX, bs->geval_bytes);
         if (ge_mode >= kLookupRank) co_await big_groups(ng, ge_mode, ge_lds, ge_wmax, gscale, nx);
+        if (prof.on && !bs) prof.alg_bytes += 24.0 * prof_device_sum(ng, grp_mcnt.ptr, b2n.ptr, surv.ptr);
         int nr_[2];
         co_await rd(counters.ptr + 18, nr_, 2);
         const int nsort = nr_[0];
@@ -4115,6 +4147,8 @@ void mce_profile_read(void* handle, double* out) {
     out[kNumPhases + 4] = p.groups;
     out[kNumPhases + 5] = p.members;
     out[kNumPhases + 6] = p.max_group;
+    out[kNumPhases + 7] = p.alg_bytes;
+    out[kNumPhases + 8] = p.alg_updates;
 }
 
 void mce_last_fz_raw(void* handle, double* out2) {
@@ -4154,6 +4188,8 @@ void mce_profile_reset(void* handle) {
     p.max_group = 0;
     p.geval_bytes = 0;
     p.geval_ms = 0;
+    p.alg_bytes = 0;
+    p.alg_updates = 0;
 }
 
 const char* mce_last_error(void) { return g_last_error.c_str(); }

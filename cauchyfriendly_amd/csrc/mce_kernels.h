@@ -234,6 +234,15 @@ __device__ inline int wave_excl_scan(int v, int* total) {
 
 __device__ inline bool wave_all(bool pred) { return __all(pred); }
 
+// Orders the LDS accesses of the lanes of one wavefront (for workgroups of one wavefront):
+// a wavefront's LDS operations complete in issue order, so only the compiler has to be
+// held back, and the wavefront does not wait for its global loads and stores as a
+// workgroup barrier makes it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Inclusive scan of one value per lane across a wavefront (shuffles, no barrier).
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T x) {
@@ -2082,7 +2091,7 @@ __device__ inline double dev_psum2(const TermPool& T, int64_t s, int m) {
     return ps * ps;
 }
 
-// Stages term s for evaluation (one wavefront, blockDim == 64).  `lds` is the
+// Stages term s for evaluation (a workgroup of one wavefront, blockDim == 64).  `lds` is the
 // kernel's dynamic LDS: parent keys (kLookupSearchLds), or the rank structure of
 // `wmax` words of bits and `wmax` prefix counts, then `wmax` words of the group's
 // own key bitset (built by the kernel), then the parent values.
@@ -2109,7 +2118,7 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
     T.dv = O.dv[s];
     T.gscale = gscale;
     T.klim = 1u << (T.phc - 1);
-    __syncthreads();
+    wave_lds_sync();
     const int lane = threadIdx.x;
     if (MODE == kLookupSearchLds) {
         const uint32_t* src = gk_prev + go;
@@ -2145,7 +2154,7 @@ __device__ inline EvalTerm stage_term(const TermPool& O, int64_t s, int m, const
         cm_s[lane] = O.cmap[s * MM + lane];
         cs_s[lane] = O.csmap[s * MM + lane];
     }
-    __syncthreads();
+    wave_lds_sync();
     return T;
 }
 
@@ -2254,12 +2263,12 @@ __device__ __forceinline__ void k_group_eval(MG<64, 1> mg, const TermPool OUT, i
         uint32_t* bset = lds_dyn + 2 * wmax;
         const int WB = (int)((top + 31u) >> 5);
         for (int w = lane; w < WB; w += kWave) bset[w] = 0;
-        __syncthreads();
+        wave_lds_sync();
         for (int i = lane; i < n; i += kWave) {
             const uint32_t key = B[i];
             if (key < top) atomicOr(&bset[key >> 5], 1u << (key & 31));
         }
-        // made visible by the barriers of stage_term
+        // ordered before its reads by the LDS syncs of stage_term
     }
     {
         EvalTerm T = stage_term<MODE, KM>(OUT, r, m, gk_prev, gv_prev, rank, gscale, lds_dyn, wmax, qr, cm_s, cs_s, rare);
@@ -2677,6 +2686,18 @@ __device__ __forceinline__ void k_geval_bytes(MG<1024, 1> mg, int ng, int d, con
         v = 24ull * n + evals * (40ull * n + rec);
     }
     // one atomic per wavefront: the counter is shared by every group of the launch
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd(acc, v);
+}
+
+// acc += sum over i < n of a[i] * (b ? b[i] : 1), over the i with keep[i] >= 0 when keep is
+// given: the cell counts of the update's byte accounting (profiling only).
+__device__ __forceinline__ void k_prof_sum(MG<1024, 1> mg, int n, const int* __restrict__ a, const int* __restrict__ b,
+                                           const int* __restrict__ keep, unsigned long long* __restrict__ acc) {
+    const int i = mg.bx * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (i < n && (!keep || keep[i] >= 0))
+        v = (unsigned long long)max(a[i], 0) * (unsigned long long)(b ? max(b[i], 0) : 1);
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
     if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd(acc, v);
 }

@@ -138,7 +138,7 @@ class GpuEstimator:
             L.mce_profile_enable(self.h, int(bool(enable)))
         if reset:
             L.mce_profile_reset(self.h)
-        out = np.zeros(16)
+        out = np.zeros(18)
         L.mce_profile_read(self.h, out.ctypes.data_as(_dp))
         return out
 

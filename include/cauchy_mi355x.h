@@ -382,10 +382,13 @@ void mce_get_random_draws(void* handle, double* root_point, double* b_pert_8xmax
  * reduction, group B-tables, group g-tables, compaction], out[9] their total, out[10] the
  * launch count of the group g-table kernel, out[11] its algorithmic bytes, out[12] its
  * kernel time, out[13] reduction groups evaluated, out[14] their members, out[15] the
- * largest group (root + members).  Recorded only after mce_profile_enable(h, 1); out must
- * hold 16 doubles. */
+ * largest group (root + members), out[16] the algorithmic bytes of the profiled updates
+ * (every parent's record and g-table read once, every child written and read, every
+ * surviving g-table and B-table written, every merged member's table read and added),
+ * out[17] the updates profiled.  Recorded only after mce_profile_enable(h, 1); out must
+ * hold 18 doubles. */
 void mce_profile_enable(void* handle, int on);
-void mce_profile_read(void* handle, double* out_ms12);
+void mce_profile_read(void* handle, double* out);
 void mce_profile_reset(void* handle);
 
 /* The last scalar update's fz (re, im) before the numeric checks, which replace fz by the
