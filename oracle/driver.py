@@ -238,3 +238,13 @@ def run_system(sysobj, num_steps=None, seed=None, collect_terms=False, kind="ref
                 r["terms"] = est.terms()
             recs.append(r)
     return est, recs
+
+
+def system_records(sysobj):
+    """The restatement over a System's whole record, one tuple per update: (terms after
+    MU+MUC, terms after reduction, error bits, public fz, public x_hat, public P_hat).  A
+    module-level function, so that process pools can run it."""
+    est, recs = run_system(sysobj, kind="oracle")
+    est.close()
+    return [(r["nt_mu"], r["nt_ftr"], r["err"], float(r["fz_public"].real), r["mean"].real.copy(),
+             r["cov"].real.reshape(-1).copy()) for r in recs]

@@ -168,3 +168,42 @@ def test_batch_rejects_mixed_modes_and_finished_estimators():
     with pytest.raises(RuntimeError):
         step_estimators([e], [s.zs[0]])
     e.shutdown()
+
+
+@pytest.mark.gpu
+def test_full_batch_of_512_matches_the_restatement_on_every_update():
+    """The bench's whole batch (512 estimators) with merged launches at the tuning table's
+    settings, against the restatement on 16 host processes: every update's term count after
+    reduction and error bits exact, fz, x_hat and P_hat within 1e-9."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from cauchyfriendly_amd import autotune
+    from cauchyfriendly_amd.cauchy_estimator import step_estimators
+    ss = W.batch_systems(count=512)
+    print("full batch: 512 estimators, merged launches", flush=True)
+    with ProcessPoolExecutor(max_workers=16, mp_context=mp.get_context("spawn")) as ex:
+        futs = [ex.submit(dr.system_records, s) for s in ss]
+        ests = [_make(s) for s in ss]
+        st = autotune.load_table("batch512") or dict(autotune.DEFAULTS)
+        autotune.set_shapes(ests, st["enum_threads"], st["big_group_members"], st["geval_mode"])
+        got = [[] for _ in ss]
+        for k in range(max(s.num_steps for s in ss)):
+            live = [i for i, s in enumerate(ss) if k < s.num_steps]
+            step_estimators([ests[i] for i in live], [ss[i].zs[k] for i in live], merged_groups=st["groups"])
+            for i in live:
+                got[i].append(_record(ests[i]))
+        for e in ests:
+            e.shutdown()
+        want = [f.result(timeout=600) for f in futs]
+    worst = 0.0
+    for i, s in enumerate(ss):
+        assert len(got[i]) == len(want[i]) == s.num_steps, f"{s.name}: update count"
+        for k, ((fz, x, P, err, nt), (_, nt_ftr, oerr, ofz, ox, oP)) in enumerate(zip(got[i], want[i])):
+            where = f"{s.name} update {k}"
+            assert nt == nt_ftr, f"{where}: terms {nt} != restatement {nt_ftr}"
+            assert int(err[-1]) == oerr, f"{where}: error bits {int(err[-1])} != {oerr}"
+            for a, b, what in ((np.array([fz[-1]]), np.array([ofz]), "fz"), (x, ox, "x_hat"), (P, oP, "P_hat")):
+                r = rel_err(a, b)
+                worst = max(worst, r)
+                assert r <= TOL, f"{where}: {what} rel err {r:.2e}"
+    print(f"  512 estimators, worst relative error {worst:.2e}", flush=True)
