@@ -76,11 +76,11 @@ def test_tangent_likelihood_equals_forward_and_restatement():
 
 
 def test_tangent_gradient_matches_central_differences_on_lti_windows():
-    """All 14 components on 16 of the 3-state windows against the GPU's central differences
-    (components whose structure holds at theta +/- h, or at a smaller step)."""
+    """All 14 components on all 64 of the 3-state windows against the GPU's central
+    differences (components whose structure holds at theta +/- h, or at a smaller step)."""
     from cauchyfriendly_amd.likelihood import loglik_gradient, tangent_gradients
-    ss = W.lti_sequences(count=16)
-    print("case 16 windows x 14 parameters", flush=True)
+    ss = W.lti_sequences(count=64)
+    print("case 64 windows x 14 parameters", flush=True)
     Lt, Gt = tangent_gradients(ss, num_threads=8)
     d = loglik_gradient(ss, h=H_STEP, num_threads=8, details=True)
     assert np.array_equal(Lt, d["L"])
