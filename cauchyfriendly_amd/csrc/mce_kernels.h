@@ -2762,9 +2762,10 @@ __device__ __forceinline__ T block_excl_scan(T v, T* total) {
     return (w > 0 ? wsum[w - 1] : (T)0) + x - v;
 }
 
-// Items per thread of one tile of the small scan: tiles of kSmallScanThreads x kScanItems
-// items are loaded coalesced (consecutive threads, consecutive items) into LDS, each thread
-// scans its kScanItems consecutive items there, and the results go back out coalesced.
+// Items per thread of one tile of the small scan.  Above one tile, tiles of
+// kSmallScanThreads x kScanItems items are loaded coalesced (consecutive threads,
+// consecutive items) into LDS, each thread scans its kScanItems consecutive items there,
+// and the results go back out coalesced.
 constexpr int kScanItems = 4;
 
 template <class In, class Out, bool Inclusive>
@@ -2775,6 +2776,32 @@ __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const 
     constexpr int kTile = kSmallScanThreads * kScanItems;
     __shared__ Out tile[kTile];
     const int t = threadIdx.x;
+    if (n <= kTile) {
+        // one tile or less: each thread scans at most kScanItems consecutive items in place,
+        // which costs fewer barriers than staging them
+        const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
+        const int b = min(n, t * per), e = min(n, b + per);
+        Out sum = 0;
+        for (int i = b; i < e; i++) sum += (Out)in[i];
+        Out tot;
+        Out run = block_excl_scan(sum, &tot);
+        for (int i = b; i < e; i++) {
+            const Out x = (Out)in[i];
+            if (wide) wide[i] = x;
+            if (Inclusive) {
+                run += x;
+                out[i] = run;
+            } else {
+                out[i] = run;
+                run += x;
+            }
+        }
+        if (t == kSmallScanThreads - 1) {
+            if (out_total) out_total[0] = (int64_t)tot;
+            if (host_total) *(volatile int64_t*)host_total = (int64_t)tot;
+        }
+        return;
+    }
     Out carry = 0;
     for (int base = 0; base < n; base += kTile) {
 #pragma unroll
