@@ -463,7 +463,7 @@ __device__ __forceinline__ void enum_vertex_cells(const double* __restrict__ A, 
 // worked off T at a time, so that the vertex work and the full condition number run on
 // full wavefronts rather than on the lanes that one combination in each wavefront keeps.
 template <int D>
-__device__ __forceinline__ void k_btab_enum(MG<256, 1> mg, TermPool P, const int* __restrict__ list,
+__device__ __forceinline__ void k_btab_enum(MG<256, 3> mg, TermPool P, const int* __restrict__ list,
                                                    const double* __restrict__ bpert, const uint8_t* __restrict__ combos,
                                                    const int64_t* __restrict__ combo_off, const int* __restrict__ combo_cnt,
                                                    const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ Barena,
