@@ -992,7 +992,7 @@ __device__ inline double dev_child_row(const double* A, const double* p, const d
 // are kept, for the coalignment filter.  When the filter finds a possible merge (a
 // rare case) the rows are rebuilt in memory and coaligned there, as k_mu does.
 template <int D, int KM>
-__device__ __forceinline__ void k_mu_r(MG<256, 4> mg, TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
+__device__ __forceinline__ void k_mu_r(MG<256, 3> mg, TermPool IN, int np, const int* __restrict__ slot_base, TermPool OUT,
                                              int nslots, const StepConst* __restrict__ sc,
                                              const uint32_t* __restrict__ gk_prev, const cplx* __restrict__ gv_prev,
                                              double* __restrict__ gy, uint32_t* __restrict__ keys_out,
