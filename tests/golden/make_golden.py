@@ -87,7 +87,7 @@ CASES = [
     # the largest state dimensions the engine takes (6 and its maximum, 8)
     (lambda: W.lti_system(d=6, steps=6, seed_offset=106, name="lti6_w6"), None, (1,)),
     (lambda: W.lti_system(d=8, steps=5, seed_offset=108, name="lti8_w5"), None, (0,)),
-    # the orbit window at the 1 s step of the largest survey configuration: a window of 6
+    # the orbit window at the 1 s step of the largest configuration (7-state orbit): a window of 6
     # steps (largest shape 12), stepped for its first 4 (8 updates, 750k terms at the last;
     # the reference's own covariance check fails from the 5th update on)
     (lambda: _named(W.leo7_windows(count=1, window=6, dt=1.0)[0], "leo7_dt1_w0"), 6, (), 4),
