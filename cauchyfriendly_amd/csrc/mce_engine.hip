@@ -885,13 +885,14 @@ struct Engine {
         }
         HIPCHK(f(tmp, bytes, st));
     }
-    // Stable sort of n (key, value) pairs by key bits [b0, b1): the library radix sort, or one
-    // workgroup per estimator in a merged launch for small inputs under a batch scheduler
-    // (the same order: both are stable sorts over the same bits).
+    // Stable sort of n (key, value) pairs by key bits [b0, b1): one workgroup for small inputs
+    // (one launch, merged with other estimators' under a batch scheduler), the multi-block
+    // radix sort up to kRadixMax items under a batch scheduler, else the library radix sort
+    // (the same order: all are stable sorts over the same bits).
     template <class K, class V>
     void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, int n, int b0, int b1) {
         if (n <= 0) return;
-        if (bs && n <= kSmallSort) {
+        if (n <= kSmallSort) {
             MCE_RUN((k_small_sort<K, V>), dim3(1), dim3(kSmallSortThreads), 0, st, kin, kout, vin, vout, n, b0, b1);
             return;
         }
@@ -939,7 +940,7 @@ struct Engine {
     }
     // out <- in[i] where flags[i] != 0, in order; the count into *count.
     void select_flagged(const uint32_t* in_, const uint8_t* flags, uint32_t* out_, int* count, int n) {
-        if (bs && n <= kSmallScan) {
+        if (n <= kSmallScan) {
             MCE_RUN(k_small_select, dim3(1), dim3(kSmallScanThreads), 0, st, in_, flags, out_, count, n);
             return;
         }
@@ -949,7 +950,7 @@ struct Engine {
     }
     // *out <- max of n ints.
     void reduce_max(const int* in_, int* out_, int n) {
-        if (bs && n <= kSmallScan) {
+        if (n <= kSmallScan) {
             MCE_RUN(k_small_max, dim3(1), dim3(1024), 0, st, in_, out_, n);
             return;
         }
