@@ -1643,7 +1643,7 @@ struct Engine {
                 // combinations come first; with the largest sets, fewer lanes per parent
                 const int words = mt >= 5 ? (1 << (mt - 5)) : 1;
                 const int pwords = ph - 1 >= 5 ? (1 << (ph - 6)) : 1;
-                const size_t enum_static = (256 + 4) * sizeof(int);
+                const size_t enum_static = 4 * sizeof(int);
                 int threads = enum_threads;
                 auto queues = [&](int t) { return (size_t)2 * t * (8 * (size_t)d + 8); };
                 int psearch = 0;

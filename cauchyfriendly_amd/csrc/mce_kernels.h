@@ -470,7 +470,6 @@ __device__ __forceinline__ void k_btab_enum(MG<256, 3> mg, TermPool P, const int
                                                    const int* __restrict__ btcap, int* __restrict__ err, int pofs,
                                                    int psearch) {
     extern __shared__ double enum_lds[];
-    __shared__ int scan_buf[256];
     __shared__ int wsum[4];
     const int T = (int)blockDim.x, Q = 2 * T;
     // queues: kept vertices (combination, solution) and undecided combinations, 2T each
@@ -647,16 +646,8 @@ __device__ __forceinline__ void k_btab_enum(MG<256, 3> mg, TermPool P, const int
             if ((bits[pr >> 5] >> (pr & 31)) & 1u) cnt++;
         }
     }
-    scan_buf[threadIdx.x] = cnt;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        int v = (threadIdx.x >= (unsigned)o) ? scan_buf[threadIdx.x - o] : 0;
-        __syncthreads();
-        scan_buf[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int pos = scan_buf[threadIdx.x] - cnt;
-    const int total = scan_buf[blockDim.x - 1];
+    int total;
+    int pos = block4_excl_scan(cnt, &total, wsum);
     uint32_t* out = Barena + P.bt_off[i];
     const int cap = btcap[i];
     for (int w = w0; w < w1; w++) {
