@@ -24,6 +24,7 @@ sys.path.insert(0, REPO)
 
 from cauchyfriendly_amd import workloads as W  # noqa: E402
 from oracle import driver as dr  # noqa: E402
+from tests.golden.demo_systems import reference_demo_systems  # noqa: E402
 
 
 def cx(z):
@@ -110,7 +111,7 @@ def main(only=()):
     for sy in W.edge_systems():
         systems.append((lambda sy=sy: sy, None, (1, 3) if sy.name in ("horthog3", "scalar1", "tpcoalign3") else (),
                         None))
-    for sy in W.reference_demo_systems():
+    for sy in reference_demo_systems():
         systems.append((lambda sy=sy: sy, None, (2,) if sy.name == "ref_3state_3msmts_horth" else (), None))
     for factory, steps, tu, run_steps in systems:
         sysobj = factory()
