@@ -2769,22 +2769,30 @@ __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const 
     const int t = threadIdx.x;
     if (n <= kTile) {
         // one tile or less: each thread scans at most kScanItems consecutive items in place,
-        // which costs fewer barriers than staging them
+        // which costs fewer barriers than staging them.  The items are loaded once, all
+        // loads issued before the first is used.
         const int per = (n + kSmallScanThreads - 1) / kSmallScanThreads;
         const int b = min(n, t * per), e = min(n, b + per);
+        Out x[kScanItems];
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) x[j] = b + j < e ? (Out)in[b + j] : (Out)0;
         Out sum = 0;
-        for (int i = b; i < e; i++) sum += (Out)in[i];
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) sum += x[j];
         Out tot;
         Out run = block_excl_scan(sum, &tot);
-        for (int i = b; i < e; i++) {
-            const Out x = (Out)in[i];
-            if (wide) wide[i] = x;
-            if (Inclusive) {
-                run += x;
-                out[i] = run;
-            } else {
-                out[i] = run;
-                run += x;
+#pragma unroll
+        for (int j = 0; j < kScanItems; j++) {
+            const int i = b + j;
+            if (i < e) {
+                if (wide) wide[i] = x[j];
+                if (Inclusive) {
+                    run += x[j];
+                    out[i] = run;
+                } else {
+                    out[i] = run;
+                    run += x[j];
+                }
             }
         }
         if (t == kSmallScanThreads - 1) {
@@ -2793,14 +2801,28 @@ __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const 
         }
         return;
     }
+    // Tiles: the next tile's loads are issued before the current tile is scanned, so their
+    // latency overlaps the scan's barriers and stores.
+    Out nx[kScanItems];
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+        const int i = j * kSmallScanThreads + t;
+        nx[j] = i < n ? (Out)in[i] : (Out)0;
+    }
     Out carry = 0;
     for (int base = 0; base < n; base += kTile) {
 #pragma unroll
         for (int j = 0; j < kScanItems; j++) {
             const int i = base + j * kSmallScanThreads + t;
-            const Out x = i < n ? (Out)in[i] : (Out)0;
-            if (wide && i < n) wide[i] = x;
-            tile[j * kSmallScanThreads + t] = x;
+            if (wide && i < n) wide[i] = nx[j];
+            tile[j * kSmallScanThreads + t] = nx[j];
+        }
+        if (base + kTile < n) {
+#pragma unroll
+            for (int j = 0; j < kScanItems; j++) {
+                const int i = base + kTile + j * kSmallScanThreads + t;
+                nx[j] = i < n ? (Out)in[i] : (Out)0;
+            }
         }
         __syncthreads();
         Out v[kScanItems], sum = 0;
@@ -2828,7 +2850,7 @@ __device__ __forceinline__ void k_small_scan(MG<kSmallScanThreads, 1> mg, const 
             if (i < n) out[i] = tile[j * kSmallScanThreads + t];
         }
         carry += tot;
-        __syncthreads();  // the tile is reloaded
+        __syncthreads();  // the tile is rewritten
     }
     if (t == kSmallScanThreads - 1) {
         if (out_total) out_total[0] = (int64_t)carry;
