@@ -5,6 +5,7 @@ shape after MU/MUC and after reduction)."""
 from __future__ import annotations
 
 import ctypes as ct
+import threading
 
 import numpy as np
 
@@ -12,6 +13,12 @@ from . import _native
 
 _dp = ct.POINTER(ct.c_double)
 _ip = ct.POINTER(ct.c_int)
+
+# The engine draws its construction-time random values (the root point and the first
+# perturbation vector) from glibc's rand(), whose state is shared by every thread of the
+# process.  Seeding and creating under one lock keeps each estimator's draws its own seed's
+# when estimators are created from several threads at once.
+_CREATE_LOCK = threading.Lock()
 
 
 def _d(a):
@@ -22,11 +29,12 @@ def _d(a):
 class GpuEstimator:
     def __init__(self, num_steps, d, pncc, p, A0, p0, b0, cmcc=0, seed=None):
         L = _native.lib()
-        if seed is not None:
-            _native.libc().srand(ct.c_uint(seed))
         self.d, self.pncc, self.p, self.cmcc = d, pncc, p, cmcc
         a0, p0_, b0_ = _d(A0), _d(p0), _d(b0)
-        self.h = L.mce_estimator_create(a0[1], p0_[1], b0_[1], int(num_steps), d, cmcc, pncc, p, False)
+        with _CREATE_LOCK:
+            if seed is not None:
+                _native.libc().srand(ct.c_uint(seed))
+            self.h = L.mce_estimator_create(a0[1], p0_[1], b0_[1], int(num_steps), d, cmcc, pncc, p, False)
         if not self.h:
             raise RuntimeError("mce_estimator_create failed: " + L.mce_last_error().decode())
         self.shape_range = L.mce_shape_range(self.h)

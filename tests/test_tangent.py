@@ -83,7 +83,9 @@ def test_tangent_gradient_matches_central_differences_on_lti_windows():
     print("case 64 windows x 14 parameters", flush=True)
     Lt, Gt = tangent_gradients(ss, num_threads=8)
     d = loglik_gradient(ss, h=H_STEP, num_threads=8, details=True)
-    assert np.array_equal(Lt, d["L"])
+    diff = np.flatnonzero(Lt != d["L"])
+    assert diff.size == 0, ("windows whose forward log-likelihoods differ", diff.tolist(),
+                            (Lt[diff] - d["L"][diff]).tolist())
     assert d["valid"].mean() > 0.9
     worst = 0.0
     for i in range(len(ss)):
@@ -95,6 +97,20 @@ def test_tangent_gradient_matches_central_differences_on_lti_windows():
             assert err <= _fd_tol(d["L"][i], hj, go, 1e-5), f"{ss[i].name} param {j}: {Gt[i, j]} vs {go} (h {hj})"
             worst = max(worst, err / max(1.0, abs(go)))
     print(f"  worst relative difference {worst:.2e}", flush=True)
+
+
+def test_windows_built_on_several_threads_equal_one_thread():
+    """Estimators created from 8 host threads at once draw their construction-time random
+    values from their own seeds: L and the gradient equal the one-thread run bit for bit,
+    on every one of three repeats."""
+    from cauchyfriendly_amd.likelihood import tangent_gradients
+    ss = W.lti_sequences(count=64)
+    print("case 64 windows, 1 thread against 8 threads x 3", flush=True)
+    L1, G1 = tangent_gradients(ss, num_threads=1)
+    for rep in range(3):
+        L8, G8 = tangent_gradients(ss, num_threads=8)
+        diff = np.flatnonzero((L8 != L1) | np.any(G8 != G1, axis=1))
+        assert diff.size == 0, ("repeat", rep, "windows that differ", diff.tolist())
 
 
 def test_tangent_gradient_matches_restatement_central_differences():
@@ -130,7 +146,9 @@ def test_scale_tangent_gradient_with_two_noises_and_two_measurements():
     ws = [base, dataclasses.replace(base, zs=base.zs * 1.1, name="ltv5_p2_z11")]
     Lt, Gt = tangent_gradients(ws, dirs_of=scale_directions, num_threads=2)
     d = scale_gradient(ws, h=H_STEP, num_threads=2, details=True)
-    assert np.array_equal(Lt, d["L"])
+    diff = np.flatnonzero(Lt != d["L"])
+    assert diff.size == 0, ("windows whose forward log-likelihoods differ", diff.tolist(),
+                            (Lt[diff] - d["L"][diff]).tolist())
     for i, s in enumerate(ws):
         print(f"case {s.name}: tangent {Gt[i]} fd {d['grad'][i]}", flush=True)
         for j in range(Gt.shape[1]):
