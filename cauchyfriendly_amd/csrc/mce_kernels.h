@@ -1672,14 +1672,33 @@ __device__ __forceinline__ void k_ftr_inedges(MG<1024, 1> mg, int n, const int* 
     const int bk = bucket_of(boff, nb, ig);
     const int beg = boff[bk], nbk = boff[bk + 1] - beg;
     const int ci = ftr_chunk_of(ig - beg, nbk);
-    for (int e = eoff[ig]; e < eoff[ig] + ecnt[ig]; e++) {
-        const int jg = edges[e];
-        if (ftr_chunk_of(jg - beg, nbk) != ci) continue;
+    // Edges in batches of kInBatch: a batch's loads, then its atomics, are all in flight
+    // before the first result is used.  The order of a term's in-edges is immaterial (the
+    // rounds only ask whether any of them is a root, or all are claimed).
+    constexpr int kInBatch = 4;
+    const int e0 = eoff[ig], e1 = e0 + ecnt[ig];
+    for (int e = e0; e < e1; e += kInBatch) {
+        int jg[kInBatch];
+#pragma unroll
+        for (int u = 0; u < kInBatch; u++) jg[u] = e + u < e1 ? edges[e + u] : -1;
+#pragma unroll
+        for (int u = 0; u < kInBatch; u++)
+            if (jg[u] >= 0 && ftr_chunk_of(jg[u] - beg, nbk) != ci) jg[u] = -1;
         if (pass == 0) {
-            atomicAdd(&indeg[jg], 1);
+#pragma unroll
+            for (int u = 0; u < kInBatch; u++)
+                if (jg[u] >= 0) atomicAdd(&indeg[jg[u]], 1);
         } else {
-            const int slot = atomicAdd(&cursor[jg], 1);
-            inedges[inoff[jg] + slot] = ig;
+            int slot[kInBatch], base[kInBatch];
+#pragma unroll
+            for (int u = 0; u < kInBatch; u++)
+                if (jg[u] >= 0) {
+                    slot[u] = atomicAdd(&cursor[jg[u]], 1);
+                    base[u] = inoff[jg[u]];
+                }
+#pragma unroll
+            for (int u = 0; u < kInBatch; u++)
+                if (jg[u] >= 0) inedges[base[u] + slot[u]] = ig;
         }
     }
 }
