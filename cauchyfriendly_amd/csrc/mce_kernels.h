@@ -1745,14 +1745,23 @@ __device__ __forceinline__ void k_ftr_round(MG<1024, 1> mg, int n, const int* __
     if (j >= n) return;
     uint8_t s = st_in[j];
     if (s == kUnknown) {
+        // in-edges in batches of kRoundBatch: a batch's index loads, then its state loads,
+        // are all in flight before the first is tested; padding counts as claimed
+        constexpr int kRoundBatch = 4;
         bool any_root = false, all_claimed = true;
-        for (int e = inoff[j]; e < inoff[j] + indeg[j]; e++) {
-            const uint8_t si = st_in[inedges[e]];
-            if (si == kRoot) {
-                any_root = true;
-                break;
+        const int e0 = inoff[j], e1 = e0 + indeg[j];
+        for (int e = e0; e < e1 && !any_root; e += kRoundBatch) {
+            int src[kRoundBatch];
+#pragma unroll
+            for (int u = 0; u < kRoundBatch; u++) src[u] = e + u < e1 ? inedges[e + u] : -1;
+            uint8_t si[kRoundBatch];
+#pragma unroll
+            for (int u = 0; u < kRoundBatch; u++) si[u] = src[u] >= 0 ? st_in[src[u]] : (uint8_t)kClaimed;
+#pragma unroll
+            for (int u = 0; u < kRoundBatch; u++) {
+                any_root |= si[u] == kRoot;
+                all_claimed &= si[u] == kClaimed;
             }
-            if (si != kClaimed) all_claimed = false;
         }
         s = any_root ? kClaimed : (all_claimed ? kRoot : kUnknown);
         if (s == kUnknown) atomicAdd(unresolved, 1);
