@@ -1790,7 +1790,16 @@ __device__ __forceinline__ void k_ftr_claim(MG<1024, 1> mg, int n, const int* __
     const int beg = boff[bk], nbk = boff[bk + 1] - beg;
     const int il = ig - beg;
     int* Fc = F + (int64_t)ftr_chunk_of(il, nbk) * n;
-    for (int e = eoff[ig]; e < eoff[ig] + ecnt[ig]; e++) atomicMin(&Fc[edges[e]], il);
+    // edges in batches of four: the batch's loads are in flight together, then its minima
+    const int e0 = eoff[ig], e1 = e0 + ecnt[ig];
+    for (int e = e0; e < e1; e += 4) {
+        int jg[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) jg[u] = e + u < e1 ? edges[e + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (jg[u] >= 0) atomicMin(&Fc[jg[u]], il);
+    }
 }
 
 // One level of the flag-array merge tree of every 8-chunk bucket,
